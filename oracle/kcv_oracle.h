@@ -1,0 +1,119 @@
+/*
+ * CPU oracle for the KleidiCV u8 image hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a scalar restatement of the reference
+ * algorithms, used by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py as the checker.  The product library never links, loads or
+ * calls it.
+ *
+ * Enum values follow kleidicv/include/kleidicv/ctypes.h:60-87 (border and
+ * interpolation) and :149-360 (colour conversion flags).
+ */
+#ifndef KCV_ORACLE_H
+#define KCV_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a1: border index mapping (median_blur_border_handling.h:14-56). */
+long orc_border_index(long index, long limit, int border);
+
+/* a2-a4: Gaussian blur, binomial (sigma == 0) and sigma (fixed kernels). */
+int orc_gaussian_blur_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                         size_t dst_stride, size_t width, size_t height,
+                         size_t channels, size_t kernel_size, float sigma,
+                         int border);
+
+/* a4: host half-kernel generator (sigma.h:27-79). Returns 0 for "copy". */
+int orc_gaussian_half_kernel(uint8_t *half_kernel, size_t half_size,
+                             float sigma);
+
+/* a5: separable filter, any odd size (closed form min(255, sum ky kx src)). */
+int orc_separable_filter_2d_u8(const uint8_t *src, size_t src_stride,
+                               uint8_t *dst, size_t dst_stride, size_t width,
+                               size_t height, size_t channels,
+                               const uint8_t *kernel_x, size_t kernel_width,
+                               const uint8_t *kernel_y, size_t kernel_height,
+                               int border);
+
+/* a6: median blur by brute-force sort. */
+int orc_median_blur_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                       size_t dst_stride, size_t width, size_t height,
+                       size_t channels, size_t kernel_size, int border);
+
+/* a7-a9: linear resize with the reference's path selection. */
+int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
+                         size_t src_width, size_t src_height, uint8_t *dst,
+                         size_t dst_stride, size_t dst_width,
+                         size_t dst_height, size_t channels);
+
+/* a9: the generic path's column table, one entry per destination element
+ * after the row kernel's store order: table base + tbl index of the left and
+ * right tap (unmasked), the 8-bit fraction, and zero-flags (bit 0: the left
+ * tbl index is beyond the vector load and reads 0; bit 1: same for the
+ * right).  ratio <= 0 selects the dispatcher's choice; otherwise ratio and
+ * upsize force a generator variant.  Returns dst_width*C. */
+size_t orc_resize_generic_columns(size_t src_width, size_t dst_width,
+                                  size_t channels, int ratio, int upsize,
+                                  int32_t *left, int32_t *right,
+                                  uint8_t *frac, uint8_t *zero);
+
+/* a10: remap with 16-bit integer coordinates and 5+5 bit fractions. */
+int orc_remap_s16point5_u8(const uint8_t *src, size_t src_stride,
+                           size_t src_width, size_t src_height, uint8_t *dst,
+                           size_t dst_stride, size_t dst_width,
+                           size_t dst_height, size_t channels,
+                           const int16_t *mapxy, size_t mapxy_stride,
+                           const uint16_t *mapfrac, size_t mapfrac_stride,
+                           int border, const uint8_t *border_value);
+
+/* a11: remap with float maps (1..4 channels; 3-4 are per-plane extensions). */
+int orc_remap_f32_u8(const uint8_t *src, size_t src_stride, size_t src_width,
+                     size_t src_height, uint8_t *dst, size_t dst_stride,
+                     size_t dst_width, size_t dst_height, size_t channels,
+                     const float *mapx, size_t mapx_stride, const float *mapy,
+                     size_t mapy_stride, int interpolation, int border,
+                     const uint8_t *border_value);
+
+/* a12: warp perspective (1..4 channels; 2-4 are per-plane extensions). */
+int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
+                            size_t src_width, size_t src_height, uint8_t *dst,
+                            size_t dst_stride, size_t dst_width,
+                            size_t dst_height, const float transformation[9],
+                            size_t channels, int interpolation, int border,
+                            const uint8_t *border_value);
+
+/* a13: NV12/NV21 to RGB/BGR/RGBA/BGRA. */
+int orc_yuv_semiplanar_to_rgb_u8(const uint8_t *src_y, size_t src_y_stride,
+                                 const uint8_t *src_uv, size_t src_uv_stride,
+                                 uint8_t *dst, size_t dst_stride, size_t width,
+                                 size_t height, int color_format);
+
+/* Stripe variants: destination rows [y_begin, y_end) only, the rest of the
+ * image is read as in the full call.  The CPU baseline runs these on
+ * std::thread-style row stripes (kleidicv_thread.cpp:60-75). */
+int orc_gaussian_blur_u8_stripe(const uint8_t *src, size_t src_stride,
+                                uint8_t *dst, size_t dst_stride, size_t width,
+                                size_t height, size_t channels,
+                                size_t kernel_size, float sigma, int border,
+                                size_t y_begin, size_t y_end);
+int orc_median_blur_u8_stripe(const uint8_t *src, size_t src_stride,
+                              uint8_t *dst, size_t dst_stride, size_t width,
+                              size_t height, size_t channels,
+                              size_t kernel_size, int border, size_t y_begin,
+                              size_t y_end);
+int orc_separable_filter_2d_u8_stripe(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    int border, size_t y_begin, size_t y_end);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KCV_ORACLE_H */

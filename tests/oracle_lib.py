@@ -1,0 +1,159 @@
+"""ctypes bindings for the CPU oracle (oracle/_build/libkcv_oracle.so).
+
+Test infrastructure: the oracle is the checker, never the product.  Each
+wrapper takes and returns numpy arrays in the reference's interleaved HWC
+layout.
+"""
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libkcv_oracle.so")
+
+BORDER = {"CONSTANT": 0, "REPLICATE": 1, "REFLECT": 2, "WRAP": 3,
+          "REVERSE": 4, "TRANSPARENT": 5, "NONE": 6}
+INTERP = {"NEAREST": 0, "LINEAR": 1}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        _lib = C.CDLL(ORACLE_SO)
+        sz, u8p, vp = C.c_size_t, C.POINTER(C.c_uint8), C.c_void_p
+        _lib.orc_border_index.restype = C.c_long
+        _lib.orc_border_index.argtypes = [C.c_long, C.c_long, C.c_int]
+        _lib.orc_resize_generic_columns.restype = sz
+        _lib.orc_resize_generic_columns.argtypes = [sz, sz, sz, C.c_int, C.c_int,
+                                                    vp, vp, vp, vp]
+        _lib.orc_gaussian_half_kernel.argtypes = [vp, sz, C.c_float]
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else C.c_void_p(0)
+
+
+def _sz(*vals):
+    return [C.c_size_t(int(v)) for v in vals]
+
+
+def hwc(img):
+    """(h, w, c) view of a 2-D or 3-D uint8 array."""
+    return img if img.ndim == 3 else img[:, :, None]
+
+
+def gaussian_blur(src, kernel, sigma=0.0, border="REVERSE", y_range=None):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    if y_range is None:
+        err = lib().orc_gaussian_blur_u8(_p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c, kernel),
+                                         C.c_float(sigma), C.c_int(BORDER[border]))
+    else:
+        err = lib().orc_gaussian_blur_u8_stripe(
+            _p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c, kernel), C.c_float(sigma),
+            C.c_int(BORDER[border]), *_sz(*y_range))
+    return err, dst.reshape(src.shape)
+
+
+def separable_filter_2d(src, kx, ky, border="REPLICATE"):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    kx = np.ascontiguousarray(kx, dtype=np.uint8)
+    ky = np.ascontiguousarray(ky, dtype=np.uint8)
+    err = lib().orc_separable_filter_2d_u8(_p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c),
+                                           _p(kx), *_sz(len(kx)), _p(ky), *_sz(len(ky)),
+                                           C.c_int(BORDER[border]))
+    return err, dst.reshape(src.shape)
+
+
+def median_blur(src, kernel, border="REPLICATE"):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    err = lib().orc_median_blur_u8(_p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c, kernel),
+                                   C.c_int(BORDER[border]))
+    return err, dst.reshape(src.shape)
+
+
+def resize_linear(src, dst_w, dst_h):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros((dst_h, dst_w, c), dtype=np.uint8)
+    err = lib().orc_resize_linear_u8(_p(s), *_sz(w * c, w, h), _p(dst),
+                                     *_sz(dst_w * c, dst_w, dst_h, c))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def resize_columns(src_w, dst_w, channels, ratio=0, upsize=False):
+    n = dst_w * channels
+    left = np.zeros(n, np.int32)
+    right = np.zeros(n, np.int32)
+    frac = np.zeros(n, np.uint8)
+    zero = np.zeros(n, np.uint8)
+    lib().orc_resize_generic_columns(src_w, dst_w, channels, int(ratio), int(upsize),
+                                     _p(left), _p(right), _p(frac), _p(zero))
+    return left, right, frac, zero
+
+
+def remap_s16point5(src, mapxy, mapfrac, border="CONSTANT", border_value=(0, 0, 0, 0)):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dh, dw = mapfrac.shape
+    dst = np.zeros((dh, dw, c), dtype=np.uint8)
+    mapxy = np.ascontiguousarray(mapxy, dtype=np.int16)
+    mapfrac = np.ascontiguousarray(mapfrac, dtype=np.uint16)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = lib().orc_remap_s16point5_u8(_p(s), *_sz(w * c, w, h), _p(dst), *_sz(dw * c, dw, dh, c),
+                                       _p(mapxy), *_sz(dw * 4), _p(mapfrac), *_sz(dw * 2),
+                                       C.c_int(BORDER[border]), _p(bv))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def remap_f32(src, mapx, mapy, interpolation="LINEAR", border="REPLICATE",
+              border_value=(0, 0, 0, 0)):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dh, dw = mapx.shape
+    dst = np.zeros((dh, dw, c), dtype=np.uint8)
+    mapx = np.ascontiguousarray(mapx, dtype=np.float32)
+    mapy = np.ascontiguousarray(mapy, dtype=np.float32)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = lib().orc_remap_f32_u8(_p(s), *_sz(w * c, w, h), _p(dst), *_sz(dw * c, dw, dh, c),
+                                 _p(mapx), *_sz(dw * 4), _p(mapy), *_sz(dw * 4),
+                                 C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]), _p(bv))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def warp_perspective(src, transform, dst_w, dst_h, interpolation="LINEAR",
+                     border="REPLICATE", border_value=(0, 0, 0, 0)):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros((dst_h, dst_w, c), dtype=np.uint8)
+    t = np.ascontiguousarray(transform, dtype=np.float32)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = lib().orc_warp_perspective_u8(_p(s), *_sz(w * c, w, h), _p(dst),
+                                        *_sz(dst_w * c, dst_w, dst_h), _p(t), *_sz(c),
+                                        C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]),
+                                        _p(bv))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def yuv_sp_to_rgb(y, uv, color_format=0x02):
+    y = np.ascontiguousarray(y, dtype=np.uint8)
+    uv = np.ascontiguousarray(uv, dtype=np.uint8)
+    h, w = y.shape
+    oc = 4 if color_format & 0x40 else 3
+    dst = np.zeros((h, w, oc), dtype=np.uint8)
+    err = lib().orc_yuv_semiplanar_to_rgb_u8(_p(y), *_sz(w), _p(uv), *_sz(uv.shape[1]), _p(dst),
+                                             *_sz(w * oc, w, h), C.c_int(color_format))
+    return err, dst
