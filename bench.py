@@ -1,0 +1,249 @@
+"""Benchmark of the u8 image hot path on one MI355X.
+
+A step is one pass of the full plain op set on 3840x2160 u8 images, one
+image per launch, inputs resident in HBM:
+  separable_filter_2d 7x7 (1 ch, seeded integer taps, REPLICATE)
+  median_blur 5x5 (1 ch, REPLICATE)
+  resize_linear 4 ch 2x up (-> 7680x4320) and 0.5x down (-> 1920x1080)
+  remap_s16point5 (1 ch, smooth seeded displacement <= 40 px, CONSTANT)
+  warp_perspective (4 ch, seeded homography: rotation <= 15 deg,
+                    perspective terms <= 1e-4, LINEAR, REPLICATE)
+  yuv_semiplanar_to_rgb (NV12 -> RGB)
+Inputs rotate over several distinct sets (> 512 MB in total) so reads come
+from HBM rather than the 256 MB Infinity Cache.  Kernel time per op is taken
+with HIP events on the stream the kernels run on.
+
+Prints one JSON line (metric, value in destination Mpix/s, per-op GB/s and
+% of the 8 TB/s HBM peak, the dominant kernel's roofline, a CPU baseline).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "yocto-mirrors_gitlab_kleidi_kleidicv_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+W, H = 3840, 2160
+METRIC = "Mpix/s + % HBM peak per op; batched/served vs plain Mpix/s; p99 latency"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--sets", type=int, default=3, help="distinct input sets rotated")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ops", default="", help="comma-separated subset of ops (profiling)")
+    return ap.parse_args()
+
+
+def make_inputs(torch, gen, dev):
+    """One seeded input set for every op, generated on the device."""
+    def u8(*shape):
+        return torch.randint(0, 256, shape, dtype=torch.uint8, device=dev, generator=gen)
+
+    s = {}
+    s["sep_src"] = u8(H, W)
+    s["med_src"] = u8(H, W)
+    s["up_src"] = u8(H, W, 4)
+    s["down_src"] = u8(H, W, 4)
+    s["remap_src"] = u8(H, W)
+    s["warp_src"] = u8(H, W, 4)
+    s["nv12_y"] = u8(H, W)
+    s["nv12_uv"] = u8(H // 2, W)
+    # smooth displacement, amplitude <= 40 px: sum of 4 seeded sinusoids
+    yy, xx = torch.meshgrid(torch.arange(H, device=dev, dtype=torch.float64),
+                            torch.arange(W, device=dev, dtype=torch.float64), indexing="ij")
+    r = torch.rand(16, generator=gen, device=dev, dtype=torch.float64)
+    fx = sum(10.0 * torch.sin((0.001 + 0.01 * r[4 * i]) * xx + (0.001 + 0.01 * r[4 * i + 1]) * yy
+                              + 6.28 * r[4 * i + 2]) for i in range(4))
+    fy = sum(10.0 * torch.cos((0.001 + 0.01 * r[4 * i + 1]) * xx + (0.001 + 0.01 * r[4 * i]) * yy
+                              + 6.28 * r[4 * i + 3]) for i in range(4))
+    ix = torch.floor((xx + fx) * 32).to(torch.int64)
+    iy = torch.floor((yy + fy) * 32).to(torch.int64)
+    s["mapxy"] = torch.stack([ix >> 5, iy >> 5], dim=-1).to(torch.int16).contiguous()
+    s["mapfrac"] = ((ix & 31) | ((iy & 31) << 5)).to(torch.int16).contiguous()
+    return s
+
+
+def seeded_params(np):
+    rng = np.random.default_rng(7)
+    kx = rng.integers(0, 3, 7).astype(np.uint8)
+    ky = rng.integers(0, 3, 7).astype(np.uint8)
+    a = np.deg2rad(rng.uniform(-15, 15))
+    cx, cy = W / 2, H / 2
+    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    Tc = np.array([[1, 0, cx], [0, 1, cy], [0, 0, 1]])
+    Ti = np.array([[1, 0, -cx], [0, 1, -cy], [0, 0, 1]])
+    P = np.array([[1, 0, 0], [0, 1, 0],
+                  [rng.uniform(-1e-4, 1e-4), rng.uniform(-1e-4, 1e-4), 1]])
+    T = (Tc @ R @ Ti) @ P
+    T = T / T[2, 2]
+    return kx, ky, T.astype(np.float32)
+
+
+# destination pixels and algorithmic (compulsory) bytes per launch
+OPS = [
+    # name, dst pixels, bytes
+    ("separable_7x7_1ch", W * H, 2 * W * H),
+    ("median_5x5_1ch", W * H, 2 * W * H),
+    ("resize_2x_up_4ch", 4 * W * H, 4 * W * H + 16 * W * H),
+    ("resize_0.5x_4ch", W * H // 4, 4 * W * H + W * H),
+    ("remap_s16point5_1ch", W * H, 8 * W * H),
+    ("warp_perspective_4ch", W * H, 8 * W * H),
+    ("nv12_to_rgb", W * H, (W * H * 9) // 2),
+]
+
+
+def run_op(K, torch, name, s, out, kx, ky, T, stream):
+    if name == "separable_7x7_1ch":
+        K.separable_filter_2d(s["sep_src"], kx, ky, "REPLICATE", dst=out[name], stream=stream,
+                              device_api=True)
+    elif name == "median_5x5_1ch":
+        K.median_blur(s["med_src"], 5, "REPLICATE", dst=out[name], stream=stream, device_api=True)
+    elif name == "resize_2x_up_4ch":
+        K.resize_linear(s["up_src"], 2 * W, 2 * H, dst=out[name], stream=stream, device_api=True)
+    elif name == "resize_0.5x_4ch":
+        K.resize_linear(s["down_src"], W // 2, H // 2, dst=out[name], stream=stream,
+                        device_api=True)
+    elif name == "remap_s16point5_1ch":
+        K.remap_s16point5(s["remap_src"], s["mapxy"], s["mapfrac"], "CONSTANT", (0, 0, 0, 0),
+                          dst=out[name], stream=stream, device_api=True)
+    elif name == "warp_perspective_4ch":
+        K.warp_perspective(s["warp_src"], T, W, H, "LINEAR", "REPLICATE", dst=out[name],
+                           stream=stream, device_api=True)
+    elif name == "nv12_to_rgb":
+        K.yuv_sp_to_rgb(s["nv12_y"], s["nv12_uv"], K.NV12_TO_RGB, dst=out[name], stream=stream,
+                        device_api=True)
+
+
+def cpu_baseline(np, s_host, kx, ky, T, ops):
+    """Single-threaded oracle on the host cores, one full 4K pass of each op."""
+    import oracle_lib as O
+    t_total, pix = 0.0, 0
+    for name, dst_pix, _ in ops:
+        t0 = time.perf_counter()
+        if name == "separable_7x7_1ch":
+            O.separable_filter_2d(s_host["sep_src"], kx, ky, "REPLICATE")
+        elif name == "median_5x5_1ch":
+            O.median_blur(s_host["med_src"], 5, "REPLICATE")
+        elif name == "resize_2x_up_4ch":
+            O.resize_linear(s_host["up_src"], 2 * W, 2 * H)
+        elif name == "resize_0.5x_4ch":
+            O.resize_linear(s_host["down_src"], W // 2, H // 2)
+        elif name == "remap_s16point5_1ch":
+            O.remap_s16point5(s_host["remap_src"], s_host["mapxy"],
+                              s_host["mapfrac"].view(np.uint16), "CONSTANT")
+        elif name == "warp_perspective_4ch":
+            O.warp_perspective(s_host["warp_src"], T, W, H, "LINEAR", "REPLICATE")
+        elif name == "nv12_to_rgb":
+            O.yuv_sp_to_rgb(s_host["nv12_y"], s_host["nv12_uv"], 0x02)
+        t_total += time.perf_counter() - t0
+        pix += dst_pix
+    return {"value": round(pix / t_total / 1e6, 2), "unit": "Mpix/s", "cores": 1,
+            "kind": "port",
+            "sample": "one full 3840x2160 pass of each of the %d ops, scalar C++ oracle, "
+                      "1 thread (%.1f s)" % (len(ops), t_total)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every op is per image and shards by "
+              "image/request with no exchange; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import kleidicv_hip as K
+
+    dev = torch.device("cuda:0")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1234)
+    kx, ky, T = seeded_params(np)
+    ops = [o for o in OPS if not args.ops or o[0] in args.ops.split(",")]
+    sets = [make_inputs(torch, gen, dev) for _ in range(args.sets)]
+    out = {
+        "separable_7x7_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
+        "median_5x5_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
+        "resize_2x_up_4ch": torch.empty(2 * H, 2 * W, 4, dtype=torch.uint8, device=dev),
+        "resize_0.5x_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
+        "remap_s16point5_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
+        "warp_perspective_4ch": torch.empty(H, W, 4, dtype=torch.uint8, device=dev),
+        "nv12_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
+    }
+    stream = torch.cuda.current_stream()
+    for i in range(args.warmup):
+        for name, _, _ in ops:
+            run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
+    torch.cuda.synchronize()
+
+    ev = {name: [] for name, _, _ in ops}
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for i in range(args.steps):
+        s = sets[i % len(sets)]
+        for name, _, _ in ops:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run_op(K, torch, name, s, out, kx, ky, T, stream)
+            e1.record(stream)
+            ev[name].append((e0, e1))
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = start.elapsed_time(stop) / args.steps
+
+    per_op = {}
+    total_pix = 0
+    for name, dst_pix, nbytes in ops:
+        ms = sum(a.elapsed_time(b) for a, b in ev[name]) / len(ev[name])
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        per_op[name] = {"ms": round(ms, 4), "mpix_s": round(dst_pix / (ms * 1e-3) / 1e6, 1),
+                        "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                        "bytes": nbytes}
+        total_pix += dst_pix
+    value = total_pix / (step_ms * 1e-3) / 1e6
+    dom = max(per_op, key=lambda n: per_op[n]["ms"])
+    d = per_op[dom]
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4),
+        "wall_s": round(wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded uniform u8 pixels, seeded maps and homography)",
+        "config": {"workload": "full plain op set, one 3840x2160 u8 image per launch "
+                               "(%d ops per step, %d rotating input sets)" % (len(ops), args.sets),
+                   "ops": [o[0] for o in ops]},
+        "per_op": per_op,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gb_s"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
+                     "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
+        result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,291 @@
+/*
+ * kleidicv_hip: MI355X (gfx950) implementation of KleidiCV's u8 image hot
+ * path behind the reference's C ABI.
+ *
+ * Three layers are declared here:
+ *   1. Drop-in `kleidicv_*` entry points with the reference's signatures,
+ *      argument meaning and error codes.  They accept host pointers (staged
+ *      through device memory) or device pointers (used in place) and return
+ *      when the result is in `dst`, as the reference's CPU functions do.
+ *   2. `kleidicv_thread_*` entry points with the reference's scheduler
+ *      argument.  The row-stripe scheduler is replaced by the GPU grid: `mt`
+ *      is validated and otherwise unused.
+ *   3. `kleidicv_hip_*` device-resident entry points: device pointers, an
+ *      explicit HIP stream, asynchronous.  They carry no counterpart in the
+ *      reference; they exist so callers can keep images in HBM and overlap
+ *      work (the batched and served paths build on them).
+ *
+ * Enum values are the reference's (kleidicv/include/kleidicv/ctypes.h).
+ */
+#ifndef KLEIDICV_HIP_KLEIDICV_H
+#define KLEIDICV_HIP_KLEIDICV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ctypes.h:26-41 */
+typedef enum {
+  KLEIDICV_OK = 0,
+  KLEIDICV_ERROR_NOT_IMPLEMENTED,
+  KLEIDICV_ERROR_NULL_POINTER,
+  KLEIDICV_ERROR_RANGE,
+  KLEIDICV_ERROR_ALLOCATION,
+  KLEIDICV_ERROR_ALIGNMENT,
+  KLEIDICV_ERROR_CONTEXT_MISMATCH,
+} kleidicv_error_t;
+
+/* ctypes.h:60-79 */
+typedef enum {
+  KLEIDICV_BORDER_TYPE_CONSTANT,
+  KLEIDICV_BORDER_TYPE_REPLICATE,
+  KLEIDICV_BORDER_TYPE_REFLECT,
+  KLEIDICV_BORDER_TYPE_WRAP,
+  KLEIDICV_BORDER_TYPE_REVERSE,
+  KLEIDICV_BORDER_TYPE_TRANSPARENT,
+  KLEIDICV_BORDER_TYPE_NONE,
+} kleidicv_border_type_t;
+
+/* ctypes.h:82-87 */
+typedef enum {
+  KLEIDICV_INTERPOLATION_NEAREST,
+  KLEIDICV_INTERPOLATION_LINEAR,
+} kleidicv_interpolation_type_t;
+
+/* ctypes.h:149-360: base format in the low nibble, modifier flags above.
+ * Only the semi-planar (NV12/NV21) conversions are on this path. */
+typedef enum {
+  KLEIDICV_COLOR_CONVERSION_FMT_YUV444 = 0x01,
+  KLEIDICV_COLOR_CONVERSION_FMT_YUV420SP = 0x02,
+  KLEIDICV_COLOR_CONVERSION_FMT_YUV420P = 0x03,
+  KLEIDICV_COLOR_CONVERSION_FMT_YUV422 = 0x04,
+  KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK = 0x0F,
+  KLEIDICV_COLOR_CONVERSION_FLAG_VU = 0x10,
+  KLEIDICV_COLOR_CONVERSION_FLAG_BGR = 0x20,
+  KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA = 0x40,
+  KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST = 0x80,
+  KLEIDICV_NV12_TO_RGB = 0x02,
+  KLEIDICV_NV12_TO_BGR = 0x22,
+  KLEIDICV_NV12_TO_RGBA = 0x42,
+  KLEIDICV_NV12_TO_BGRA = 0x62,
+  KLEIDICV_NV21_TO_RGB = 0x12,
+  KLEIDICV_NV21_TO_BGR = 0x32,
+  KLEIDICV_NV21_TO_RGBA = 0x52,
+  KLEIDICV_NV21_TO_BGRA = 0x72,
+} kleidicv_color_conversion_t;
+
+/* kleidicv.h:103-113 */
+#define KLEIDICV_MAX_IMAGE_PIXELS ((1ULL << 48) - 1)
+#define KLEIDICV_MAXIMUM_CHANNEL_COUNT (4)
+#define KLEIDICV_RESIZE_MAX_WIDTH_OR_HEIGHT ((1ULL << 24) - 1)
+
+/* ------------------------------------------------------------------------
+ * 1. Drop-in C API
+ * ------------------------------------------------------------------------ */
+
+/* Replaces kleidicv.h:2159-2162 (src/filters/gaussian_blur_api.cpp:59-67).
+ * Kernel sizes 3, 5, 7, 9 (binomial when sigma == 0), 15 and 21; borders
+ * REPLICATE, REFLECT, WRAP, REVERSE. */
+kleidicv_error_t kleidicv_gaussian_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, float sigma_x, float sigma_y,
+    kleidicv_border_type_t border_type);
+
+/* Replaces kleidicv.h:2091-2095 (src/filters/separable_filter_2d_api.cpp:
+ * 46-54).  5x5 as the reference; 3x3 and 7x7 are extensions computed with the
+ * same saturating arithmetic. */
+kleidicv_error_t kleidicv_separable_filter_2d_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type);
+
+/* Replaces kleidicv.h:2271 (src/filters/median_blur_api.cpp:121-130).
+ * Kernel sizes 3, 5 and 7 (the reference's sorting-network sizes). */
+kleidicv_error_t kleidicv_median_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, kleidicv_border_type_t border_type);
+
+/* Replaces kleidicv.h:2483-2487 (src/resize/resize_linear_api.cpp:269-277). */
+kleidicv_error_t kleidicv_resize_linear_u8(const uint8_t *src,
+                                           size_t src_stride, size_t src_width,
+                                           size_t src_height, uint8_t *dst,
+                                           size_t dst_stride, size_t dst_width,
+                                           size_t dst_height, size_t channels);
+
+/* Replaces kleidicv.h:2899-2904 (src/transform/warp_perspective_api.cpp:
+ * 17-34).  1 channel as the reference; 2-4 channels are per-plane
+ * extensions. */
+kleidicv_error_t kleidicv_warp_perspective_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    const float transformation[9], size_t channels,
+    kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value);
+
+/* Function-pointer data symbols, as KLEIDICV_API_DECLARATION declares them
+ * (kleidicv.h:126-130); initialised to this library's implementations. */
+
+/* kleidicv.h:1640-1645 (src/conversions/yuv_to_rgb_api.cpp:23-24) */
+extern kleidicv_error_t (*kleidicv_yuv_semiplanar_to_rgb_u8)(
+    const uint8_t *src_y, size_t src_y_stride, const uint8_t *src_uv,
+    size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
+    size_t height, kleidicv_color_conversion_t color_format);
+
+/* kleidicv.h:2715-2721 (src/transform/remap_api.cpp:17-19) */
+extern kleidicv_error_t (*kleidicv_remap_s16point5_u8)(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const int16_t *mapxy, size_t mapxy_stride,
+    const uint16_t *mapfrac, size_t mapfrac_stride,
+    kleidicv_border_type_t border_type, const uint8_t *border_value);
+
+/* kleidicv.h:2782-2789 (src/transform/remap_api.cpp:25-27).  1-2 channels
+ * as the reference; 3-4 channels are per-plane extensions. */
+extern kleidicv_error_t (*kleidicv_remap_f32_u8)(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const float *mapx, size_t mapx_stride, const float *mapy,
+    size_t mapy_stride, kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value);
+
+/* ------------------------------------------------------------------------
+ * 2. Threaded API (kleidicv_thread/include/kleidicv_thread/kleidicv_thread.h)
+ * ------------------------------------------------------------------------ */
+
+/* kleidicv_thread.h:55-97 */
+typedef kleidicv_error_t (*kleidicv_thread_callback)(unsigned begin,
+                                                     unsigned end, void *data);
+typedef kleidicv_error_t (*kleidicv_thread_parallel)(
+    kleidicv_thread_callback callback, void *callback_data, void *parallel_data,
+    unsigned task_count);
+typedef struct {
+  kleidicv_thread_parallel parallel;
+  void *parallel_data;
+} kleidicv_thread_multithreading;
+
+/* kleidicv_thread.h:459 */
+kleidicv_error_t kleidicv_thread_gaussian_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, float sigma_x, float sigma_y,
+    kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:467 */
+kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:496 */
+kleidicv_error_t kleidicv_thread_median_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, kleidicv_border_type_t border_type,
+    kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:542 */
+kleidicv_error_t kleidicv_thread_resize_linear_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:578 */
+kleidicv_error_t kleidicv_thread_remap_s16point5_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const int16_t *mapxy, size_t mapxy_stride,
+    const uint16_t *mapfrac, size_t mapfrac_stride,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:602 */
+kleidicv_error_t kleidicv_thread_remap_f32_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const float *mapx, size_t mapx_stride, const float *mapy,
+    size_t mapy_stride, kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:622 */
+kleidicv_error_t kleidicv_thread_warp_perspective_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    const float transformation[9], size_t channels,
+    kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:215 */
+kleidicv_error_t kleidicv_thread_yuv_semiplanar_to_rgb_u8(
+    const uint8_t *src_y, size_t src_y_stride, const uint8_t *src_uv,
+    size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
+    size_t height, kleidicv_color_conversion_t color_format,
+    kleidicv_thread_multithreading mt);
+
+/* ------------------------------------------------------------------------
+ * 3. Device-resident API: device pointers, `stream` is a hipStream_t (NULL =
+ *    the library's own stream), no synchronisation.  Same validation and
+ *    error codes as the drop-in functions.
+ * ------------------------------------------------------------------------ */
+
+kleidicv_error_t kleidicv_hip_gaussian_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, float sigma_x, float sigma_y,
+    kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_median_blur_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_resize_linear_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, void *stream);
+
+kleidicv_error_t kleidicv_hip_remap_s16point5_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const int16_t *mapxy, size_t mapxy_stride,
+    const uint16_t *mapfrac, size_t mapfrac_stride,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    void *stream);
+
+kleidicv_error_t kleidicv_hip_remap_f32_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, const float *mapx, size_t mapx_stride, const float *mapy,
+    size_t mapy_stride, kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    void *stream);
+
+kleidicv_error_t kleidicv_hip_warp_perspective_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    const float transformation[9], size_t channels,
+    kleidicv_interpolation_type_t interpolation,
+    kleidicv_border_type_t border_type, const uint8_t *border_value,
+    void *stream);
+
+kleidicv_error_t kleidicv_hip_yuv_semiplanar_to_rgb_u8(
+    const uint8_t *src_y, size_t src_y_stride, const uint8_t *src_uv,
+    size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
+    size_t height, kleidicv_color_conversion_t color_format, void *stream);
+
+/* Library stream used when `stream` is NULL, and a helper to wait on it. */
+void *kleidicv_hip_default_stream(void);
+kleidicv_error_t kleidicv_hip_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KLEIDICV_HIP_KLEIDICV_H */

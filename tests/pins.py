@@ -1,0 +1,209 @@
+"""Independent numpy formulations used to pin the oracle.
+
+Each function restates what the reference's own tests compute for an op, or
+what the mathematics fixes, in a form that does not share code or structure
+with the oracle: borders come from numpy.pad, convolutions from
+sliding-window sums, medians from numpy.median, float references in double
+precision.  They run only on small inputs.
+"""
+
+import numpy as np
+
+# KleidiCV border -> numpy.pad mode (adapters/opencv/kleidicv_hal.cpp:405-435
+# and test/framework/border.cpp:117-285 give the correspondences).
+PAD_MODE = {"REPLICATE": "edge", "REFLECT": "symmetric", "WRAP": "wrap",
+            "REVERSE": "reflect"}
+
+BINOMIAL = {3: ([1, 2, 1], 4), 5: ([1, 4, 6, 4, 1], 8),
+            7: ([2, 7, 14, 18, 14, 7, 2], 12), 9: ([4, 13, 30, 51, 60, 51, 30, 13, 4], 16)}
+
+
+def _hwc(a):
+    return a if a.ndim == 3 else a[:, :, None]
+
+
+def padded(src, r, border):
+    s = _hwc(src)
+    return np.pad(s, ((r, r), (r, r), (0, 0)), mode=PAD_MODE[border])
+
+
+def windows(src, k, border):
+    """(h, w, c, k, k) view of every k x k window."""
+    p = padded(src, k // 2, border)
+    return np.lib.stride_tricks.sliding_window_view(p, (k, k), axis=(0, 1))
+
+
+def binomial_blur(src, k, border):
+    taps, shift = BINOMIAL[k]
+    w2 = np.outer(taps, taps).astype(np.int64)
+    acc = (windows(src, k, border).astype(np.int64) * w2).sum(axis=(-1, -2))
+    out = ((acc + (1 << (shift - 1))) >> shift).astype(np.uint8)
+    return out.reshape(src.shape)
+
+
+def separable_two_stage(src, kx, ky, border):
+    """The reference's vector path: u16-saturated vertical sum, then a
+    horizontal sum saturated to u8 (separable_filter_2d_neon.cpp:37-69)."""
+    k = len(kx)
+    r = k // 2
+    p = padded(src, r, border).astype(np.int64)
+    h, w = src.shape[0], src.shape[1]
+    vert = np.zeros((h, p.shape[1], p.shape[2]), np.int64)
+    for i in range(k):
+        vert += int(ky[i]) * p[i:i + h]
+    vert = np.minimum(vert, 65535)
+    horiz = np.zeros((h, w, p.shape[2]), np.int64)
+    for j in range(k):
+        horiz += int(kx[j]) * vert[:, j:j + w]
+    return np.minimum(horiz, 255).astype(np.uint8).reshape(src.shape)
+
+
+def median(src, k, border):
+    win = windows(src, k, border).reshape(src.shape[0], src.shape[1], -1, k * k)
+    return np.median(win, axis=-1).astype(np.uint8).reshape(src.shape)
+
+
+def remap_s16point5(src, mapxy, mapfrac, border, border_value):
+    """Closed form of test/api/test_remap.cpp:638-688."""
+    s = _hwc(src).astype(np.int64)
+    h, w, c = s.shape
+    x = mapxy[..., 0].astype(np.int64)
+    y = mapxy[..., 1].astype(np.int64)
+    fx = (mapfrac & 31).astype(np.int64)[..., None]
+    fy = ((mapfrac >> 5) & 31).astype(np.int64)[..., None]
+
+    def get(xx, yy):
+        if border == "REPLICATE":
+            return s[np.clip(yy, 0, h - 1), np.clip(xx, 0, w - 1)]
+        inside = (xx >= 0) & (xx < w) & (yy >= 0) & (yy < h)
+        v = s[np.clip(yy, 0, h - 1), np.clip(xx, 0, w - 1)]
+        bv = np.array(border_value[:c], np.int64)
+        return np.where(inside[..., None], v, bv)
+
+    a, b, cc, d = get(x, y), get(x + 1, y), get(x, y + 1), get(x + 1, y + 1)
+    r = ((32 - fx) * (32 - fy) * a + fx * (32 - fy) * b + (32 - fx) * fy * cc +
+         fx * fy * d + 512) // 1024
+    out = r.astype(np.uint8)
+    return out if src.ndim == 3 else out[..., 0]
+
+
+def _double_sample(src, xf, yf, border, border_value, interpolation):
+    s = _hwc(src).astype(np.float64)
+    h, w, c = s.shape
+
+    def get(xx, yy):
+        if border == "REPLICATE":
+            return s[np.clip(yy, 0, h - 1), np.clip(xx, 0, w - 1)]
+        inside = (xx >= 0) & (xx < w) & (yy >= 0) & (yy < h)
+        v = s[np.clip(yy, 0, h - 1), np.clip(xx, 0, w - 1)]
+        return np.where(inside[..., None], v, np.array(border_value[:c], np.float64))
+
+    xf = np.asarray(xf, np.float64)
+    yf = np.asarray(yf, np.float64)
+    if interpolation == "NEAREST":
+        # round half away from zero, as FCVTA does
+        xi = np.clip(np.sign(xf) * np.floor(np.abs(xf) + 0.5), -2 ** 40, 2 ** 40).astype(np.int64)
+        yi = np.clip(np.sign(yf) * np.floor(np.abs(yf) + 0.5), -2 ** 40, 2 ** 40).astype(np.int64)
+        out = get(xi, yi)
+    else:
+        x0 = np.clip(np.floor(xf), -2 ** 40, 2 ** 40).astype(np.int64)
+        y0 = np.clip(np.floor(yf), -2 ** 40, 2 ** 40).astype(np.int64)
+        ax = (xf - np.floor(xf))[..., None]
+        ay = (yf - np.floor(yf))[..., None]
+        a, b = get(x0, y0), get(x0 + 1, y0)
+        cc, d = get(x0, y0 + 1), get(x0 + 1, y0 + 1)
+        top = a + (b - a) * ax
+        bot = cc + (d - cc) * ax
+        out = np.floor(top + (bot - top) * ay + 0.5)
+    out = np.clip(out, 0, 255).astype(np.uint8)
+    return out if src.ndim == 3 else out[..., 0]
+
+
+def remap_f32_double(src, mapx, mapy, interpolation, border, border_value):
+    """Double-precision counterpart of test/api/test_remap.cpp:1183-1244."""
+    return _double_sample(src, mapx, mapy, border, border_value, interpolation)
+
+
+def warp_double(src, t, dst_w, dst_h, interpolation, border, border_value):
+    """Double-precision homography (test/api/test_warp_perspective.cpp:122-160)."""
+    T = np.asarray(t, np.float64).reshape(3, 3)
+    yy, xx = np.mgrid[0:dst_h, 0:dst_w].astype(np.float64)
+    tw = T[2, 0] * xx + T[2, 1] * yy + T[2, 2]
+    xf = (T[0, 0] * xx + T[0, 1] * yy + T[0, 2]) / tw
+    yf = (T[1, 0] * xx + T[1, 1] * yy + T[1, 2]) / tw
+    return _double_sample(src, xf, yf, border, border_value, interpolation)
+
+
+def nv12_float(y, uv, vu=False, bgr=False, alpha=False):
+    """BT.601 limited range with the coefficients the reference rounds to
+    20 bits (yuv42x_coefficients.h:46-71), evaluated in double."""
+    h, w = y.shape
+    Y = np.maximum(y.astype(np.float64) - 16, 0)
+    U = uv[np.arange(h) // 2][:, (np.arange(w) // 2) * 2 + (1 if vu else 0)].astype(np.float64) - 128
+    V = uv[np.arange(h) // 2][:, (np.arange(w) // 2) * 2 + (0 if vu else 1)].astype(np.float64) - 128
+    R = 1.164 * Y + 1.596 * V
+    G = 1.164 * Y - 0.391 * U - 0.813 * V
+    B = 1.164 * Y + 2.018 * U
+    if bgr:
+        R, B = B, R
+    planes = [R, G, B]
+    out = np.stack([np.clip(np.floor(p + 0.5), 0, 255) for p in planes], axis=-1)
+    if alpha:
+        out = np.concatenate([out, np.full((h, w, 1), 255.0)], axis=-1)
+    return out.astype(np.uint8)
+
+
+def resize_generator_direct(C, sw, dw, R, up):
+    """Direct-formula column table of the reference's generator unit test
+    (test/unit_neon/test_resize_linear_generic_u8.cpp:18-136), flattened to
+    one (left, right, xfrac) entry per destination element in store order."""
+    def rdiv(n, d):
+        return (n + d // 2) // d
+
+    def to(dx):
+        return rdiv(((dx << 16) + 32768) * sw, dw) - 32768 + 128
+
+    def sat8(v):
+        return max(-128, min(127, v))
+
+    two_x = (dw * C) // 32 if sw * C >= 16 * R else 0
+    n = dw * C
+    left, right, frac = [0] * n, [0] * n, [0] * n
+    di = 0
+    for v in range(two_x * 2):
+        base = (to(di // C) >> 16) * C + di % C
+        mx = sw * C - 16 * R
+        base = min(max(base, 0), mx)
+        lanes = []
+        for i in range(16):
+            sx = to(di // C)
+            inpx = di % C
+            s0 = min(max(sx >> 16, 0), sw - 1)
+            s1 = min(max((sx >> 16) + 1, 0), sw - 1)
+            lanes.append((sat8(s0 * C + inpx - base), sat8(s1 * C + inpx - base),
+                          (sx & 0xFFFF) >> 8))
+            di += 1
+        if C == 3:
+            ml = min(lanes[0][0], lanes[1][0], lanes[2][0])
+            fb = min(max(base + ml, 0), mx)
+            adj = fb - base
+            base = fb
+            lanes = [(l0 - adj, l1 - adj, f) for l0, l1, f in lanes]
+        for i, (l0, l1, f) in enumerate(lanes):
+            left[v * 16 + i], right[v * 16 + i], frac[v * 16 + i] = base + l0, base + l1, f
+    half = (n - di + 7) // 8
+    for _ in range(half):
+        di = min(di, n - 8)
+        base = ((to(di // C) >> 16) * C) + ((di % C) if not up else 0)
+        rs = 16 * (2 if (not up and ((R == 2 and C == 3) or R == 3)) else 1)
+        base = min(max(base, 0), sw * C - rs)
+        for _ in range(8):
+            sx = to(di // C)
+            inpx = di % C
+            s0 = min(max(sx >> 16, 0), sw - 1)
+            s1 = min(max((sx >> 16) + 1, 0), sw - 1)
+            left[di] = base + sat8(s0 * C + inpx - base)
+            right[di] = base + sat8(s1 * C + inpx - base)
+            frac[di] = (sx & 0xFFFF) >> 8
+            di += 1
+    return np.array(left), np.array(right), np.array(frac)

@@ -1,0 +1,165 @@
+"""C-ABI checks that need no GPU: the library loads, exports every entry
+point that include/kleidicv_hip/kleidicv.h declares, and rejects invalid
+arguments with the reference's error codes before touching the device."""
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import kleidicv_hip as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kleidicv_hip", "kleidicv.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    funcs = re.findall(r"^(?:kleidicv_error_t|void \*|void)\s*\**\s*(kleidicv_\w+)\(", text, re.M)
+    data = re.findall(r"^extern kleidicv_error_t \(\*(kleidicv_\w+)\)", text, re.M)
+    return sorted(set(funcs)), sorted(set(data))
+
+
+def exported():
+    out = os.popen("nm -D --defined-only '%s'" % K.LIB_PATH).read()
+    syms = {}
+    for line in out.splitlines():
+        parts = line.split()
+        if len(parts) == 3:
+            syms[parts[2]] = parts[1]
+    return syms
+
+
+def test_library_exports_every_declared_symbol():
+    funcs, data = declared_symbols()
+    assert len(funcs) >= 20 and len(data) == 3
+    syms = exported()
+    for f in funcs:
+        assert syms.get(f) == "T", f
+    for d in data:
+        assert syms.get(d) in ("D", "B"), d
+    lib = K.lib()
+    for f in funcs:
+        getattr(lib, f)
+    for d in data:
+        assert C.c_void_p.in_dll(lib, d).value
+
+
+def test_function_pointer_symbols_are_initialised():
+    lib = K.lib()
+    for name in ("kleidicv_yuv_semiplanar_to_rgb_u8", "kleidicv_remap_s16point5_u8",
+                 "kleidicv_remap_f32_u8"):
+        assert C.c_void_p.in_dll(lib, name).value != 0
+
+
+def _gauss(src, sstride, dst, dstride, w, h, c, kw, kh, sx, sy, border):
+    return K.lib().kleidicv_gaussian_blur_u8(
+        C.c_void_p(src), C.c_size_t(sstride), C.c_void_p(dst), C.c_size_t(dstride),
+        C.c_size_t(w), C.c_size_t(h), C.c_size_t(c), C.c_size_t(kw), C.c_size_t(kh),
+        C.c_float(sx), C.c_float(sy), C.c_int(border))
+
+
+def test_gaussian_error_codes():
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+    # border first (gaussian_blur_api.cpp:28-31)
+    for b in (0, 5, 6):
+        assert _gauss(p, 64, p + 1, 64, 16, 16, 1, 5, 5, 0, 0, b) == K.NOT_IMPLEMENTED
+    assert _gauss(p, 64, p + 1, 64, 16, 16, 1, 5, 3, 0, 0, 1) == K.NOT_IMPLEMENTED  # kw != kh
+    assert _gauss(p, 64, p + 1, 64, 16, 16, 1, 4, 4, 0, 0, 1) == K.NOT_IMPLEMENTED  # even
+    assert _gauss(p, 64, p + 1, 64, 3, 16, 1, 5, 5, 0, 0, 1) == K.NOT_IMPLEMENTED   # w < k-1
+    assert _gauss(p, 64, p + 1, 64, 16, 16, 5, 5, 5, 0, 0, 1) == K.NOT_IMPLEMENTED  # channels
+    assert _gauss(p, 64, p + 1, 64, 16, 16, 1, 5, 5, 1.0, 2.0, 1) == K.NOT_IMPLEMENTED
+    assert _gauss(0, 64, p + 1, 64, 16, 16, 1, 5, 5, 0, 0, 1) == K.NULL_POINTER
+    assert _gauss(p, 64, 0, 64, 16, 16, 1, 5, 5, 0, 0, 1) == K.NULL_POINTER
+    assert _gauss(p, 64, p + 1, 64, 1 << 40, 1 << 40, 1, 5, 5, 0, 0, 1) == K.RANGE
+
+
+def test_median_error_codes():
+    lib = K.lib()
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+
+    def med(src, dst, w, h, c, k, b):
+        return lib.kleidicv_median_blur_u8(C.c_void_p(src), C.c_size_t(64), C.c_void_p(dst),
+                                           C.c_size_t(64), C.c_size_t(w), C.c_size_t(h),
+                                           C.c_size_t(c), C.c_size_t(k), C.c_size_t(k),
+                                           C.c_int(b))
+    assert med(0, p, 16, 16, 1, 5, 1) == K.NULL_POINTER
+    assert med(p, p, 16, 16, 1, 5, 1) == K.NOT_IMPLEMENTED  # in place
+    assert med(p, p + 1, 16, 16, 1, 5, 0) == K.NOT_IMPLEMENTED  # constant border
+    assert med(p, p + 1, 16, 16, 1, 4, 1) == K.NOT_IMPLEMENTED
+
+
+def test_resize_error_codes():
+    lib = K.lib()
+    buf = np.zeros(4096, np.uint8)
+    p = buf.ctypes.data
+
+    def rs(src, sw, sh, dst, dw, dh, c):
+        return lib.kleidicv_resize_linear_u8(C.c_void_p(src), C.c_size_t(sw * max(c, 1)),
+                                             C.c_size_t(sw), C.c_size_t(sh), C.c_void_p(dst),
+                                             C.c_size_t(dw * max(c, 1)), C.c_size_t(dw),
+                                             C.c_size_t(dh), C.c_size_t(c))
+    assert rs(p, 48, 2, p, 32, 1, 5) == K.NOT_IMPLEMENTED   # test_resize_linear.cpp:996-1014
+    assert rs(p, 48, 2, p, 32, 1, 0) == K.NOT_IMPLEMENTED
+    assert rs(p, 48, 2, p, 15, 1, 1) == K.NOT_IMPLEMENTED   # below 1/3
+    assert rs(0, 32, 2, p, 16, 1, 1) == K.NULL_POINTER
+    assert rs(p, 0, 0, 0, 0, 0, 1) == K.NULL_POINTER
+    assert rs(p, 0, 0, p, 0, 0, 1) == K.OK
+
+
+def test_remap_error_codes():
+    buf = np.zeros(4096, np.uint8)
+    p = buf.ctypes.data
+    fn = K.data_function("kleidicv_remap_s16point5_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
+                         [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]))
+    assert fn(p, 16, 16, 4, p, 16, 16, 4, 1, p, 64, p, 32, 0, None) == K.NULL_POINTER
+    assert fn(p, 16, 16, 4, p, 16, 16, 4, 1, p, 63, p, 32, 1, None) == K.ALIGNMENT
+    assert fn(p, 16, 16, 4, p, 16, 16, 4, 2, p, 64, p, 32, 1, None) == K.NOT_IMPLEMENTED
+    assert fn(p, 16, 16, 4, p, 7, 7, 4, 1, p, 64, p, 32, 1, None) == K.NOT_IMPLEMENTED
+
+
+def test_warp_error_codes():
+    buf = np.zeros(4096, np.uint8)
+    p = buf.ctypes.data
+    t = np.eye(3, dtype=np.float32)
+    lib = K.lib()
+
+    def warp(src, sw, sh, dst, dw, dh, tp, c, interp, border, bv):
+        return lib.kleidicv_warp_perspective_u8(
+            C.c_void_p(src), C.c_size_t(max(sw, 1)), C.c_size_t(sw), C.c_size_t(sh),
+            C.c_void_p(dst), C.c_size_t(dw), C.c_size_t(dw), C.c_size_t(dh),
+            C.c_void_p(tp), C.c_size_t(c), C.c_int(interp), C.c_int(border), C.c_void_p(bv))
+    tp = t.ctypes.data
+    assert warp(p, 16, 16, p, 7, 4, tp, 1, 1, 1, None) == K.NOT_IMPLEMENTED  # dst_w < 8
+    assert warp(p, 16, 16, p, 16, 4, tp, 1, 2, 1, None) == K.NOT_IMPLEMENTED
+    assert warp(p, 16, 16, p, 16, 4, tp, 5, 1, 1, None) == K.NOT_IMPLEMENTED
+    assert warp(p, 16, 16, p, 16, 4, None, 1, 1, 1, None) == K.NULL_POINTER
+    assert warp(p, 16, 16, p, 16, 4, tp, 1, 1, 0, None) == K.NULL_POINTER
+    assert warp(p, 0, 16, p, 16, 4, tp, 1, 1, 1, None) == K.RANGE
+
+
+def test_yuv_error_codes():
+    buf = np.zeros(4096, np.uint8)
+    p = buf.ctypes.data
+    fn = K.data_function("kleidicv_yuv_semiplanar_to_rgb_u8", C.c_void_p, C.c_size_t,
+                         C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
+                         C.c_size_t, C.c_int)
+    assert fn(p, 16, p, 16, p, 48, 16, 4, 0x03) == K.NOT_IMPLEMENTED  # I420 is not semi-planar
+    assert fn(p, 16, p, 16, p, 48, 16, 4, 0x82) == K.NOT_IMPLEMENTED
+    assert fn(None, 16, p, 16, p, 48, 16, 4, 0x02) == K.NULL_POINTER
+    assert fn(p, 16, None, 16, p, 48, 16, 4, 0x02) == K.NULL_POINTER
+
+
+def test_thread_api_requires_scheduler():
+    class MT(C.Structure):
+        _fields_ = [("parallel", C.c_void_p), ("parallel_data", C.c_void_p)]
+    buf = np.zeros(4096, np.uint8)
+    p = buf.ctypes.data
+    r = K.lib().kleidicv_thread_median_blur_u8(
+        C.c_void_p(p), C.c_size_t(16), C.c_void_p(p + 1), C.c_size_t(16), C.c_size_t(16),
+        C.c_size_t(16), C.c_size_t(1), C.c_size_t(5), C.c_size_t(5), C.c_int(1), MT(None, None))
+    assert r == K.NULL_POINTER

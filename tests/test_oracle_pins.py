@@ -1,0 +1,271 @@
+"""Pins the CPU oracle (test infrastructure) before anything is compared
+against it: the reference tests' known-answer tables, the generic resize
+generator's direct formula from the reference's unit test, and independent
+numpy formulations (numpy.pad borders, window sums, numpy.median, double
+precision geometry).  CPU only."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import pins
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+BORDERS = ["REPLICATE", "REFLECT", "WRAP", "REVERSE"]
+
+
+def test_border_index_matches_numpy_pad():
+    for border, mode in pins.PAD_MODE.items():
+        for n in (3, 4, 7):
+            r = n - 1  # a k x k filter needs n >= k-1, so overhangs stay below n
+            idx = np.pad(np.arange(n), (r, r), mode=mode)
+            got = [O.lib().orc_border_index(i, n, O.BORDER[border]) for i in range(-r, n + r)]
+            assert list(idx) == got, (border, n)
+
+
+@pytest.mark.parametrize("kat", KATS["gaussian_valid_image_size"], ids=lambda k: "k%d" % k["kernel"])
+def test_gaussian_kats(kat):
+    src = np.array(kat["src"], np.uint8)
+    err, out = O.gaussian_blur(src, kat["kernel"], 0.0, kat["border"])
+    assert err == 0 and (out == np.array(kat["expected_sigma0"])).all()
+    err, out = O.gaussian_blur(src, kat["kernel"], 2.25, kat["border"])
+    assert err == 0
+    assert np.abs(out.astype(int) - np.array(kat["expected_sigma2_25"])).max() <= kat["sigma2_25_tolerance"]
+
+
+@pytest.mark.parametrize("k", [3, 5, 7, 9])
+@pytest.mark.parametrize("border", BORDERS)
+@pytest.mark.parametrize("channels", [1, 3, 4])
+def test_gaussian_binomial_vs_padded_convolution(rng, k, border, channels):
+    for h, w in [(k - 1, k - 1), (11, 23), (17, 9)]:
+        src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+        err, out = O.gaussian_blur(src, k, 0.0, border)
+        assert err == 0
+        np.testing.assert_array_equal(out, pins.binomial_blur(src, k, border))
+
+
+def test_gaussian_sigma_rounds_twice(rng):
+    # The sigma path rounds the vertical pass to u8 before the horizontal
+    # one (gaussian_blur_fixed_neon.cpp:461-529): check on a case where a
+    # single rounding would differ.
+    half = np.zeros(3, np.uint8)
+    assert O.lib().orc_gaussian_half_kernel(half.ctypes.data, 3, 1.0)
+    w = np.concatenate([half, half[-2::-1]]).astype(np.int64)
+    assert w.sum() == 256
+    src = rng.integers(0, 256, (12, 12), dtype=np.uint8)
+    err, out = O.gaussian_blur(src, 5, 1.0, "REPLICATE")
+    p = np.pad(src, 2, mode="edge").astype(np.int64)
+    vert = sum(w[i] * p[i:i + 12] for i in range(5))
+    vert = (vert + 128) >> 8
+    hor = sum(w[j] * vert[:, j:j + 12] for j in range(5))
+    np.testing.assert_array_equal(out, ((hor + 128) >> 8).astype(np.uint8))
+
+
+@pytest.mark.parametrize("kat", KATS["separable_overflow"], ids=lambda k: k["name"])
+def test_separable_kats(kat):
+    err, out = O.separable_filter_2d(np.array(kat["src"], np.uint8), kat["kernel_x"],
+                                     kat["kernel_y"], kat["border"])
+    assert err == 0 and (out == np.array(kat["expected"])).all()
+
+
+@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("border", BORDERS)
+def test_separable_vs_two_stage_saturation(rng, k, border):
+    for taps_hi in (3, 40, 256):
+        for channels in (1, 2, 4):
+            src = rng.integers(0, 256, (13, 19, channels), dtype=np.uint8)
+            kx = rng.integers(0, taps_hi, k)
+            ky = rng.integers(0, taps_hi, k)
+            err, out = O.separable_filter_2d(src, kx, ky, border)
+            assert err == 0
+            np.testing.assert_array_equal(out, pins.separable_two_stage(src, kx, ky, border))
+
+
+@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("border", BORDERS)
+def test_median_vs_numpy(rng, k, border):
+    for h, w, c in [(k - 1, k - 1, 1), (9, 21, 3), (14, 6, 4)]:
+        src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+        err, out = O.median_blur(src, k, border)
+        assert err == 0
+        np.testing.assert_array_equal(out, pins.median(src, k, border))
+
+
+@pytest.mark.parametrize("case", KATS["resize_u8_params"], ids=lambda c: "%dx%d" % (len(c["src"][0]), len(c["src"])))
+def test_resize_param_tables(case):
+    src = np.array(case["src"], np.uint8)
+    exp = np.array(case["expected"], np.uint8)
+    err, out = O.resize_linear(src, exp.shape[1], exp.shape[0])
+    assert err == 0
+    np.testing.assert_array_equal(out, exp)
+
+
+@pytest.mark.parametrize("g", KATS["resize_generator_sizes"],
+                         ids=lambda g: "c%d_%d_%d" % (g["channels"], g["src_width"], g["dst_width"]))
+def test_resize_generator_vs_direct_formula(g):
+    C, sw, dw, tol = g["channels"], g["src_width"], g["dst_width"], g["xfrac_tolerance"]
+    # the unit test's own choice of generator variant
+    R = 2 if (dw > sw and dw * 14 <= sw * 15) else (sw // dw) + 1
+    up = dw >= sw
+    left, right, frac, _ = O.resize_columns(sw, dw, C, R, up)
+    l_ref, r_ref, f_ref = pins.resize_generator_direct(C, sw, dw, R, up)
+    np.testing.assert_array_equal(left, l_ref)
+    np.testing.assert_array_equal(right, r_ref)
+    assert np.abs(frac.astype(int) - f_ref).max() <= tol
+
+
+@pytest.mark.parametrize("channels", [1, 2, 3, 4])
+def test_resize_quarter_closed_form(rng, channels):
+    src = rng.integers(0, 256, (18, 26, channels), dtype=np.uint8)
+    err, out = O.resize_linear(src, 13, 9)
+    s = src.astype(np.int64)
+    ref = ((s[0::2, 0::2] + s[0::2, 1::2] + s[1::2, 0::2] + s[1::2, 1::2] + 2) >> 2)
+    assert err == 0
+    np.testing.assert_array_equal(out, ref.astype(np.uint8))
+
+
+@pytest.mark.parametrize("channels", [2, 3, 4])
+def test_resize_exact_2x_generic_equals_direct_lerp(rng, channels):
+    # Exact 2x upsizing goes through the generic path for C > 1; every
+    # fraction is exact (64 or 192 in 8 bits), so the two-stage lerp on the
+    # textbook coordinates must reproduce it.
+    h, w = 6, 24
+    src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+    err, out = O.resize_linear(src, 2 * w, 2 * h)
+    assert err == 0
+
+    def coords(n_dst, n_src):
+        s = (np.arange(n_dst) + 0.5) / 2 - 0.5
+        i0 = np.floor(s).astype(int)
+        f = np.round((s - np.floor(s)) * 256).astype(int)
+        return np.clip(i0, 0, n_src - 1), np.clip(i0 + 1, 0, n_src - 1), f
+
+    x0, x1, fx = coords(2 * w, w)
+    y0, y1, fy = coords(2 * h, h)
+    s = src.astype(np.int64)
+    lerp = lambda a, b, t: (256 * a + (b - a) * t + 128) >> 8
+    left = lerp(s[y0][:, x0], s[y1][:, x0], fy[:, None, None])
+    right = lerp(s[y0][:, x1], s[y1][:, x1], fy[:, None, None])
+    ref = lerp(left, right, fx[None, :, None])
+    np.testing.assert_array_equal(out, ref.astype(np.uint8))
+
+
+@pytest.mark.parametrize("channels", [1, 4])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_s16point5_vs_reference_closed_form(rng, channels, border):
+    h, w, dh, dw = 20, 31, 17, 29
+    src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+    mapxy = np.stack([rng.integers(-5, w + 5, (dh, dw)), rng.integers(-5, h + 5, (dh, dw))],
+                     axis=-1).astype(np.int16)
+    mapfrac = rng.integers(0, 1024, (dh, dw)).astype(np.uint16)
+    bv = (7, 99, 200, 3)
+    err, out = O.remap_s16point5(src, mapxy, mapfrac, border, bv)
+    assert err == 0
+    np.testing.assert_array_equal(out, pins.remap_s16point5(src, mapxy, mapfrac, border, bv))
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+@pytest.mark.parametrize("channels", [1, 2, 4])
+def test_remap_f32_within_one_of_double(rng, interp, border, channels):
+    h, w, dh, dw = 23, 37, 19, 33
+    src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+    mapx = rng.uniform(-3, w + 3, (dh, dw)).astype(np.float32)
+    mapy = rng.uniform(-3, h + 3, (dh, dw)).astype(np.float32)
+    bv = (11, 22, 33, 44)
+    err, out = O.remap_f32(src, mapx, mapy, interp, border, bv)
+    assert err == 0
+    ref = pins.remap_f32_double(src, mapx, mapy, interp, border, bv)
+    assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
+    # integer coordinates copy pixels exactly
+    ix = rng.integers(0, w, (dh, dw)).astype(np.float32)
+    iy = rng.integers(0, h, (dh, dw)).astype(np.float32)
+    err, out = O.remap_f32(src, ix, iy, interp, border, bv)
+    np.testing.assert_array_equal(out, src[iy.astype(int), ix.astype(int)])
+
+
+def test_remap_f32_per_plane_invariant(rng):
+    src = rng.integers(0, 256, (21, 30, 4), dtype=np.uint8)
+    mapx = rng.uniform(-4, 34, (15, 26)).astype(np.float32)
+    mapy = rng.uniform(-4, 25, (15, 26)).astype(np.float32)
+    bv = (1, 2, 3, 4)
+    for interp in ("LINEAR", "NEAREST"):
+        for border in ("REPLICATE", "CONSTANT"):
+            _, four = O.remap_f32(src, mapx, mapy, interp, border, bv)
+            for c in range(4):
+                _, one = O.remap_f32(np.ascontiguousarray(src[..., c]), mapx, mapy, interp,
+                                     border, (bv[c],))
+                np.testing.assert_array_equal(four[..., c], one)
+
+
+def test_warp_simple_resize_kat():
+    kat = KATS["warp_simple_resize"]
+    src = np.array(kat["src"], np.uint8)
+    exp = np.array(kat["expected"], np.uint8)
+    err, out = O.warp_perspective(src, kat["transform"], exp.shape[1], exp.shape[0],
+                                  kat["interpolation"], kat["border"])
+    assert err == 0
+    np.testing.assert_array_equal(out, exp)
+
+
+def random_homography(rng, w, h, max_deg=15.0, persp=1e-4):
+    a = np.deg2rad(rng.uniform(-max_deg, max_deg))
+    s = rng.uniform(0.8, 1.2)
+    T = np.array([[s * np.cos(a), -s * np.sin(a), rng.uniform(-0.1, 0.1) * w],
+                  [s * np.sin(a), s * np.cos(a), rng.uniform(-0.1, 0.1) * h],
+                  [rng.uniform(-persp, persp), rng.uniform(-persp, persp), 1.0]])
+    return T.astype(np.float32)
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_warp_within_one_of_double(rng, interp, border):
+    src = rng.integers(0, 256, (40, 56), dtype=np.uint8)
+    T = random_homography(rng, 56, 40, persp=3e-3)
+    err, out = O.warp_perspective(src, T, 48, 36, interp, border, (77,))
+    assert err == 0
+    ref = pins.warp_double(src, T, 48, 36, interp, border, (77,))
+    diff = np.abs(out.astype(int) - ref.astype(int))
+    if interp == "LINEAR":
+        assert diff.max() <= 1
+    else:
+        # float vs double coordinates may straddle a rounding tie
+        assert (diff > 1).mean() < 0.01
+
+
+def test_warp_per_plane_invariant(rng):
+    src = rng.integers(0, 256, (30, 40, 4), dtype=np.uint8)
+    T = random_homography(rng, 40, 30, persp=2e-3)
+    bv = (9, 8, 7, 6)
+    for interp in ("LINEAR", "NEAREST"):
+        for border in ("REPLICATE", "CONSTANT"):
+            _, four = O.warp_perspective(src, T, 36, 28, interp, border, bv)
+            for c in range(4):
+                _, one = O.warp_perspective(np.ascontiguousarray(src[..., c]), T, 36, 28,
+                                            interp, border, (bv[c],))
+                np.testing.assert_array_equal(four[..., c], one)
+
+
+@pytest.mark.parametrize("fmt", [0x02, 0x22, 0x42, 0x62, 0x12, 0x32, 0x52, 0x72])
+def test_nv12_fixed_inputs_and_bt601(fmt):
+    fx = KATS["nv12_fixed_inputs"]
+    y = np.array(fx["y"], np.uint8)
+    uv = np.array(fx["uv"], np.uint8)
+    err, out = O.yuv_sp_to_rgb(y, uv, fmt)
+    assert err == 0
+    ref = pins.nv12_float(y, uv, vu=bool(fmt & 0x10), bgr=bool(fmt & 0x20), alpha=bool(fmt & 0x40))
+    assert np.abs(out.astype(int) - ref.astype(int)).max() <= 1
+
+
+def test_nv12_gray_axis_exact(rng):
+    # U = V = 128 removes the chroma terms: R = G = B = round(1220542 Y' / 2^20)
+    y = rng.integers(0, 256, (6, 10), dtype=np.uint8)
+    uv = np.full((3, 10), 128, np.uint8)
+    _, out = O.yuv_sp_to_rgb(y, uv, 0x02)
+    g = (1220542 * np.maximum(y.astype(np.int64) - 16, 0) + (1 << 19)) >> 20
+    g = np.clip(g, 0, 255)
+    for c in range(3):
+        np.testing.assert_array_equal(out[..., c], g)

@@ -1,0 +1,326 @@
+"""Bit-exact parity of the HIP library against the CPU oracle.
+
+Every case runs the product through its C ABI (host buffers through the
+drop-in `kleidicv_*` entry points, device buffers through `kleidicv_hip_*`)
+on seeded inputs and compares with the oracle at tolerance 0.  Float ops
+(remap f32, warp) are compared bit-exactly too: both sides implement the
+reference's fused multiply-adds and correctly rounded division.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+import kleidicv_hip as K
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+BORDERS = ["REPLICATE", "REFLECT", "WRAP", "REVERSE"]
+
+
+def assert_same(got, want, what=""):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, (what, got.shape, want.shape)
+    if not np.array_equal(got, want):
+        bad = np.argwhere(got != want)
+        i = tuple(bad[0])
+        raise AssertionError("%s: %d mismatches, first at %s: got %d want %d" %
+                             (what, len(bad), i, got[i], want[i]))
+
+
+def padded_rows(rng, h, w, c, pad):
+    """(h, w, c) view into a buffer whose rows carry `pad` extra bytes."""
+    buf = rng.integers(0, 256, (h, w * c + pad), dtype=np.uint8)
+    return buf[:, :w * c].reshape(h, w, c) if c > 1 else buf[:, :w]
+
+
+# ---------------------------------------------------------------------------
+# Gaussian
+GAUSS_SIZES = [(4, 4), (9, 5), (33, 70), (257, 40), (300, 19), (513, 37)]
+
+
+@pytest.mark.parametrize("k", [3, 5, 7, 9])
+@pytest.mark.parametrize("border", BORDERS)
+def test_gaussian_binomial(rng, k, border):
+    for (w, h) in GAUSS_SIZES:
+        if w < k - 1 or h < k - 1:
+            continue
+        for c in (1, 2, 3, 4):
+            src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+            got = K.gaussian_blur(src, k, 0.0, border)
+            _, want = O.gaussian_blur(src, k, 0.0, border)
+            assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
+
+
+@pytest.mark.parametrize("k,sigma", [(3, 0.8), (5, 2.25), (7, 1.3), (9, 3.0), (15, 0.0),
+                                     (21, 0.0), (15, 4.0), (21, 6.5), (5, 0.1)])
+def test_gaussian_sigma(rng, k, sigma):
+    for border in BORDERS:
+        for (w, h) in [(k - 1, k - 1), (130, 45), (301, 23)]:
+            c = int(rng.integers(1, 5))
+            src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+            got = K.gaussian_blur(src, k, sigma, border)
+            _, want = O.gaussian_blur(src, k, sigma, border)
+            assert_same(got, want, "k%d s%.2f %s %dx%dx%d" % (k, sigma, border, w, h, c))
+
+
+@pytest.mark.parametrize("kat", KATS["gaussian_valid_image_size"], ids=lambda k: "k%d" % k["kernel"])
+def test_gaussian_kats(kat):
+    src = np.array(kat["src"], np.uint8)
+    assert_same(K.gaussian_blur(src, kat["kernel"], 0.0, "REVERSE"), np.array(kat["expected_sigma0"]))
+
+
+def test_gaussian_padded_strides(rng):
+    for c in (1, 3, 4):
+        src = padded_rows(rng, 37, 61, c, 13)
+        dst_buf = np.full((37, 61 * c + 7), 0xA5, np.uint8)
+        dst = dst_buf[:, :61 * c].reshape(37, 61, c) if c > 1 else dst_buf[:, :61]
+        K.gaussian_blur(src, 5, 0.0, "REVERSE", dst=dst)
+        _, want = O.gaussian_blur(np.ascontiguousarray(src), 5, 0.0, "REVERSE")
+        assert_same(dst, want)
+        assert (dst_buf[:, 61 * c:] == 0xA5).all(), "row padding must be left alone"
+
+
+# ---------------------------------------------------------------------------
+# Separable filter
+@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("border", BORDERS)
+def test_separable(rng, k, border):
+    for (w, h) in [(k - 1, k - 1), (100, 31), (300, 17)]:
+        for c in (1, 3, 4):
+            for hi in (2, 9, 256):
+                src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+                kx = rng.integers(0, hi, k)
+                ky = rng.integers(0, hi, k)
+                got = K.separable_filter_2d(src, kx, ky, border)
+                _, want = O.separable_filter_2d(src, kx, ky, border)
+                assert_same(got, want, "k%d %s %dx%dx%d hi%d" % (k, border, w, h, c, hi))
+
+
+@pytest.mark.parametrize("kat", KATS["separable_overflow"], ids=lambda k: k["name"])
+def test_separable_kats(kat):
+    got = K.separable_filter_2d(np.array(kat["src"], np.uint8), kat["kernel_x"], kat["kernel_y"])
+    assert_same(got, np.array(kat["expected"]))
+
+
+# ---------------------------------------------------------------------------
+# Median
+@pytest.mark.parametrize("k", [3, 5, 7])
+@pytest.mark.parametrize("border", BORDERS)
+def test_median(rng, k, border):
+    for (w, h) in [(k - 1, k - 1), (70, 23), (260, 33)]:
+        for c in (1, 2, 3, 4):
+            src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+            got = K.median_blur(src, k, border)
+            _, want = O.median_blur(src, k, border)
+            assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
+
+
+def test_median_low_entropy(rng):
+    # many ties: values drawn from {0, 1, 255}
+    src = rng.choice(np.array([0, 1, 255], np.uint8), (40, 90))
+    for k in (3, 5, 7):
+        got = K.median_blur(src, k, "REPLICATE")
+        _, want = O.median_blur(src, k, "REPLICATE")
+        assert_same(got, want)
+
+
+# ---------------------------------------------------------------------------
+# Resize
+def resize_case(rng, sw, sh, dw, dh, c):
+    src = rng.integers(0, 256, (sh, sw, c), dtype=np.uint8)
+    got = K.resize_linear(src, dw, dh)
+    err, want = O.resize_linear(src, dw, dh)
+    assert err == 0
+    assert_same(got, want, "%dx%d->%dx%d c%d" % (sw, sh, dw, dh, c))
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+def test_resize_quarter(rng, c):
+    for sw, sh in [(2, 2), (34, 10), (640, 96), (1026, 18)]:
+        resize_case(rng, sw, sh, sw // 2, sh // 2, c)
+
+
+def test_resize_exact_1ch_2x_4x(rng):
+    for sw, sh in [(1, 1), (2, 3), (9, 7), (300, 20)]:
+        resize_case(rng, sw, sh, 2 * sw, 2 * sh, 1)
+        resize_case(rng, sw, sh, 4 * sw, 4 * sh, 1)
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+def test_resize_generic_sweep(rng, c):
+    # upsizing (r1/r2), downsizing (r2/r3) and the edge cases the reference
+    # tests: widths just above the vector minimum, pull-backs, odd sizes
+    min_w = 16 // c + (1 if 16 % c else 0)
+    widths = sorted({min_w, min_w + 1, 17, 23, 33, 64, 101, 149, 333, 1479})
+    for sw in widths:
+        if sw * c < 16:
+            continue
+        for dw in {sw + 1, sw * 2, (sw * 15) // 14 + 1, sw * 3 - 1, (sw * 2) // 3 + 1,
+                   sw // 2 + 1, (sw + 2) // 3 + 1, sw - 1}:
+            if dw <= 0:
+                continue
+            sh = int(rng.integers(1, 9))
+            dh = int(rng.integers(1, 11))
+            err, _ = O.resize_linear(np.zeros((sh, sw, c), np.uint8), dw, dh)
+            if err != 0:
+                with pytest.raises(K.KleidiCVError):
+                    K.resize_linear(np.zeros((sh, sw, c), np.uint8), dw, dh)
+                continue
+            resize_case(rng, sw, sh, dw, dh, c)
+
+
+@pytest.mark.parametrize("kat", KATS["resize_u8_params"], ids=lambda c: "%dx%d" % (len(c["src"][0]), len(c["src"])))
+def test_resize_kats(kat):
+    src = np.array(kat["src"], np.uint8)
+    exp = np.array(kat["expected"], np.uint8)
+    assert_same(K.resize_linear(src, exp.shape[1], exp.shape[0]), exp)
+
+
+def test_resize_pyramid_odd_sizes(rng):
+    # cfg3-style 0.5x pyramids on odd sizes leave the quarter path
+    for _ in range(12):
+        w, h = int(rng.integers(64, 700)), int(rng.integers(8, 80))
+        c = int(rng.choice([1, 3, 4]))
+        for _lvl in range(3):
+            dw, dh = max(w // 2, 1), max(h // 2, 1)
+            err, _ = O.resize_linear(np.zeros((h, w, c), np.uint8), dw, dh)
+            if err == 0:
+                resize_case(rng, w, h, dw, dh, c)
+            w, h = dw, dh
+
+
+# ---------------------------------------------------------------------------
+# Remap
+def smooth_maps(rng, dw, dh, sw, sh, amp):
+    yy, xx = np.mgrid[0:dh, 0:dw].astype(np.float64)
+    fx = np.zeros_like(xx)
+    fy = np.zeros_like(yy)
+    for _ in range(4):
+        a, b, p, q = rng.uniform(0.002, 0.05, 4)
+        fx += amp / 4 * np.sin(a * xx + b * yy + p * 10)
+        fy += amp / 4 * np.cos(b * xx + a * yy + q * 10)
+    return xx * sw / dw + fx, yy * sh / dh + fy
+
+
+def s16point5_maps(mx, my):
+    ix = np.floor(mx * 32).astype(np.int64)
+    iy = np.floor(my * 32).astype(np.int64)
+    mapxy = np.stack([ix >> 5, iy >> 5], axis=-1).astype(np.int16)
+    mapfrac = ((ix & 31) | ((iy & 31) << 5)).astype(np.uint16)
+    return mapxy, mapfrac
+
+
+@pytest.mark.parametrize("c", [1, 4])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_s16point5(rng, c, border):
+    for (sw, sh, dw, dh) in [(31, 17, 8, 3), (200, 90, 150, 70), (64, 64, 333, 41)]:
+        src = rng.integers(0, 256, (sh, sw, c), dtype=np.uint8)
+        mx, my = smooth_maps(rng, dw, dh, sw, sh, 12)
+        mapxy, mapfrac = s16point5_maps(mx, my)
+        # sprinkle far out-of-range and extreme coordinates
+        sel = rng.random((dh, dw)) < 0.05
+        mapxy[sel] = rng.integers(-32768, 32767, (int(sel.sum()), 2))
+        mapfrac = (mapfrac | (rng.integers(0, 64, (dh, dw)) << 10)).astype(np.uint16)
+        bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+        got = K.remap_s16point5(src, mapxy, mapfrac, border, bv)
+        _, want = O.remap_s16point5(src, mapxy, mapfrac, border, bv)
+        assert_same(got, want, "%s c%d %dx%d" % (border, c, dw, dh))
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_f32(rng, c, interp, border):
+    for (sw, sh, dw, dh) in [(29, 13, 4, 2), (180, 96, 141, 77), (64, 40, 300, 9)]:
+        src = rng.integers(0, 256, (sh, sw, c), dtype=np.uint8)
+        mx, my = smooth_maps(rng, dw, dh, sw, sh, 20)
+        mapx, mapy = mx.astype(np.float32), my.astype(np.float32)
+        sel = rng.random((dh, dw)) < 0.03
+        mapx[sel] = rng.choice(np.array([np.nan, np.inf, -np.inf, -1e9, 3e9, -0.5, 0.5,
+                                         sw - 0.5, sw - 1 + 1e-3], np.float32), int(sel.sum()))
+        bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+        got = K.remap_f32(src, mapx, mapy, interp, border, bv)
+        _, want = O.remap_f32(src, mapx, mapy, interp, border, bv)
+        assert_same(got, want, "%s %s c%d" % (interp, border, c))
+
+
+# ---------------------------------------------------------------------------
+# Warp perspective
+def homography(rng, w, h, deg=15.0, persp=1e-4):
+    a = np.deg2rad(rng.uniform(-deg, deg))
+    s = rng.uniform(0.7, 1.3)
+    return np.array([[s * np.cos(a), -s * np.sin(a), rng.uniform(-0.2, 0.2) * w],
+                     [s * np.sin(a), s * np.cos(a), rng.uniform(-0.2, 0.2) * h],
+                     [rng.uniform(-persp, persp), rng.uniform(-persp, persp), 1.0]], np.float32)
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_warp(rng, c, interp, border):
+    for (sw, sh, dw, dh) in [(16, 4, 8, 2), (220, 130, 211, 97), (97, 61, 333, 12)]:
+        src = rng.integers(0, 256, (sh, sw, c), dtype=np.uint8)
+        for persp in (1e-4, 3e-3):
+            T = homography(rng, sw, sh, persp=persp)
+            bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+            got = K.warp_perspective(src, T, dw, dh, interp, border, bv)
+            _, want = O.warp_perspective(src, T, dw, dh, interp, border, bv)
+            assert_same(got, want, "%s %s c%d %g" % (interp, border, c, persp))
+
+
+def test_warp_zero_weight_row(rng):
+    # a homography whose weight crosses 0 inside the image takes the checked path
+    src = rng.integers(0, 256, (40, 60), dtype=np.uint8)
+    T = np.array([[1, 0, 0], [0, 1, 0], [0.02, 0, -0.5]], np.float32)
+    for interp in ("LINEAR", "NEAREST"):
+        for border in ("REPLICATE", "CONSTANT"):
+            got = K.warp_perspective(src, T, 64, 40, interp, border, (5,))
+            _, want = O.warp_perspective(src, T, 64, 40, interp, border, (5,))
+            assert_same(got, want)
+
+
+def test_warp_kat():
+    kat = KATS["warp_simple_resize"]
+    src = np.array(kat["src"], np.uint8)
+    exp = np.array(kat["expected"], np.uint8)
+    assert_same(K.warp_perspective(src, kat["transform"], exp.shape[1], exp.shape[0]), exp)
+
+
+# ---------------------------------------------------------------------------
+# NV12
+@pytest.mark.parametrize("fmt", [0x02, 0x22, 0x42, 0x62, 0x12, 0x32, 0x52, 0x72])
+def test_nv12(rng, fmt):
+    for (w, h) in [(1, 1), (3, 5), (16, 2), (101, 37), (640, 33)]:
+        y = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        uv = rng.integers(0, 256, ((h + 1) // 2, (w + 1) // 2 * 2), dtype=np.uint8)
+        got = K.yuv_sp_to_rgb(y, uv, fmt)
+        _, want = O.yuv_sp_to_rgb(y, uv, fmt)
+        assert_same(got, want, "fmt %#x %dx%d" % (fmt, w, h))
+
+
+# ---------------------------------------------------------------------------
+# Device-resident API on a torch stream gives the same bytes as the host API.
+def test_device_api_matches_host_api(rng):
+    import torch
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    src = rng.integers(0, 256, (120, 250, 3), dtype=np.uint8)
+    d_src = torch.from_numpy(src).to(dev)
+    with torch.cuda.stream(stream):
+        outs = {
+            "gauss": K.gaussian_blur(d_src, 5, 0.0, "REVERSE", stream=stream, device_api=True),
+            "median": K.median_blur(d_src, 5, "REPLICATE", stream=stream, device_api=True),
+            "resize": K.resize_linear(d_src, 125, 60, stream=stream, device_api=True),
+            "resize_up": K.resize_linear(d_src, 333, 200, stream=stream, device_api=True),
+        }
+    stream.synchronize()
+    assert_same(outs["gauss"].cpu().numpy(), K.gaussian_blur(src, 5, 0.0, "REVERSE"))
+    assert_same(outs["median"].cpu().numpy(), K.median_blur(src, 5, "REPLICATE"))
+    assert_same(outs["resize"].cpu().numpy(), K.resize_linear(src, 125, 60))
+    assert_same(outs["resize_up"].cpu().numpy(), K.resize_linear(src, 333, 200))

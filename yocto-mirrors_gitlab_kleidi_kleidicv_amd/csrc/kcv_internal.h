@@ -1,0 +1,134 @@
+// Internal interfaces between the C ABI layer (api.cpp) and the HIP kernel
+// launchers.  Launchers take validated arguments and device pointers, enqueue
+// on `stream` and return hipSuccess or the launch error.
+#ifndef KCV_INTERNAL_H
+#define KCV_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace kcv {
+
+enum Border : int {
+  kConstant = 0,
+  kReplicate = 1,
+  kReflect = 2,
+  kWrap = 3,
+  kReverse = 4,
+};
+
+// Separable filter flavours sharing one tiled kernel.
+enum SepMode : int {
+  kSepBinomial = 0,   // single-rounded 2-D integer convolution, >> shift
+  kSepSigma = 1,      // vertical pass rounded to u8, then horizontal
+  kSepSaturate = 2,   // min(255, sum) with unsigned taps, no shift
+};
+
+struct SepArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  uint8_t *dst;
+  size_t dst_stride;
+  int width, height, channels, border;
+  int ksize;
+  int mode;
+  int shift;
+  int wy[21];
+  int wx[21];
+};
+
+hipError_t launch_separable(const SepArgs &a, hipStream_t stream);
+
+struct MedianArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  uint8_t *dst;
+  size_t dst_stride;
+  int width, height, channels, border, ksize;
+};
+
+hipError_t launch_median(const MedianArgs &a, hipStream_t stream);
+
+struct ResizeArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  int src_width, src_height;
+  uint8_t *dst;
+  size_t dst_stride;
+  int dst_width, dst_height, channels;
+};
+
+// Path selection follows src/resize/resize_linear_api.cpp:82-216.  The
+// generic path needs a per-column table; `workspace` must hold
+// resize_workspace_bytes() bytes of device memory.
+size_t resize_workspace_bytes(const ResizeArgs &a);
+hipError_t launch_resize(const ResizeArgs &a, void *workspace,
+                         hipStream_t stream);
+
+struct RemapS16Args {
+  const uint8_t *src;
+  size_t src_stride;
+  int src_width, src_height;
+  uint8_t *dst;
+  size_t dst_stride;
+  int dst_width, dst_height, channels;
+  const int16_t *mapxy;
+  size_t mapxy_stride;
+  const uint16_t *mapfrac;
+  size_t mapfrac_stride;
+  int border;
+  uint8_t border_value[4];
+};
+
+hipError_t launch_remap_s16point5(const RemapS16Args &a, hipStream_t stream);
+
+struct RemapF32Args {
+  const uint8_t *src;
+  size_t src_stride;
+  int src_width, src_height;
+  uint8_t *dst;
+  size_t dst_stride;
+  int dst_width, dst_height, channels;
+  const float *mapx;
+  size_t mapx_stride;
+  const float *mapy;
+  size_t mapy_stride;
+  int interpolation, border;
+  uint8_t border_value[4];
+};
+
+hipError_t launch_remap_f32(const RemapF32Args &a, hipStream_t stream);
+
+struct WarpArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  int src_width, src_height;
+  uint8_t *dst;
+  size_t dst_stride;
+  int dst_width, dst_height, channels;
+  float t[9];
+  int interpolation, border;
+  int check_zero;  // transform_common.h:17-43 decided on the host
+  uint8_t border_value[4];
+};
+
+hipError_t launch_warp_perspective(const WarpArgs &a, hipStream_t stream);
+
+struct YuvArgs {
+  const uint8_t *y;
+  size_t y_stride;
+  const uint8_t *uv;
+  size_t uv_stride;
+  uint8_t *dst;
+  size_t dst_stride;
+  int width, height;
+  int vu, bgr, alpha;
+};
+
+hipError_t launch_yuv_sp_to_rgb(const YuvArgs &a, hipStream_t stream);
+
+}  // namespace kcv
+
+#endif  // KCV_INTERNAL_H

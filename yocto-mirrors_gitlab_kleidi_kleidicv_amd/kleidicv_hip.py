@@ -1,0 +1,250 @@
+"""Python host binding of libkleidicv_hip.so (the C ABI in include/kleidicv_hip).
+
+Thin ctypes layer used by the tests, the benchmark and the serving loop.
+Images are interleaved HWC uint8 arrays: numpy arrays (host memory, staged by
+the library) or torch CUDA tensors (device memory, used in place).  Every
+call goes to the HIP library; there is no CPU fallback, and loading fails
+loudly when the library has not been built.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkleidicv_hip.so")
+
+OK, NOT_IMPLEMENTED, NULL_POINTER, RANGE, ALLOCATION, ALIGNMENT = range(6)
+ERROR_NAMES = ["OK", "NOT_IMPLEMENTED", "NULL_POINTER", "RANGE", "ALLOCATION",
+               "ALIGNMENT", "CONTEXT_MISMATCH"]
+BORDER = {"CONSTANT": 0, "REPLICATE": 1, "REFLECT": 2, "WRAP": 3,
+          "REVERSE": 4, "TRANSPARENT": 5, "NONE": 6}
+INTERP = {"NEAREST": 0, "LINEAR": 1}
+NV12_TO_RGB, NV12_TO_BGR, NV12_TO_RGBA, NV12_TO_BGRA = 0x02, 0x22, 0x42, 0x62
+NV21_TO_RGB, NV21_TO_BGR, NV21_TO_RGBA, NV21_TO_BGRA = 0x12, 0x32, 0x52, 0x72
+
+
+class KleidiCVError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s returned %s" % (what, ERROR_NAMES[code] if code < len(ERROR_NAMES) else code))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded C library; raises when it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libkleidicv_hip.so is not built: run "
+                               "python yocto-mirrors_gitlab_kleidi_kleidicv_amd/build.py")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.kleidicv_hip_default_stream.restype = C.c_void_p
+    return _lib
+
+
+def data_function(name, *argtypes):
+    """Callable behind a KLEIDICV_API_DECLARATION function-pointer variable."""
+    ptr = C.c_void_p.in_dll(lib(), name)
+    return C.CFUNCTYPE(C.c_int, *argtypes)(ptr.value)
+
+
+# ---------------------------------------------------------------------------
+# helpers
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return C.c_void_p(0)
+    if _is_torch(x):
+        return C.c_void_p(x.data_ptr())
+    return C.c_void_p(x.ctypes.data)
+
+
+def _shape(x):
+    s = tuple(x.shape)
+    return s if len(s) == 3 else (s[0], s[1], 1)
+
+
+def _stride(x):
+    """Row stride in bytes of a (h, w[, c]) array whose rows may be padded."""
+    if _is_torch(x):
+        return x.stride(0) * x.element_size()
+    return x.strides[0]
+
+
+def _empty_like_shape(like, shape, dtype=np.uint8):
+    if _is_torch(like):
+        import torch
+        return torch.empty(shape, dtype=torch.uint8, device=like.device)
+    return np.empty(shape, dtype=dtype)
+
+
+def _sz(*vals):
+    return [C.c_size_t(int(v)) for v in vals]
+
+
+def _check(code, what):
+    if code != OK:
+        raise KleidiCVError(code, what)
+
+
+def _stream_arg(stream):
+    if stream is None:
+        return C.c_void_p(0)
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(int(stream))
+
+
+# ---------------------------------------------------------------------------
+# Drop-in API (host or device buffers, synchronous) and device API (stream)
+def gaussian_blur(src, kernel_size, sigma=0.0, border="REVERSE", dst=None,
+                  stream=None, device_api=False):
+    h, w, c = _shape(src)
+    dst = _empty_like_shape(src, tuple(src.shape)) if dst is None else dst
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(dst), *_sz(_stride(dst), w, h, c,
+            kernel_size, kernel_size), C.c_float(sigma), C.c_float(sigma),
+            C.c_int(BORDER[border])]
+    if device_api:
+        _check(lib().kleidicv_hip_gaussian_blur_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_gaussian_blur_u8")
+    else:
+        _check(lib().kleidicv_gaussian_blur_u8(*args), "kleidicv_gaussian_blur_u8")
+    return dst
+
+
+def separable_filter_2d(src, kernel_x, kernel_y, border="REPLICATE", dst=None,
+                        stream=None, device_api=False):
+    h, w, c = _shape(src)
+    dst = _empty_like_shape(src, tuple(src.shape)) if dst is None else dst
+    kx = np.ascontiguousarray(kernel_x, dtype=np.uint8)
+    ky = np.ascontiguousarray(kernel_y, dtype=np.uint8)
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(dst), *_sz(_stride(dst), w, h, c),
+            _ptr(kx), *_sz(len(kx)), _ptr(ky), *_sz(len(ky)), C.c_int(BORDER[border])]
+    if device_api:
+        _check(lib().kleidicv_hip_separable_filter_2d_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_separable_filter_2d_u8")
+    else:
+        _check(lib().kleidicv_separable_filter_2d_u8(*args),
+               "kleidicv_separable_filter_2d_u8")
+    return dst
+
+
+def median_blur(src, kernel_size, border="REPLICATE", dst=None, stream=None,
+                device_api=False):
+    h, w, c = _shape(src)
+    dst = _empty_like_shape(src, tuple(src.shape)) if dst is None else dst
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(dst), *_sz(_stride(dst), w, h, c,
+            kernel_size, kernel_size), C.c_int(BORDER[border])]
+    if device_api:
+        _check(lib().kleidicv_hip_median_blur_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_median_blur_u8")
+    else:
+        _check(lib().kleidicv_median_blur_u8(*args), "kleidicv_median_blur_u8")
+    return dst
+
+
+def resize_linear(src, dst_width, dst_height, dst=None, stream=None,
+                  device_api=False):
+    h, w, c = _shape(src)
+    if dst is None:
+        shape = (dst_height, dst_width, c) if len(src.shape) == 3 else (dst_height, dst_width)
+        dst = _empty_like_shape(src, shape)
+    args = [_ptr(src), *_sz(_stride(src), w, h), _ptr(dst),
+            *_sz(_stride(dst), dst_width, dst_height, c)]
+    if device_api:
+        _check(lib().kleidicv_hip_resize_linear_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_resize_linear_u8")
+    else:
+        _check(lib().kleidicv_resize_linear_u8(*args), "kleidicv_resize_linear_u8")
+    return dst
+
+
+def remap_s16point5(src, mapxy, mapfrac, border="CONSTANT", border_value=(0, 0, 0, 0),
+                    dst=None, stream=None, device_api=False):
+    h, w, c = _shape(src)
+    dh, dw = mapfrac.shape[0], mapfrac.shape[1]
+    if dst is None:
+        dst = _empty_like_shape(src, (dh, dw, c) if len(src.shape) == 3 else (dh, dw))
+    bv = np.array(border_value, dtype=np.uint8)
+    args = [_ptr(src), *_sz(_stride(src), w, h), _ptr(dst),
+            *_sz(_stride(dst), dw, dh, c), _ptr(mapxy), *_sz(_stride(mapxy)),
+            _ptr(mapfrac), *_sz(_stride(mapfrac)), C.c_int(BORDER[border]), _ptr(bv)]
+    if device_api:
+        _check(lib().kleidicv_hip_remap_s16point5_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_remap_s16point5_u8")
+    else:
+        fn = data_function("kleidicv_remap_s16point5_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
+                           [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int,
+                            C.c_void_p]))
+        _check(fn(*args), "kleidicv_remap_s16point5_u8")
+    return dst
+
+
+def remap_f32(src, mapx, mapy, interpolation="LINEAR", border="REPLICATE",
+              border_value=(0, 0, 0, 0), dst=None, stream=None, device_api=False):
+    h, w, c = _shape(src)
+    dh, dw = mapx.shape[0], mapx.shape[1]
+    if dst is None:
+        dst = _empty_like_shape(src, (dh, dw, c) if len(src.shape) == 3 else (dh, dw))
+    bv = np.array(border_value, dtype=np.uint8)
+    args = [_ptr(src), *_sz(_stride(src), w, h), _ptr(dst), *_sz(_stride(dst), dw, dh, c),
+            _ptr(mapx), *_sz(_stride(mapx)), _ptr(mapy), *_sz(_stride(mapy)),
+            C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]), _ptr(bv)]
+    if device_api:
+        _check(lib().kleidicv_hip_remap_f32_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_remap_f32_u8")
+    else:
+        fn = data_function("kleidicv_remap_f32_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
+                           [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int,
+                            C.c_int, C.c_void_p]))
+        _check(fn(*args), "kleidicv_remap_f32_u8")
+    return dst
+
+
+def warp_perspective(src, transform, dst_width, dst_height, interpolation="LINEAR",
+                     border="REPLICATE", border_value=(0, 0, 0, 0), dst=None,
+                     stream=None, device_api=False):
+    h, w, c = _shape(src)
+    if dst is None:
+        dst = _empty_like_shape(src, (dst_height, dst_width, c) if len(src.shape) == 3
+                                else (dst_height, dst_width))
+    t = np.ascontiguousarray(transform, dtype=np.float32).reshape(9)
+    bv = np.array(border_value, dtype=np.uint8)
+    args = [_ptr(src), *_sz(_stride(src), w, h), _ptr(dst),
+            *_sz(_stride(dst), dst_width, dst_height), _ptr(t), *_sz(c),
+            C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]), _ptr(bv)]
+    if device_api:
+        _check(lib().kleidicv_hip_warp_perspective_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_warp_perspective_u8")
+    else:
+        _check(lib().kleidicv_warp_perspective_u8(*args), "kleidicv_warp_perspective_u8")
+    return dst
+
+
+def yuv_sp_to_rgb(y, uv, color_format=NV12_TO_RGB, dst=None, stream=None,
+                  device_api=False):
+    h, w = y.shape[0], y.shape[1]
+    oc = 4 if color_format & 0x40 else 3
+    dst = _empty_like_shape(y, (h, w, oc)) if dst is None else dst
+    args = [_ptr(y), *_sz(_stride(y)), _ptr(uv), *_sz(_stride(uv)), _ptr(dst),
+            *_sz(_stride(dst), w, h), C.c_int(color_format)]
+    if device_api:
+        _check(lib().kleidicv_hip_yuv_semiplanar_to_rgb_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_yuv_semiplanar_to_rgb_u8")
+    else:
+        fn = data_function("kleidicv_yuv_semiplanar_to_rgb_u8",
+                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
+                           C.c_size_t, C.c_size_t, C.c_size_t, C.c_int)
+        _check(fn(*args), "kleidicv_yuv_semiplanar_to_rgb_u8")
+    return dst
+
+
+def synchronize(stream=None):
+    _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
