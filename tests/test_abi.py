@@ -114,8 +114,7 @@ def test_resize_error_codes():
 def test_remap_error_codes():
     buf = np.zeros(4096, np.uint8)
     p = buf.ctypes.data
-    fn = K.data_function("kleidicv_remap_s16point5_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
-                         [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]))
+    fn = K.data_function("kleidicv_remap_s16point5_u8", *K.REMAP_S16_ARGTYPES)
     assert fn(p, 16, 16, 4, p, 16, 16, 4, 1, p, 64, p, 32, 0, None) == K.NULL_POINTER
     assert fn(p, 16, 16, 4, p, 16, 16, 4, 1, p, 63, p, 32, 1, None) == K.ALIGNMENT
     assert fn(p, 16, 16, 4, p, 16, 16, 4, 2, p, 64, p, 32, 1, None) == K.NOT_IMPLEMENTED
@@ -145,9 +144,7 @@ def test_warp_error_codes():
 def test_yuv_error_codes():
     buf = np.zeros(4096, np.uint8)
     p = buf.ctypes.data
-    fn = K.data_function("kleidicv_yuv_semiplanar_to_rgb_u8", C.c_void_p, C.c_size_t,
-                         C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_size_t,
-                         C.c_size_t, C.c_int)
+    fn = K.data_function("kleidicv_yuv_semiplanar_to_rgb_u8", *K.YUV_SP_ARGTYPES)
     assert fn(p, 16, p, 16, p, 48, 16, 4, 0x03) == K.NOT_IMPLEMENTED  # I420 is not semi-planar
     assert fn(p, 16, p, 16, p, 48, 16, 4, 0x82) == K.NOT_IMPLEMENTED
     assert fn(None, 16, p, 16, p, 48, 16, 4, 0x02) == K.NULL_POINTER

@@ -23,7 +23,6 @@ namespace kcv {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kRowsPerThread = 8;
 
 // ---------------------------------------------------------------------------
 // Host/device-shared coordinate arithmetic
@@ -316,142 +315,215 @@ __global__ __launch_bounds__(kThreads) void resize_tables_kernel(
   }
 }
 
-// vraddhn_u16(a << 8, (b - a) * w): exact in 16 bits for 0 <= w <= 255.
-__device__ __forceinline__ int lerp8(int a, int b, int w) {
-  return (256 * a + (b - a) * w + 128) >> 8;
-}
-
-__global__ __launch_bounds__(kThreads) void resize_generic_kernel(
-    ResizeArgs a, const int2 *__restrict__ cols, const int2 *__restrict__ rows) {
-  const int n = a.dst_width * a.channels;
-  const int e0 = (blockIdx.x * kThreads + threadIdx.x) * 4;
-  if (e0 >= n) return;
-  int left[4], right[4], frac[4], zero[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int2 c = cols[min(e0 + k, n - 1)];
-    left[k] = c.x;
-    right[k] = c.x + (c.y >> 16);
-    frac[k] = c.y & 0xFF;
-    zero[k] = (c.y >> 8) & 3;
-  }
-  const int y_begin = blockIdx.y * kRowsPerThread;
-  const int y_end = min(y_begin + kRowsPerThread, a.dst_height);
-  for (int y = y_begin; y < y_end; ++y) {
-    const int2 r = rows[y];
-    const uint8_t *top = a.src + static_cast<size_t>(r.x) * a.src_stride;
-    const uint8_t *bot =
-        a.src + static_cast<size_t>(r.y & 0xFFFFFF) * a.src_stride;
-    const int yfrac = (r.y >> 24) & 0xFF;
-    uint32_t packed = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int pa = (zero[k] & 1) ? 0 : top[left[k]];
-      const int pc = (zero[k] & 1) ? 0 : bot[left[k]];
-      const int pb = (zero[k] & 2) ? 0 : top[right[k]];
-      const int pd = (zero[k] & 2) ? 0 : bot[right[k]];
-      const int l = lerp8(pa, pc, yfrac), rr = lerp8(pb, pd, yfrac);
-      packed |= static_cast<uint32_t>(lerp8(l, rr, frac[k]) & 0xFF) << (8 * k);
-    }
-    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
-    if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
-      *reinterpret_cast<uint32_t *>(out + e0) = packed;
-    } else {
-      for (int k = 0; k < 4 && e0 + k < n; ++k)
-        out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Exact 0.5 x 0.5: 4 output bytes per thread per row.
-__global__ __launch_bounds__(kThreads) void resize_quarter_kernel(ResizeArgs a) {
-  const int C = a.channels;
-  const int n = a.dst_width * C;
-  const int e0 = (blockIdx.x * kThreads + threadIdx.x) * 4;
-  if (e0 >= n) return;
-  const int y_begin = blockIdx.y * kRowsPerThread;
-  const int y_end = min(y_begin + kRowsPerThread, a.dst_height);
-  const bool fast = C != 3 && e0 + 3 < n &&
-                    ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
-                      reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 7) == 0;
-  for (int y = y_begin; y < y_end; ++y) {
-    const uint8_t *r0 = a.src + static_cast<size_t>(2 * y) * a.src_stride;
-    const uint8_t *r1 = r0 + a.src_stride;
-    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
-    if (fast) {
-      const uint2 t = *reinterpret_cast<const uint2 *>(r0 + 2 * e0);
-      const uint2 b = *reinterpret_cast<const uint2 *>(r1 + 2 * e0);
-      const uint64_t tv = (static_cast<uint64_t>(t.y) << 32) | t.x;
-      const uint64_t bv = (static_cast<uint64_t>(b.y) << 32) | b.x;
-      uint32_t packed = 0;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int i0 = 2 * k - k % C, i1 = i0 + C;
-        const uint32_t s = ((tv >> (8 * i0)) & 0xFF) + ((tv >> (8 * i1)) & 0xFF) +
-                           ((bv >> (8 * i0)) & 0xFF) + ((bv >> (8 * i1)) & 0xFF);
-        packed |= ((s + 2) >> 2) << (8 * k);
-      }
-      *reinterpret_cast<uint32_t *>(out + e0) = packed;
-    } else {
-      for (int k = 0; k < 4 && e0 + k < n; ++k) {
-        const int e = e0 + k;
-        const int i0 = 2 * e - e % C, i1 = i0 + C;
-        const uint32_t s = r0[i0] + r0[i1] + r1[i0] + r1[i1];
-        out[e] = static_cast<uint8_t>((s + 2) >> 2);
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // Exact 1-channel 2x / 4x upsizers: separable weights (3,1)/4 resp.
-// (7,1),(5,3),(3,5),(1,7)/8 per axis with clamped taps and one rounding.
+// (7,1),(5,3),(3,5),(1,7)/8 per axis with clamped taps and one rounding
+// (the edge rows' 1-D formulas of resize_linear_neon.cpp are this closed form
+// with the other axis' weight all on one tap).  Their tables use the same
+// layout as the generic path's, with the right/bottom tap weight in place of
+// the 8-bit fraction.
 template <int F>
-__device__ __forceinline__ void taps(int d, int n, int &i0, int &i1, int &w0,
-                                     int &w1) {
-  constexpr int kUnit = F == 2 ? 4 : 8;
+__device__ __forceinline__ void exact_taps(int d, int n, int &i0, int &i1,
+                                           int &w1) {
   const int q = d - F / 2;
   if (q < 0) {
     i0 = i1 = 0;
-    w0 = kUnit;
     w1 = 0;
     return;
   }
   const int s = q / F, ph = q - s * F;
   i0 = s;
   i1 = min(s + 1, n - 1);
-  if (F == 2) {
-    w0 = ph == 0 ? 3 : 1;
-  } else {
-    w0 = 7 - 2 * ph;
-  }
-  w1 = kUnit - w0;
+  w1 = F == 2 ? (ph == 0 ? 1 : 3) : 1 + 2 * ph;
 }
 
 template <int F>
-__global__ __launch_bounds__(kThreads) void resize_exact_up_kernel(ResizeArgs a) {
-  constexpr int kShift = F == 2 ? 4 : 6;
-  const int n = a.dst_width;
-  const int e0 = (blockIdx.x * kThreads + threadIdx.x) * 4;
+__global__ __launch_bounds__(kThreads) void exact_tables_kernel(ResizeArgs a,
+                                                                int2 *cols,
+                                                                int2 *rows) {
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  int i0, i1, w1;
+  if (e < a.dst_width) {
+    exact_taps<F>(e, a.src_width, i0, i1, w1);
+    cols[e] = make_int2(i0, ((i1 - i0) << 16) | w1);
+  }
+  if (e < a.dst_height) {
+    exact_taps<F>(e, a.src_height, i0, i1, w1);
+    rows[e] = make_int2(i0, static_cast<int>(static_cast<unsigned>(i1) |
+                                             (static_cast<unsigned>(w1) << 24)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Staged gather.  A workgroup owns kTileE destination bytes of `band` rows.
+// It reads its column taps once, finds the source byte span they cover and
+// the source rows the band needs, copies that window into LDS with dword
+// loads, then every output is four LDS byte reads and two (or one) lerps.
+constexpr int kTileE = 4 * kThreads;       // destination bytes per workgroup
+constexpr int kStageBytes = 24 * 1024;     // LDS window capacity
+
+enum { kLerpGeneric = 0, kExact2 = 2, kExact4 = 4 };
+
+// vraddhn_u16(a << 8, (b - a) * w): exact in 16 bits for 0 <= w <= 255.
+__device__ __forceinline__ int lerp8(int a, int b, int w) {
+  return (256 * a + (b - a) * w + 128) >> 8;
+}
+
+template <int MODE>
+__device__ __forceinline__ uint32_t combine(int pa, int pb, int pc, int pd,
+                                            int wx, int wy) {
+  if (MODE == kLerpGeneric) {
+    return static_cast<uint32_t>(lerp8(lerp8(pa, pc, wy), lerp8(pb, pd, wy), wx));
+  } else {
+    constexpr int kUnit = MODE == kExact2 ? 4 : 8;
+    constexpr int kShift = MODE == kExact2 ? 4 : 6;
+    const int acc = (kUnit - wy) * ((kUnit - wx) * pa + wx * pb) +
+                    wy * ((kUnit - wx) * pc + wx * pd);
+    return static_cast<uint32_t>((acc + (1 << (kShift - 1))) >> kShift);
+  }
+}
+
+__device__ __forceinline__ int block_reduce(int v, bool is_max, int *scratch) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = is_max ? max(v, o) : min(v, o);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) scratch[wave] = v;
+  __syncthreads();
+  int r = scratch[0];
+#pragma unroll
+  for (int i = 1; i < kThreads / 64; ++i)
+    r = is_max ? max(r, scratch[i]) : min(r, scratch[i]);
+  __syncthreads();
+  return r;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void resize_staged_kernel(
+    ResizeArgs a, const int2 *__restrict__ cols, const int2 *__restrict__ rows,
+    int band) {
+  // 8 spare bytes: the 8-byte reads of the narrow path may start in the
+  // last dword of the window.
+  __shared__ __align__(16) uint8_t win[kStageBytes + 8];
+  __shared__ int scratch[kThreads / 64];
+  const int n = a.dst_width * a.channels;
+  const int row_bytes = a.src_width * a.channels;
+  const int e0 = blockIdx.x * kTileE + threadIdx.x * 4;
+
+  int left[4], right[4], wx[4], zero[4];
+  int lo = 0x7FFFFFFF, hi = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = e0 + k;
+    const int2 c = cols[min(e, n - 1)];
+    left[k] = c.x;
+    right[k] = c.x + (c.y >> 16);
+    wx[k] = c.y & 0xFF;
+    zero[k] = e < n ? (c.y >> 8) & 3 : 3;
+    if (!(zero[k] & 1)) { lo = min(lo, left[k]); hi = max(hi, left[k]); }
+    if (!(zero[k] & 2)) { lo = min(lo, right[k]); hi = max(hi, right[k]); }
+  }
+  lo = block_reduce(lo, false, scratch);
+  hi = block_reduce(hi, true, scratch);
+  if (hi < lo) { lo = 0; hi = 0; }
+  const int lo4 = lo & ~3;
+  const int pitch = ((hi + 1 - lo4) + 3) & ~3;
+
+  const int y_begin = blockIdx.y * band;
+  const int y_end = min(y_begin + band, a.dst_height);
+  const int r_lo = rows[y_begin].x;
+  const int r_hi = rows[y_end - 1].y & 0xFFFFFF;
+  const int nrows = r_hi - r_lo + 1;
+  const bool staged = nrows * pitch <= kStageBytes;
+
+  if (staged) {
+    const bool aligned =
+      ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
+    const int quads = pitch >> 2;
+    for (int i = threadIdx.x; i < nrows * quads; i += kThreads) {
+      const int r = i / quads, q = i - r * quads;
+      const uint8_t *srow = a.src + static_cast<size_t>(r_lo + r) * a.src_stride;
+      const int off = lo4 + 4 * q;
+      uint32_t v;
+      if (aligned && off + 4 <= row_bytes) {
+        v = *reinterpret_cast<const uint32_t *>(srow + off);
+      } else {
+        v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (off + b < row_bytes) v |= static_cast<uint32_t>(srow[off + b]) << (8 * b);
+      }
+      *reinterpret_cast<uint32_t *>(win + r * pitch + 4 * q) = v;
+    }
+    __syncthreads();
+  }
   if (e0 >= n) return;
-  int x0[4], x1[4], wx0[4], wx1[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) taps<F>(min(e0 + k, n - 1), a.src_width, x0[k], x1[k], wx0[k], wx1[k]);
-  const int y_begin = blockIdx.y * kRowsPerThread;
-  const int y_end = min(y_begin + kRowsPerThread, a.dst_height);
+  for (int k = 0; k < 4; ++k) {
+    left[k] -= lo4;
+    right[k] -= lo4;
+  }
+
+  // When a thread's taps lie within 8 bytes of a dword boundary (upsizing,
+  // mild downsizing) each source row is one 8-byte LDS read plus bit-field
+  // extracts instead of eight byte reads.
+  int tmin = 0x7FFFFFFF, tmax = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!(zero[k] & 1)) { tmin = min(tmin, left[k]); tmax = max(tmax, left[k]); }
+    if (!(zero[k] & 2)) { tmin = min(tmin, right[k]); tmax = max(tmax, right[k]); }
+  }
+  const int tbase = tmin & ~3;
+  const bool narrow = staged && tmax >= 0 && tmax - tbase < 8;
+  int sl[4], sr[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    sl[k] = 8 * (left[k] - tbase);
+    sr[k] = 8 * (right[k] - tbase);
+  }
+
   for (int y = y_begin; y < y_end; ++y) {
-    int y0, y1, wy0, wy1;
-    taps<F>(y, a.src_height, y0, y1, wy0, wy1);
-    const uint8_t *r0 = a.src + static_cast<size_t>(y0) * a.src_stride;
-    const uint8_t *r1 = a.src + static_cast<size_t>(y1) * a.src_stride;
+    const int2 r = rows[y];
+    const int top = r.x, bot = r.y & 0xFFFFFF;
+    const int wy = (r.y >> 24) & 0xFF;
     uint32_t packed = 0;
+    if (narrow) {
+      const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
+      const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
+      const uint64_t t = (static_cast<uint64_t>(tp[1]) << 32) | tp[0];
+      const uint64_t b = (static_cast<uint64_t>(bp[1]) << 32) | bp[0];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int acc = wy0 * (wx0[k] * r0[x0[k]] + wx1[k] * r0[x1[k]]) +
-                      wy1 * (wx0[k] * r1[x0[k]] + wx1[k] * r1[x1[k]]);
-      packed |= static_cast<uint32_t>((acc + (1 << (kShift - 1))) >> kShift)
-                << (8 * k);
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (zero[k] & 1) ? 0 : static_cast<int>((t >> sl[k]) & 0xFF);
+        const int pc = (zero[k] & 1) ? 0 : static_cast<int>((b >> sl[k]) & 0xFF);
+        const int pb = (zero[k] & 2) ? 0 : static_cast<int>((t >> sr[k]) & 0xFF);
+        const int pd = (zero[k] & 2) ? 0 : static_cast<int>((b >> sr[k]) & 0xFF);
+        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      }
+    } else if (staged) {
+      const uint8_t *t = win + (top - r_lo) * pitch;
+      const uint8_t *b = win + (bot - r_lo) * pitch;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
+        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
+        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
+        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      }
+    } else {
+      const uint8_t *t = a.src + static_cast<size_t>(top) * a.src_stride + lo4;
+      const uint8_t *b = a.src + static_cast<size_t>(bot) * a.src_stride + lo4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
+        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
+        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
+        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      }
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
     if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
@@ -459,6 +531,58 @@ __global__ __launch_bounds__(kThreads) void resize_exact_up_kernel(ResizeArgs a)
     } else {
       for (int k = 0; k < 4 && e0 + k < n; ++k)
         out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact 0.5 x 0.5: 16 output bytes per thread per row for 1, 2 and 4
+// channels (two 16-byte loads from each source row), bytewise for 3.
+__device__ __forceinline__ uint32_t quarter4(uint64_t t, uint64_t b, int C) {
+  uint32_t packed = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i0 = 2 * k - k % C, i1 = i0 + C;
+    const uint32_t s = ((t >> (8 * i0)) & 0xFF) + ((t >> (8 * i1)) & 0xFF) +
+                       ((b >> (8 * i0)) & 0xFF) + ((b >> (8 * i1)) & 0xFF);
+    packed |= ((s + 2) >> 2) << (8 * k);
+  }
+  return packed;
+}
+
+__global__ __launch_bounds__(kThreads) void resize_quarter_kernel(ResizeArgs a) {
+  const int C = a.channels;
+  const int n = a.dst_width * C;
+  const int e0 = (blockIdx.x * kThreads + threadIdx.x) * 16;
+  if (e0 >= n) return;
+  const bool fast = C != 3 && e0 + 15 < n &&
+                    ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
+                      reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+  for (int y = blockIdx.y; y < a.dst_height; y += gridDim.y) {
+    const uint8_t *r0 = a.src + static_cast<size_t>(2 * y) * a.src_stride;
+    const uint8_t *r1 = r0 + a.src_stride;
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
+    if (fast) {
+      const uint4 t0 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0);
+      const uint4 t1 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0 + 16);
+      const uint4 b0 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0);
+      const uint4 b1 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0 + 16);
+      auto u64 = [](uint32_t lo, uint32_t hi) {
+        return (static_cast<uint64_t>(hi) << 32) | lo;
+      };
+      uint4 o;
+      o.x = quarter4(u64(t0.x, t0.y), u64(b0.x, b0.y), C);
+      o.y = quarter4(u64(t0.z, t0.w), u64(b0.z, b0.w), C);
+      o.z = quarter4(u64(t1.x, t1.y), u64(b1.x, b1.y), C);
+      o.w = quarter4(u64(t1.z, t1.w), u64(b1.z, b1.w), C);
+      *reinterpret_cast<uint4 *>(out + e0) = o;
+    } else {
+      for (int k = 0; k < 16 && e0 + k < n; ++k) {
+        const int e = e0 + k;
+        const int i0 = 2 * e - e % C, i1 = i0 + C;
+        const uint32_t s = r0[i0] + r0[i1] + r1[i0] + r1[i1];
+        out[e] = static_cast<uint8_t>((s + 2) >> 2);
+      }
     }
   }
 }
@@ -475,52 +599,65 @@ Path pick(const ResizeArgs &a) {
   return Path::kGeneric;
 }
 
-dim3 quad_grid(int64_t elements, int rows) {
-  return dim3(static_cast<unsigned>((elements / 4 + kThreads) / kThreads),
-              static_cast<unsigned>((rows + kRowsPerThread - 1) / kRowsPerThread));
+// Rows per workgroup band so that the staged window fits in LDS for a
+// typical tile; tiles whose window does not fit read from global memory.
+int band_rows(const ResizeArgs &a) {
+  const double sx = static_cast<double>(a.src_width) / a.dst_width;
+  const double sy = static_cast<double>(a.src_height) / a.dst_height;
+  const double pitch = kTileE * sx + 8.0 * a.channels + 8.0;
+  const double rows_fit = kStageBytes / pitch;
+  int band = static_cast<int>((rows_fit - 2.0) / (sy > 0 ? sy : 1.0)) + 1;
+  band = band < 1 ? 1 : band > 64 ? 64 : band;
+  return band;
+}
+
+template <int MODE>
+hipError_t launch_staged(const ResizeArgs &a, const int2 *cols, const int2 *rows,
+                         hipStream_t stream) {
+  const int64_t n = int64_t{a.dst_width} * a.channels;
+  const int band = band_rows(a);
+  const dim3 grid(static_cast<unsigned>((n + kTileE - 1) / kTileE),
+                  static_cast<unsigned>((a.dst_height + band - 1) / band));
+  resize_staged_kernel<MODE><<<grid, kThreads, 0, stream>>>(a, cols, rows, band);
+  return hipGetLastError();
 }
 
 }  // namespace
 
 size_t resize_workspace_bytes(const ResizeArgs &a) {
-  if (pick(a) != Path::kGeneric) return 0;
+  const Path p = pick(a);
+  if (p == Path::kNone || p == Path::kQuarter) return 0;
   return (static_cast<size_t>(a.dst_width) * a.channels + a.dst_height) *
          sizeof(int2);
 }
 
 hipError_t launch_resize(const ResizeArgs &a, void *workspace,
                          hipStream_t stream) {
-  switch (pick(a)) {
-    case Path::kNone:
-      return hipSuccess;
-    case Path::kQuarter:
-      if (a.dst_width == 0 || a.dst_height == 0) return hipSuccess;
-      resize_quarter_kernel<<<quad_grid(int64_t{a.dst_width} * a.channels,
-                                        a.dst_height),
-                              kThreads, 0, stream>>>(a);
-      return hipGetLastError();
-    case Path::kUp2:
-      resize_exact_up_kernel<2><<<quad_grid(a.dst_width, a.dst_height),
-                                  kThreads, 0, stream>>>(a);
-      return hipGetLastError();
-    case Path::kUp4:
-      resize_exact_up_kernel<4><<<quad_grid(a.dst_width, a.dst_height),
-                                  kThreads, 0, stream>>>(a);
-      return hipGetLastError();
-    case Path::kGeneric: {
-      const GenericPlan p = make_plan(a);
-      const int64_t n = int64_t{a.dst_width} * a.channels;
-      int2 *cols = static_cast<int2 *>(workspace);
-      int2 *rows = cols + n;
-      const int64_t work = n > a.dst_height ? n : a.dst_height;
-      resize_tables_kernel<<<static_cast<unsigned>((work + kThreads - 1) / kThreads),
-                             kThreads, 0, stream>>>(p, cols, rows);
-      resize_generic_kernel<<<quad_grid(n, a.dst_height), kThreads, 0,
-                              stream>>>(a, cols, rows);
-      return hipGetLastError();
-    }
+  const Path path = pick(a);
+  if (path == Path::kNone || a.dst_width == 0 || a.dst_height == 0) return hipSuccess;
+  if (path == Path::kQuarter) {
+    const int64_t n = int64_t{a.dst_width} * a.channels;
+    const dim3 grid(static_cast<unsigned>((n + 16 * kThreads - 1) / (16 * kThreads)),
+                    static_cast<unsigned>(a.dst_height < 65535 ? a.dst_height : 65535));
+    resize_quarter_kernel<<<grid, kThreads, 0, stream>>>(a);
+    return hipGetLastError();
   }
-  return hipSuccess;
+  const int64_t n = int64_t{a.dst_width} * a.channels;
+  int2 *cols = static_cast<int2 *>(workspace);
+  int2 *rows = cols + n;
+  const int64_t work = n > a.dst_height ? n : a.dst_height;
+  const unsigned tgrid = static_cast<unsigned>((work + kThreads - 1) / kThreads);
+  if (path == Path::kUp2) {
+    exact_tables_kernel<2><<<tgrid, kThreads, 0, stream>>>(a, cols, rows);
+    return launch_staged<kExact2>(a, cols, rows, stream);
+  }
+  if (path == Path::kUp4) {
+    exact_tables_kernel<4><<<tgrid, kThreads, 0, stream>>>(a, cols, rows);
+    return launch_staged<kExact4>(a, cols, rows, stream);
+  }
+  const GenericPlan p = make_plan(a);
+  resize_tables_kernel<<<tgrid, kThreads, 0, stream>>>(p, cols, rows);
+  return launch_staged<kLerpGeneric>(a, cols, rows, stream);
 }
 
 }  // namespace kcv

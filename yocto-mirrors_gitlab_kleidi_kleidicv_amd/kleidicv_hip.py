@@ -41,6 +41,14 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libkleidicv_hip.so is not built: run "
                                "python yocto-mirrors_gitlab_kleidi_kleidicv_amd/build.py")
+        # PyTorch ships its own copy of the HIP runtime under the same soname
+        # (libamdhip64.so.7).  Loading torch first makes this library bind to
+        # that copy, so one runtime serves both and torch tensors, streams
+        # and events are valid handles here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = C.CDLL(LIB_PATH)
         _lib.kleidicv_hip_default_stream.restype = C.c_void_p
     return _lib
@@ -50,6 +58,13 @@ def data_function(name, *argtypes):
     """Callable behind a KLEIDICV_API_DECLARATION function-pointer variable."""
     ptr = C.c_void_p.in_dll(lib(), name)
     return C.CFUNCTYPE(C.c_int, *argtypes)(ptr.value)
+
+
+# Argument types of the function-pointer data symbols (kleidicv.h:2715, 2782).
+_P, _S = C.c_void_p, C.c_size_t
+REMAP_S16_ARGTYPES = [_P, _S, _S, _S, _P, _S, _S, _S, _S, _P, _S, _P, _S, C.c_int, _P]
+REMAP_F32_ARGTYPES = [_P, _S, _S, _S, _P, _S, _S, _S, _S, _P, _S, _P, _S, C.c_int, C.c_int, _P]
+YUV_SP_ARGTYPES = [_P, _S, _P, _S, _P, _S, _S, _S, C.c_int]
 
 
 # ---------------------------------------------------------------------------
@@ -180,9 +195,7 @@ def remap_s16point5(src, mapxy, mapfrac, border="CONSTANT", border_value=(0, 0, 
         _check(lib().kleidicv_hip_remap_s16point5_u8(*args, _stream_arg(stream)),
                "kleidicv_hip_remap_s16point5_u8")
     else:
-        fn = data_function("kleidicv_remap_s16point5_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
-                           [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int,
-                            C.c_void_p]))
+        fn = data_function("kleidicv_remap_s16point5_u8", *REMAP_S16_ARGTYPES)
         _check(fn(*args), "kleidicv_remap_s16point5_u8")
     return dst
 
@@ -201,9 +214,7 @@ def remap_f32(src, mapx, mapy, interpolation="LINEAR", border="REPLICATE",
         _check(lib().kleidicv_hip_remap_f32_u8(*args, _stream_arg(stream)),
                "kleidicv_hip_remap_f32_u8")
     else:
-        fn = data_function("kleidicv_remap_f32_u8", *([C.c_void_p] + [C.c_size_t] * 8 +
-                           [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_int,
-                            C.c_int, C.c_void_p]))
+        fn = data_function("kleidicv_remap_f32_u8", *REMAP_F32_ARGTYPES)
         _check(fn(*args), "kleidicv_remap_f32_u8")
     return dst
 
@@ -239,9 +250,7 @@ def yuv_sp_to_rgb(y, uv, color_format=NV12_TO_RGB, dst=None, stream=None,
         _check(lib().kleidicv_hip_yuv_semiplanar_to_rgb_u8(*args, _stream_arg(stream)),
                "kleidicv_hip_yuv_semiplanar_to_rgb_u8")
     else:
-        fn = data_function("kleidicv_yuv_semiplanar_to_rgb_u8",
-                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p,
-                           C.c_size_t, C.c_size_t, C.c_size_t, C.c_int)
+        fn = data_function("kleidicv_yuv_semiplanar_to_rgb_u8", *YUV_SP_ARGTYPES)
         _check(fn(*args), "kleidicv_yuv_semiplanar_to_rgb_u8")
     return dst
 
