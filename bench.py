@@ -185,7 +185,7 @@ def main():
             run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
     torch.cuda.synchronize()
 
-    ev = {name: [] for name, _, _ in ops}
+    # Whole-step time: every op of a step back to back on one stream.
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -194,21 +194,29 @@ def main():
     for i in range(args.steps):
         s = sets[i % len(sets)]
         for name, _, _ in ops:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
             run_op(K, torch, name, s, out, kx, ky, T, stream)
-            e1.record(stream)
-            ev[name].append((e0, e1))
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_time(stop) / args.steps
 
+    # Per-op launch duration: the op alone, `steps` launches back to back on
+    # the same stream between two events, rotating the input sets.
+    op_ms = {}
+    for name, _, _ in ops:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(args.steps):
+            run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        op_ms[name] = e0.elapsed_time(e1) / args.steps
+
     per_op = {}
     total_pix = 0
     for name, dst_pix, nbytes in ops:
-        ms = sum(a.elapsed_time(b) for a, b in ev[name]) / len(ev[name])
+        ms = op_ms[name]
         gbs = nbytes / (ms * 1e-3) / 1e9
         per_op[name] = {"ms": round(ms, 4), "mpix_s": round(dst_pix / (ms * 1e-3) / 1e6, 1),
                         "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
