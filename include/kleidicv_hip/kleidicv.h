@@ -225,8 +225,8 @@ kleidicv_error_t kleidicv_thread_yuv_semiplanar_to_rgb_u8(
 
 /* ------------------------------------------------------------------------
  * 3. Device-resident API: device pointers, `stream` is a hipStream_t (NULL =
- *    the library's own stream), no synchronisation.  Same validation and
- *    error codes as the drop-in functions.
+ *    HIP's null stream), no synchronisation.  Same validation and error
+ *    codes as the drop-in functions.
  * ------------------------------------------------------------------------ */
 
 kleidicv_error_t kleidicv_hip_gaussian_blur_u8(
@@ -280,7 +280,8 @@ kleidicv_error_t kleidicv_hip_yuv_semiplanar_to_rgb_u8(
     size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
     size_t height, kleidicv_color_conversion_t color_format, void *stream);
 
-/* Library stream used when `stream` is NULL, and a helper to wait on it. */
+/* The library's own non-blocking stream (used by the drop-in entry points),
+ * and a helper that waits for a stream (NULL = HIP's null stream). */
 void *kleidicv_hip_default_stream(void);
 kleidicv_error_t kleidicv_hip_synchronize(void *stream);
 

@@ -50,9 +50,9 @@ kleidicv_error_t from_hip(hipError_t e) {
   return KLEIDICV_ERROR_NOT_IMPLEMENTED;
 }
 
-hipStream_t pick_stream(void *stream) {
-  return stream ? static_cast<hipStream_t>(stream) : context().stream;
-}
+// The device API launches on the caller's stream as given; NULL is HIP's
+// null stream, as everywhere in HIP.
+hipStream_t pick_stream(void *stream) { return static_cast<hipStream_t>(stream); }
 
 // ---------------------------------------------------------------------------
 // Checks shared by the entry points (include/kleidicv/utils.h:374-429).
@@ -268,7 +268,7 @@ kleidicv_error_t gaussian_impl(const uint8_t *src, size_t src_stride, uint8_t *d
     }
   }
 
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   const size_t row_bytes = width * channels;
   if (!sync) {
     a.src = src;
@@ -333,7 +333,7 @@ kleidicv_error_t separable_impl(const uint8_t *src, size_t src_stride, uint8_t *
   // The taps are read on the host: for the device API they must therefore
   // be host memory, as in the reference's signature.
   uint8_t kx[7], ky[7];
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   if (is_device_pointer(kernel_x) || is_device_pointer(kernel_y)) {
     KCV_TRY(from_hip(hipMemcpy(kx, kernel_x, kernel_width, hipMemcpyDefault)));
     KCV_TRY(from_hip(hipMemcpy(ky, kernel_y, kernel_width, hipMemcpyDefault)));
@@ -394,7 +394,7 @@ kleidicv_error_t median_impl(const uint8_t *src, size_t src_stride, uint8_t *dst
   a.channels = static_cast<int>(channels);
   a.border = static_cast<int>(border_type);
   a.ksize = static_cast<int>(kernel_width);
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   const size_t row_bytes = width * channels;
   if (!sync) {
     a.src = src;
@@ -455,7 +455,7 @@ kleidicv_error_t resize_impl(const uint8_t *src, size_t src_stride, size_t src_w
   a.channels = static_cast<int>(channels);
   a.src_stride = src_stride;
   a.dst_stride = dst_stride;
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   const size_t ws_bytes = kcv::resize_workspace_bytes(a);
   Session ss(s);
   void *ws = nullptr;
@@ -519,7 +519,7 @@ kleidicv_error_t remap_s16point5_impl(
     else
       std::memcpy(a.border_value, border_value, channels);
   }
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   if (!sync) {
     a.src = src;
     a.dst = dst;
@@ -585,7 +585,7 @@ kleidicv_error_t remap_f32_impl(const uint8_t *src, size_t src_stride,
     else
       std::memcpy(a.border_value, border_value, channels);
   }
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   if (!sync) {
     a.src = src;
     a.dst = dst;
@@ -668,7 +668,7 @@ kleidicv_error_t warp_impl(const uint8_t *src, size_t src_stride, size_t src_wid
   }
   a.check_zero = (mn * mx <= 0.0f) ? 1 : 0;
 
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   if (!sync) {
     a.src = src;
     a.dst = dst;
@@ -714,7 +714,7 @@ kleidicv_error_t yuv_sp_impl(const uint8_t *src_y, size_t src_y_stride,
   a.uv_stride = src_uv_stride;
   a.dst_stride = dst_stride;
   const size_t oc = a.alpha ? 4 : 3;
-  hipStream_t s = pick_stream(stream);
+  hipStream_t s = sync ? context().stream : pick_stream(stream);
   if (!sync) {
     a.y = src_y;
     a.uv = src_uv;
@@ -1016,7 +1016,7 @@ void *kleidicv_hip_default_stream(void) { return context().stream; }
 
 kleidicv_error_t kleidicv_hip_synchronize(void *stream) {
   if (!context().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
-  return from_hip(hipStreamSynchronize(pick_stream(stream)));
+  return from_hip(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
 }
 
 }  // extern "C"

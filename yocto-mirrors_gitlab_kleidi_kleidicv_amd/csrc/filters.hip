@@ -208,12 +208,39 @@ struct Forgetful {
   }
 };
 
+// Forgetful prefix: the same steps, stopping once a[..END) have been
+// admitted.  The remaining candidates are a[LO_OUT..HI].
+template <int LO, int HI, int NXT, int END>
+struct ForgetfulPrefix {
+  static constexpr int kLoOut = LO + (END - NXT);
+  static __device__ __forceinline__ void run(us2 *a) {
+    if constexpr (NXT < END) {
+#pragma unroll
+      for (int i = LO + 1; i <= HI; ++i) cmpex(a[LO], a[i]);
+#pragma unroll
+      for (int i = LO + 1; i < HI; ++i) cmpex(a[i], a[HI]);
+      a[HI] = a[NXT];
+      ForgetfulPrefix<LO + 1, HI, NXT + 1, END>::run(a);
+    }
+  }
+};
+
+// Two vertically adjacent outputs share K-1 of their K window rows.  The
+// shared K(K-1) values are admitted first, in one forgetful-selection pass
+// valid for both windows (the candidate count and the number of unseen
+// values are the same for each); each window then finishes with its own
+// row.  About 105 instead of 165 compare-exchanges per 5x5 output.
 template <int K>
 __global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a) {
   constexpr int R = K / 2;
   constexpr int N = K * K;
   constexpr int M0 = N / 2 + 2;
+  constexpr int NC = K * (K - 1);    // shared values
+  constexpr int M1 = 2 * M0 - NC;    // candidates left after the shared part
+  constexpr int N2 = M1 + K;         // per-window remainder
+  static_assert(NC >= M0 && N2 / 2 + 2 == M1, "shared prefix shape");
   using G = Window<R>;
+  using Prefix = ForgetfulPrefix<0, M0 - 1, M0, NC>;
   __shared__ __align__(16) uint8_t win[G::kRows * G::kCols];
 
   const int e0 = blockIdx.x * kTileW;
@@ -226,28 +253,46 @@ __global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a) {
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int row_bytes = a.width * C;
 #pragma unroll 1
-  for (int rr = 0; rr < kTileH / 4; ++rr) {
-    const int r = ty + 4 * rr;
-    const int y = y0 + r;
-    if (y >= a.height) break;
-    uint32_t packed = 0;
+  for (int rp = 0; rp < kTileH / 8; ++rp) {
+    const int r = 4 * ty + 2 * rp;  // this pass: tile rows r and r + 1
+    if (y0 + r >= a.height) break;
+    uint32_t packed[2] = {0, 0};
 #pragma unroll
     for (int pair = 0; pair < 2; ++pair) {
       const int base = r * G::kCols + G::kPad + 4 * tx + 2 * pair;
-      us2 v[N];
+      auto at = [&](int i, int j) {
+        const int o = base + i * G::kCols + (j - R) * C;
+        return us2{win[o], win[o + 1]};
+      };
+      // window rows 1..K-1 of the upper output = rows 0..K-2 of the lower
+      us2 v[NC];
 #pragma unroll
-      for (int i = 0; i < K; ++i)
+      for (int i = 0; i < K - 1; ++i)
 #pragma unroll
-        for (int j = 0; j < K; ++j) {
-          const int o = base + i * G::kCols + (j - R) * C;
-          v[i * K + j] = us2{win[o], win[o + 1]};
-        }
-      const us2 m = Forgetful<N, 0, M0 - 1, M0>::run(v);
-      packed |= (static_cast<uint32_t>(m.x) | (static_cast<uint32_t>(m.y) << 8))
-                << (16 * pair);
+        for (int j = 0; j < K; ++j) v[i * K + j] = at(i + 1, j);
+      Prefix::run(v);
+      us2 up[N2], dn[N2];
+#pragma unroll
+      for (int i = 0; i < M1; ++i) up[i] = dn[i] = v[Prefix::kLoOut + i];
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        up[M1 + j] = at(0, j);
+        dn[M1 + j] = at(K, j);
+      }
+      const us2 mu = Forgetful<N2, 0, M1 - 1, M1>::run(up);
+      const us2 md = Forgetful<N2, 0, M1 - 1, M1>::run(dn);
+      packed[0] |= (static_cast<uint32_t>(mu.x) | (static_cast<uint32_t>(mu.y) << 8))
+                   << (16 * pair);
+      packed[1] |= (static_cast<uint32_t>(md.x) | (static_cast<uint32_t>(md.y) << 8))
+                   << (16 * pair);
     }
-    store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
-               row_bytes, packed);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int y = y0 + r + h;
+      if (y < a.height)
+        store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
+                   row_bytes, packed[h]);
+    }
   }
 }
 

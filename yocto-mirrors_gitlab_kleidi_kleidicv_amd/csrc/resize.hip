@@ -383,6 +383,33 @@ __device__ __forceinline__ uint32_t combine(int pa, int pb, int pc, int pd,
   }
 }
 
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
+__device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// combine<MODE> on two taps at once in 16-bit halves (v_pk_* arithmetic).
+// Every intermediate fits 16 bits: the lerp's 256a + (b-a)w + 128 lies in
+// [0, 65408] and wraps exactly as the reference's vraddhn_u16 operands do;
+// the exact upsizers' sums stay below 2^14.
+template <int MODE>
+__device__ __forceinline__ us2 combine2(us2 pa, us2 pb, us2 pc, us2 pd, us2 wx, us2 wy) {
+  const us2 k256 = us2{256, 256}, k128 = us2{128, 128};
+  if (MODE == kLerpGeneric) {
+    const us2 left = (pa * k256 + (pc - pa) * wy + k128) >> 8;
+    const us2 right = (pb * k256 + (pd - pb) * wy + k128) >> 8;
+    return (left * k256 + (right - left) * wx + k128) >> 8;
+  } else {
+    constexpr uint16_t kUnit = MODE == kExact2 ? 4 : 8;
+    constexpr uint16_t kShift = MODE == kExact2 ? 4 : 6;
+    const us2 unit = us2{kUnit, kUnit};
+    const us2 h0 = pa * (unit - wx) + pb * wx;
+    const us2 h1 = pc * (unit - wx) + pd * wx;
+    const us2 rnd = us2{1 << (kShift - 1), 1 << (kShift - 1)};
+    return (h0 * (unit - wy) + h1 * wy + rnd) >> kShift;
+  }
+}
+
 __device__ __forceinline__ int block_reduce(int v, bool is_max, int *scratch) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -477,12 +504,21 @@ __global__ __launch_bounds__(kThreads) void resize_staged_kernel(
   }
   const int tbase = tmin & ~3;
   const bool narrow = staged && tmax >= 0 && tmax - tbase < 8;
-  int sl[4], sr[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    sl[k] = 8 * (left[k] - tbase);
-    sr[k] = 8 * (right[k] - tbase);
-  }
+  // Byte selectors for v_perm_b32 over the 8-byte window {hi, lo}: two taps
+  // per selector land in the low bytes of a 16-bit pair; 0x0C selects zero,
+  // which also realises the vector code's "index beyond the load reads 0".
+  auto sel = [&](const int *idx, int flag, int k0) {
+    uint32_t v = 0x0C000C00u;
+    if (!(zero[k0] & flag)) v |= static_cast<uint32_t>(idx[k0] - tbase);
+    else v |= 0x0Cu;
+    if (!(zero[k0 + 1] & flag)) v |= static_cast<uint32_t>(idx[k0 + 1] - tbase) << 16;
+    else v |= 0x0Cu << 16;
+    return v;
+  };
+  const uint32_t sel_l01 = sel(left, 1, 0), sel_l23 = sel(left, 1, 2);
+  const uint32_t sel_r01 = sel(right, 2, 0), sel_r23 = sel(right, 2, 2);
+  const us2 wx01 = us2{static_cast<uint16_t>(wx[0]), static_cast<uint16_t>(wx[1])};
+  const us2 wx23 = us2{static_cast<uint16_t>(wx[2]), static_cast<uint16_t>(wx[3])};
 
   for (int y = y_begin; y < y_end; ++y) {
     const int2 r = rows[y];
@@ -492,16 +528,18 @@ __global__ __launch_bounds__(kThreads) void resize_staged_kernel(
     if (narrow) {
       const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
       const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
-      const uint64_t t = (static_cast<uint64_t>(tp[1]) << 32) | tp[0];
-      const uint64_t b = (static_cast<uint64_t>(bp[1]) << 32) | bp[0];
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int pa = (zero[k] & 1) ? 0 : static_cast<int>((t >> sl[k]) & 0xFF);
-        const int pc = (zero[k] & 1) ? 0 : static_cast<int>((b >> sl[k]) & 0xFF);
-        const int pb = (zero[k] & 2) ? 0 : static_cast<int>((t >> sr[k]) & 0xFF);
-        const int pd = (zero[k] & 2) ? 0 : static_cast<int>((b >> sr[k]) & 0xFF);
-        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
-      }
+      const uint32_t t0 = tp[0], t1 = tp[1], b0 = bp[0], b1 = bp[1];
+      const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
+      const us2 r01 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l01)),
+                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r01)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l01)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r01)), wx01, wy2);
+      const us2 r23 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l23)),
+                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r23)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l23)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r23)), wx23, wy2);
+      // bytes 0 and 2 of each pair word -> one dword
+      packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
     } else if (staged) {
       const uint8_t *t = win + (top - r_lo) * pitch;
       const uint8_t *b = win + (bot - r_lo) * pitch;
