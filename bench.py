@@ -18,9 +18,14 @@ Prints one JSON line (metric, value in destination Mpix/s, per-op GB/s and
 """
 
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -41,6 +46,10 @@ def parse():
     ap.add_argument("--sets", type=int, default=3, help="distinct input sets rotated")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ops", default="", help="comma-separated subset of ops (profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of 512 MiB (HBM counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     return ap.parse_args()
 
 
@@ -102,6 +111,70 @@ OPS = [
 ]
 
 
+# Library kernels launched by each op (substrings of the kernel names).
+OP_KERNELS = {
+    "separable_7x7_1ch": ["separable_kernel"],
+    "median_5x5_1ch": ["median_kernel"],
+    "resize_2x_up_4ch": ["resize_tables_kernel", "resize_staged_kernel"],
+    "resize_0.5x_4ch": ["resize_quarter_kernel"],
+    "remap_s16point5_1ch": ["remap_s16point5_kernel"],
+    "warp_perspective_4ch": ["warp_kernel"],
+    "nv12_to_rgb": ["nv12_kernel"],
+}
+
+
+def _pmc_pass(exe, counter, extra, td, tag):
+    """One rocprofv3 --pmc run of this script; mean counter value per kernel name."""
+    out = os.path.join(td, counter + "_" + tag)
+    env = dict(os.environ, TMPDIR="/tmp")
+    cmd = [exe, "--pmc", counter, "-f", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__)] + extra
+    subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL, timeout=240, check=True)
+    files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
+    acc = {}
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def measure_traffic(args, ops):
+    """HBM bytes per launch of each op from two rocprofv3 --pmc passes.
+
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are read in separate runs of
+    this script over the same ops, and each is divided by what the same
+    counter reads per byte for a plain 512 MiB device copy.  Runs before this
+    process touches the GPU, so the profiled children never share it with
+    the timed run.
+    """
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    names = ",".join(o[0] for o in ops)
+    op_args = ["--steps", "2", "--warmup", "1", "--sets", str(args.sets), "--no-cpu-baseline",
+               "--no-traffic", "--ops", names]
+    try:
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            per = {}
+            for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+                copy = _pmc_pass(exe, counter, ["--copy-calibration"], td, "copy")
+                kib = [v for k, v in copy.items() if "copyBuffer" in k or "copy" in k.lower()]
+                if not kib:
+                    return None, "copy kernel not seen by rocprofv3"
+                factor = max(kib) * 1024.0 / (512 << 20)
+                vals = _pmc_pass(exe, counter, op_args, td, "ops")
+                for name, _, _ in ops:
+                    tot = 0.0
+                    for sub in OP_KERNELS[name]:
+                        tot += sum(v for k, v in vals.items() if sub in k and "kcv::" in k)
+                    per.setdefault(name, {})[counter] = tot * 1024.0 / factor
+                per.setdefault("_factor", {})[counter] = round(factor, 4)
+    except (subprocess.SubprocessError, OSError) as e:
+        return None, "rocprofv3 pass failed: %s" % type(e).__name__
+    return per, None
+
+
 def run_op(K, torch, name, s, out, kx, ky, T, stream):
     if name == "separable_7x7_1ch":
         K.separable_filter_2d(s["sep_src"], kx, ky, "REPLICATE", dst=out[name], stream=stream,
@@ -159,8 +232,25 @@ def main():
         print("bench.py measures one GPU only: every op is per image and shards by "
               "image/request with no exchange; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    ops = [o for o in OPS if not args.ops or o[0] in args.ops.split(",")]
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not args.no_traffic and not args.copy_calibration:
+        traffic, traffic_note = measure_traffic(args, ops)
+
     import numpy as np
     import torch
+
+    if args.copy_calibration:
+        # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+        # scale the counters read for the image kernels.
+        n = 512 << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
 
     import kleidicv_hip as K
 
@@ -168,7 +258,6 @@ def main():
     gen = torch.Generator(device=dev)
     gen.manual_seed(1234)
     kx, ky, T = seeded_params(np)
-    ops = [o for o in OPS if not args.ops or o[0] in args.ops.split(",")]
     sets = [make_inputs(torch, gen, dev) for _ in range(args.sets)]
     out = {
         "separable_7x7_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
@@ -224,6 +313,9 @@ def main():
         per_op[name] = {"ms": round(ms, 4), "mpix_s": round(dst_pix / (ms * 1e-3) / 1e6, 1),
                         "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                         "bytes": nbytes}
+        if traffic is not None:
+            per_op[name]["traffic"] = int(traffic[name]["FETCH_SIZE"] +
+                                          traffic[name]["WRITE_SIZE"])
         total_pix += dst_pix
     value = total_pix / (step_ms * 1e-3) / 1e6
     dom = max(per_op, key=lambda n: per_op[n]["ms"])
@@ -248,8 +340,14 @@ def main():
         "per_op": per_op,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gb_s"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
-                     "traffic": None},
+                     "traffic": d.get("traffic")},
     }
+    if traffic is None:
+        result["roofline"]["traffic_note"] = traffic_note
+    else:
+        result["roofline"]["traffic_note"] = (
+            "HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, each divided by its "
+            "reading per byte of a plain 512 MiB copy (factors %s)" % json.dumps(traffic["_factor"]))
     if not args.no_cpu_baseline:
         s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
         result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)

@@ -1,299 +1,22 @@
-// Tiled 2-D filters for gfx950: separable integer filters (binomial and
-// sigma Gaussian, saturating separable filter) and the median filter.
-//
-// One workgroup of 256 threads (4 wave64) produces a tile of 256 bytes x 16
-// rows of the interleaved u8 image.  The tile's source window, widened by the
-// filter radius in rows and by a 16-byte-aligned pad in bytes, is staged in
-// LDS once; the border mode is applied while staging, so the arithmetic that
-// follows never tests for borders.  Interior windows are fetched with
-// 16-byte loads, edge windows byte by byte through the border map.
-//
-// Reference semantics:
-//   borders      include/kleidicv/workspace/border_{3x3,5x5,7x7}.h,
-//                src/filters/median_blur_border_handling.h:14-56
-//   binomial     src/filters/gaussian_blur_fixed_neon.cpp:27-459
-//   sigma        src/filters/gaussian_blur_fixed_neon.cpp:461-529
-//   separable    src/filters/separable_filter_2d_neon.cpp:19-119
-//   median       src/filters/median_blur_sorting_network_neon.cpp:173-204
+// Per-image launches of the tiled 2-D filters (bodies in filters_body.h).
 
-#include "kcv_internal.h"
+#include "filters_body.h"
 
 namespace kcv {
 namespace {
 
-constexpr int kTileW = 256;
-constexpr int kTileH = 16;
-constexpr int kThreads = 256;
+using namespace filt;
 
-// Border index map with a single reflection (valid for overhang <= n, the
-// reference's w,h >= k-1 precondition).  The final clamp only affects window
-// positions no output reads (tiles hanging over the image's right/bottom).
-__device__ __forceinline__ int map_index(int i, int n, int border) {
-  if (static_cast<unsigned>(i) < static_cast<unsigned>(n)) return i;
-  int r;
-  switch (border) {
-    case kReflect:
-      r = i < 0 ? -i - 1 : 2 * n - i - 1;
-      break;
-    case kWrap:
-      r = i < 0 ? n + i : i - n;
-      break;
-    case kReverse:
-      r = i < 0 ? min(-i, n - 1) : 2 * n - i - 2;
-      break;
-    default:
-      r = i < 0 ? 0 : n - 1;
-      break;
-  }
-  return min(max(r, 0), n - 1);
-}
-
-template <int R>
-struct Window {
-  static constexpr int kPad = 16 * ((R + 3) / 4);  // >= R*C bytes for C <= 4
-  static constexpr int kCols = kTileW + 2 * kPad;
-  static constexpr int kRows = kTileH + 2 * R;
-};
-
-// Stages rows [y0-R, y0+kTileH+R) x bytes [e0-pad, e0+kTileW+pad) of the
-// border-extended image into `win` (row pitch Window<R>::kCols).
-template <int R>
-__device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
-                                             size_t stride, int width,
-                                             int height, int channels,
-                                             int border, int e0, int y0) {
-  using G = Window<R>;
-  const int row_bytes = width * channels;
-  const int c0 = e0 - G::kPad;
-  const bool interior =
-      c0 >= 0 && c0 + G::kCols <= row_bytes && y0 - R >= 0 &&
-      y0 + kTileH + R <= height &&
-      ((reinterpret_cast<uintptr_t>(src) | stride) & 15) == 0;
-  if (interior) {
-    constexpr int kChunks = G::kCols / 16;
-    const uint8_t *base = src + static_cast<size_t>(y0 - R) * stride + c0;
-    for (int i = threadIdx.x; i < G::kRows * kChunks; i += kThreads) {
-      const int r = i / kChunks, q = i - r * kChunks;
-      const uint4 v = *reinterpret_cast<const uint4 *>(
-          base + static_cast<size_t>(r) * stride + q * 16);
-      *reinterpret_cast<uint4 *>(win + r * G::kCols + q * 16) = v;
-    }
-  } else {
-    for (int i = threadIdx.x; i < G::kRows * G::kCols; i += kThreads) {
-      const int r = i / G::kCols, w = i - r * G::kCols;
-      const int y = map_index(y0 - R + r, height, border);
-      const int e = c0 + w;
-      const int x = e >= 0 ? e / channels : -((channels - 1 - e) / channels);
-      const int c = e - x * channels;
-      const int xm = map_index(x, width, border);
-      win[i] = src[static_cast<size_t>(y) * stride + xm * channels + c];
-    }
-  }
-}
-
-__device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
-                                           uint32_t packed) {
-  if (e + 3 < row_bytes && (reinterpret_cast<uintptr_t>(row + e) & 3) == 0) {
-    *reinterpret_cast<uint32_t *>(row + e) = packed;
-  } else {
-#pragma unroll
-    for (int b = 0; b < 4; ++b)
-      if (e + b < row_bytes) row[e + b] = static_cast<uint8_t>(packed >> (8 * b));
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Separable integer filters.
 template <int MODE, int K>
 __global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a) {
-  constexpr int R = K / 2;
-  using G = Window<R>;
-  __shared__ __align__(16) uint8_t win[G::kRows * G::kCols];
-  __shared__ __align__(16) uint32_t vert[kTileH * G::kCols];
-
-  const int e0 = blockIdx.x * kTileW;
-  const int y0 = blockIdx.y * kTileH;
-  const int C = a.channels;
-  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0,
-                  y0);
-  __syncthreads();
-
-  // Vertical pass: four columns per work item, one LDS dword per tap row.
-  constexpr int kQuads = G::kCols / 4;
-  for (int i = threadIdx.x; i < kTileH * kQuads; i += kThreads) {
-    const int r = i / kQuads, q = i - r * kQuads;
-    uint32_t acc[4] = {0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t v =
-          *reinterpret_cast<const uint32_t *>(win + (r + k) * G::kCols + 4 * q);
-      const uint32_t w = static_cast<uint32_t>(a.wy[k]);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[b] += w * ((v >> (8 * b)) & 0xFFu);
-    }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      uint32_t v = acc[b];
-      if (MODE == kSepSigma) v = (v + 128u) >> 8;
-      vert[r * G::kCols + 4 * q + b] = v;
-    }
-  }
-  __syncthreads();
-
-  // Horizontal pass: each thread writes 4 bytes in each of 4 rows.
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row_bytes = a.width * C;
-#pragma unroll
-  for (int rr = 0; rr < kTileH / 4; ++rr) {
-    const int r = ty + 4 * rr;
-    const int y = y0 + r;
-    if (y >= a.height) break;
-    uint32_t packed = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int base = r * G::kCols + G::kPad + 4 * tx + b;
-      uint32_t acc = 0;
-#pragma unroll
-      for (int j = 0; j < K; ++j)
-        acc += static_cast<uint32_t>(a.wx[j]) * vert[base + (j - R) * C];
-      uint32_t out;
-      if (MODE == kSepBinomial)
-        out = (acc + (1u << (a.shift - 1))) >> a.shift;
-      else if (MODE == kSepSigma)
-        out = (acc + 128u) >> 8;
-      else
-        out = min(acc, 255u);
-      packed |= (out & 0xFFu) << (8 * b);
-    }
-    store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
-               row_bytes, packed);
-  }
+  __shared__ __align__(16) uint8_t smem[separable_smem_bytes<K>()];
+  separable_tile<MODE, K>(a, blockIdx.x, blockIdx.y, smem);
 }
 
-// ---------------------------------------------------------------------------
-// Median: exact selection by the forgetful-selection scheme on pairs of
-// outputs packed in 16-bit halves (v_pk_min_u16 / v_pk_max_u16).  Any exact
-// selection returns the reference sorting network's value.
-typedef unsigned short us2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ void cmpex(us2 &lo, us2 &hi) {
-  const us2 a = lo;
-  lo = __builtin_elementwise_min(a, hi);
-  hi = __builtin_elementwise_max(a, hi);
-}
-
-__device__ __forceinline__ us2 med3(us2 a, us2 b, us2 c) {
-  return __builtin_elementwise_max(
-      __builtin_elementwise_min(a, b),
-      __builtin_elementwise_min(__builtin_elementwise_max(a, b), c));
-}
-
-// Candidates live in a[LO..HI]; a[NXT..N) are not seen yet.  Each step moves
-// the minimum to a[LO] and the maximum to a[HI], drops both and admits one
-// new element, which keeps the median of all N inside the candidate set.
-template <int N, int LO, int HI, int NXT>
-struct Forgetful {
-  static __device__ __forceinline__ us2 run(us2 *a) {
-    if constexpr (NXT == N) {
-      static_assert(HI - LO == 2, "three candidates remain");
-      return med3(a[LO], a[LO + 1], a[LO + 2]);
-    } else {
-#pragma unroll
-      for (int i = LO + 1; i <= HI; ++i) cmpex(a[LO], a[i]);
-#pragma unroll
-      for (int i = LO + 1; i < HI; ++i) cmpex(a[i], a[HI]);
-      a[HI] = a[NXT];
-      return Forgetful<N, LO + 1, HI, NXT + 1>::run(a);
-    }
-  }
-};
-
-// Forgetful prefix: the same steps, stopping once a[..END) have been
-// admitted.  The remaining candidates are a[LO_OUT..HI].
-template <int LO, int HI, int NXT, int END>
-struct ForgetfulPrefix {
-  static constexpr int kLoOut = LO + (END - NXT);
-  static __device__ __forceinline__ void run(us2 *a) {
-    if constexpr (NXT < END) {
-#pragma unroll
-      for (int i = LO + 1; i <= HI; ++i) cmpex(a[LO], a[i]);
-#pragma unroll
-      for (int i = LO + 1; i < HI; ++i) cmpex(a[i], a[HI]);
-      a[HI] = a[NXT];
-      ForgetfulPrefix<LO + 1, HI, NXT + 1, END>::run(a);
-    }
-  }
-};
-
-// Two vertically adjacent outputs share K-1 of their K window rows.  The
-// shared K(K-1) values are admitted first, in one forgetful-selection pass
-// valid for both windows (the candidate count and the number of unseen
-// values are the same for each); each window then finishes with its own
-// row.  About 105 instead of 165 compare-exchanges per 5x5 output.
 template <int K>
 __global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a) {
-  constexpr int R = K / 2;
-  constexpr int N = K * K;
-  constexpr int M0 = N / 2 + 2;
-  constexpr int NC = K * (K - 1);    // shared values
-  constexpr int M1 = 2 * M0 - NC;    // candidates left after the shared part
-  constexpr int N2 = M1 + K;         // per-window remainder
-  static_assert(NC >= M0 && N2 / 2 + 2 == M1, "shared prefix shape");
-  using G = Window<R>;
-  using Prefix = ForgetfulPrefix<0, M0 - 1, M0, NC>;
-  __shared__ __align__(16) uint8_t win[G::kRows * G::kCols];
-
-  const int e0 = blockIdx.x * kTileW;
-  const int y0 = blockIdx.y * kTileH;
-  const int C = a.channels;
-  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0,
-                  y0);
-  __syncthreads();
-
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row_bytes = a.width * C;
-#pragma unroll 1
-  for (int rp = 0; rp < kTileH / 8; ++rp) {
-    const int r = 4 * ty + 2 * rp;  // this pass: tile rows r and r + 1
-    if (y0 + r >= a.height) break;
-    uint32_t packed[2] = {0, 0};
-#pragma unroll
-    for (int pair = 0; pair < 2; ++pair) {
-      const int base = r * G::kCols + G::kPad + 4 * tx + 2 * pair;
-      auto at = [&](int i, int j) {
-        const int o = base + i * G::kCols + (j - R) * C;
-        return us2{win[o], win[o + 1]};
-      };
-      // window rows 1..K-1 of the upper output = rows 0..K-2 of the lower
-      us2 v[NC];
-#pragma unroll
-      for (int i = 0; i < K - 1; ++i)
-#pragma unroll
-        for (int j = 0; j < K; ++j) v[i * K + j] = at(i + 1, j);
-      Prefix::run(v);
-      us2 up[N2], dn[N2];
-#pragma unroll
-      for (int i = 0; i < M1; ++i) up[i] = dn[i] = v[Prefix::kLoOut + i];
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        up[M1 + j] = at(0, j);
-        dn[M1 + j] = at(K, j);
-      }
-      const us2 mu = Forgetful<N2, 0, M1 - 1, M1>::run(up);
-      const us2 md = Forgetful<N2, 0, M1 - 1, M1>::run(dn);
-      packed[0] |= (static_cast<uint32_t>(mu.x) | (static_cast<uint32_t>(mu.y) << 8))
-                   << (16 * pair);
-      packed[1] |= (static_cast<uint32_t>(md.x) | (static_cast<uint32_t>(md.y) << 8))
-                   << (16 * pair);
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int y = y0 + r + h;
-      if (y < a.height)
-        store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
-                   row_bytes, packed[h]);
-    }
-  }
+  __shared__ __align__(16) uint8_t smem[median_smem_bytes<K>()];
+  median_tile<K>(a, blockIdx.x, blockIdx.y, smem);
 }
 
 dim3 tile_grid(int width, int height, int channels) {

@@ -1,0 +1,670 @@
+// Device bodies and host helpers of the linear resize (per-image kernels in
+// resize.hip, batched kernels in batch.hip).
+//
+// Path selection follows src/resize/resize_linear_api.cpp:82-216:
+//   * exact 0.5 x 0.5            -> quarter kernel, (a+b+c+d+2) >> 2
+//                                   (src/resize/resize_to_quarter_neon.cpp:179-213)
+//   * 1 channel, exact 2x or 4x  -> closed-form upsizer
+//                                   (src/resize/resize_linear_neon.cpp:61-...)
+//   * everything else            -> generic fixed-point lerp
+//                                   (src/resize/resize_linear_generic_u8_neon.h)
+//
+// The generic path reproduces the reference's column coordinates exactly,
+// including the accumulated 16.16 step with its periodic recalibration, the
+// pull-backs at the right edge and the table-lookup semantics of the vector
+// code (an index beyond the loaded bytes reads 0).  A small kernel evaluates
+// that generator in closed form per destination element (every lane of every
+// vector is a pure function of its vector index) and writes a column table
+// and a row table to the workspace; the main kernel then gathers and does the
+// two 8-bit lerps, vertical first.
+
+#ifndef KCV_RESIZE_BODY_H
+#define KCV_RESIZE_BODY_H
+
+#include "kcv_internal.h"
+
+namespace kcv {
+namespace rsz {
+
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Host/device-shared coordinate arithmetic
+// (resize_linear_generic_u8_neon.h:73-88).
+__host__ __device__ __forceinline__ int64_t rounding_div(uint64_t nom,
+                                                         uint64_t denom) {
+  return static_cast<int64_t>((nom + denom / 2) / denom);
+}
+
+__host__ __device__ __forceinline__ int64_t aligned_scale(uint64_t x,
+                                                          uint64_t nom,
+                                                          uint64_t denom) {
+  return rounding_div(((x << 16) + 32768) * nom, denom) - 32768 + 128;
+}
+
+struct GenericPlan {
+  int src_width, dst_width, channels;
+  int ratio;           // source vectors loaded per 16 destination bytes
+  int upsize;          // horizontal upsizing variant
+  int set_rightmost;   // 3ch r3 with inverse scale >= 2.8
+  int64_t two_x;       // pairs of full vectors
+  int64_t half_n;      // half vectors
+  int full_read;       // bytes loaded per full vector
+  int half_read;       // bytes loaded per half vector
+  int dst_height, src_height;
+};
+
+inline GenericPlan make_plan(const ResizeArgs &a) {
+  GenericPlan p{};
+  const int64_t sw = a.src_width, dw = a.dst_width, C = a.channels;
+  p.src_width = a.src_width;
+  p.dst_width = a.dst_width;
+  p.channels = a.channels;
+  p.src_height = a.src_height;
+  p.dst_height = a.dst_height;
+  auto set = [&](int r, int up) {
+    p.ratio = r;
+    p.upsize = up;
+  };
+  if (C == 1) {
+    if (dw * 14 > sw * 15) set(1, 1);
+    else if (dw >= sw) set(2, 1);
+    else if (dw * 2 >= sw) set(2, 0);
+    else set(3, 0);
+  } else if (C == 2) {
+    if (dw * 6 > sw * 7) set(1, 1);
+    else if (dw >= sw) set(2, 1);
+    else if (dw * 2 >= sw) set(2, 0);
+    else set(3, 0);
+  } else if (C == 3) {
+    if (dw * 2 > sw * 3) set(1, 1);
+    else if (dw >= sw) set(2, 1);
+    else {
+      const double inv = static_cast<double>(sw) / static_cast<double>(dw);
+      if (inv < 1.8) set(2, 0);
+      else {
+        set(3, 0);
+        p.set_rightmost = inv >= 2.8;  // resize_linear_generic_neon.cpp:39-52
+      }
+    }
+  } else {
+    if (dw * 2 > sw * 3) set(1, 1);
+    else if (dw >= sw) set(2, 1);
+    else if (dw * 2 >= sw) set(2, 0);
+    else set(3, 0);
+  }
+  p.full_read = 16 * p.ratio;
+  if (C == 3)
+    p.half_read = p.upsize ? 16 : 32;
+  else
+    p.half_read = (p.ratio == 3 && !p.upsize) ? 32 : 16;
+  // calculate_num_of_vector_paths (resize_linear_generic_u8_neon.h:151-160)
+  p.two_x = (sw * C >= p.full_read) ? (dw * C) / 32 : 0;
+  const int64_t remaining = dw * C - p.two_x * 32;
+  p.half_n = (remaining + 7) / 8;
+  return p;
+}
+
+struct Tap {
+  int64_t base;
+  int idx0, idx1;  // tbl indices as unsigned bytes
+  int frac;
+  int load;
+  bool direct_right;
+};
+
+__device__ __forceinline__ int to_u8(int v) { return v & 0xFF; }
+__device__ __forceinline__ int to_s8(int v) {
+  v &= 0xFF;
+  return v >= 128 ? v - 256 : v;
+}
+
+__device__ __forceinline__ uint32_t scale_x(const GenericPlan &p, int64_t dx) {
+  return static_cast<uint32_t>(
+      rounding_div((static_cast<uint64_t>(dx) * p.src_width) << 16, p.dst_width));
+}
+
+// One lane of a full or half vector for 1, 2 or 4 channels
+// (resize_linear_generic_u8_neon.h:301-401).
+__device__ Tap lane_124(const GenericPlan &p, int64_t sx_base, int64_t sx_fixp,
+                        int lane, int load) {
+  const int C = p.channels;
+  const int32_t xfrac0 = static_cast<int32_t>(sx_fixp - (sx_base << 16));
+  const uint32_t delta = scale_x(p, lane / C) + static_cast<uint32_t>(xfrac0);
+  int i0 = to_s8(static_cast<int>(delta >> 16));
+  int i1 = to_s8(i0 + 1);
+  if (p.upsize) {
+    const int mx = (load - 1) / C;
+    i0 = min(max(i0, 0), mx);
+    i1 = min(max(i1, 0), mx);
+  }
+  int u0 = to_u8(i0), u1 = to_u8(i1);
+  if (C > 1) {
+    const int sh = C / 2;
+    const int off = lane % C;
+    u0 = to_u8((u0 << sh) + off);
+    u1 = to_u8((u1 << sh) + off);
+  }
+  return Tap{sx_base * C, u0, u1, static_cast<int>((delta >> 8) & 0xFF), load,
+             false};
+}
+
+__device__ Tap full_124(const GenericPlan &p, int64_t v, int lane) {
+  const int C = p.channels;
+  const int64_t i = v >> 1;
+  const int64_t i0 = i & ~int64_t{255};
+  const int64_t dx0 = i0 * 2 * (16 / C);
+  const int64_t step =
+      rounding_div((static_cast<uint64_t>(p.src_width) * 16 / C) << 16,
+                   p.dst_width);
+  const int64_t sx_fixp =
+      aligned_scale(dx0, p.src_width, p.dst_width) + (v - 2 * i0) * step;
+  const int64_t max_sx =
+      (static_cast<int64_t>(p.src_width) * C - p.full_read) / C;
+  const int64_t base = min(max(sx_fixp >> 16, int64_t{0}), max_sx);
+  return lane_124(p, base, sx_fixp, lane, p.full_read);
+}
+
+__device__ Tap half_124(const GenericPlan &p, int64_t h, int lane) {
+  const int C = p.channels;
+  const int64_t d0 = p.two_x * 32 / C;  // first half vector's pixel
+  const int64_t step_px = 8 / C;
+  const int64_t dx = d0 + h * step_px;
+  const int64_t dxp = min(dx, static_cast<int64_t>(p.dst_width) - step_px);
+  const int64_t one_px =
+      rounding_div(static_cast<uint64_t>(p.src_width) << 16, p.dst_width);
+  const int64_t half_step =
+      rounding_div((static_cast<uint64_t>(p.src_width) * 8 / C) << 16,
+                   p.dst_width);
+  const int64_t sx_fixp = aligned_scale(d0, p.src_width, p.dst_width) +
+                          h * half_step - (dx - dxp) * one_px;
+  const int64_t max_sx =
+      (static_cast<int64_t>(p.src_width) * C - p.half_read) / C;
+  const int64_t base = min(max(sx_fixp >> 16, int64_t{0}), max_sx);
+  return lane_124(p, base, sx_fixp, lane, p.half_read);
+}
+
+// 3-channel full vectors (resize_linear_generic_u8_neon.h:404-628): lanes
+// of the r, g and b vectors of a 16-pixel triplet.
+__device__ Tap full_3(const GenericPlan &p, int64_t v, int lane) {
+  const int64_t t = v / 3;
+  const int phase = static_cast<int>(v - 3 * t);
+  int64_t tr = 0;
+  if (t >= 512) tr = 512 + 513 * ((t - 512) / 513);
+  const int64_t triplet_step =
+      rounding_div(static_cast<uint64_t>(p.src_width) << 20, p.dst_width);
+  const int64_t sx_fixp =
+      aligned_scale(16 * tr, p.src_width, p.dst_width) + (t - tr) * triplet_step;
+  const int64_t sx_base = sx_fixp >> 16;
+  const int64_t src_base = sx_base * 3;
+  const int32_t xfrac0 = static_cast<int32_t>(sx_fixp & 0xFFFF);
+  const int64_t sw3 = static_cast<int64_t>(p.src_width) * 3;
+  const int64_t max_base = sw3 > p.full_read ? sw3 - p.full_read : 0;
+  // lane -> pixel offset within the triplet and channel offset
+  auto px_of = [&](int l) { return (16 * phase + l) / 3; };
+  auto ch_of = [&](int l) { return (16 * phase + l) % 3; };
+  auto idx0_of = [&](int l) {
+    const uint32_t delta = scale_x(p, px_of(l)) + static_cast<uint32_t>(xfrac0);
+    int i0 = to_s8(static_cast<int>(delta >> 16));
+    if (p.upsize) {
+      const int mn = static_cast<int>(min(max(-sx_base, int64_t{-128}), int64_t{127}));
+      const uint64_t room = static_cast<uint64_t>(p.src_width) - 1 -
+                            static_cast<uint64_t>(sx_base);
+      const int mx = static_cast<int>(room > 127 ? 127 : room);
+      i0 = min(max(i0, mn), mx);
+    }
+    return to_s8(to_s8(i0 * 3) + ch_of(l));
+  };
+  const uint32_t delta =
+      scale_x(p, px_of(lane)) + static_cast<uint32_t>(xfrac0);
+  int i0 = to_s8(static_cast<int>(delta >> 16));
+  int i1 = to_s8(i0 + 1);
+  if (p.upsize) {
+    const int mn = static_cast<int>(min(max(-sx_base, int64_t{-128}), int64_t{127}));
+    const uint64_t room =
+        static_cast<uint64_t>(p.src_width) - 1 - static_cast<uint64_t>(sx_base);
+    const int mx = static_cast<int>(room > 127 ? 127 : room);
+    i0 = min(max(i0, mn), mx);
+    i1 = min(max(i1, mn), mx);
+  }
+  i0 = to_s8(to_s8(i0 * 3) + ch_of(lane));
+  i1 = to_s8(to_s8(i1 * 3) + ch_of(lane));
+  const int mn_idx = min(min(idx0_of(0), idx0_of(1)), idx0_of(2));
+  const int64_t final_base =
+      min(max(src_base + mn_idx, int64_t{0}), max_base);
+  const int adj = to_s8(static_cast<int>(final_base - src_base));
+  return Tap{final_base, to_u8(i0 - adj), to_u8(i1 - adj),
+             static_cast<int>((delta >> 8) & 0xFF), p.full_read,
+             p.set_rightmost && lane == 15};
+}
+
+// 3-channel half vectors, filled scalarly (:630-703).
+__device__ Tap half_3(const GenericPlan &p, int64_t start, int lane) {
+  const int64_t sw = p.src_width;
+  const int64_t sw3 = sw * 3;
+  const int64_t max_base = sw3 > p.half_read ? sw3 - p.half_read : 0;
+  const int64_t dx = start / 3;
+  const int inpx = static_cast<int>(start % 3);
+  const int64_t sxp0 = aligned_scale(dx, p.src_width, p.dst_width);
+  const int64_t sxp1 = aligned_scale(dx + 1, p.src_width, p.dst_width);
+  const int64_t sei0 = ((sxp0 >> 16) * 3) + inpx;
+  const int64_t sei1 = (sxp1 >> 16) * 3;
+  const int64_t base = min(max(min(sei0, sei1), int64_t{0}), max_base);
+  const int64_t one_px =
+      rounding_div(static_cast<uint64_t>(p.src_width) << 16, p.dst_width);
+  // Which pixel of the half vector this lane belongs to, and its channel.
+  int pix, k;
+  if (lane < 3 - inpx) {
+    pix = 0;
+    k = inpx + lane;
+  } else {
+    const int l = lane - (3 - inpx);
+    pix = 1 + l / 3;
+    k = l % 3;
+  }
+  const int64_t sx = sxp0 + pix * one_px;
+  int64_t s0 = sx >> 16, s1 = s0 + 1;
+  if (p.upsize) {
+    s0 = min(max(s0, int64_t{0}), sw - 1);
+    s1 = min(max(s1, int64_t{0}), sw - 1);
+  }
+  int frac = static_cast<int>((sx & 0xFFFF) >> 8);
+  // The first pixel tests against the base, the second does not, later
+  // ones do again (fill_half_constants_scalarly, :654-702).
+  const int64_t sei = pix == 0 ? sei0 : (sx >> 16) * 3;
+  if (pix != 1 && sei < base) frac = 0;
+  return Tap{base, to_u8(static_cast<int>(s0 * 3 + k - base)),
+             to_u8(static_cast<int>(s1 * 3 + k - base)), frac, p.half_read,
+             false};
+}
+
+// Column entry e (e < dst_width * channels) and row entry e (e < dst_height)
+// of the generic path's tables.
+__device__ __forceinline__ void resize_tables_elem(const GenericPlan &p, int2 *cols,
+                                                   int2 *rows, int64_t e) {
+  const int64_t n = static_cast<int64_t>(p.dst_width) * p.channels;
+  if (e < n) {
+    Tap t{};
+    bool found = false;
+    for (int64_t h = p.half_n - 1; h >= 0 && !found; --h) {
+      const int64_t start = min(p.two_x * 32 + 8 * h, n - 8);
+      if (e >= start && e < start + 8) {
+        t = p.channels == 3 ? half_3(p, start, static_cast<int>(e - start))
+                            : half_124(p, h, static_cast<int>(e - start));
+        found = true;
+      }
+    }
+    if (!found) {
+      t = p.channels == 3 ? full_3(p, e / 16, static_cast<int>(e % 16))
+                          : full_124(p, e / 16, static_cast<int>(e % 16));
+    }
+    // Taps stay inside the row even where the vector code would read beyond
+    // it (only the kSetRightmostLanes lane reads outside its tbl range).
+    const int64_t last = static_cast<int64_t>(p.src_width) * p.channels - 1;
+    const int left = static_cast<int>(min(t.base + t.idx0, last));
+    const int right = static_cast<int>(min(t.base + t.idx1, last));
+    const int zero = (t.idx0 < t.load ? 0 : 1) |
+                     ((t.idx1 < t.load || t.direct_right) ? 0 : 2);
+    cols[e] = make_int2(left, ((right - left) << 16) | (zero << 8) | t.frac);
+  }
+  if (e < p.dst_height) {
+    // ResizeGenericU8Operation::process_row (:730-754)
+    const int64_t sy_fixp = aligned_scale(e, p.src_height, p.dst_height);
+    const int64_t sy = sy_fixp >> 16;
+    const int64_t max_sy = p.src_height - 1;
+    const int top = static_cast<int>(min(max(sy, int64_t{0}), max_sy));
+    const int bot = static_cast<int>(min(max(sy + 1, int64_t{0}), max_sy));
+    const int yfrac = static_cast<int>((sy_fixp - sy * 65536) >> 8);
+    rows[e] = make_int2(top, static_cast<int>(static_cast<unsigned>(bot) |
+                                              (static_cast<unsigned>(yfrac) << 24)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact 1-channel 2x / 4x upsizers: separable weights (3,1)/4 resp.
+// (7,1),(5,3),(3,5),(1,7)/8 per axis with clamped taps and one rounding
+// (the edge rows' 1-D formulas of resize_linear_neon.cpp are this closed form
+// with the other axis' weight all on one tap).  Their tables use the same
+// layout as the generic path's, with the right/bottom tap weight in place of
+// the 8-bit fraction.
+template <int F>
+__device__ __forceinline__ void exact_taps(int d, int n, int &i0, int &i1,
+                                           int &w1) {
+  const int q = d - F / 2;
+  if (q < 0) {
+    i0 = i1 = 0;
+    w1 = 0;
+    return;
+  }
+  const int s = q / F, ph = q - s * F;
+  i0 = s;
+  i1 = min(s + 1, n - 1);
+  w1 = F == 2 ? (ph == 0 ? 1 : 3) : 1 + 2 * ph;
+}
+
+template <int F>
+__device__ __forceinline__ void exact_tables_elem(const ResizeArgs &a, int2 *cols,
+                                                  int2 *rows, int e) {
+  int i0, i1, w1;
+  if (e < a.dst_width) {
+    exact_taps<F>(e, a.src_width, i0, i1, w1);
+    cols[e] = make_int2(i0, ((i1 - i0) << 16) | w1);
+  }
+  if (e < a.dst_height) {
+    exact_taps<F>(e, a.src_height, i0, i1, w1);
+    rows[e] = make_int2(i0, static_cast<int>(static_cast<unsigned>(i1) |
+                                             (static_cast<unsigned>(w1) << 24)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Staged gather.  A workgroup owns kTileE destination bytes of `band` rows.
+// It reads its column taps once, finds the source byte span they cover and
+// the source rows the band needs, copies that window into LDS with dword
+// loads, then every output is four LDS byte reads and two (or one) lerps.
+constexpr int kTileE = 4 * kThreads;       // destination bytes per workgroup
+constexpr int kStageBytes = 24 * 1024;     // LDS window capacity
+
+enum { kLerpGeneric = 0, kExact2 = 2, kExact4 = 4 };
+
+// vraddhn_u16(a << 8, (b - a) * w): exact in 16 bits for 0 <= w <= 255.
+__device__ __forceinline__ int lerp8(int a, int b, int w) {
+  return (256 * a + (b - a) * w + 128) >> 8;
+}
+
+template <int MODE>
+__device__ __forceinline__ uint32_t combine(int pa, int pb, int pc, int pd,
+                                            int wx, int wy) {
+  if (MODE == kLerpGeneric) {
+    return static_cast<uint32_t>(lerp8(lerp8(pa, pc, wy), lerp8(pb, pd, wy), wx));
+  } else {
+    constexpr int kUnit = MODE == kExact2 ? 4 : 8;
+    constexpr int kShift = MODE == kExact2 ? 4 : 6;
+    const int acc = (kUnit - wy) * ((kUnit - wx) * pa + wx * pb) +
+                    wy * ((kUnit - wx) * pc + wx * pd);
+    return static_cast<uint32_t>((acc + (1 << (kShift - 1))) >> kShift);
+  }
+}
+
+typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ us2 as_us2(uint32_t v) { return __builtin_bit_cast(us2, v); }
+__device__ __forceinline__ uint32_t as_u32(us2 v) { return __builtin_bit_cast(uint32_t, v); }
+
+// combine<MODE> on two taps at once in 16-bit halves (v_pk_* arithmetic).
+// Every intermediate fits 16 bits: the lerp's 256a + (b-a)w + 128 lies in
+// [0, 65408] and wraps exactly as the reference's vraddhn_u16 operands do;
+// the exact upsizers' sums stay below 2^14.
+template <int MODE>
+__device__ __forceinline__ us2 combine2(us2 pa, us2 pb, us2 pc, us2 pd, us2 wx, us2 wy) {
+  const us2 k256 = us2{256, 256}, k128 = us2{128, 128};
+  if (MODE == kLerpGeneric) {
+    // 256a + (b - a)w + 128 == a(256 - w) + bw + 128: two multiply-adds
+    const us2 iwy = k256 - wy, iwx = k256 - wx;
+    const us2 left = (pa * iwy + pc * wy + k128) >> 8;
+    const us2 right = (pb * iwy + pd * wy + k128) >> 8;
+    return (left * iwx + right * wx + k128) >> 8;
+  } else {
+    constexpr uint16_t kUnit = MODE == kExact2 ? 4 : 8;
+    constexpr uint16_t kShift = MODE == kExact2 ? 4 : 6;
+    const us2 unit = us2{kUnit, kUnit};
+    const us2 h0 = pa * (unit - wx) + pb * wx;
+    const us2 h1 = pc * (unit - wx) + pd * wx;
+    const us2 rnd = us2{1 << (kShift - 1), 1 << (kShift - 1)};
+    return (h0 * (unit - wy) + h1 * wy + rnd) >> kShift;
+  }
+}
+
+__device__ __forceinline__ int block_reduce(int v, bool is_max, int *scratch) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const int o = __shfl_xor(v, off);
+    v = is_max ? max(v, o) : min(v, o);
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) scratch[wave] = v;
+  __syncthreads();
+  int r = scratch[0];
+#pragma unroll
+  for (int i = 1; i < kThreads / 64; ++i)
+    r = is_max ? max(r, scratch[i]) : min(r, scratch[i]);
+  __syncthreads();
+  return r;
+}
+
+// LDS of one staged tile: the window plus 8 spare bytes (the 8-byte reads of
+// the narrow path may start in the window's last dword) and the reduction
+// scratch.
+constexpr int kStagedSmem = kStageBytes + 16 + 4 * (kThreads / 64);
+
+// Destination bytes [bx*kTileE, +kTileE) of rows [by*band, +band).
+template <int MODE>
+__device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
+                                                   const int2 *__restrict__ cols,
+                                                   const int2 *__restrict__ rows,
+                                                   int band, int bx, int by,
+                                                   uint8_t *smem) {
+  uint8_t *win = smem;
+  int *scratch = reinterpret_cast<int *>(smem + kStageBytes + 16);
+  const int n = a.dst_width * a.channels;
+  const int row_bytes = a.src_width * a.channels;
+  const int e0 = bx * kTileE + threadIdx.x * 4;
+  __syncthreads();  // LDS may still hold the previous tile of a batched loop
+
+  int left[4], right[4], wx[4], zero[4];
+  int lo = 0x7FFFFFFF, hi = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = e0 + k;
+    const int2 c = cols[min(e, n - 1)];
+    left[k] = c.x;
+    right[k] = c.x + (c.y >> 16);
+    wx[k] = c.y & 0xFF;
+    zero[k] = e < n ? (c.y >> 8) & 3 : 3;
+    if (!(zero[k] & 1)) { lo = min(lo, left[k]); hi = max(hi, left[k]); }
+    if (!(zero[k] & 2)) { lo = min(lo, right[k]); hi = max(hi, right[k]); }
+  }
+  lo = block_reduce(lo, false, scratch);
+  hi = block_reduce(hi, true, scratch);
+  if (hi < lo) { lo = 0; hi = 0; }
+  const int lo4 = lo & ~3;
+  const int pitch = ((hi + 1 - lo4) + 3) & ~3;
+
+  const int y_begin = by * band;
+  const int y_end = min(y_begin + band, a.dst_height);
+  const int r_lo = rows[y_begin].x;
+  const int r_hi = rows[y_end - 1].y & 0xFFFFFF;
+  const int nrows = r_hi - r_lo + 1;
+  const bool staged = nrows * pitch <= kStageBytes;
+
+  if (staged) {
+    const bool aligned =
+      ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
+    const int quads = pitch >> 2;
+    for (int i = threadIdx.x; i < nrows * quads; i += kThreads) {
+      const int r = i / quads, q = i - r * quads;
+      const uint8_t *srow = a.src + static_cast<size_t>(r_lo + r) * a.src_stride;
+      const int off = lo4 + 4 * q;
+      uint32_t v;
+      if (aligned && off + 4 <= row_bytes) {
+        v = *reinterpret_cast<const uint32_t *>(srow + off);
+      } else {
+        v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          if (off + b < row_bytes) v |= static_cast<uint32_t>(srow[off + b]) << (8 * b);
+      }
+      *reinterpret_cast<uint32_t *>(win + r * pitch + 4 * q) = v;
+    }
+    __syncthreads();
+  }
+  if (e0 >= n) return;  // no barrier follows in this tile
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    left[k] -= lo4;
+    right[k] -= lo4;
+  }
+
+  // When a thread's taps lie within 8 bytes of a dword boundary (upsizing,
+  // mild downsizing) each source row is one 8-byte LDS read plus bit-field
+  // extracts instead of eight byte reads.
+  int tmin = 0x7FFFFFFF, tmax = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!(zero[k] & 1)) { tmin = min(tmin, left[k]); tmax = max(tmax, left[k]); }
+    if (!(zero[k] & 2)) { tmin = min(tmin, right[k]); tmax = max(tmax, right[k]); }
+  }
+  const int tbase = tmin & ~3;
+  const bool narrow = staged && tmax >= 0 && tmax - tbase < 8;
+  // Byte selectors for v_perm_b32 over the 8-byte window {hi, lo}: two taps
+  // per selector land in the low bytes of a 16-bit pair; 0x0C selects zero,
+  // which also realises the vector code's "index beyond the load reads 0".
+  auto sel = [&](const int *idx, int flag, int k0) {
+    uint32_t v = 0x0C000C00u;
+    if (!(zero[k0] & flag)) v |= static_cast<uint32_t>(idx[k0] - tbase);
+    else v |= 0x0Cu;
+    if (!(zero[k0 + 1] & flag)) v |= static_cast<uint32_t>(idx[k0 + 1] - tbase) << 16;
+    else v |= 0x0Cu << 16;
+    return v;
+  };
+  const uint32_t sel_l01 = sel(left, 1, 0), sel_l23 = sel(left, 1, 2);
+  const uint32_t sel_r01 = sel(right, 2, 0), sel_r23 = sel(right, 2, 2);
+  const us2 wx01 = us2{static_cast<uint16_t>(wx[0]), static_cast<uint16_t>(wx[1])};
+  const us2 wx23 = us2{static_cast<uint16_t>(wx[2]), static_cast<uint16_t>(wx[3])};
+
+  for (int y = y_begin; y < y_end; ++y) {
+    const int2 r = rows[y];
+    const int top = r.x, bot = r.y & 0xFFFFFF;
+    const int wy = (r.y >> 24) & 0xFF;
+    uint32_t packed = 0;
+    if (narrow) {
+      const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
+      const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
+      const uint32_t t0 = tp[0], t1 = tp[1], b0 = bp[0], b1 = bp[1];
+      const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
+      const us2 r01 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l01)),
+                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r01)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l01)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r01)), wx01, wy2);
+      const us2 r23 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l23)),
+                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r23)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l23)),
+                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r23)), wx23, wy2);
+      // bytes 0 and 2 of each pair word -> one dword
+      packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
+    } else if (staged) {
+      const uint8_t *t = win + (top - r_lo) * pitch;
+      const uint8_t *b = win + (bot - r_lo) * pitch;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
+        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
+        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
+        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      }
+    } else {
+      const uint8_t *t = a.src + static_cast<size_t>(top) * a.src_stride + lo4;
+      const uint8_t *b = a.src + static_cast<size_t>(bot) * a.src_stride + lo4;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
+        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
+        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
+        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      }
+    }
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
+    if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
+      *reinterpret_cast<uint32_t *>(out + e0) = packed;
+    } else {
+      for (int k = 0; k < 4 && e0 + k < n; ++k)
+        out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact 0.5 x 0.5: 16 output bytes per thread per row for 1, 2 and 4
+// channels (two 16-byte loads from each source row), bytewise for 3.
+__device__ __forceinline__ uint32_t quarter4(uint64_t t, uint64_t b, int C) {
+  uint32_t packed = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int i0 = 2 * k - k % C, i1 = i0 + C;
+    const uint32_t s = ((t >> (8 * i0)) & 0xFF) + ((t >> (8 * i1)) & 0xFF) +
+                       ((b >> (8 * i0)) & 0xFF) + ((b >> (8 * i1)) & 0xFF);
+    packed |= ((s + 2) >> 2) << (8 * k);
+  }
+  return packed;
+}
+
+// Bytes [bx*16*kThreads, +16*kThreads) of rows y_begin, y_begin + y_step, ...
+__device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
+                                                    int y_begin, int y_end, int y_step) {
+  const int C = a.channels;
+  const int n = a.dst_width * C;
+  const int e0 = (bx * kThreads + threadIdx.x) * 16;
+  if (e0 >= n) return;
+  const bool fast = C != 3 && e0 + 15 < n &&
+                    ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
+                      reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+  for (int y = y_begin; y < y_end; y += y_step) {
+    const uint8_t *r0 = a.src + static_cast<size_t>(2 * y) * a.src_stride;
+    const uint8_t *r1 = r0 + a.src_stride;
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
+    if (fast) {
+      const uint4 t0 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0);
+      const uint4 t1 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0 + 16);
+      const uint4 b0 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0);
+      const uint4 b1 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0 + 16);
+      auto u64 = [](uint32_t lo, uint32_t hi) {
+        return (static_cast<uint64_t>(hi) << 32) | lo;
+      };
+      uint4 o;
+      o.x = quarter4(u64(t0.x, t0.y), u64(b0.x, b0.y), C);
+      o.y = quarter4(u64(t0.z, t0.w), u64(b0.z, b0.w), C);
+      o.z = quarter4(u64(t1.x, t1.y), u64(b1.x, b1.y), C);
+      o.w = quarter4(u64(t1.z, t1.w), u64(b1.z, b1.w), C);
+      *reinterpret_cast<uint4 *>(out + e0) = o;
+    } else {
+      for (int k = 0; k < 16 && e0 + k < n; ++k) {
+        const int e = e0 + k;
+        const int i0 = 2 * e - e % C, i1 = i0 + C;
+        const uint32_t s = r0[i0] + r0[i1] + r1[i0] + r1[i1];
+        out[e] = static_cast<uint8_t>((s + 2) >> 2);
+      }
+    }
+  }
+}
+
+enum class Path { kNone, kQuarter, kUp2, kUp4, kGeneric };
+
+inline Path pick(const ResizeArgs &a) {
+  const int64_t sw = a.src_width, sh = a.src_height, dw = a.dst_width,
+                dh = a.dst_height;
+  if (sw == 0 || sh == 0) return Path::kNone;
+  if (sw == 2 * dw && sh == 2 * dh) return Path::kQuarter;
+  if (a.channels == 1 && sw * 2 == dw && sh * 2 == dh) return Path::kUp2;
+  if (a.channels == 1 && sw * 4 == dw && sh * 4 == dh) return Path::kUp4;
+  return Path::kGeneric;
+}
+
+// Rows per workgroup band so that the staged window fits in LDS for a
+// typical tile; tiles whose window does not fit read from global memory.
+inline int band_rows(const ResizeArgs &a) {
+  const double sx = static_cast<double>(a.src_width) / a.dst_width;
+  const double sy = static_cast<double>(a.src_height) / a.dst_height;
+  const double pitch = kTileE * sx + 8.0 * a.channels + 8.0;
+  const double rows_fit = kStageBytes / pitch;
+  int band = static_cast<int>((rows_fit - 2.0) / (sy > 0 ? sy : 1.0)) + 1;
+  band = band < 1 ? 1 : band > 64 ? 64 : band;
+  return band;
+}
+
+}  // namespace rsz
+}  // namespace kcv
+
+#endif  // KCV_RESIZE_BODY_H
