@@ -50,6 +50,8 @@ def parse():
                     help="only run 3 device copies of 512 MiB (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched workloads (pyramid and transform batches)")
     return ap.parse_args()
 
 
@@ -226,6 +228,81 @@ def cpu_baseline(np, s_host, kx, ky, T, ops):
                       "1 thread (%.1f s)" % (len(ops), t_total)}
 
 
+def bench_batched(K, torch, WL, dev, stream, steps, warmup):
+    """Batched workloads: one graph launch per step vs the same work as
+    per-image device-API calls, both timed with HIP events on `stream`."""
+    results = {}
+    specs = [
+        ("pyramid_256_uneven", "256 u8 images, W,H uniform in [64,2048], C in {1,3,4}, "
+         "Gaussian k in {3,5,7} (sigma 0), 4 filter borders, then a 3-level 0.5x linear "
+         "pyramid (level size ceil(s/2)); 1024 ops in 4 stages",
+         lambda: WL.pyramid_batch(torch, dev, n=256, seed=3), WL.pyramid_descs,
+         WL.pyramid_plain_calls, WL.pyramid_counts,
+         lambda r: [r["gauss"]] + r["levels"]),
+        ("transform_512_rgba_720p", "512 RGBA 1280x720 frames warped through per-frame "
+         "homographies + 512 frames remapped through per-frame f32 maps (<= 20 px), "
+         "LINEAR, REPLICATE; 1024 ops in 1 stage",
+         lambda: WL.transform_batch(torch, dev, n=512, seed=5), WL.transform_descs,
+         WL.transform_plain_calls, WL.transform_counts,
+         lambda r: [r["warp_dst"], r["remap_dst"]]),
+    ]
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    for name, what, build, descs_fn, calls_fn, counts_fn, outs_fn in specs:
+        recs = build()
+        torch.cuda.synchronize()
+        pix, nbytes = counts_fn(recs)
+        calls = calls_fn(recs, stream)
+        batch = K.Batch(descs_fn(recs), keep=recs)
+        tuned, tune_ms = batch.tune(stream, iterations=3)
+        # identical results: plain once, snapshot, batched once, compare on device
+        WL.run_calls(calls)
+        torch.cuda.synchronize()
+        snap = [t.clone() for r in recs for t in outs_fn(r)]
+        for r in recs:
+            for t in outs_fn(r):
+                t.fill_(0x5A)
+        batch.run(stream)
+        torch.cuda.synchronize()
+        identical = all(torch.equal(a, b) for a, b in
+                        zip(snap, [t for r in recs for t in outs_fn(r)]))
+        del snap
+        for _ in range(warmup):
+            batch.run(stream)
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for _ in range(steps):
+            batch.run(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        b_ms = e0.elapsed_time(e1) / steps
+        for _ in range(max(1, warmup)):
+            WL.run_calls(calls)
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for _ in range(steps):
+            WL.run_calls(calls)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        p_ms = e0.elapsed_time(e1) / steps
+        gbs = nbytes / (b_ms * 1e-3) / 1e9
+        results[name] = {
+            "workload": what, "ops": len(calls), "work_items": batch.work_items,
+            "tuning": tuned.as_dict(), "tuned_ms": round(tune_ms, 4),
+            "batched_ms": round(b_ms, 4), "plain_ms": round(p_ms, 4),
+            "batched_mpix_s": round(pix / (b_ms * 1e-3) / 1e6, 1),
+            "plain_mpix_s": round(pix / (p_ms * 1e-3) / 1e6, 1),
+            "speedup": round(p_ms / b_ms, 2),
+            "bytes": nbytes, "batched_gb_s": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+            "identical_to_plain": bool(identical),
+        }
+        batch.close()
+        del recs, calls, batch
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return results
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -348,6 +425,9 @@ def main():
         result["roofline"]["traffic_note"] = (
             "HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, each divided by its "
             "reading per byte of a plain 512 MiB copy (factors %s)" % json.dumps(traffic["_factor"]))
+    if not args.no_batched and not args.ops:
+        import workloads as WL
+        result["batched"] = bench_batched(K, torch, WL, dev, stream, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
         result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)

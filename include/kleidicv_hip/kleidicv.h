@@ -285,6 +285,114 @@ kleidicv_error_t kleidicv_hip_yuv_semiplanar_to_rgb_u8(
 void *kleidicv_hip_default_stream(void);
 kleidicv_error_t kleidicv_hip_synchronize(void *stream);
 
+/* ------------------------------------------------------------------------
+ * 4. Batched execution.  No counterpart in the reference: it replaces many
+ *    per-image calls (and the reference's per-call stripe scheduling) with
+ *    one submission.  Every op keeps exactly the arithmetic, validation and
+ *    error codes of its single-image entry point; only the scheduling
+ *    differs.
+ *
+ *    A batch is a list of op descriptors with device pointers.  Ops carry a
+ *    stage number: every op of stage s starts after every op of stages < s
+ *    has finished (a pyramid level reads the previous level), while ops of
+ *    one stage run together.  For each (stage, kernel family, kernel
+ *    variant) the batch builds a device-side work list of tiles across all
+ *    of its images and runs it with one persistent kernel, so a batch of
+ *    hundreds of images is a handful of launches.
+ * ------------------------------------------------------------------------ */
+
+typedef enum {
+  KLEIDICV_HIP_OP_GAUSSIAN_BLUR = 1,
+  KLEIDICV_HIP_OP_SEPARABLE_FILTER_2D = 2,
+  KLEIDICV_HIP_OP_MEDIAN_BLUR = 3,
+  KLEIDICV_HIP_OP_RESIZE_LINEAR = 4,
+  KLEIDICV_HIP_OP_REMAP_S16POINT5 = 5,
+  KLEIDICV_HIP_OP_REMAP_F32 = 6,
+  KLEIDICV_HIP_OP_WARP_PERSPECTIVE = 7,
+  KLEIDICV_HIP_OP_YUV_SEMIPLANAR_TO_RGB = 8,
+} kleidicv_hip_op_t;
+
+/* One op.  Fields an op does not use are ignored.
+ *   src/src_stride/src_width/src_height: source image (for NV12: the Y plane)
+ *   dst/dst_stride/dst_width/dst_height: destination image; filters and
+ *       NV12 use src_width/src_height for both
+ *   aux0/aux0_stride: mapxy (remap s16point5), mapx (remap f32), UV plane
+ *       (NV12)
+ *   aux1/aux1_stride: mapfrac (remap s16point5), mapy (remap f32)
+ *   kernel_x/kernel_y: separable taps (kernel_size of them, by value)
+ *   transformation: warp-perspective matrix (by value)
+ *   border_value: CONSTANT border value per channel (by value) */
+typedef struct {
+  kleidicv_hip_op_t op;
+  unsigned stage;
+  const uint8_t *src;
+  size_t src_stride, src_width, src_height;
+  uint8_t *dst;
+  size_t dst_stride, dst_width, dst_height;
+  size_t channels;
+  size_t kernel_size;
+  float sigma;
+  uint8_t kernel_x[7], kernel_y[7];
+  kleidicv_border_type_t border_type;
+  kleidicv_interpolation_type_t interpolation;
+  uint8_t border_value[4];
+  const void *aux0;
+  size_t aux0_stride;
+  const void *aux1;
+  size_t aux1_stride;
+  float transformation[9];
+  kleidicv_color_conversion_t color_format;
+} kleidicv_hip_op_desc;
+
+/* Scheduling parameters; results never depend on them.
+ *   blocks_per_cu: persistent workgroups per compute unit for each stage
+ *       kernel (1..16)
+ *   resize_band_rows: destination rows per resize work item (0: as many as
+ *       fit the LDS window, at most 64)
+ *   xcd_order: 1 = hand consecutive work items (neighbouring tiles) to the
+ *       workgroups of one XCD so their halos share that XCD's L2
+ *   use_graph: must be 0 (stage kernels launched directly); 1 (graph
+ *       replay) returns KLEIDICV_ERROR_NOT_IMPLEMENTED */
+typedef struct {
+  unsigned blocks_per_cu;
+  unsigned resize_band_rows;
+  unsigned xcd_order;
+  unsigned use_graph;
+} kleidicv_hip_batch_tuning;
+
+typedef struct kleidicv_hip_batch kleidicv_hip_batch;
+
+/* Validates every op (as its single-image entry point would), builds the
+ * work lists.  On a validation error *failed_index (if not
+ * NULL) names the first failing op and no batch is created.  tuning may be
+ * NULL for the defaults. */
+kleidicv_error_t kleidicv_hip_batch_create(const kleidicv_hip_op_desc *ops, size_t count,
+                                           const kleidicv_hip_batch_tuning *tuning,
+                                           kleidicv_hip_batch **batch,
+                                           size_t *failed_index);
+
+/* Enqueues the whole batch on `stream` (a hipStream_t; NULL = null stream).
+ * Asynchronous.  Runs of one batch must not overlap each other. */
+kleidicv_error_t kleidicv_hip_batch_run(kleidicv_hip_batch *batch, void *stream);
+
+/* Times each candidate tuning over `iterations` runs on `stream`, keeps the
+ * fastest in the batch and reports it (and its mean time per run in
+ * milliseconds) when the out-pointers are not NULL.  Synchronous. */
+kleidicv_error_t kleidicv_hip_batch_tune(kleidicv_hip_batch *batch, void *stream,
+                                         unsigned iterations,
+                                         kleidicv_hip_batch_tuning *best, float *best_ms);
+
+kleidicv_error_t kleidicv_hip_batch_set_tuning(kleidicv_hip_batch *batch,
+                                               const kleidicv_hip_batch_tuning *tuning);
+kleidicv_error_t kleidicv_hip_batch_get_tuning(const kleidicv_hip_batch *batch,
+                                               kleidicv_hip_batch_tuning *tuning);
+
+/* Work items (tiles) of the batch under its current tuning. */
+size_t kleidicv_hip_batch_work_items(const kleidicv_hip_batch *batch);
+
+/* Waits for the device to finish outstanding work, then frees the batch. */
+void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

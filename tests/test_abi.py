@@ -17,7 +17,8 @@ HEADER = os.path.join(ROOT, "include", "kleidicv_hip", "kleidicv.h")
 
 def declared_symbols():
     text = open(HEADER).read()
-    funcs = re.findall(r"^(?:kleidicv_error_t|void \*|void)\s*\**\s*(kleidicv_\w+)\(", text, re.M)
+    funcs = re.findall(r"^(?:kleidicv_error_t|void \*|void|size_t)\s*\**\s*(kleidicv_\w+)\(",
+                       text, re.M)
     data = re.findall(r"^extern kleidicv_error_t \(\*(kleidicv_\w+)\)", text, re.M)
     return sorted(set(funcs)), sorted(set(data))
 
@@ -160,3 +161,52 @@ def test_thread_api_requires_scheduler():
         C.c_void_p(p), C.c_size_t(16), C.c_void_p(p + 1), C.c_size_t(16), C.c_size_t(16),
         C.c_size_t(16), C.c_size_t(1), C.c_size_t(5), C.c_size_t(5), C.c_int(1), MT(None, None))
     assert r == K.NULL_POINTER
+
+
+LAYOUT_PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "kleidicv_hip/kleidicv.h"
+#define F(T, m) printf(#T " " #m " %zu\n", offsetof(T, m));
+int main(void) {
+  printf("kleidicv_hip_op_desc size %zu\n", sizeof(kleidicv_hip_op_desc));
+  printf("kleidicv_hip_batch_tuning size %zu\n", sizeof(kleidicv_hip_batch_tuning));
+  %s
+  return 0;
+}
+"""
+
+
+def test_batch_structs_match_the_c_layout(tmp_path):
+    fields = ["F(kleidicv_hip_op_desc, %s)" % n for n, _ in K.OpDesc._fields_]
+    fields += ["F(kleidicv_hip_batch_tuning, %s)" % n for n, _ in K.Tuning._fields_]
+    src = tmp_path / "probe.c"
+    src.write_text(LAYOUT_PROBE.replace("%s", "\n  ".join(fields)))
+    exe = tmp_path / "probe"
+    rc = os.system("gcc -std=c99 -I'%s' '%s' -o '%s'" % (os.path.join(ROOT, "include"), src, exe))
+    assert rc == 0
+    want = {}
+    for line in os.popen(str(exe)).read().splitlines():
+        t, m, v = line.split()
+        want[(t, m)] = int(v)
+    assert want[("kleidicv_hip_op_desc", "size")] == C.sizeof(K.OpDesc)
+    assert want[("kleidicv_hip_batch_tuning", "size")] == C.sizeof(K.Tuning)
+    for n, _ in K.OpDesc._fields_:
+        assert want[("kleidicv_hip_op_desc", n)] == getattr(K.OpDesc, n).offset, n
+    for n, _ in K.Tuning._fields_:
+        assert want[("kleidicv_hip_batch_tuning", n)] == getattr(K.Tuning, n).offset, n
+
+
+def test_batch_create_argument_errors():
+    lib = K.lib()
+    h = C.c_void_p(0)
+    assert lib.kleidicv_hip_batch_create(None, C.c_size_t(0), None, None, None) == \
+        K.NULL_POINTER
+    assert lib.kleidicv_hip_batch_create(None, C.c_size_t(3), None, C.byref(h), None) == \
+        K.NULL_POINTER
+    bad = K.Tuning(0, 0, 1, 1)  # blocks_per_cu must be 1..16
+    assert lib.kleidicv_hip_batch_create(None, C.c_size_t(0), C.byref(bad), C.byref(h),
+                                         None) == K.RANGE
+    assert lib.kleidicv_hip_batch_run(None, None) == K.NULL_POINTER
+    assert lib.kleidicv_hip_batch_tune(None, None, C.c_uint(1), None, None) == K.NULL_POINTER
+    lib.kleidicv_hip_batch_destroy(None)  # no-op

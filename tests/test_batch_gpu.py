@@ -1,0 +1,294 @@
+"""Batched execution: every op of a batch gives exactly what its single-image
+entry point gives, whatever the batch's composition, order or tuning.
+
+Each case builds device tensors, runs the work as one batch (a few
+persistent stage launches) and as plain per-image device-API launches, and
+compares byte for byte; a few images are also checked against the CPU
+oracle directly.  Inputs are produced on torch's stream and the batch runs
+on another, so every case synchronises in between.
+"""
+
+import numpy as np
+import pytest
+
+import kleidicv_hip as K
+import oracle_lib as O
+import workloads as WL
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def assert_same(got, want, what=""):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape, (what, got.shape, want.shape)
+    if not np.array_equal(got, want):
+        bad = np.argwhere(got != want)
+        i = tuple(bad[0])
+        raise AssertionError("%s: %d mismatches, first at %s: got %d want %d" %
+                             (what, len(bad), i, got[i], want[i]))
+
+
+def pyramid_outputs(recs):
+    out = []
+    for r in recs:
+        out.append(host(r["gauss"]))
+        out.extend(host(lv) for lv in r["levels"])
+    return out
+
+
+def clear_pyramid(recs):
+    for r in recs:
+        r["gauss"].fill_(0xA5)
+        for lv in r["levels"]:
+            lv.fill_(0x5A)
+
+
+def test_pyramid_batch_matches_plain_and_oracle():
+    recs = WL.pyramid_batch(torch, dev(), n=24, seed=11, lo=64, hi=300, padded=True)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    WL.pyramid_plain(recs, stream)
+    torch.cuda.synchronize()
+    want = pyramid_outputs(recs)
+    clear_pyramid(recs)
+    torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    with K.Batch(WL.pyramid_descs(recs), keep=recs) as b:
+        assert b.work_items > 0
+        b.run(stream)
+        torch.cuda.synchronize()
+    got = pyramid_outputs(recs)
+    for i, (g, w) in enumerate(zip(got, want)):
+        assert_same(g, w, "output %d" % i)
+    # the chain of the first images against the oracle
+    for r in recs[:4]:
+        src = np.ascontiguousarray(host(r["src"]))
+        _, g = O.gaussian_blur(src, r["ksize"], 0.0, r["border"])
+        assert_same(host(r["gauss"]), g, "oracle gaussian")
+        prev = g
+        for lv in r["levels"]:
+            err, want_lv = O.resize_linear(prev, lv.shape[1], lv.shape[0])
+            assert err == 0
+            assert_same(host(lv), want_lv, "oracle level")
+            prev = want_lv
+
+
+def test_pyramid_batch_order_and_tuning_invariant():
+    recs = WL.pyramid_batch(torch, dev(), n=12, seed=12, lo=64, hi=200)
+    descs = WL.pyramid_descs(recs)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with K.Batch(descs, keep=recs) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+        want = pyramid_outputs(recs)
+        tunings = [K.Tuning(1, 0, 0, 0), K.Tuning(2, 4, 1, 0), K.Tuning(8, 1, 1, 0),
+                   K.Tuning(3, 37, 0, 0)]
+        for t in tunings:
+            clear_pyramid(recs)
+            b.tuning = t
+            b.run(stream)
+            torch.cuda.synchronize()
+            for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+                assert_same(g, w, "tuning %s output %d" % (t.as_dict(), i))
+        best, ms = b.tune(stream, iterations=2)
+        assert ms > 0
+        clear_pyramid(recs)
+        b.run(stream)
+        torch.cuda.synchronize()
+        for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+            assert_same(g, w, "tuned output %d" % i)
+    # permuted descriptor order (stages keep the dependency order)
+    perm = np.random.default_rng(5).permutation(len(descs))
+    clear_pyramid(recs)
+    with K.Batch([descs[i] for i in perm], keep=recs) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+    for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+        assert_same(g, w, "permuted output %d" % i)
+
+
+@pytest.mark.skip(reason="graph replay of a batch is switched off: its first run here "
+                         "ended in a host segmentation fault inside hipGraphLaunch and the "
+                         "cause is not yet known; kleidicv_hip_batch_create rejects "
+                         "use_graph=1")
+def test_graph_replays_agree_with_eager_work_between():
+    recs = WL.pyramid_batch(torch, dev(), n=6, seed=13, lo=64, hi=160)
+    stream = torch.cuda.Stream()
+    with K.Batch(WL.pyramid_descs(recs), keep=recs) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+        want = pyramid_outputs(recs)
+        for rep in range(3):
+            clear_pyramid(recs)
+            scratch = torch.ones(1 << 20, device=dev()).sum()  # eager work between replays
+            b.run(stream)
+            torch.cuda.synchronize()
+            assert float(scratch) == float(1 << 20)
+            for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+                assert_same(g, w, "replay %d output %d" % (rep, i))
+
+
+def test_batch_reruns_agree_with_eager_work_between():
+    recs = WL.pyramid_batch(torch, dev(), n=8, seed=15, lo=64, hi=180)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with K.Batch(WL.pyramid_descs(recs), keep=recs) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+        want = pyramid_outputs(recs)
+        for rep in range(3):
+            clear_pyramid(recs)
+            scratch = torch.ones(1 << 20, device=dev()).sum()  # eager work between runs
+            torch.cuda.synchronize()
+            b.run(stream)
+            torch.cuda.synchronize()
+            assert float(scratch) == float(1 << 20)
+            for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+                assert_same(g, w, "rerun %d output %d" % (rep, i))
+
+
+def test_graph_replay_is_rejected():
+    src = torch.zeros((32, 32, 1), dtype=torch.uint8, device=dev())
+    dst = torch.zeros((32, 32, 1), dtype=torch.uint8, device=dev())
+    with pytest.raises(K.KleidiCVError) as e:
+        K.Batch([K.op_gaussian_blur(src, dst, 5)], tuning=K.Tuning(4, 0, 1, 1))
+    assert e.value.code == K.NOT_IMPLEMENTED
+
+
+def test_transform_batch_matches_plain():
+    recs = WL.transform_batch(torch, dev(), n=6, seed=14, w=320, h=180)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    WL.transform_plain(recs, stream)
+    torch.cuda.synchronize()
+    want = [(host(r["warp_dst"]), host(r["remap_dst"])) for r in recs]
+    for r in recs:
+        r["warp_dst"].zero_()
+        r["remap_dst"].zero_()
+    with K.Batch(WL.transform_descs(recs), keep=recs) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+    for i, r in enumerate(recs):
+        assert_same(host(r["warp_dst"]), want[i][0], "warp %d" % i)
+        assert_same(host(r["remap_dst"]), want[i][1], "remap %d" % i)
+    # one frame against the oracle
+    r = recs[0]
+    _, want_w = O.warp_perspective(host(r["warp_src"]), r["T"], 320, 180, "LINEAR", "REPLICATE")
+    assert_same(host(r["warp_dst"]), want_w, "oracle warp")
+    _, want_r = O.remap_f32(host(r["remap_src"]), host(r["mapx"]), host(r["mapy"]), "LINEAR",
+                            "REPLICATE")
+    assert_same(host(r["remap_dst"]), want_r, "oracle remap")
+
+
+def test_every_op_in_one_batch_matches_plain():
+    g = torch.Generator(device=dev())
+    g.manual_seed(21)
+    u8 = lambda *s: torch.randint(0, 256, s, dtype=torch.uint8, device=dev(), generator=g)
+    stream = torch.cuda.Stream()
+    src4 = u8(97, 131, 4)
+    src1 = u8(97, 131, 1)
+    src3 = u8(64, 90, 3)
+    mapxy = torch.randint(-20, 150, (40, 60, 2), dtype=torch.int16, device=dev(), generator=g)
+    mapfrac = torch.randint(0, 1024, (40, 60), dtype=torch.int16, device=dev(), generator=g)
+    mx = (torch.rand(40, 60, device=dev(), generator=g) * 160 - 15).contiguous()
+    my = (torch.rand(40, 60, device=dev(), generator=g) * 120 - 15).contiguous()
+    ny, nuv = u8(50, 66), u8(25, 66)
+    T = WL.homography(np.random.default_rng(3), 131, 97)
+    torch.cuda.synchronize()
+    cases = [
+        ("gauss5", lambda d: K.op_gaussian_blur(src4, d, 5, 0.0, "REFLECT"),
+         lambda d: K.gaussian_blur(src4, 5, 0.0, "REFLECT", dst=d, stream=stream, device_api=True),
+         (97, 131, 4)),
+        ("gauss_sigma", lambda d: K.op_gaussian_blur(src3, d, 7, 1.7, "WRAP"),
+         lambda d: K.gaussian_blur(src3, 7, 1.7, "WRAP", dst=d, stream=stream, device_api=True),
+         (64, 90, 3)),
+        ("gauss_copy", lambda d: K.op_gaussian_blur(src1, d, 3, 0.01, "REPLICATE"),
+         lambda d: K.gaussian_blur(src1, 3, 0.01, "REPLICATE", dst=d, stream=stream,
+                                   device_api=True),
+         (97, 131, 1)),
+        ("sep7", lambda d: K.op_separable_filter_2d(src1, d, [0, 1, 2, 1, 0, 1, 0],
+                                                    [1, 0, 1, 2, 0, 0, 1], "REVERSE"),
+         lambda d: K.separable_filter_2d(src1, np.array([0, 1, 2, 1, 0, 1, 0], np.uint8),
+                                         np.array([1, 0, 1, 2, 0, 0, 1], np.uint8), "REVERSE",
+                                         dst=d, stream=stream, device_api=True),
+         (97, 131, 1)),
+        ("median7", lambda d: K.op_median_blur(src3, d, 7, "REFLECT"),
+         lambda d: K.median_blur(src3, 7, "REFLECT", dst=d, stream=stream, device_api=True),
+         (64, 90, 3)),
+        ("resize_up", lambda d: K.op_resize_linear(src3, d),
+         lambda d: K.resize_linear(src3, 150, 101, dst=d, stream=stream, device_api=True),
+         (101, 150, 3)),
+        ("resize_up4_1ch", lambda d: K.op_resize_linear(src1, d),
+         lambda d: K.resize_linear(src1, 524, 388, dst=d, stream=stream, device_api=True),
+         (388, 524, 1)),
+        ("resize_quarter", lambda d: K.op_resize_linear(src4[:96, :130], d),
+         lambda d: K.resize_linear(src4[:96, :130], 65, 48, dst=d, stream=stream,
+                                   device_api=True),
+         (48, 65, 4)),
+        ("remap_s16", lambda d: K.op_remap_s16point5(src1, d, mapxy, mapfrac, "CONSTANT",
+                                                     (7, 0, 0, 0)),
+         lambda d: K.remap_s16point5(src1, mapxy, mapfrac, "CONSTANT", (7, 0, 0, 0), dst=d,
+                                     stream=stream, device_api=True),
+         (40, 60, 1)),
+        ("remap_f32", lambda d: K.op_remap_f32(src4, d, mx, my, "NEAREST", "CONSTANT",
+                                               (1, 2, 3, 4)),
+         lambda d: K.remap_f32(src4, mx, my, "NEAREST", "CONSTANT", (1, 2, 3, 4), dst=d,
+                               stream=stream, device_api=True),
+         (40, 60, 4)),
+        ("warp", lambda d: K.op_warp_perspective(src3, d, T, "LINEAR", "CONSTANT", (9, 8, 7)),
+         lambda d: K.warp_perspective(src3, T, 77, 55, "LINEAR", "CONSTANT", (9, 8, 7), dst=d,
+                                      stream=stream, device_api=True),
+         (55, 77, 3)),
+        ("nv21_bgra", lambda d: K.op_yuv_sp_to_rgb(ny, nuv, d, K.NV21_TO_BGRA),
+         lambda d: K.yuv_sp_to_rgb(ny, nuv, K.NV21_TO_BGRA, dst=d, stream=stream,
+                                   device_api=True),
+         (50, 66, 4)),
+    ]
+    plain_out, batch_out, descs = [], [], []
+    for name, mk, run, shape in cases:
+        p = torch.empty(shape, dtype=torch.uint8, device=dev())
+        run(p)
+        q = torch.full(shape, 0x33, dtype=torch.uint8, device=dev())
+        descs.append(mk(q))
+        plain_out.append(p)
+        batch_out.append(q)
+    torch.cuda.synchronize()
+    with K.Batch(descs, keep=batch_out) as b:
+        b.run(stream)
+        torch.cuda.synchronize()
+    for (name, _, _, _), p, q in zip(cases, plain_out, batch_out):
+        assert_same(host(q), host(p), name)
+
+
+def test_batch_validation_names_failing_op():
+    src = torch.zeros((32, 32, 1), dtype=torch.uint8, device=dev())
+    dst = torch.zeros((32, 32, 1), dtype=torch.uint8, device=dev())
+    torch.cuda.synchronize()
+    ok = K.op_gaussian_blur(src, dst, 5)
+    bad = K.op_median_blur(src, src, 5)  # in place: NOT_IMPLEMENTED as in the reference
+    with pytest.raises(K.KleidiCVError) as e:
+        K.Batch([ok, ok, bad, ok])
+    assert e.value.code == K.NOT_IMPLEMENTED and e.value.failed_index == 2
+    hsrc = np.zeros((32, 32, 1), np.uint8)
+    host_op = K.op_gaussian_blur(hsrc, dst, 5)  # host memory is not accepted
+    with pytest.raises(K.KleidiCVError) as e:
+        K.Batch([ok, host_op])
+    assert e.value.failed_index == 1
+    too_small = K.op_gaussian_blur(src[:3, :3], dst[:3, :3], 7)
+    with pytest.raises(K.KleidiCVError) as e:
+        K.Batch([too_small])
+    assert e.value.code == K.NOT_IMPLEMENTED
+    with K.Batch([]) as b:  # empty batch is valid and does nothing
+        b.run()
+        assert b.work_items == 0

@@ -1,0 +1,237 @@
+// Persistent stage kernels of the batch engine.  Each kernel walks the work
+// items of one segment (tiles of many images, each with its own sizes,
+// channels and border) and runs the same tile bodies as the per-image
+// kernels.  There is one kernel per (family, variant) so that every
+// instantiation keeps the register footprint of its per-image counterpart.
+//
+// Workgroups loop until the segment's items are exhausted; every workgroup
+// reaches the end of the list, and the only barriers are inside the tile
+// bodies, which every thread of the workgroup enters.
+
+#include "batch.h"
+#include "filters_body.h"
+#include "resize_body.h"
+#include "transform_body.h"
+#include "yuv_body.h"
+
+namespace kcv {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+
+// Item handled by this workgroup at loop step t, or >= nitems when done.
+__device__ __forceinline__ uint32_t item_at(const Segment &s, uint32_t t) {
+  if (!s.xcd_order) return blockIdx.x + t * gridDim.x;
+  // Grid is a multiple of 8: XCD x owns items [x*chunk, (x+1)*chunk) and its
+  // gridDim.x/8 workgroups take consecutive items of that slice together.
+  const uint32_t per_xcd = gridDim.x >> 3;
+  const uint32_t x = blockIdx.x & 7, lb = blockIdx.x >> 3;
+  const uint32_t chunk = (s.nitems + 7) >> 3;
+  const uint32_t o = t * per_xcd + lb;
+  return o < chunk ? x * chunk + o : kNoItem;
+}
+
+// Last job whose first item is <= item (first[0] == 0).
+__device__ __forceinline__ int find_job(const Segment &s, uint32_t item) {
+  int lo = 0, hi = s.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (s.first[mid] <= item) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Calls body(job, bx, by) for each item of the segment owned by this
+// workgroup.  Jobs are read through the constant address space: they are
+// never written while a segment runs, so their fields load as scalars.
+template <class Body>
+__device__ __forceinline__ void for_each_item(const Segment &s, const Body &body) {
+  typedef const __attribute__((address_space(4))) Job ConstJob;
+  for (uint32_t t = 0;; ++t) {
+    const uint32_t item = item_at(s, t);
+    if (item >= s.nitems) break;
+    const int j = find_job(s, item);
+    const Job &job = *(const Job *)((ConstJob *)s.jobs + j);
+    const int local = static_cast<int>(item - s.first[j]);
+    const int by = local / job.tiles_x;
+    body(job, local - by * job.tiles_x, by);
+  }
+}
+
+template <int MODE, int K>
+__global__ __launch_bounds__(kThreads) void batch_sep_kernel(Segment s) {
+  __shared__ __align__(16) uint8_t smem[filt::separable_smem_bytes<K>()];
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    filt::separable_tile<MODE, K>(job.u.sep, bx, by, smem);
+  });
+}
+
+template <int K>
+__global__ __launch_bounds__(kThreads) void batch_median_kernel(Segment s) {
+  __shared__ __align__(16) uint8_t smem[filt::median_smem_bytes<K>()];
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    filt::median_tile<K>(job.u.median, bx, by, smem);
+  });
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void batch_tables_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int) {
+    const TablesJob &tj = job.u.tables;
+    const int64_t e = static_cast<int64_t>(bx) * kTablesPerItem + threadIdx.x;
+    if (KIND == kRsGeneric)
+      rsz::resize_tables_elem(tj.plan, tj.cols, tj.rows, e);
+    else
+      rsz::exact_tables_elem<KIND == kRsExact2 ? 2 : 4>(tj.a, tj.cols, tj.rows,
+                                                        static_cast<int>(e));
+  });
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void batch_resize_kernel(Segment s) {
+  __shared__ __align__(16) uint8_t smem[KIND == kRsQuarter ? 16 : rsz::kStagedSmem];
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    const ResizeJob &rj = job.u.resize;
+    if (KIND == kRsQuarter) {
+      const int y0 = by * job.band;
+      rsz::resize_quarter_rows(rj.a, bx, y0, min(y0 + job.band, rj.a.dst_height), 1);
+    } else {
+      constexpr int kMode = KIND == kRsGeneric  ? rsz::kLerpGeneric
+                            : KIND == kRsExact2 ? rsz::kExact2
+                                                : rsz::kExact4;
+      rsz::resize_staged_tile<kMode>(rj.a, rj.cols, rj.rows, job.band, bx, by, smem);
+    }
+  });
+}
+
+template <int C, int BORDER>
+__global__ __launch_bounds__(kThreads) void batch_remap_s16_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    xform::remap_s16point5_tile<C, BORDER>(job.u.remap_s16, bx, by);
+  });
+}
+
+template <int C, int INTERP, int BORDER>
+__global__ __launch_bounds__(kThreads) void batch_remap_f32_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    xform::remap_f32_tile<C, INTERP, BORDER>(job.u.remap_f32, bx, by);
+  });
+}
+
+template <int C, int INTERP, int BORDER>
+__global__ __launch_bounds__(kThreads) void batch_warp_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    xform::warp_tile<C, INTERP, BORDER>(job.u.warp, bx, by);
+  });
+}
+
+template <int OC>
+__global__ __launch_bounds__(kThreads) void batch_yuv_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    const int p0 = by * job.band;
+    yuv::nv12_rows<OC>(job.u.yuv, bx, p0, p0 + job.band, 1);
+  });
+}
+
+// Bytes [bx*kCopyTileE, +kCopyTileE) of rows [by*band, +band).
+__global__ __launch_bounds__(kThreads) void batch_copy_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    const CopyArgs &a = job.u.copy;
+    const int64_t e0 = static_cast<int64_t>(bx) * kCopyTileE + threadIdx.x * 16;
+    const int y0 = by * job.band, y1 = min(y0 + job.band, a.height);
+    if (e0 >= a.row_bytes) return;
+    const bool vec = e0 + 16 <= a.row_bytes &&
+                     ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
+                       reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+    for (int y = y0; y < y1; ++y) {
+      const uint8_t *sp = a.src + static_cast<size_t>(y) * a.src_stride + e0;
+      uint8_t *dp = a.dst + static_cast<size_t>(y) * a.dst_stride + e0;
+      if (vec) {
+        *reinterpret_cast<uint4 *>(dp) = *reinterpret_cast<const uint4 *>(sp);
+      } else {
+        for (int k = 0; k < 16 && e0 + k < a.row_bytes; ++k) dp[k] = sp[k];
+      }
+    }
+  });
+}
+
+#define KCV_K(...) reinterpret_cast<const void *>(&__VA_ARGS__)
+
+const void *gather_kernel(bool warp, int variant) {
+  const int c = variant >> 3, i = (variant >> 2) & 1, b = variant & 3;
+#define KCV_G(C, I, B)                                                   \
+  if (c == C && i == I && b == B)                                       \
+    return warp ? KCV_K(batch_warp_kernel<C, I, B>)                     \
+                : KCV_K(batch_remap_f32_kernel<C, I, B>);
+  KCV_G(1, 0, 0) KCV_G(1, 0, 1) KCV_G(1, 1, 0) KCV_G(1, 1, 1)
+  KCV_G(2, 0, 0) KCV_G(2, 0, 1) KCV_G(2, 1, 0) KCV_G(2, 1, 1)
+  KCV_G(3, 0, 0) KCV_G(3, 0, 1) KCV_G(3, 1, 0) KCV_G(3, 1, 1)
+  KCV_G(4, 0, 0) KCV_G(4, 0, 1) KCV_G(4, 1, 0) KCV_G(4, 1, 1)
+#undef KCV_G
+  return nullptr;
+}
+
+}  // namespace
+
+const void *segment_kernel(Family family, int variant) {
+  switch (family) {
+    case kFamTables:
+      return variant == kRsGeneric  ? KCV_K(batch_tables_kernel<kRsGeneric>)
+             : variant == kRsExact2 ? KCV_K(batch_tables_kernel<kRsExact2>)
+             : variant == kRsExact4 ? KCV_K(batch_tables_kernel<kRsExact4>)
+                                    : nullptr;
+    case kFamCopy:
+      return KCV_K(batch_copy_kernel);
+    case kFamSep:
+      switch (variant) {
+#define KCV_S(MODE, K) \
+  case MODE * 32 + K: return KCV_K(batch_sep_kernel<MODE, K>);
+        KCV_S(kSepBinomial, 3) KCV_S(kSepBinomial, 5) KCV_S(kSepBinomial, 7)
+        KCV_S(kSepBinomial, 9) KCV_S(kSepSigma, 3) KCV_S(kSepSigma, 5)
+        KCV_S(kSepSigma, 7) KCV_S(kSepSigma, 9) KCV_S(kSepSigma, 15)
+        KCV_S(kSepSigma, 21) KCV_S(kSepSaturate, 3) KCV_S(kSepSaturate, 5)
+        KCV_S(kSepSaturate, 7)
+#undef KCV_S
+        default: return nullptr;
+      }
+    case kFamMedian:
+      return variant == 3   ? KCV_K(batch_median_kernel<3>)
+             : variant == 5 ? KCV_K(batch_median_kernel<5>)
+             : variant == 7 ? KCV_K(batch_median_kernel<7>)
+                            : nullptr;
+    case kFamResize:
+      return variant == kRsGeneric   ? KCV_K(batch_resize_kernel<kRsGeneric>)
+             : variant == kRsExact2  ? KCV_K(batch_resize_kernel<kRsExact2>)
+             : variant == kRsExact4  ? KCV_K(batch_resize_kernel<kRsExact4>)
+             : variant == kRsQuarter ? KCV_K(batch_resize_kernel<kRsQuarter>)
+                                     : nullptr;
+    case kFamRemapS16:
+      return variant == 1 * 8 + kReplicate   ? KCV_K(batch_remap_s16_kernel<1, kReplicate>)
+             : variant == 1 * 8 + kConstant  ? KCV_K(batch_remap_s16_kernel<1, kConstant>)
+             : variant == 4 * 8 + kReplicate ? KCV_K(batch_remap_s16_kernel<4, kReplicate>)
+             : variant == 4 * 8 + kConstant  ? KCV_K(batch_remap_s16_kernel<4, kConstant>)
+                                             : nullptr;
+    case kFamRemapF32:
+      return gather_kernel(false, variant);
+    case kFamWarp:
+      return gather_kernel(true, variant);
+    case kFamYuv:
+      return variant ? KCV_K(batch_yuv_kernel<4>) : KCV_K(batch_yuv_kernel<3>);
+    default:
+      return nullptr;
+  }
+}
+
+hipError_t launch_segment(Family family, int variant, const Segment &seg, int grid,
+                          hipStream_t stream) {
+  if (seg.nitems == 0) return hipSuccess;
+  const void *fn = segment_kernel(family, variant);
+  if (fn == nullptr) return hipErrorInvalidValue;
+  Segment arg = seg;
+  void *args[] = {&arg};
+  return hipLaunchKernel(fn, dim3(static_cast<unsigned>(grid)), dim3(kThreads), args, 0,
+                         stream);
+}
+
+}  // namespace kcv

@@ -1,0 +1,528 @@
+// Batch engine host side: validates op descriptors with the single-image
+// preparation code, builds per-(stage, family, variant) work lists in device
+// memory and runs them as one persistent kernel launch per segment, in stage
+// order on the caller's stream; it also tunes the scheduling parameters.
+//
+// Graph replay (use_graph = 1) is switched off: replaying the batch as an
+// explicit multi-branch hipGraph ended once in a host segmentation fault
+// inside hipGraphLaunch, with no defect found in this code; until the cause
+// is known the option is rejected with KLEIDICV_ERROR_NOT_IMPLEMENTED.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "batch.h"
+#include "kleidicv_hip/kleidicv.h"
+#include "prepare.h"
+
+namespace {
+
+using kcv::Action;
+using kcv::Family;
+using kcv::from_hip;
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+int64_t div_up(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// One validated op, ready to be turned into jobs.
+struct Prepared {
+  Family family;
+  unsigned stage;
+  kcv::Job job;        // variant and args filled; geometry set at build time
+  int resize_kind;     // for kFamResize
+  size_t table_bytes;  // resize coordinate tables (0 when none)
+};
+
+struct SegmentPlan {
+  Family family;
+  int variant;
+  unsigned stage;  // tables: ignored (run before everything that needs them)
+  kcv::Segment seg;
+  int grid;
+};
+
+kleidicv_hip_batch_tuning default_tuning() {
+  kleidicv_hip_batch_tuning t;
+  t.blocks_per_cu = 4;
+  t.resize_band_rows = 0;
+  t.xcd_order = 1;
+  t.use_graph = 0;
+  return t;
+}
+
+bool valid_tuning(const kleidicv_hip_batch_tuning &t) {
+  return t.blocks_per_cu >= 1 && t.blocks_per_cu <= 16 && t.resize_band_rows <= 256 &&
+         t.xcd_order <= 1 && t.use_graph <= 1;
+}
+
+// Graph replay is switched off (see the top of this file).
+kleidicv_error_t check_supported(const kleidicv_hip_batch_tuning &t) {
+  return t.use_graph ? KLEIDICV_ERROR_NOT_IMPLEMENTED : KLEIDICV_OK;
+}
+
+int gather_variant(int channels, int interpolation, int border) {
+  return channels * 8 + interpolation * 4 + border;
+}
+
+// Validation and argument preparation for one descriptor.
+kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *skip) {
+  *skip = false;
+  std::memset(p, 0, sizeof(*p));
+  p->stage = d.stage;
+  Action act = Action::kNone;
+  kcv::Job &j = p->job;
+  switch (d.op) {
+    case KLEIDICV_HIP_OP_GAUSSIAN_BLUR: {
+      KCV_TRY(kcv::prepare_gaussian(d.src, d.src_stride, d.dst, d.dst_stride, d.src_width,
+                                    d.src_height, d.channels, d.kernel_size, d.kernel_size,
+                                    d.sigma, d.sigma, d.border_type, &j.u.sep, &act));
+      if (act == Action::kCopy) {
+        p->family = kcv::kFamCopy;
+        kcv::CopyArgs c{};
+        c.src = d.src;
+        c.src_stride = d.src_stride;
+        c.dst = d.dst;
+        c.dst_stride = d.dst_stride;
+        c.row_bytes = static_cast<int64_t>(d.src_width * d.channels);
+        c.height = static_cast<int>(d.src_height);
+        j.u.copy = c;
+      } else {
+        p->family = kcv::kFamSep;
+        j.variant = j.u.sep.mode * 32 + j.u.sep.ksize;
+      }
+      break;
+    }
+    case KLEIDICV_HIP_OP_SEPARABLE_FILTER_2D:
+      KCV_TRY(kcv::prepare_separable(d.src, d.src_stride, d.dst, d.dst_stride, d.src_width,
+                                     d.src_height, d.channels, d.kernel_x, d.kernel_size,
+                                     d.kernel_y, d.kernel_size, d.border_type, &j.u.sep,
+                                     &act));
+      p->family = kcv::kFamSep;
+      j.variant = j.u.sep.mode * 32 + j.u.sep.ksize;
+      break;
+    case KLEIDICV_HIP_OP_MEDIAN_BLUR:
+      KCV_TRY(kcv::prepare_median(d.src, d.src_stride, d.dst, d.dst_stride, d.src_width,
+                                  d.src_height, d.channels, d.kernel_size, d.kernel_size,
+                                  d.border_type, &j.u.median, &act));
+      p->family = kcv::kFamMedian;
+      j.variant = j.u.median.ksize;
+      break;
+    case KLEIDICV_HIP_OP_RESIZE_LINEAR: {
+      kcv::ResizeArgs a;
+      KCV_TRY(kcv::prepare_resize(d.src, d.src_stride, d.src_width, d.src_height, d.dst,
+                                  d.dst_stride, d.dst_width, d.dst_height, d.channels, &a,
+                                  &act));
+      if (act != Action::kLaunch) break;
+      if (a.dst_width == 0 || a.dst_height == 0) {
+        act = Action::kNone;
+        break;
+      }
+      p->family = kcv::kFamResize;
+      switch (kcv::rsz::pick(a)) {
+        case kcv::rsz::Path::kQuarter: p->resize_kind = kcv::kRsQuarter; break;
+        case kcv::rsz::Path::kUp2: p->resize_kind = kcv::kRsExact2; break;
+        case kcv::rsz::Path::kUp4: p->resize_kind = kcv::kRsExact4; break;
+        default: p->resize_kind = kcv::kRsGeneric; break;
+      }
+      j.u.resize.a = a;
+      j.u.resize.kind = p->resize_kind;
+      j.variant = p->resize_kind;
+      p->table_bytes = kcv::resize_workspace_bytes(a);
+      break;
+    }
+    case KLEIDICV_HIP_OP_REMAP_S16POINT5:
+      KCV_TRY(kcv::prepare_remap_s16point5(
+          d.src, d.src_stride, d.src_width, d.src_height, d.dst, d.dst_stride, d.dst_width,
+          d.dst_height, d.channels, static_cast<const int16_t *>(d.aux0), d.aux0_stride,
+          static_cast<const uint16_t *>(d.aux1), d.aux1_stride, d.border_type, d.border_value,
+          &j.u.remap_s16, &act));
+      p->family = kcv::kFamRemapS16;
+      j.variant = j.u.remap_s16.channels * 8 + j.u.remap_s16.border;
+      break;
+    case KLEIDICV_HIP_OP_REMAP_F32:
+      KCV_TRY(kcv::prepare_remap_f32(d.src, d.src_stride, d.src_width, d.src_height, d.dst,
+                                     d.dst_stride, d.dst_width, d.dst_height, d.channels,
+                                     static_cast<const float *>(d.aux0), d.aux0_stride,
+                                     static_cast<const float *>(d.aux1), d.aux1_stride,
+                                     d.interpolation, d.border_type, d.border_value,
+                                     &j.u.remap_f32, &act));
+      p->family = kcv::kFamRemapF32;
+      j.variant = gather_variant(j.u.remap_f32.channels, j.u.remap_f32.interpolation,
+                                 j.u.remap_f32.border);
+      break;
+    case KLEIDICV_HIP_OP_WARP_PERSPECTIVE:
+      KCV_TRY(kcv::prepare_warp(d.src, d.src_stride, d.src_width, d.src_height, d.dst,
+                                d.dst_stride, d.dst_width, d.dst_height, d.transformation,
+                                d.channels, d.interpolation, d.border_type, d.border_value,
+                                &j.u.warp, &act));
+      p->family = kcv::kFamWarp;
+      j.variant = gather_variant(j.u.warp.channels, j.u.warp.interpolation, j.u.warp.border);
+      break;
+    case KLEIDICV_HIP_OP_YUV_SEMIPLANAR_TO_RGB:
+      KCV_TRY(kcv::prepare_yuv_sp(d.src, d.src_stride, static_cast<const uint8_t *>(d.aux0),
+                                  d.aux0_stride, d.dst, d.dst_stride, d.src_width,
+                                  d.src_height, d.color_format, &j.u.yuv, &act));
+      p->family = kcv::kFamYuv;
+      j.variant = j.u.yuv.alpha;
+      break;
+    default:
+      return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  }
+  if (act == Action::kNone) {
+    *skip = true;
+    return KLEIDICV_OK;
+  }
+  // The batch runs asynchronously on device memory only.
+  const void *ptrs[3] = {d.src, d.dst, d.aux0};
+  const int nptr = (d.op == KLEIDICV_HIP_OP_YUV_SEMIPLANAR_TO_RGB ||
+                    d.op == KLEIDICV_HIP_OP_REMAP_S16POINT5 ||
+                    d.op == KLEIDICV_HIP_OP_REMAP_F32)
+                       ? 3
+                       : 2;
+  for (int i = 0; i < nptr; ++i)
+    if (!kcv::is_device_pointer(ptrs[i])) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if ((d.op == KLEIDICV_HIP_OP_REMAP_S16POINT5 || d.op == KLEIDICV_HIP_OP_REMAP_F32) &&
+      !kcv::is_device_pointer(d.aux1))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  return KLEIDICV_OK;
+}
+
+// Work items of a job under `t`; sets the job's tiles_x and band.
+int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Job *j) {
+  using namespace kcv;
+  switch (p.family) {
+    case kFamSep: {
+      const SepArgs &a = j->u.sep;
+      j->tiles_x = static_cast<int>(div_up(int64_t{a.width} * a.channels, kFilterTileW));
+      return j->tiles_x * div_up(a.height, kFilterTileH);
+    }
+    case kFamMedian: {
+      const MedianArgs &a = j->u.median;
+      j->tiles_x = static_cast<int>(div_up(int64_t{a.width} * a.channels, kFilterTileW));
+      return j->tiles_x * div_up(a.height, kFilterTileH);
+    }
+    case kFamCopy: {
+      const CopyArgs &a = j->u.copy;
+      j->band = kFilterTileH;
+      j->tiles_x = static_cast<int>(div_up(a.row_bytes, kCopyTileE));
+      return j->tiles_x * div_up(a.height, j->band);
+    }
+    case kFamResize: {
+      const ResizeArgs &a = j->u.resize.a;
+      const int64_t n = int64_t{a.dst_width} * a.channels;
+      if (p.resize_kind == kRsQuarter) {
+        j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows) : 2;
+        j->tiles_x = static_cast<int>(div_up(n, kQuarterTileE));
+      } else {
+        j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows)
+                                     : rsz::band_rows(a);
+        j->tiles_x = static_cast<int>(div_up(n, kResizeTileE));
+      }
+      return j->tiles_x * div_up(a.dst_height, j->band);
+    }
+    case kFamRemapS16: {
+      const RemapS16Args &a = j->u.remap_s16;
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, kXformTile));
+      return j->tiles_x * div_up(a.dst_height, kXformTile);
+    }
+    case kFamRemapF32: {
+      const RemapF32Args &a = j->u.remap_f32;
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, kXformTile));
+      return j->tiles_x * div_up(a.dst_height, kXformTile);
+    }
+    case kFamWarp: {
+      const WarpArgs &a = j->u.warp;
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, kXformTile));
+      return j->tiles_x * div_up(a.dst_height, kXformTile);
+    }
+    case kFamYuv: {
+      const YuvArgs &a = j->u.yuv;
+      j->band = 4;  // row pairs per item
+      j->tiles_x = static_cast<int>(div_up(a.width, kYuvTileW));
+      return j->tiles_x * div_up(div_up(a.height, 2), j->band);
+    }
+    default:
+      return 0;
+  }
+}
+
+}  // namespace
+
+struct kleidicv_hip_batch {
+  std::vector<Prepared> prep;
+  kleidicv_hip_batch_tuning tuning{};
+  void *dev = nullptr;
+  std::vector<SegmentPlan> segs;
+  size_t items = 0;
+
+  void release() {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    segs.clear();
+    items = 0;
+  }
+
+  kleidicv_error_t build(const kleidicv_hip_batch_tuning &t);
+  kleidicv_error_t enqueue(hipStream_t s) const;
+};
+
+kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
+  release();
+  tuning = t;
+  const int cus = kcv::runtime().compute_units > 0 ? kcv::runtime().compute_units : 256;
+
+  // Jobs in (stage, family) order; resize ops add a tables job in front.
+  struct Entry {
+    Family family;
+    unsigned stage;
+    kcv::Job job;
+    int64_t items;
+    size_t prep_index;
+  };
+  std::vector<Entry> entries;
+  entries.reserve(prep.size() * 2);
+  size_t table_total = 0;
+  std::vector<size_t> table_offset(prep.size(), 0);
+  for (size_t i = 0; i < prep.size(); ++i) {
+    const Prepared &p = prep[i];
+    Entry e{p.family, p.stage, p.job, 0, i};
+    e.items = job_items(p, t, &e.job);
+    entries.push_back(e);
+    if (p.table_bytes) {
+      table_offset[i] = table_total;
+      table_total += align_up(p.table_bytes);
+      Entry te{kcv::kFamTables, 0, kcv::Job{}, 0, i};
+      const kcv::ResizeArgs &a = p.job.u.resize.a;
+      te.job.u.tables.a = a;
+      te.job.u.tables.kind = p.resize_kind;
+      te.job.variant = p.resize_kind;
+      if (p.resize_kind == kcv::kRsGeneric) te.job.u.tables.plan = kcv::rsz::make_plan(a);
+      const int64_t n = int64_t{a.dst_width} * a.channels;
+      te.job.tiles_x = static_cast<int>(
+          div_up(std::max<int64_t>(n, a.dst_height), kcv::kTablesPerItem));
+      te.items = te.job.tiles_x;
+      entries.push_back(te);
+    }
+  }
+  std::stable_sort(entries.begin(), entries.end(), [](const Entry &a, const Entry &b) {
+    const bool ta = a.family == kcv::kFamTables, tb = b.family == kcv::kFamTables;
+    if (ta != tb) return ta;  // tables first
+    if (a.stage != b.stage) return a.stage < b.stage;
+    if (a.family != b.family) return a.family < b.family;
+    return a.job.variant < b.job.variant;
+  });
+
+  const size_t jobs_bytes = align_up(entries.size() * sizeof(kcv::Job));
+  const size_t first_bytes = align_up(entries.size() * sizeof(uint32_t));
+  const size_t total = jobs_bytes + first_bytes + table_total;
+  if (total == 0) return KLEIDICV_OK;
+  KCV_TRY(from_hip(hipMalloc(&dev, total)));
+  auto *djobs = static_cast<kcv::Job *>(dev);
+  auto *dfirst = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + jobs_bytes);
+  char *dtables = static_cast<char *>(dev) + jobs_bytes + first_bytes;
+
+  // Point resize and tables jobs at their tables.
+  for (Entry &e : entries) {
+    const Prepared &p = prep[e.prep_index];
+    if (p.table_bytes == 0) continue;
+    auto *cols = reinterpret_cast<int2 *>(dtables + table_offset[e.prep_index]);
+    const kcv::ResizeArgs &a = p.job.u.resize.a;
+    int2 *rows = cols + int64_t{a.dst_width} * a.channels;
+    if (e.family == kcv::kFamTables) {
+      e.job.u.tables.cols = cols;
+      e.job.u.tables.rows = rows;
+    } else {
+      e.job.u.resize.cols = cols;
+      e.job.u.resize.rows = rows;
+    }
+  }
+
+  std::vector<kcv::Job> hjobs(entries.size());
+  std::vector<uint32_t> hfirst(entries.size());
+  size_t begin = 0;
+  while (begin < entries.size()) {
+    size_t end = begin;
+    int64_t count = 0;
+    while (end < entries.size() && entries[end].family == entries[begin].family &&
+           entries[end].job.variant == entries[begin].job.variant &&
+           (entries[end].stage == entries[begin].stage ||
+            entries[begin].family == kcv::kFamTables)) {
+      hjobs[end] = entries[end].job;
+      hfirst[end] = static_cast<uint32_t>(count);
+      count += entries[end].items;
+      if (count >= (int64_t{1} << 31)) {
+        release();
+        return KLEIDICV_ERROR_RANGE;
+      }
+      ++end;
+    }
+    if (count > 0) {
+      SegmentPlan sp{};
+      sp.family = entries[begin].family;
+      sp.variant = entries[begin].job.variant;
+      sp.stage = entries[begin].stage;
+      sp.seg.jobs = djobs + begin;
+      sp.seg.first = dfirst + begin;
+      sp.seg.njobs = static_cast<int>(end - begin);
+      sp.seg.nitems = static_cast<uint32_t>(count);
+      sp.seg.xcd_order = static_cast<int>(t.xcd_order);
+      int64_t grid = std::min<int64_t>(count, int64_t{cus} * t.blocks_per_cu);
+      if (t.xcd_order) grid = div_up(grid, 8) * 8;
+      sp.grid = static_cast<int>(grid);
+      segs.push_back(sp);
+      items += static_cast<size_t>(count);
+    }
+    begin = end;
+  }
+  // Stream-ordered upload on the library's stream, then wait for it, so the
+  // work lists are in place before any caller stream can run the batch.
+  hipStream_t up = kcv::runtime().stream;
+  KCV_TRY(from_hip(hipMemcpyAsync(djobs, hjobs.data(), hjobs.size() * sizeof(kcv::Job),
+                                  hipMemcpyHostToDevice, up)));
+  KCV_TRY(from_hip(hipMemcpyAsync(dfirst, hfirst.data(), hfirst.size() * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, up)));
+  return from_hip(hipStreamSynchronize(up));
+}
+
+kleidicv_error_t kleidicv_hip_batch::enqueue(hipStream_t s) const {
+  for (const SegmentPlan &sp : segs)
+    KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
+  return KLEIDICV_OK;
+}
+
+extern "C" {
+
+kleidicv_error_t kleidicv_hip_batch_create(const kleidicv_hip_op_desc *ops, size_t count,
+                                           const kleidicv_hip_batch_tuning *tuning,
+                                           kleidicv_hip_batch **batch,
+                                           size_t *failed_index) {
+  if (batch == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  *batch = nullptr;
+  if (ops == nullptr && count > 0) return KLEIDICV_ERROR_NULL_POINTER;
+  const kleidicv_hip_batch_tuning t = tuning ? *tuning : default_tuning();
+  if (!valid_tuning(t)) return KLEIDICV_ERROR_RANGE;
+  KCV_TRY(check_supported(t));
+  if (!kcv::runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  auto *b = new (std::nothrow) kleidicv_hip_batch;
+  if (b == nullptr) return KLEIDICV_ERROR_ALLOCATION;
+  b->prep.reserve(count);
+  for (size_t i = 0; i < count; ++i) {
+    Prepared p;
+    bool skip = false;
+    const kleidicv_error_t e = prepare_op(ops[i], &p, &skip);
+    if (e != KLEIDICV_OK) {
+      if (failed_index) *failed_index = i;
+      delete b;
+      return e;
+    }
+    if (!skip) b->prep.push_back(p);
+  }
+  const kleidicv_error_t e = b->build(t);
+  if (e != KLEIDICV_OK) {
+    b->release();
+    delete b;
+    return e;
+  }
+  *batch = b;
+  return KLEIDICV_OK;
+}
+
+kleidicv_error_t kleidicv_hip_batch_run(kleidicv_hip_batch *batch, void *stream) {
+  if (batch == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  return batch->enqueue(static_cast<hipStream_t>(stream));
+}
+
+kleidicv_error_t kleidicv_hip_batch_set_tuning(kleidicv_hip_batch *batch,
+                                               const kleidicv_hip_batch_tuning *tuning) {
+  if (batch == nullptr || tuning == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  if (!valid_tuning(*tuning)) return KLEIDICV_ERROR_RANGE;
+  KCV_TRY(check_supported(*tuning));
+  // Earlier runs may still read the work lists that build() frees.
+  KCV_TRY(from_hip(hipDeviceSynchronize()));
+  return batch->build(*tuning);
+}
+
+kleidicv_error_t kleidicv_hip_batch_get_tuning(const kleidicv_hip_batch *batch,
+                                               kleidicv_hip_batch_tuning *tuning) {
+  if (batch == nullptr || tuning == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  *tuning = batch->tuning;
+  return KLEIDICV_OK;
+}
+
+size_t kleidicv_hip_batch_work_items(const kleidicv_hip_batch *batch) {
+  return batch ? batch->items : 0;
+}
+
+kleidicv_error_t kleidicv_hip_batch_tune(kleidicv_hip_batch *batch, void *stream,
+                                         unsigned iterations,
+                                         kleidicv_hip_batch_tuning *best, float *best_ms) {
+  if (batch == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  if (iterations == 0) iterations = 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  bool has_resize = false;
+  for (const Prepared &p : batch->prep) has_resize |= p.family == kcv::kFamResize;
+
+  std::vector<kleidicv_hip_batch_tuning> cands;
+  const unsigned bpc[] = {1, 2, 4, 8};
+  const unsigned bands_resize[] = {0, 4, 16};
+  const unsigned bands_none[] = {0};
+  for (unsigned b : bpc)
+    for (unsigned x = 0; x <= 1; ++x)
+      for (unsigned band : has_resize ? std::vector<unsigned>(bands_resize, bands_resize + 3)
+                                      : std::vector<unsigned>(bands_none, bands_none + 1)) {
+        kleidicv_hip_batch_tuning t;
+        t.blocks_per_cu = b;
+        t.xcd_order = x;
+        t.resize_band_rows = band;
+        t.use_graph = 0;
+        cands.push_back(t);
+      }
+
+  KCV_TRY(from_hip(hipDeviceSynchronize()));
+  hipEvent_t e0, e1;
+  KCV_TRY(from_hip(hipEventCreate(&e0)));
+  KCV_TRY(from_hip(hipEventCreate(&e1)));
+  float best_time = -1.0f;
+  kleidicv_hip_batch_tuning best_t = batch->tuning;
+  kleidicv_error_t err = KLEIDICV_OK;
+  for (const kleidicv_hip_batch_tuning &t : cands) {
+    if ((err = batch->build(t)) != KLEIDICV_OK) break;
+    if ((err = batch->enqueue(s)) != KLEIDICV_OK) break;  // warm-up
+    if ((err = from_hip(hipEventRecord(e0, s))) != KLEIDICV_OK) break;
+    for (unsigned i = 0; i < iterations && err == KLEIDICV_OK; ++i) err = batch->enqueue(s);
+    if (err != KLEIDICV_OK) break;
+    if ((err = from_hip(hipEventRecord(e1, s))) != KLEIDICV_OK) break;
+    if ((err = from_hip(hipEventSynchronize(e1))) != KLEIDICV_OK) break;
+    float ms = 0.0f;
+    if ((err = from_hip(hipEventElapsedTime(&ms, e0, e1))) != KLEIDICV_OK) break;
+    ms /= static_cast<float>(iterations);
+    if (best_time < 0.0f || ms < best_time) {
+      best_time = ms;
+      best_t = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err == KLEIDICV_OK) err = from_hip(hipDeviceSynchronize());
+  const kleidicv_error_t e2 = batch->build(best_t);
+  if (err == KLEIDICV_OK) err = e2;
+  if (err != KLEIDICV_OK) return err;
+  if (best) *best = best_t;
+  if (best_ms) *best_ms = best_time;
+  return KLEIDICV_OK;
+}
+
+void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch) {
+  if (batch == nullptr) return;
+  (void)hipDeviceSynchronize();  // runs still in flight read the work lists
+  batch->release();
+  delete batch;
+}
+
+}  // extern "C"

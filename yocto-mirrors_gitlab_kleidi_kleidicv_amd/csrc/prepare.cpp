@@ -1,0 +1,520 @@
+// Argument validation and kernel-argument preparation (declared in
+// prepare.h).  Validation order per entry point follows the reference's API
+// files, cited at each function; the first failing check decides the code.
+
+#include "prepare.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+namespace kcv {
+
+Runtime &runtime() {
+  static Runtime rt;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return;
+    if (hipGetDevice(&rt.device) != hipSuccess) return;
+    if (hipDeviceGetAttribute(&rt.compute_units, hipDeviceAttributeMultiprocessorCount,
+                              rt.device) != hipSuccess)
+      return;
+    if (hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking) != hipSuccess) return;
+    rt.ok = true;
+  });
+  return rt;
+}
+
+kleidicv_error_t from_hip(hipError_t e) {
+  if (e == hipSuccess) return KLEIDICV_OK;
+  if (std::getenv("KLEIDICV_HIP_VERBOSE"))
+    std::fprintf(stderr, "kleidicv_hip: %s\n", hipGetErrorString(e));
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    return KLEIDICV_ERROR_ALLOCATION;
+  return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+}
+
+// Pointers that HIP knows as device or managed memory; anything else is
+// host memory.
+bool is_device_pointer(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+namespace {
+
+// Checks shared by the entry points (include/kleidicv/utils.h:374-429).
+template <typename T>
+kleidicv_error_t check_ptr_stride(const T *p, size_t stride, size_t height) {
+  if (p == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  if (alignof(T) > 1 && height > 1 && (stride % alignof(T)) != 0)
+    return KLEIDICV_ERROR_ALIGNMENT;
+  return KLEIDICV_OK;
+}
+
+kleidicv_error_t check_image_size(size_t width, size_t height) {
+  size_t n = 0;
+  if (__builtin_mul_overflow(width, height, &n)) return KLEIDICV_ERROR_RANGE;
+  if (n > KLEIDICV_MAX_IMAGE_PIXELS) return KLEIDICV_ERROR_RANGE;
+  return KLEIDICV_OK;
+}
+
+bool is_filter_border(kleidicv_border_type_t b) {
+  return b == KLEIDICV_BORDER_TYPE_REPLICATE || b == KLEIDICV_BORDER_TYPE_REFLECT ||
+         b == KLEIDICV_BORDER_TYPE_WRAP || b == KLEIDICV_BORDER_TYPE_REVERSE;
+}
+
+// Small parameter arrays may live in host or device memory.
+kleidicv_error_t read_small(void *dst, const void *src, size_t bytes) {
+  if (is_device_pointer(src)) return from_hip(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+  std::memcpy(dst, src, bytes);
+  return KLEIDICV_OK;
+}
+
+// include/kleidicv/filters/gaussian_blur.h:38-82.
+bool gaussian_is_implemented(size_t w, size_t h, size_t kw, size_t kh, float sx, float sy,
+                             size_t c, kleidicv_border_type_t b) {
+  if (kw != kh || kw < 3 || kw > 255 || (kw & 1) != 1 || sx != sy) return false;
+  if (w < kw - 1 || h < kw - 1 || c > KLEIDICV_MAXIMUM_CHANNEL_COUNT) return false;
+  if (kw > 9 && kw != 15 && kw != 21) {
+    if (b != KLEIDICV_BORDER_TYPE_REPLICATE) return false;
+    const size_t margin = kw / 2;
+    const size_t aligned_margin = (margin + 7) / 8 * 8;
+    if (w < aligned_margin + margin) return false;
+  }
+  return true;
+}
+
+// Host half-kernel for sigma != 0 (include/kleidicv/filters/sigma.h:27-79).
+// The reference computes `hk * multiplier - error` as one contracted
+// expression, hence the explicit fused multiply-subtract.
+bool gaussian_half_kernel(uint8_t *half, size_t half_size, float sigma) {
+  const size_t mid = half_size - 1;
+  const size_t ksize = mid * 2 + 1;
+  if (sigma == 0.0f) sigma = static_cast<float>(ksize) * 0.15f + 0.35f;
+  float vals[128];
+  const float coefficient = 1 / -(2 * sigma * sigma);
+  float sum = 0.0f;
+  for (size_t i = 0, j = mid; i < mid; i++, j--) {
+    vals[i] = std::exp(static_cast<float>(j) * static_cast<float>(j) * coefficient);
+    sum += vals[i];
+  }
+  const float multiplier = 256 / std::fmaf(sum, 2.0f, 1.0f);
+  float error = 0.0f;
+  for (size_t i = 0; i < mid; i++) {
+    const float value = std::fmaf(vals[i], multiplier, -error);
+    const float rounded = std::round(value);
+    half[i] = static_cast<uint8_t>(rounded);
+    error = rounded - value;
+  }
+  const uint16_t mid_value = static_cast<uint16_t>(std::round(multiplier - error));
+  if (mid_value == 256) return false;
+  half[mid] = static_cast<uint8_t>(mid_value);
+  return true;
+}
+
+// include/kleidicv/resize/resize_linear.h:16-65.
+bool resize_is_implemented(size_t sw, size_t sh, size_t dw, size_t dh, size_t c) {
+  const size_t kMax = KLEIDICV_RESIZE_MAX_WIDTH_OR_HEIGHT;
+  if (sw > kMax || sh > kMax || dw > kMax || dh > kMax || c == 0) return false;
+  if (sw == 0 || sh == 0) return true;
+  if (sw == 2 * dw && sh == 2 * dh && c <= 4) return true;
+  if (c == 1 && ((sw * 2 == dw && sh * 2 == dh) || (sw * 4 == dw && sh * 4 == dh)))
+    return true;
+  if (c <= 4 && dw >= sw) return sw * c >= 16;
+  if (c <= 4 && dw * 3 >= sw && dw < sw) {
+    if (dw * c < 8) return false;
+    if (dw * 2 >= sw && c != 3) return sw * c >= 16;
+    return sw * c >= 32;
+  }
+  return false;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Gaussian blur.  src/filters/gaussian_blur_api.cpp:21-53,
+// include/kleidicv/filters/gaussian_blur.h:38-95.
+kleidicv_error_t prepare_gaussian(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                  size_t dst_stride, size_t width, size_t height,
+                                  size_t channels, size_t kernel_width,
+                                  size_t kernel_height, float sigma_x, float sigma_y,
+                                  kleidicv_border_type_t border_type, SepArgs *a,
+                                  Action *act) {
+  *act = Action::kNone;
+  if (!is_filter_border(border_type)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (!gaussian_is_implemented(width, height, kernel_width, kernel_height, sigma_x,
+                               sigma_y, channels, border_type))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  // The arbitrary-size path (gaussian_blur_arbitrary_neon.cpp) is not built.
+  if (!(kernel_width <= 9 || kernel_width == 15 || kernel_width == 21))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, height));
+  KCV_TRY(check_image_size(width, height));
+  if (width == 0 || height == 0 || channels == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = SepArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->channels = static_cast<int>(channels);
+  a->border = static_cast<int>(border_type);
+  a->ksize = static_cast<int>(kernel_width);
+  if (sigma_x == 0.0f && kernel_width <= 9) {
+    static const int kTaps[4][9] = {{1, 2, 1},
+                                    {1, 4, 6, 4, 1},
+                                    {2, 7, 14, 18, 14, 7, 2},
+                                    {4, 13, 30, 51, 60, 51, 30, 13, 4}};
+    static const int kShift[4] = {4, 8, 12, 16};
+    const int k = static_cast<int>(kernel_width / 2) - 1;
+    a->mode = kSepBinomial;
+    a->shift = kShift[k];
+    for (size_t i = 0; i < kernel_width; ++i) a->wx[i] = a->wy[i] = kTaps[k][i];
+  } else {
+    uint8_t half[16];
+    const size_t half_size = kernel_width / 2 + 1;
+    if (!gaussian_half_kernel(half, half_size, sigma_x)) {
+      *act = Action::kCopy;  // gaussian_blur_fixed_neon.cpp:565-573
+      return KLEIDICV_OK;
+    }
+    a->mode = kSepSigma;
+    for (size_t i = 0; i < half_size; ++i)
+      a->wx[i] = a->wy[i] = a->wx[kernel_width - 1 - i] = a->wy[kernel_width - 1 - i] =
+          half[i];
+  }
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Separable filter.  src/filters/separable_filter_2d_api.cpp:12-31,
+// src/filters/separable_filter_2d_neon.cpp:223-236.
+kleidicv_error_t prepare_separable(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                   size_t dst_stride, size_t width, size_t height,
+                                   size_t channels, const uint8_t *kernel_x,
+                                   size_t kernel_width, const uint8_t *kernel_y,
+                                   size_t kernel_height, kleidicv_border_type_t border_type,
+                                   SepArgs *a, Action *act) {
+  *act = Action::kNone;
+  // 5x5 as the reference; 3x3 and 7x7 extend it with the same arithmetic.
+  if (kernel_width != kernel_height ||
+      !(kernel_width == 3 || kernel_width == 5 || kernel_width == 7))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (width < kernel_width - 1 || height < kernel_width - 1)
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (!is_filter_border(border_type)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, height));
+  KCV_TRY(check_image_size(width, height));
+  if (kernel_x == nullptr || kernel_y == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  if (channels > KLEIDICV_MAXIMUM_CHANNEL_COUNT) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (width == 0 || height == 0 || channels == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = SepArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->channels = static_cast<int>(channels);
+  a->border = static_cast<int>(border_type);
+  a->ksize = static_cast<int>(kernel_width);
+  a->mode = kSepSaturate;
+  uint8_t kx[7], ky[7];
+  KCV_TRY(read_small(kx, kernel_x, kernel_width));
+  KCV_TRY(read_small(ky, kernel_y, kernel_width));
+  for (size_t i = 0; i < kernel_width; ++i) {
+    a->wx[i] = kx[i];
+    a->wy[i] = ky[i];
+  }
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Median blur.  src/filters/median_blur_api.cpp:62-95,
+// include/kleidicv/filters/median_blur.h:149-195.
+kleidicv_error_t prepare_median(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                size_t dst_stride, size_t width, size_t height,
+                                size_t channels, size_t kernel_width, size_t kernel_height,
+                                kleidicv_border_type_t border_type, MedianArgs *a,
+                                Action *act) {
+  *act = Action::kNone;
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, height));
+  KCV_TRY(check_image_size(width, height));
+  const bool size_ok = kernel_width == kernel_height && kernel_width >= 3 &&
+                       kernel_width <= 255 && (kernel_width % 2) != 0;
+  if (!(src != dst && channels <= KLEIDICV_MAXIMUM_CHANNEL_COUNT &&
+        height >= kernel_height - 1 && width >= kernel_width - 1 && size_ok &&
+        is_filter_border(border_type)))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  // Histogram medians for kernels above 7x7 are not built.
+  if (kernel_width > 7) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (width == 0 || height == 0 || channels == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = MedianArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->channels = static_cast<int>(channels);
+  a->border = static_cast<int>(border_type);
+  a->ksize = static_cast<int>(kernel_width);
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Resize.  src/resize/resize_linear_api.cpp:82-247,
+// include/kleidicv/resize/resize_linear.h:16-65.
+kleidicv_error_t prepare_resize(const uint8_t *src, size_t src_stride, size_t src_width,
+                                size_t src_height, uint8_t *dst, size_t dst_stride,
+                                size_t dst_width, size_t dst_height, size_t channels,
+                                ResizeArgs *a, Action *act) {
+  *act = Action::kNone;
+  if (!resize_is_implemented(src_width, src_height, dst_width, dst_height, channels))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(check_ptr_stride(src, src_stride, src_height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, dst_height));
+  if (src_width == 0 || src_height == 0) return KLEIDICV_OK;
+  if (dst_width == 0 || dst_height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = ResizeArgs{};
+  a->src = src;
+  a->dst = dst;
+  a->src_width = static_cast<int>(src_width);
+  a->src_height = static_cast<int>(src_height);
+  a->dst_width = static_cast<int>(dst_width);
+  a->dst_height = static_cast<int>(dst_height);
+  a->channels = static_cast<int>(channels);
+  a->src_stride = src_stride;
+  a->dst_stride = dst_stride;
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Remap s16point5.  src/transform/remap_s16point5_neon.cpp:1212-1235,
+// include/kleidicv/transform/remap.h:35-53.
+kleidicv_error_t prepare_remap_s16point5(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height, size_t channels,
+    const int16_t *mapxy, size_t mapxy_stride, const uint16_t *mapfrac,
+    size_t mapfrac_stride, kleidicv_border_type_t border_type, const uint8_t *border_value,
+    RemapS16Args *a, Action *act) {
+  *act = Action::kNone;
+  KCV_TRY(check_ptr_stride(src, src_stride, src_height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, dst_height));
+  KCV_TRY(check_ptr_stride(mapxy, mapxy_stride, dst_height));
+  KCV_TRY(check_ptr_stride(mapfrac, mapfrac_stride, dst_height));
+  KCV_TRY(check_image_size(src_width, src_height));
+  KCV_TRY(check_image_size(dst_width, dst_height));
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT && border_value == nullptr)
+    return KLEIDICV_ERROR_NULL_POINTER;
+  if (!((channels == 1 || channels == 4) && src_stride <= 65535 / channels &&
+        dst_width >= 8 && src_width <= 32768 && src_height <= 32768 &&
+        (border_type == KLEIDICV_BORDER_TYPE_REPLICATE ||
+         border_type == KLEIDICV_BORDER_TYPE_CONSTANT)))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (dst_height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = RemapS16Args{};
+  a->src = src;
+  a->dst = dst;
+  a->mapxy = mapxy;
+  a->mapfrac = mapfrac;
+  a->src_width = static_cast<int>(src_width);
+  a->src_height = static_cast<int>(src_height);
+  a->dst_width = static_cast<int>(dst_width);
+  a->dst_height = static_cast<int>(dst_height);
+  a->channels = static_cast<int>(channels);
+  a->src_stride = src_stride;
+  a->dst_stride = dst_stride;
+  a->mapxy_stride = mapxy_stride;
+  a->mapfrac_stride = mapfrac_stride;
+  a->border = static_cast<int>(border_type);
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT)
+    KCV_TRY(read_small(a->border_value, border_value, channels));
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Remap f32.  src/transform/remap_f32_neon.cpp:183-209,
+// include/kleidicv/transform/remap.h:55-74 (channels extended to 4).
+kleidicv_error_t prepare_remap_f32(const uint8_t *src, size_t src_stride, size_t src_width,
+                                   size_t src_height, uint8_t *dst, size_t dst_stride,
+                                   size_t dst_width, size_t dst_height, size_t channels,
+                                   const float *mapx, size_t mapx_stride, const float *mapy,
+                                   size_t mapy_stride,
+                                   kleidicv_interpolation_type_t interpolation,
+                                   kleidicv_border_type_t border_type,
+                                   const uint8_t *border_value, RemapF32Args *a,
+                                   Action *act) {
+  *act = Action::kNone;
+  KCV_TRY(check_ptr_stride(src, src_stride, src_height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, dst_height));
+  KCV_TRY(check_ptr_stride(mapx, mapx_stride, dst_height));
+  KCV_TRY(check_ptr_stride(mapy, mapy_stride, dst_height));
+  KCV_TRY(check_image_size(src_width, src_height));
+  KCV_TRY(check_image_size(dst_width, dst_height));
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT && border_value == nullptr)
+    return KLEIDICV_ERROR_NULL_POINTER;
+  if (!(src_stride <= 0xFFFFFFFFull && dst_width >= 4 && src_width < (1ull << 24) &&
+        src_height < (1ull << 24) && dst_width < (1ull << 24) &&
+        dst_height < (1ull << 24) && src_width > 0 && src_height > 0 &&
+        (border_type == KLEIDICV_BORDER_TYPE_REPLICATE ||
+         border_type == KLEIDICV_BORDER_TYPE_CONSTANT) &&
+        channels >= 1 && channels <= 4 &&
+        (interpolation == KLEIDICV_INTERPOLATION_LINEAR ||
+         interpolation == KLEIDICV_INTERPOLATION_NEAREST)))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (dst_height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = RemapF32Args{};
+  a->src = src;
+  a->dst = dst;
+  a->mapx = mapx;
+  a->mapy = mapy;
+  a->src_width = static_cast<int>(src_width);
+  a->src_height = static_cast<int>(src_height);
+  a->dst_width = static_cast<int>(dst_width);
+  a->dst_height = static_cast<int>(dst_height);
+  a->channels = static_cast<int>(channels);
+  a->src_stride = src_stride;
+  a->dst_stride = dst_stride;
+  a->mapx_stride = mapx_stride;
+  a->mapy_stride = mapy_stride;
+  a->interpolation = static_cast<int>(interpolation);
+  a->border = static_cast<int>(border_type);
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT)
+    KCV_TRY(read_small(a->border_value, border_value, channels));
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Warp perspective.  src/transform/warp_perspective_api.cpp:17-34,
+// src/transform/warp_perspective_neon.cpp:158-193, transform_common.h:17-43.
+kleidicv_error_t prepare_warp(const uint8_t *src, size_t src_stride, size_t src_width,
+                              size_t src_height, uint8_t *dst, size_t dst_stride,
+                              size_t dst_width, size_t dst_height,
+                              const float transformation[9], size_t channels,
+                              kleidicv_interpolation_type_t interpolation,
+                              kleidicv_border_type_t border_type,
+                              const uint8_t *border_value, WarpArgs *a, Action *act) {
+  *act = Action::kNone;
+  if (!(dst_width >= 8 &&
+        (interpolation == KLEIDICV_INTERPOLATION_NEAREST ||
+         interpolation == KLEIDICV_INTERPOLATION_LINEAR) &&
+        (border_type == KLEIDICV_BORDER_TYPE_REPLICATE ||
+         border_type == KLEIDICV_BORDER_TYPE_CONSTANT) &&
+        channels >= 1 && channels <= 4))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(check_ptr_stride(src, src_stride, src_height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, dst_height));
+  if (transformation == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  KCV_TRY(check_image_size(src_width, src_height));
+  KCV_TRY(check_image_size(dst_width, dst_height));
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT && border_value == nullptr)
+    return KLEIDICV_ERROR_NULL_POINTER;
+  if (src_width >= (1ull << 24) || src_height >= (1ull << 24) ||
+      dst_width >= (1ull << 24) || dst_height >= (1ull << 24) ||
+      src_stride >= (1ull << 32) || src_width == 0 || src_height == 0)
+    return KLEIDICV_ERROR_RANGE;
+  if (dst_height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = WarpArgs{};
+  a->src = src;
+  a->dst = dst;
+  a->src_width = static_cast<int>(src_width);
+  a->src_height = static_cast<int>(src_height);
+  a->dst_width = static_cast<int>(dst_width);
+  a->dst_height = static_cast<int>(dst_height);
+  a->channels = static_cast<int>(channels);
+  a->src_stride = src_stride;
+  a->dst_stride = dst_stride;
+  a->interpolation = static_cast<int>(interpolation);
+  a->border = static_cast<int>(border_type);
+  KCV_TRY(read_small(a->t, transformation, sizeof(a->t)));
+  if (border_type == KLEIDICV_BORDER_TYPE_CONSTANT)
+    KCV_TRY(read_small(a->border_value, border_value, channels));
+  // weight_may_be_zero (transform_common.h:17-43): T6*x + T7*y + T8 parses
+  // as (T6*x + T7*y) + T8 and clang fuses the first product into the first
+  // sum.
+  auto weight = [&](float x, float y) { return std::fmaf(a->t[6], x, a->t[7] * y) + a->t[8]; };
+  const float x1 = static_cast<float>(dst_width - 1);
+  const float y1 = static_cast<float>(dst_height - 1);
+  float mn = weight(0.0f, 0.0f), mx = mn;
+  for (float w : {weight(x1, 0.0f), weight(0.0f, y1), weight(x1, y1)}) {
+    mn = w < mn ? w : mn;
+    mx = w > mx ? w : mx;
+  }
+  a->check_zero = (mn * mx <= 0.0f) ? 1 : 0;
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// ---------------------------------------------------------------------------
+// NV12 / NV21.  src/conversions/yuv420sp_to_rgb_neon.cpp:75-160.
+kleidicv_error_t prepare_yuv_sp(const uint8_t *src_y, size_t src_y_stride,
+                                const uint8_t *src_uv, size_t src_uv_stride, uint8_t *dst,
+                                size_t dst_stride, size_t width, size_t height,
+                                kleidicv_color_conversion_t color_format, YuvArgs *a,
+                                Action *act) {
+  *act = Action::kNone;
+  switch (color_format) {
+    case KLEIDICV_NV12_TO_RGB: case KLEIDICV_NV12_TO_BGR:
+    case KLEIDICV_NV12_TO_RGBA: case KLEIDICV_NV12_TO_BGRA:
+    case KLEIDICV_NV21_TO_RGB: case KLEIDICV_NV21_TO_BGR:
+    case KLEIDICV_NV21_TO_RGBA: case KLEIDICV_NV21_TO_BGRA:
+      break;
+    default:
+      return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  }
+  KCV_TRY(check_ptr_stride(src_y, src_y_stride, height));
+  KCV_TRY(check_ptr_stride(src_uv, src_uv_stride, (height + 1) / 2));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, height));
+  KCV_TRY(check_image_size(width, height));
+  if (width == 0 || height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = YuvArgs{};
+  a->y = src_y;
+  a->uv = src_uv;
+  a->dst = dst;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->vu = (color_format & KLEIDICV_COLOR_CONVERSION_FLAG_VU) ? 1 : 0;
+  a->bgr = (color_format & KLEIDICV_COLOR_CONVERSION_FLAG_BGR) ? 1 : 0;
+  a->alpha = (color_format & KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA) ? 1 : 0;
+  a->y_stride = src_y_stride;
+  a->uv_stride = src_uv_stride;
+  a->dst_stride = dst_stride;
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+}  // namespace kcv

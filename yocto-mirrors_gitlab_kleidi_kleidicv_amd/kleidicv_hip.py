@@ -257,3 +257,185 @@ def yuv_sp_to_rgb(y, uv, color_format=NV12_TO_RGB, dst=None, stream=None,
 
 def synchronize(stream=None):
     _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# Batched execution (kleidicv_hip_batch_*): device tensors only.
+OP_GAUSSIAN_BLUR, OP_SEPARABLE_FILTER_2D, OP_MEDIAN_BLUR, OP_RESIZE_LINEAR = 1, 2, 3, 4
+OP_REMAP_S16POINT5, OP_REMAP_F32, OP_WARP_PERSPECTIVE, OP_YUV_SEMIPLANAR_TO_RGB = 5, 6, 7, 8
+
+
+class OpDesc(C.Structure):
+    """kleidicv_hip_op_desc."""
+    _fields_ = [
+        ("op", C.c_int), ("stage", C.c_uint),
+        ("src", C.c_void_p), ("src_stride", C.c_size_t), ("src_width", C.c_size_t),
+        ("src_height", C.c_size_t),
+        ("dst", C.c_void_p), ("dst_stride", C.c_size_t), ("dst_width", C.c_size_t),
+        ("dst_height", C.c_size_t),
+        ("channels", C.c_size_t), ("kernel_size", C.c_size_t), ("sigma", C.c_float),
+        ("kernel_x", C.c_uint8 * 7), ("kernel_y", C.c_uint8 * 7),
+        ("border_type", C.c_int), ("interpolation", C.c_int),
+        ("border_value", C.c_uint8 * 4),
+        ("aux0", C.c_void_p), ("aux0_stride", C.c_size_t),
+        ("aux1", C.c_void_p), ("aux1_stride", C.c_size_t),
+        ("transformation", C.c_float * 9), ("color_format", C.c_int),
+    ]
+
+
+class Tuning(C.Structure):
+    """kleidicv_hip_batch_tuning."""
+    _fields_ = [("blocks_per_cu", C.c_uint), ("resize_band_rows", C.c_uint),
+                ("xcd_order", C.c_uint), ("use_graph", C.c_uint)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        return cls(**{k: int(d[k]) for k, _ in cls._fields_})
+
+
+def _desc(op, src, dst, stage, **kw):
+    d = OpDesc()
+    d.op = op
+    d.stage = stage
+    sh, sw, sc = _shape(src)
+    d.src, d.src_stride, d.src_width, d.src_height = _ptr(src).value, _stride(src), sw, sh
+    dh, dw, _ = _shape(dst)
+    d.dst, d.dst_stride, d.dst_width, d.dst_height = _ptr(dst).value, _stride(dst), dw, dh
+    d.channels = sc
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def op_gaussian_blur(src, dst, kernel_size, sigma=0.0, border="REVERSE", stage=0):
+    return _desc(OP_GAUSSIAN_BLUR, src, dst, stage, kernel_size=kernel_size, sigma=sigma,
+                 border_type=BORDER[border])
+
+
+def op_separable_filter_2d(src, dst, kernel_x, kernel_y, border="REPLICATE", stage=0):
+    d = _desc(OP_SEPARABLE_FILTER_2D, src, dst, stage, kernel_size=len(kernel_x),
+              border_type=BORDER[border])
+    for i, (a, b) in enumerate(zip(kernel_x, kernel_y)):
+        d.kernel_x[i], d.kernel_y[i] = int(a), int(b)
+    return d
+
+
+def op_median_blur(src, dst, kernel_size, border="REPLICATE", stage=0):
+    return _desc(OP_MEDIAN_BLUR, src, dst, stage, kernel_size=kernel_size,
+                 border_type=BORDER[border])
+
+
+def op_resize_linear(src, dst, stage=0):
+    return _desc(OP_RESIZE_LINEAR, src, dst, stage)
+
+
+def _bv(d, border_value):
+    for i, v in enumerate(tuple(border_value)[:4]):
+        d.border_value[i] = int(v)
+
+
+def op_remap_s16point5(src, dst, mapxy, mapfrac, border="CONSTANT", border_value=(0, 0, 0, 0),
+                       stage=0):
+    d = _desc(OP_REMAP_S16POINT5, src, dst, stage, border_type=BORDER[border],
+              aux0=_ptr(mapxy).value, aux0_stride=_stride(mapxy),
+              aux1=_ptr(mapfrac).value, aux1_stride=_stride(mapfrac))
+    _bv(d, border_value)
+    return d
+
+
+def op_remap_f32(src, dst, mapx, mapy, interpolation="LINEAR", border="REPLICATE",
+                 border_value=(0, 0, 0, 0), stage=0):
+    d = _desc(OP_REMAP_F32, src, dst, stage, interpolation=INTERP[interpolation],
+              border_type=BORDER[border], aux0=_ptr(mapx).value, aux0_stride=_stride(mapx),
+              aux1=_ptr(mapy).value, aux1_stride=_stride(mapy))
+    _bv(d, border_value)
+    return d
+
+
+def op_warp_perspective(src, dst, transform, interpolation="LINEAR", border="REPLICATE",
+                        border_value=(0, 0, 0, 0), stage=0):
+    d = _desc(OP_WARP_PERSPECTIVE, src, dst, stage, interpolation=INTERP[interpolation],
+              border_type=BORDER[border])
+    for i, v in enumerate(np.asarray(transform, dtype=np.float32).reshape(9)):
+        d.transformation[i] = float(v)
+    _bv(d, border_value)
+    return d
+
+
+def op_yuv_sp_to_rgb(y, uv, dst, color_format=NV12_TO_RGB, stage=0):
+    d = _desc(OP_YUV_SEMIPLANAR_TO_RGB, y, dst, stage, aux0=_ptr(uv).value,
+              aux0_stride=_stride(uv), color_format=color_format)
+    d.channels = 1
+    return d
+
+
+class Batch:
+    """A validated batch of ops over device tensors, run as one persistent
+    kernel launch per (stage, kernel family, variant) segment.
+
+    The tensors referenced by the descriptors must stay alive (and in place)
+    for as long as the batch is used; `keep` holds references for that.
+    """
+
+    def __init__(self, descs, tuning=None, keep=()):
+        L = lib()
+        L.kleidicv_hip_batch_work_items.restype = C.c_size_t
+        arr = (OpDesc * len(descs))(*descs)
+        self._keep = tuple(keep)
+        self._h = C.c_void_p(0)
+        failed = C.c_size_t(0)
+        t = C.byref(tuning) if tuning is not None else None
+        code = L.kleidicv_hip_batch_create(arr, C.c_size_t(len(descs)), t, C.byref(self._h),
+                                           C.byref(failed))
+        if code != OK:
+            err = KleidiCVError(code, "kleidicv_hip_batch_create (op %d)" % failed.value)
+            err.failed_index = failed.value
+            raise err
+
+    def run(self, stream=None):
+        _check(lib().kleidicv_hip_batch_run(self._h, _stream_arg(stream)),
+               "kleidicv_hip_batch_run")
+
+    def tune(self, stream=None, iterations=5):
+        best = Tuning()
+        ms = C.c_float(0)
+        _check(lib().kleidicv_hip_batch_tune(self._h, _stream_arg(stream), C.c_uint(iterations),
+                                             C.byref(best), C.byref(ms)),
+               "kleidicv_hip_batch_tune")
+        return best, ms.value
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        _check(lib().kleidicv_hip_batch_get_tuning(self._h, C.byref(t)),
+               "kleidicv_hip_batch_get_tuning")
+        return t
+
+    @tuning.setter
+    def tuning(self, t):
+        _check(lib().kleidicv_hip_batch_set_tuning(self._h, C.byref(t)),
+               "kleidicv_hip_batch_set_tuning")
+
+    @property
+    def work_items(self):
+        return int(lib().kleidicv_hip_batch_work_items(self._h))
+
+    def close(self):
+        if self._h:
+            lib().kleidicv_hip_batch_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
