@@ -28,11 +28,15 @@ int orc_gaussian_blur_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
                          size_t channels, size_t kernel_size, float sigma,
                          int border);
 
-/* a4: host half-kernel generator (sigma.h:27-79). Returns 0 for "copy". */
+/* a4: host half-kernel generator (sigma.h:27-79). Returns 0 for "copy".
+ * Parity unpinned: the taps depend on the host expf; the reference's own
+ * tests allow +-1 for sigma != 0 and hold no exact table. */
 int orc_gaussian_half_kernel(uint8_t *half_kernel, size_t half_size,
                              float sigma);
 
-/* a5: separable filter, any odd size (closed form min(255, sum ky kx src)). */
+/* a5: separable filter, any odd size (closed form min(255, sum ky kx src)).
+ * 5x5 is pinned by the reference's KATs; 3x3 and 7x7 (not in the
+ * reference) are parity unpinned beyond that closed form. */
 int orc_separable_filter_2d_u8(const uint8_t *src, size_t src_stride,
                                uint8_t *dst, size_t dst_stride, size_t width,
                                size_t height, size_t channels,
@@ -71,7 +75,9 @@ int orc_remap_s16point5_u8(const uint8_t *src, size_t src_stride,
                            const uint16_t *mapfrac, size_t mapfrac_stride,
                            int border, const uint8_t *border_value);
 
-/* a11: remap with float maps (1..4 channels; 3-4 are per-plane extensions). */
+/* a11: remap with float maps (1..4 channels; 3-4 are per-plane extensions).
+ * Exactness against the AArch64 build (its FMA contraction) is import-only;
+ * pinned here to the reference's +-1 tolerance and the per-plane invariant. */
 int orc_remap_f32_u8(const uint8_t *src, size_t src_stride, size_t src_width,
                      size_t src_height, uint8_t *dst, size_t dst_stride,
                      size_t dst_width, size_t dst_height, size_t channels,
@@ -79,7 +85,9 @@ int orc_remap_f32_u8(const uint8_t *src, size_t src_stride, size_t src_width,
                      size_t mapy_stride, int interpolation, int border,
                      const uint8_t *border_value);
 
-/* a12: warp perspective (1..4 channels; 2-4 are per-plane extensions). */
+/* a12: warp perspective (1..4 channels; 2-4 are per-plane extensions).
+ * Pinned by the reference's exact SimpleResize KAT, +-1 against a double
+ * model, and the per-plane invariant; otherwise import-only as for a11. */
 int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
                             size_t src_width, size_t src_height, uint8_t *dst,
                             size_t dst_stride, size_t dst_width,
