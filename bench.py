@@ -52,6 +52,9 @@ def parse():
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched workloads (pyramid and transform batches)")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the serving-loop workload")
+    ap.add_argument("--serving-requests", type=int, default=1500)
     return ap.parse_args()
 
 
@@ -303,6 +306,47 @@ def bench_batched(K, torch, WL, dev, stream, steps, warmup):
     return results
 
 
+def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0)):
+    """Poisson request streams through the continuous-batching server and,
+    with the same arrivals, as plain per-request launches on one stream."""
+    import serving as S
+    import workloads as WL
+    corpus = WL.frame_corpus(torch, dev, n=48, seed=9)
+    torch.cuda.synchronize()
+    be = S.TorchBackend(torch, dev)
+    budget = 64 << 30
+    stream = torch.cuda.Stream()
+    # warm-up of both paths (kernel loading, allocator caches)
+    srv = S.Server(be, hbm_budget=budget, slots=2, max_batch_ops=512)
+    S.run_served(srv, WL.serving_requests(corpus, 100, 1e6, seed=1))
+    S.run_plain(be, WL.serving_requests(corpus, 100, 1e6, seed=2), stream)
+    torch.cuda.synchronize()
+    out = {"workload": "%d requests, Poisson arrivals; ops uniform over gaussian 5x5, median "
+                       "5x5, resize 0.5x, warp_perspective, NV12->RGB; frames from a resident "
+                       "corpus of 48, W x H log-uniform 64x64..7680x4320, C in {1,3,4}; "
+                       "latency = arrival to host-observed completion" % n,
+           "hbm_budget_gb": budget / 2 ** 30, "slots": 2, "max_batch_ops": 512}
+    for qps in qps_list:
+        reqs = WL.serving_requests(corpus, n, qps, seed=11)
+        srv = S.Server(be, hbm_budget=budget, slots=2, max_batch_ops=512)
+        t0 = S.run_served(srv, reqs)
+        torch.cuda.synchronize()
+        served = S.latency_stats(reqs, t0)
+        served["batches"] = srv.batches
+        served["peak_resident_gb"] = round(srv.peak / 2 ** 30, 3)
+        del reqs, srv
+        plain_reqs = WL.serving_requests(corpus, n, qps, seed=11)
+        t0 = S.run_plain(be, plain_reqs, stream)
+        torch.cuda.synchronize()
+        plain = S.latency_stats(plain_reqs, t0)
+        del plain_reqs
+        torch.cuda.empty_cache()
+        out["qps_%d" % int(qps)] = {
+            "served": served, "plain": plain,
+            "mpix_s_ratio": round(served["mpix_s"] / plain["mpix_s"], 2)}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -428,6 +472,8 @@ def main():
     if not args.no_batched and not args.ops:
         import workloads as WL
         result["batched"] = bench_batched(K, torch, WL, dev, stream, args.steps, args.warmup)
+    if not args.no_serving and not args.ops:
+        result["serving"] = bench_serving(torch, dev, args.serving_requests)
     if not args.no_cpu_baseline:
         s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
         result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)

@@ -390,6 +390,21 @@ kleidicv_error_t kleidicv_hip_batch_get_tuning(const kleidicv_hip_batch *batch,
 /* Work items (tiles) of the batch under its current tuning. */
 size_t kleidicv_hip_batch_work_items(const kleidicv_hip_batch *batch);
 
+/* One-shot form for callers that build a new batch per submission (the
+ * serving loop): validates the ops, writes their work lists into the
+ * caller's device `workspace` and enqueues the stage kernels on `stream`.
+ * No device allocation and no wait for `stream`.  The workspace must hold
+ * kleidicv_hip_batch_workspace_size() bytes and must not be reused until
+ * the launch has finished on `stream`. */
+kleidicv_error_t kleidicv_hip_batch_workspace_size(const kleidicv_hip_op_desc *ops,
+                                                   size_t count,
+                                                   const kleidicv_hip_batch_tuning *tuning,
+                                                   size_t *bytes, size_t *failed_index);
+kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size_t count,
+                                           const kleidicv_hip_batch_tuning *tuning,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *stream, size_t *failed_index);
+
 /* Waits for the device to finish outstanding work, then frees the batch. */
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch);
 

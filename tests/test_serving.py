@@ -1,0 +1,167 @@
+"""Scheduling logic of the serving loop, on the host.
+
+The server runs against a host-side backend whose "device" work completes a
+set number of polls after launch, so admission, batching and completion
+order can be checked without a GPU.
+"""
+
+import numpy as np
+import pytest
+
+import serving as S
+
+
+class FakeEvent:
+    def __init__(self, polls):
+        self.left = polls
+
+
+class FakeBackend:
+    def __init__(self, polls=3):
+        self.polls = polls
+        self.launches = []
+
+    def stream(self):
+        return object()
+
+    def alloc(self, shape):
+        return np.zeros(shape, np.uint8)
+
+    def workspace(self, nbytes):
+        return np.zeros(max(int(nbytes), 1), np.uint8)
+
+    def workspace_size(self, descs, tuning):
+        return 1024 * len(descs)
+
+    def launch(self, descs, workspace, stream, tuning, requests):
+        assert len(descs) == len(requests)
+        assert workspace.size >= 1024 * len(descs)
+        self.launches.append([r.rid for r in requests])
+
+    def record(self, stream):
+        return FakeEvent(self.polls)
+
+    def query(self, ev):
+        ev.left -= 1
+        return ev.left <= 0
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        self.t += 1.0
+        return self.t
+
+
+def frame(h, w, c=1):
+    return np.zeros((h, w, c), np.uint8)
+
+
+def req(rid, op="gaussian5", h=64, w=64, c=1):
+    if op == "nv12":
+        return S.Request(rid, op, np.zeros((h, w), np.uint8), uv=np.zeros((h // 2, w), np.uint8))
+    return S.Request(rid, op, frame(h, w, c), T=np.eye(3, dtype=np.float32))
+
+
+def test_output_shapes():
+    assert req(0, "resize_half", 65, 131, 3).out_shape() == (33, 66, 3)
+    assert req(1, "nv12", 64, 96).out_shape() == (64, 96, 3)
+    assert req(2, "median5", 70, 80, 4).out_shape() == (70, 80, 4)
+    with pytest.raises(ValueError):
+        S.Request(3, "sobel", frame(8, 8))
+
+
+def test_every_request_completes_once_in_fifo_admission_order():
+    be = FakeBackend(polls=2)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=4, clock=Clock())
+    reqs = [req(i, S.OPS[i % len(S.OPS)]) for i in range(23)]
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    assert sorted(r.rid for r in srv.completed) == list(range(23))
+    admitted = sorted(reqs, key=lambda r: r.admitted)
+    assert [r.rid for r in admitted] == list(range(23))
+    assert all(len(b) <= 4 for b in be.launches)
+    assert sum(len(b) for b in be.launches) == 23
+    assert all(r.admitted <= r.launched < r.done for r in reqs)
+    assert srv.out_in_use == 0 and srv.in_use == sum(srv.ws_bytes)
+
+
+def test_budget_bounds_resident_bytes_and_blocks_the_queue_head():
+    be = FakeBackend(polls=3)
+    one = 64 * 64
+    reserve = 4096   # work lists of up to 4 ops per slot
+    # outputs of 3 requests fit beside the two reserves; the 4th waits
+    budget = 3 * one + 2 * reserve
+    srv = S.Server(be, hbm_budget=budget, slots=2, max_batch_ops=8,
+                   workspace_reserve=reserve, clock=Clock())
+    for i in range(10):
+        srv.submit(req(i))
+    peak_seen = max_out = 0
+    while srv.pending():
+        srv.pump()
+        peak_seen = max(peak_seen, srv.in_use)
+        max_out = max(max_out, srv.out_in_use)
+        assert srv.in_use <= budget
+        assert srv.out_in_use <= 3 * one
+    assert srv.peak <= budget and peak_seen <= budget and max_out == 3 * one
+    assert len(srv.completed) == 10
+    assert all(len(b) <= 3 for b in be.launches)
+    with pytest.raises(ValueError):
+        big = S.Server(FakeBackend(), hbm_budget=3 * 4096 - 1, workspace_reserve=4096,
+                       clock=Clock())
+        big.submit(req(0))   # a 4096-byte output does not fit beside two reserves
+        big.pump()
+
+
+def test_batches_larger_than_the_workspace_reserve_are_split():
+    be = FakeBackend(polls=2)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=1, max_batch_ops=64,
+                   workspace_reserve=3 * 1024, clock=Clock())
+    for i in range(10):
+        srv.submit(req(i))
+    srv.drain()
+    assert sorted(r.rid for r in srv.completed) == list(range(10))
+    assert all(len(b) <= 3 for b in be.launches)
+    assert [rid for b in be.launches for rid in b] == list(range(10))
+
+
+def test_arrivals_during_a_batch_overlap_it_on_another_slot():
+    be = FakeBackend(polls=6)
+    clock = Clock()
+    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=64, clock=clock)
+    first = [req(i) for i in range(5)]
+    for r in first:
+        srv.submit(r)
+    srv.pump()                       # batch 1 launched
+    assert len(be.launches) == 1
+    late = [req(100 + i) for i in range(3)]
+    for r in late:
+        srv.submit(r)
+    srv.pump()                       # batch 2 launched while batch 1 runs
+    assert len(be.launches) == 2 and be.launches[1] == [100, 101, 102]
+    assert all(r.done is None for r in first)
+    srv.drain()
+    assert max(r.launched for r in late) < min(r.done for r in first)
+
+
+def test_single_slot_serialises_batches():
+    be = FakeBackend(polls=4)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=1, clock=Clock())
+    for i in range(3):
+        srv.submit(req(i))
+    srv.pump()
+    srv.submit(req(9))
+    srv.pump()
+    assert len(be.launches) == 1      # the only slot is busy
+    srv.drain()
+    assert be.launches == [[0, 1, 2], [9]]
+
+
+def test_poisson_arrivals_rate():
+    rng = np.random.default_rng(1)
+    t = S.poisson_arrivals(rng, 20000, 5000.0)
+    assert np.all(np.diff(t) > 0)
+    assert abs(20000 / t[-1] - 5000.0) / 5000.0 < 0.03

@@ -252,32 +252,23 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
   }
 }
 
-}  // namespace
-
-struct kleidicv_hip_batch {
-  std::vector<Prepared> prep;
-  kleidicv_hip_batch_tuning tuning{};
-  void *dev = nullptr;
+// Host-side plan of a batch placed at device address `base`: the work-list
+// bytes to upload there (jobs, then first-item offsets), the total device
+// bytes including the resize tables that the tables kernels fill, and the
+// segments to launch.  Layout does not depend on `base`.
+struct Plan {
+  std::vector<unsigned char> blob;
+  size_t bytes = 0;
   std::vector<SegmentPlan> segs;
   size_t items = 0;
-
-  void release() {
-    if (dev) (void)hipFree(dev);
-    dev = nullptr;
-    segs.clear();
-    items = 0;
-  }
-
-  kleidicv_error_t build(const kleidicv_hip_batch_tuning &t);
-  kleidicv_error_t enqueue(hipStream_t s) const;
 };
 
-kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
-  release();
-  tuning = t;
+kleidicv_error_t make_plan(const std::vector<Prepared> &prep,
+                           const kleidicv_hip_batch_tuning &t, char *base, Plan *out) {
   const int cus = kcv::runtime().compute_units > 0 ? kcv::runtime().compute_units : 256;
+  *out = Plan{};
 
-  // Jobs in (stage, family) order; resize ops add a tables job in front.
+  // Jobs in (stage, family, variant) order; resize ops add a tables job.
   struct Entry {
     Family family;
     unsigned stage;
@@ -320,20 +311,25 @@ kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
 
   const size_t jobs_bytes = align_up(entries.size() * sizeof(kcv::Job));
   const size_t first_bytes = align_up(entries.size() * sizeof(uint32_t));
-  const size_t total = jobs_bytes + first_bytes + table_total;
-  if (total == 0) return KLEIDICV_OK;
-  KCV_TRY(from_hip(hipMalloc(&dev, total)));
-  auto *djobs = static_cast<kcv::Job *>(dev);
-  auto *dfirst = reinterpret_cast<uint32_t *>(static_cast<char *>(dev) + jobs_bytes);
-  char *dtables = static_cast<char *>(dev) + jobs_bytes + first_bytes;
+  out->bytes = jobs_bytes + first_bytes + table_total;
+  out->blob.assign(jobs_bytes + first_bytes, 0);
+  auto *hjobs = reinterpret_cast<kcv::Job *>(out->blob.data());
+  auto *hfirst = reinterpret_cast<uint32_t *>(out->blob.data() + jobs_bytes);
+  // Device addresses are formed as integers: `base` is null when only the
+  // size is wanted.
+  const uintptr_t b0 = reinterpret_cast<uintptr_t>(base);
+  auto *djobs = reinterpret_cast<kcv::Job *>(b0);
+  auto *dfirst = reinterpret_cast<uint32_t *>(b0 + jobs_bytes);
+  const uintptr_t dtables = b0 + jobs_bytes + first_bytes;
 
   // Point resize and tables jobs at their tables.
   for (Entry &e : entries) {
     const Prepared &p = prep[e.prep_index];
     if (p.table_bytes == 0) continue;
-    auto *cols = reinterpret_cast<int2 *>(dtables + table_offset[e.prep_index]);
     const kcv::ResizeArgs &a = p.job.u.resize.a;
-    int2 *rows = cols + int64_t{a.dst_width} * a.channels;
+    const uintptr_t c0 = dtables + table_offset[e.prep_index];
+    auto *cols = reinterpret_cast<int2 *>(c0);
+    auto *rows = reinterpret_cast<int2 *>(c0 + sizeof(int2) * int64_t{a.dst_width} * a.channels);
     if (e.family == kcv::kFamTables) {
       e.job.u.tables.cols = cols;
       e.job.u.tables.rows = rows;
@@ -343,8 +339,6 @@ kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
     }
   }
 
-  std::vector<kcv::Job> hjobs(entries.size());
-  std::vector<uint32_t> hfirst(entries.size());
   size_t begin = 0;
   while (begin < entries.size()) {
     size_t end = begin;
@@ -353,13 +347,10 @@ kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
            entries[end].job.variant == entries[begin].job.variant &&
            (entries[end].stage == entries[begin].stage ||
             entries[begin].family == kcv::kFamTables)) {
-      hjobs[end] = entries[end].job;
+      std::memcpy(&hjobs[end], &entries[end].job, sizeof(kcv::Job));
       hfirst[end] = static_cast<uint32_t>(count);
       count += entries[end].items;
-      if (count >= (int64_t{1} << 31)) {
-        release();
-        return KLEIDICV_ERROR_RANGE;
-      }
+      if (count >= (int64_t{1} << 31)) return KLEIDICV_ERROR_RANGE;
       ++end;
     }
     if (count > 0) {
@@ -367,34 +358,82 @@ kleidicv_error_t kleidicv_hip_batch::build(const kleidicv_hip_batch_tuning &t) {
       sp.family = entries[begin].family;
       sp.variant = entries[begin].job.variant;
       sp.stage = entries[begin].stage;
-      sp.seg.jobs = djobs + begin;
-      sp.seg.first = dfirst + begin;
+      sp.seg.jobs = reinterpret_cast<const kcv::Job *>(reinterpret_cast<uintptr_t>(djobs) +
+                                                       begin * sizeof(kcv::Job));
+      sp.seg.first = reinterpret_cast<const uint32_t *>(reinterpret_cast<uintptr_t>(dfirst) +
+                                                        begin * sizeof(uint32_t));
       sp.seg.njobs = static_cast<int>(end - begin);
       sp.seg.nitems = static_cast<uint32_t>(count);
       sp.seg.xcd_order = static_cast<int>(t.xcd_order);
       int64_t grid = std::min<int64_t>(count, int64_t{cus} * t.blocks_per_cu);
       if (t.xcd_order) grid = div_up(grid, 8) * 8;
       sp.grid = static_cast<int>(grid);
-      segs.push_back(sp);
-      items += static_cast<size_t>(count);
+      out->segs.push_back(sp);
+      out->items += static_cast<size_t>(count);
     }
     begin = end;
   }
-  // Stream-ordered upload on the library's stream, then wait for it, so the
-  // work lists are in place before any caller stream can run the batch.
+  return KLEIDICV_OK;
+}
+
+// Copies a plan's work lists to `base`.  The copy runs on the library's own
+// stream and is complete on return, so any caller stream may run the plan.
+kleidicv_error_t upload(const Plan &plan, void *base) {
+  if (plan.blob.empty()) return KLEIDICV_OK;
   hipStream_t up = kcv::runtime().stream;
-  KCV_TRY(from_hip(hipMemcpyAsync(djobs, hjobs.data(), hjobs.size() * sizeof(kcv::Job),
-                                  hipMemcpyHostToDevice, up)));
-  KCV_TRY(from_hip(hipMemcpyAsync(dfirst, hfirst.data(), hfirst.size() * sizeof(uint32_t),
+  KCV_TRY(from_hip(hipMemcpyAsync(base, plan.blob.data(), plan.blob.size(),
                                   hipMemcpyHostToDevice, up)));
   return from_hip(hipStreamSynchronize(up));
 }
 
-kleidicv_error_t kleidicv_hip_batch::enqueue(hipStream_t s) const {
-  for (const SegmentPlan &sp : segs)
+kleidicv_error_t enqueue(const Plan &plan, hipStream_t s) {
+  for (const SegmentPlan &sp : plan.segs)
     KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
   return KLEIDICV_OK;
 }
+
+kleidicv_error_t prepare_all(const kleidicv_hip_op_desc *ops, size_t count,
+                             std::vector<Prepared> *prep, size_t *failed_index) {
+  prep->clear();
+  prep->reserve(count);
+  for (size_t i = 0; i < count; ++i) {
+    Prepared p;
+    bool skip = false;
+    const kleidicv_error_t e = prepare_op(ops[i], &p, &skip);
+    if (e != KLEIDICV_OK) {
+      if (failed_index) *failed_index = i;
+      return e;
+    }
+    if (!skip) prep->push_back(p);
+  }
+  return KLEIDICV_OK;
+}
+
+}  // namespace
+
+struct kleidicv_hip_batch {
+  std::vector<Prepared> prep;
+  kleidicv_hip_batch_tuning tuning{};
+  void *dev = nullptr;
+  Plan plan;
+
+  void release() {
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    plan = Plan{};
+  }
+
+  kleidicv_error_t build(const kleidicv_hip_batch_tuning &t) {
+    release();
+    tuning = t;
+    Plan sizing;
+    KCV_TRY(make_plan(prep, t, nullptr, &sizing));
+    if (sizing.bytes == 0) return KLEIDICV_OK;
+    KCV_TRY(from_hip(hipMalloc(&dev, sizing.bytes)));
+    KCV_TRY(make_plan(prep, t, static_cast<char *>(dev), &plan));
+    return upload(plan, dev);
+  }
+};
 
 extern "C" {
 
@@ -411,19 +450,12 @@ kleidicv_error_t kleidicv_hip_batch_create(const kleidicv_hip_op_desc *ops, size
   if (!kcv::runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   auto *b = new (std::nothrow) kleidicv_hip_batch;
   if (b == nullptr) return KLEIDICV_ERROR_ALLOCATION;
-  b->prep.reserve(count);
-  for (size_t i = 0; i < count; ++i) {
-    Prepared p;
-    bool skip = false;
-    const kleidicv_error_t e = prepare_op(ops[i], &p, &skip);
-    if (e != KLEIDICV_OK) {
-      if (failed_index) *failed_index = i;
-      delete b;
-      return e;
-    }
-    if (!skip) b->prep.push_back(p);
+  kleidicv_error_t e = prepare_all(ops, count, &b->prep, failed_index);
+  if (e != KLEIDICV_OK) {
+    delete b;
+    return e;
   }
-  const kleidicv_error_t e = b->build(t);
+  e = b->build(t);
   if (e != KLEIDICV_OK) {
     b->release();
     delete b;
@@ -435,7 +467,7 @@ kleidicv_error_t kleidicv_hip_batch_create(const kleidicv_hip_op_desc *ops, size
 
 kleidicv_error_t kleidicv_hip_batch_run(kleidicv_hip_batch *batch, void *stream) {
   if (batch == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
-  return batch->enqueue(static_cast<hipStream_t>(stream));
+  return enqueue(batch->plan, static_cast<hipStream_t>(stream));
 }
 
 kleidicv_error_t kleidicv_hip_batch_set_tuning(kleidicv_hip_batch *batch,
@@ -456,7 +488,7 @@ kleidicv_error_t kleidicv_hip_batch_get_tuning(const kleidicv_hip_batch *batch,
 }
 
 size_t kleidicv_hip_batch_work_items(const kleidicv_hip_batch *batch) {
-  return batch ? batch->items : 0;
+  return batch ? batch->plan.items : 0;
 }
 
 kleidicv_error_t kleidicv_hip_batch_tune(kleidicv_hip_batch *batch, void *stream,
@@ -493,9 +525,10 @@ kleidicv_error_t kleidicv_hip_batch_tune(kleidicv_hip_batch *batch, void *stream
   kleidicv_error_t err = KLEIDICV_OK;
   for (const kleidicv_hip_batch_tuning &t : cands) {
     if ((err = batch->build(t)) != KLEIDICV_OK) break;
-    if ((err = batch->enqueue(s)) != KLEIDICV_OK) break;  // warm-up
+    if ((err = enqueue(batch->plan, s)) != KLEIDICV_OK) break;  // warm-up
     if ((err = from_hip(hipEventRecord(e0, s))) != KLEIDICV_OK) break;
-    for (unsigned i = 0; i < iterations && err == KLEIDICV_OK; ++i) err = batch->enqueue(s);
+    for (unsigned i = 0; i < iterations && err == KLEIDICV_OK; ++i)
+      err = enqueue(batch->plan, s);
     if (err != KLEIDICV_OK) break;
     if ((err = from_hip(hipEventRecord(e1, s))) != KLEIDICV_OK) break;
     if ((err = from_hip(hipEventSynchronize(e1))) != KLEIDICV_OK) break;
@@ -516,6 +549,46 @@ kleidicv_error_t kleidicv_hip_batch_tune(kleidicv_hip_batch *batch, void *stream
   if (best) *best = best_t;
   if (best_ms) *best_ms = best_time;
   return KLEIDICV_OK;
+}
+
+kleidicv_error_t kleidicv_hip_batch_workspace_size(const kleidicv_hip_op_desc *ops,
+                                                   size_t count,
+                                                   const kleidicv_hip_batch_tuning *tuning,
+                                                   size_t *bytes, size_t *failed_index) {
+  if (bytes == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  *bytes = 0;
+  if (ops == nullptr && count > 0) return KLEIDICV_ERROR_NULL_POINTER;
+  const kleidicv_hip_batch_tuning t = tuning ? *tuning : default_tuning();
+  if (!valid_tuning(t)) return KLEIDICV_ERROR_RANGE;
+  KCV_TRY(check_supported(t));
+  if (!kcv::runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  std::vector<Prepared> prep;
+  KCV_TRY(prepare_all(ops, count, &prep, failed_index));
+  Plan plan;
+  KCV_TRY(make_plan(prep, t, nullptr, &plan));
+  *bytes = plan.bytes;
+  return KLEIDICV_OK;
+}
+
+kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size_t count,
+                                           const kleidicv_hip_batch_tuning *tuning,
+                                           void *workspace, size_t workspace_bytes,
+                                           void *stream, size_t *failed_index) {
+  if (ops == nullptr && count > 0) return KLEIDICV_ERROR_NULL_POINTER;
+  const kleidicv_hip_batch_tuning t = tuning ? *tuning : default_tuning();
+  if (!valid_tuning(t)) return KLEIDICV_ERROR_RANGE;
+  KCV_TRY(check_supported(t));
+  if (!kcv::runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  std::vector<Prepared> prep;
+  KCV_TRY(prepare_all(ops, count, &prep, failed_index));
+  Plan plan;
+  KCV_TRY(make_plan(prep, t, static_cast<char *>(workspace), &plan));
+  if (plan.bytes == 0) return KLEIDICV_OK;
+  if (workspace == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
+  if (workspace_bytes < plan.bytes) return KLEIDICV_ERROR_RANGE;
+  if (!kcv::is_device_pointer(workspace)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(upload(plan, workspace));
+  return enqueue(plan, static_cast<hipStream_t>(stream));
 }
 
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch) {

@@ -439,3 +439,32 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+def batch_workspace_size(descs, tuning=None):
+    """Device bytes kleidicv_hip_batch_launch needs for these descriptors."""
+    arr = (OpDesc * len(descs))(*descs)
+    n = C.c_size_t(0)
+    failed = C.c_size_t(0)
+    code = lib().kleidicv_hip_batch_workspace_size(
+        arr, C.c_size_t(len(descs)), C.byref(tuning) if tuning is not None else None,
+        C.byref(n), C.byref(failed))
+    if code != OK:
+        err = KleidiCVError(code, "kleidicv_hip_batch_workspace_size (op %d)" % failed.value)
+        err.failed_index = failed.value
+        raise err
+    return n.value
+
+
+def batch_launch(descs, workspace, stream=None, tuning=None):
+    """One-shot batch into a caller-provided device workspace (a uint8 tensor)."""
+    arr = (OpDesc * len(descs))(*descs)
+    failed = C.c_size_t(0)
+    nbytes = workspace.numel() if workspace is not None else 0
+    code = lib().kleidicv_hip_batch_launch(
+        arr, C.c_size_t(len(descs)), C.byref(tuning) if tuning is not None else None,
+        _ptr(workspace), C.c_size_t(nbytes), _stream_arg(stream), C.byref(failed))
+    if code != OK:
+        err = KleidiCVError(code, "kleidicv_hip_batch_launch (op %d)" % failed.value)
+        err.failed_index = failed.value
+        raise err

@@ -16,6 +16,10 @@ per-image device-API calls, or as individual requests.
                    (rotation <= 15 degrees about the centre, perspective terms
                    <= 1e-4) and, separately, remapped through its own float
                    maps (smooth displacement <= 20 px); bilinear, replicate
+  serving_stream   requests with Poisson arrivals at a fixed rate, ops drawn
+                   uniformly from the served set, frames from a resident
+                   corpus whose sizes are log-uniform from 64x64 to
+                   7680x4320 (one draw sets both axes) with 1, 3 or 4 channels
 """
 
 import ctypes as C
@@ -235,3 +239,40 @@ def run_calls(calls):
     bad = [i for i, (fn, args) in enumerate(calls) if fn(*args) != K.OK]
     if bad:
         raise K.KleidiCVError(K.NOT_IMPLEMENTED, "plain call %d" % bad[0])
+
+
+# ---------------------------------------------------------------------------
+# Serving stream
+def frame_corpus(torch, dev, n=48, seed=9, lo=(64, 64), hi=(7680, 4320)):
+    """Resident frames: even W, H log-uniform between lo and hi, C from
+    {1, 3, 4}; each frame also carries an NV12 chroma plane of its size."""
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    corpus = []
+    for _ in range(n):
+        u = rng.uniform()
+        w = int(round(lo[0] * (hi[0] / lo[0]) ** u)) // 2 * 2
+        h = int(round(lo[1] * (hi[1] / lo[1]) ** u)) // 2 * 2
+        c = int(rng.choice([1, 3, 4]))
+        corpus.append({"src": _u8(torch, gen, dev, (h, w, c)),
+                       "y": _u8(torch, gen, dev, (h, w)),
+                       "uv": _u8(torch, gen, dev, (h // 2, w)),
+                       "T": homography(rng, w, h)})
+    return corpus
+
+
+def serving_requests(corpus, n, qps, seed=10):
+    """n requests with Poisson arrivals at `qps` over the corpus."""
+    import serving as S
+    rng = np.random.default_rng(seed)
+    t = S.poisson_arrivals(rng, n, qps)
+    reqs = []
+    for i in range(n):
+        f = corpus[int(rng.integers(0, len(corpus)))]
+        op = S.OPS[int(rng.integers(0, len(S.OPS)))]
+        if op == "nv12":
+            reqs.append(S.Request(i, op, f["y"], uv=f["uv"], arrival=float(t[i])))
+        else:
+            reqs.append(S.Request(i, op, f["src"], T=f["T"], arrival=float(t[i])))
+    return reqs
