@@ -391,11 +391,14 @@ kleidicv_error_t kleidicv_hip_batch_get_tuning(const kleidicv_hip_batch *batch,
 size_t kleidicv_hip_batch_work_items(const kleidicv_hip_batch *batch);
 
 /* One-shot form for callers that build a new batch per submission (the
- * serving loop): validates the ops, writes their work lists into the
- * caller's device `workspace` and enqueues the stage kernels on `stream`.
- * No device allocation and no wait for `stream`.  The workspace must hold
- * kleidicv_hip_batch_workspace_size() bytes and must not be reused until
- * the launch has finished on `stream`. */
+ * serving loop): validates the ops, copies their work lists into the
+ * caller's device `workspace` (through pinned staging, on `stream`) and
+ * enqueues the stage kernels on `stream`.  No device allocation and no wait
+ * for the device.  The workspace must hold the bytes that
+ * kleidicv_hip_batch_workspace_size() reports (also returned in
+ * *required_bytes; a smaller workspace gives KLEIDICV_ERROR_RANGE and
+ * launches nothing) and must not be reused until the launch has finished on
+ * `stream`. */
 kleidicv_error_t kleidicv_hip_batch_workspace_size(const kleidicv_hip_op_desc *ops,
                                                    size_t count,
                                                    const kleidicv_hip_batch_tuning *tuning,
@@ -403,7 +406,8 @@ kleidicv_error_t kleidicv_hip_batch_workspace_size(const kleidicv_hip_op_desc *o
 kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size_t count,
                                            const kleidicv_hip_batch_tuning *tuning,
                                            void *workspace, size_t workspace_bytes,
-                                           void *stream, size_t *failed_index);
+                                           void *stream, size_t *required_bytes,
+                                           size_t *failed_index);
 
 /* Waits for the device to finish outstanding work, then frees the batch. */
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch);

@@ -229,6 +229,9 @@ def test_every_op_in_one_batch_matches_plain():
         ("resize_up", lambda d: K.op_resize_linear(src3, d),
          lambda d: K.resize_linear(src3, 150, 101, dst=d, stream=stream, device_api=True),
          (101, 150, 3)),
+        ("resize_up2_4ch", lambda d: K.op_resize_linear(src4, d),
+         lambda d: K.resize_linear(src4, 262, 194, dst=d, stream=stream, device_api=True),
+         (194, 262, 4)),
         ("resize_up4_1ch", lambda d: K.op_resize_linear(src1, d),
          lambda d: K.resize_linear(src1, 524, 388, dst=d, stream=stream, device_api=True),
          (388, 524, 1)),

@@ -175,6 +175,22 @@ def test_resize_generic_sweep(rng, c):
             resize_case(rng, sw, sh, dw, dh, c)
 
 
+def test_resize_exact_2x_4ch(rng):
+    # closed-form kernel for the generic path at exactly 2x, 4 channels
+    for sw, sh in [(4, 1), (4, 4), (5, 3), (17, 9), (255, 7), (256, 5), (1023, 3),
+                   (1025, 4), (480, 6)]:
+        resize_case(rng, sw, sh, 2 * sw, 2 * sh, 4)
+    # padded rows on both sides, host and device entry points
+    src = padded_rows(rng, 11, 37, 4, 5)
+    err, want = O.resize_linear(np.ascontiguousarray(src), 74, 22)
+    assert err == 0
+    dst = np.full((22, 74 * 4 + 3), 0xEE, np.uint8)
+    view = dst[:, :74 * 4].reshape(22, 74, 4)
+    K.resize_linear(src, 74, 22, dst=view)
+    assert_same(view, want, "padded 2x")
+    assert np.all(dst[:, 74 * 4:] == 0xEE)
+
+
 @pytest.mark.parametrize("kat", KATS["resize_u8_params"], ids=lambda c: "%dx%d" % (len(c["src"][0]), len(c["src"])))
 def test_resize_kats(kat):
     src = np.array(kat["src"], np.uint8)

@@ -30,13 +30,13 @@ class FakeBackend:
     def workspace(self, nbytes):
         return np.zeros(max(int(nbytes), 1), np.uint8)
 
-    def workspace_size(self, descs, tuning):
-        return 1024 * len(descs)
-
     def launch(self, descs, workspace, stream, tuning, requests):
         assert len(descs) == len(requests)
-        assert workspace.size >= 1024 * len(descs)
+        need = 1024 * len(descs)
+        if workspace is None or workspace.size < need:
+            return need
         self.launches.append([r.rid for r in requests])
+        return 0
 
     def record(self, stream):
         return FakeEvent(self.polls)

@@ -30,7 +30,7 @@ enum Family : int {
 
 // Resize path of a job (rsz::Path as an int so the job stays trivially
 // copyable on both sides).
-enum ResizeKind : int { kRsGeneric = 0, kRsExact2, kRsExact4, kRsQuarter };
+enum ResizeKind : int { kRsGeneric = 0, kRsExact2, kRsExact4, kRsQuarter, kRsUp2x4 };
 
 struct CopyArgs {
   const uint8_t *src;
@@ -89,6 +89,7 @@ constexpr int kFilterTileH = 16;   // rows
 constexpr int kXformTile = 32;     // pixels (remap, warp)
 constexpr int kResizeTileE = 1024;            // destination bytes (staged)
 constexpr int kQuarterTileE = 16 * 256;       // destination bytes (quarter)
+constexpr int kUp2x4TilePx = 4 * 256;         // source pixels (exact 2x, 4 channels)
 constexpr int kYuvTileW = 8 * 256;            // pixels
 constexpr int kCopyTileE = 16 * 256;          // bytes
 constexpr int kTablesPerItem = 256;           // table entries

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -49,7 +51,7 @@ struct SegmentPlan {
 
 kleidicv_hip_batch_tuning default_tuning() {
   kleidicv_hip_batch_tuning t;
-  t.blocks_per_cu = 4;
+  t.blocks_per_cu = 8;
   t.resize_band_rows = 0;
   t.xcd_order = 1;
   t.use_graph = 0;
@@ -128,6 +130,7 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
         case kcv::rsz::Path::kQuarter: p->resize_kind = kcv::kRsQuarter; break;
         case kcv::rsz::Path::kUp2: p->resize_kind = kcv::kRsExact2; break;
         case kcv::rsz::Path::kUp4: p->resize_kind = kcv::kRsExact4; break;
+        case kcv::rsz::Path::kUp2x4: p->resize_kind = kcv::kRsUp2x4; break;
         default: p->resize_kind = kcv::kRsGeneric; break;
       }
       j.u.resize.a = a;
@@ -219,6 +222,11 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
       if (p.resize_kind == kRsQuarter) {
         j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows) : 2;
         j->tiles_x = static_cast<int>(div_up(n, kQuarterTileE));
+      } else if (p.resize_kind == kRsUp2x4) {
+        // one item per row group (two output rows) of 1024 source pixels
+        j->band = 1;
+        j->tiles_x = static_cast<int>(div_up(a.src_width, kUp2x4TilePx));
+        return j->tiles_x * (int64_t{a.src_height} + 1);
       } else {
         j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows)
                                      : rsz::band_rows(a);
@@ -384,6 +392,68 @@ kleidicv_error_t upload(const Plan &plan, void *base) {
   KCV_TRY(from_hip(hipMemcpyAsync(base, plan.blob.data(), plan.blob.size(),
                                   hipMemcpyHostToDevice, up)));
   return from_hip(hipStreamSynchronize(up));
+}
+
+// Pinned staging for one-shot launches: work lists are copied into a ring of
+// pinned host memory and from there to the device on the caller's stream,
+// so a launch never waits for the device.  A region of the ring is reused
+// only after the copy that read it has completed (an event per copy).
+class StagingRing {
+ public:
+  kleidicv_error_t copy(void *dst, const void *src, size_t len, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    if (host_ == nullptr) {
+      if (hipHostMalloc(reinterpret_cast<void **>(&host_), kCapacity, hipHostMallocDefault) !=
+          hipSuccess) {
+        (void)hipGetLastError();
+        host_ = nullptr;
+        return KLEIDICV_ERROR_ALLOCATION;
+      }
+    }
+    if (len > kCapacity) return KLEIDICV_ERROR_RANGE;
+    size_t off = head_;
+    if (off + len > kCapacity) off = 0;
+    // Wait for earlier copies still reading [off, off + len).
+    for (auto it = live_.begin(); it != live_.end();) {
+      if (it->off < off + len && off < it->off + it->len) {
+        KCV_TRY(from_hip(hipEventSynchronize(it->ev)));
+        free_events_.push_back(it->ev);
+        it = live_.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    std::memcpy(host_ + off, src, len);
+    KCV_TRY(from_hip(hipMemcpyAsync(dst, host_ + off, len, hipMemcpyHostToDevice, s)));
+    hipEvent_t ev;
+    if (free_events_.empty()) {
+      KCV_TRY(from_hip(hipEventCreateWithFlags(&ev, hipEventDisableTiming)));
+    } else {
+      ev = free_events_.back();
+      free_events_.pop_back();
+    }
+    KCV_TRY(from_hip(hipEventRecord(ev, s)));
+    live_.push_back(Span{off, len, ev});
+    head_ = off + len;
+    return KLEIDICV_OK;
+  }
+
+ private:
+  static constexpr size_t kCapacity = size_t{16} << 20;
+  struct Span {
+    size_t off, len;
+    hipEvent_t ev;
+  };
+  std::mutex mutex_;
+  char *host_ = nullptr;
+  size_t head_ = 0;
+  std::deque<Span> live_;
+  std::vector<hipEvent_t> free_events_;
+};
+
+StagingRing &staging() {
+  static StagingRing *ring = new StagingRing;  // lives until the process ends
+  return *ring;
 }
 
 kleidicv_error_t enqueue(const Plan &plan, hipStream_t s) {
@@ -573,7 +643,9 @@ kleidicv_error_t kleidicv_hip_batch_workspace_size(const kleidicv_hip_op_desc *o
 kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size_t count,
                                            const kleidicv_hip_batch_tuning *tuning,
                                            void *workspace, size_t workspace_bytes,
-                                           void *stream, size_t *failed_index) {
+                                           void *stream, size_t *required_bytes,
+                                           size_t *failed_index) {
+  if (required_bytes) *required_bytes = 0;
   if (ops == nullptr && count > 0) return KLEIDICV_ERROR_NULL_POINTER;
   const kleidicv_hip_batch_tuning t = tuning ? *tuning : default_tuning();
   if (!valid_tuning(t)) return KLEIDICV_ERROR_RANGE;
@@ -583,12 +655,21 @@ kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size
   KCV_TRY(prepare_all(ops, count, &prep, failed_index));
   Plan plan;
   KCV_TRY(make_plan(prep, t, static_cast<char *>(workspace), &plan));
+  if (required_bytes) *required_bytes = plan.bytes;
   if (plan.bytes == 0) return KLEIDICV_OK;
-  if (workspace == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
   if (workspace_bytes < plan.bytes) return KLEIDICV_ERROR_RANGE;
+  if (workspace == nullptr) return KLEIDICV_ERROR_NULL_POINTER;
   if (!kcv::is_device_pointer(workspace)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
-  KCV_TRY(upload(plan, workspace));
-  return enqueue(plan, static_cast<hipStream_t>(stream));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (!plan.blob.empty()) {
+    const kleidicv_error_t e = staging().copy(workspace, plan.blob.data(), plan.blob.size(), s);
+    if (e == KLEIDICV_ERROR_RANGE) {
+      KCV_TRY(upload(plan, workspace));  // larger than the ring: synchronous copy
+    } else {
+      KCV_TRY(e);
+    }
+  }
+  return enqueue(plan, s);
 }
 
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch) {

@@ -7,45 +7,73 @@ namespace {
 
 using namespace filt;
 
+// Tile (bx, by) of this workgroup: XCD-ordered over a 1-D grid of
+// tiles_x * tiles_y workgroups, or the 2-D grid position when tiles_x == 0.
+__device__ __forceinline__ void tile_of(int tiles_x, int &bx, int &by) {
+  if (tiles_x == 0) {
+    bx = blockIdx.x;
+    by = blockIdx.y;
+    return;
+  }
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  by = t / tiles_x;
+  bx = t - by * tiles_x;
+}
+
 template <int MODE, int K>
-__global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a) {
+__global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a, int tiles_x) {
   __shared__ __align__(16) uint8_t smem[separable_smem_bytes<K>()];
-  separable_tile<MODE, K>(a, blockIdx.x, blockIdx.y, smem);
+  int bx, by;
+  tile_of(tiles_x, bx, by);
+  separable_tile<MODE, K>(a, bx, by, smem);
 }
 
 template <int K>
-__global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a) {
+__global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a, int tiles_x) {
   __shared__ __align__(16) uint8_t smem[median_smem_bytes<K>()];
-  median_tile<K>(a, blockIdx.x, blockIdx.y, smem);
+  int bx, by;
+  tile_of(tiles_x, bx, by);
+  median_tile<K>(a, bx, by, smem);
 }
 
-dim3 tile_grid(int width, int height, int channels) {
-  return dim3((width * channels + kTileW - 1) / kTileW,
-              (height + kTileH - 1) / kTileH);
+// Grid and tiles_x argument for an image: a 1-D XCD-ordered grid unless the
+// tile count is very large.
+struct TileGrid {
+  dim3 grid;
+  int tiles_x;
+};
+
+TileGrid tile_grid(int width, int height, int channels) {
+  const int tx = (width * channels + kTileW - 1) / kTileW;
+  const int ty = (height + kTileH - 1) / kTileH;
+  if (int64_t{tx} * ty <= kMaxLinearTiles) return {dim3(tx * ty), tx};
+  return {dim3(tx, ty), 0};
 }
 
 template <int MODE>
 hipError_t launch_mode(const SepArgs &a, hipStream_t s) {
-  const dim3 grid = tile_grid(a.width, a.height, a.channels);
+  const TileGrid g = tile_grid(a.width, a.height, a.channels);
+  const dim3 grid = g.grid;
+  const int tx = g.tiles_x;
   switch (a.ksize) {
-    case 3: separable_kernel<MODE, 3><<<grid, kThreads, 0, s>>>(a); break;
-    case 5: separable_kernel<MODE, 5><<<grid, kThreads, 0, s>>>(a); break;
-    case 7: separable_kernel<MODE, 7><<<grid, kThreads, 0, s>>>(a); break;
+    case 3: separable_kernel<MODE, 3><<<grid, kThreads, 0, s>>>(a, tx); break;
+    case 5: separable_kernel<MODE, 5><<<grid, kThreads, 0, s>>>(a, tx); break;
+    case 7: separable_kernel<MODE, 7><<<grid, kThreads, 0, s>>>(a, tx); break;
     case 9:
       if constexpr (MODE != kSepSaturate) {
-        separable_kernel<MODE, 9><<<grid, kThreads, 0, s>>>(a);
+        separable_kernel<MODE, 9><<<grid, kThreads, 0, s>>>(a, tx);
         break;
       }
       return hipErrorInvalidValue;
     case 15:
       if constexpr (MODE == kSepSigma) {
-        separable_kernel<MODE, 15><<<grid, kThreads, 0, s>>>(a);
+        separable_kernel<MODE, 15><<<grid, kThreads, 0, s>>>(a, tx);
         break;
       }
       return hipErrorInvalidValue;
     case 21:
       if constexpr (MODE == kSepSigma) {
-        separable_kernel<MODE, 21><<<grid, kThreads, 0, s>>>(a);
+        separable_kernel<MODE, 21><<<grid, kThreads, 0, s>>>(a, tx);
         break;
       }
       return hipErrorInvalidValue;
@@ -68,11 +96,11 @@ hipError_t launch_separable(const SepArgs &a, hipStream_t stream) {
 
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
-  const dim3 grid = tile_grid(a.width, a.height, a.channels);
+  const TileGrid g = tile_grid(a.width, a.height, a.channels);
   switch (a.ksize) {
-    case 3: median_kernel<3><<<grid, kThreads, 0, stream>>>(a); break;
-    case 5: median_kernel<5><<<grid, kThreads, 0, stream>>>(a); break;
-    case 7: median_kernel<7><<<grid, kThreads, 0, stream>>>(a); break;
+    case 3: median_kernel<3><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
+    case 5: median_kernel<5><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
+    case 7: median_kernel<7><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

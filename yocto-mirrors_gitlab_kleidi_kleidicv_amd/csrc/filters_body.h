@@ -60,38 +60,41 @@ struct Window {
 };
 
 // Stages rows [y0-R, y0+kTileH+R) x bytes [e0-pad, e0+kTileW+pad) of the
-// border-extended image into `win` (row pitch Window<R>::kCols).
+// border-extended image into `win` (row pitch Window<R>::kCols).  Rows are
+// mapped through the border first, so every 16-byte chunk that lies inside
+// the row is one 16-byte load (edge tiles included); only chunks that cross
+// the left or right edge are assembled byte by byte.
 template <int R>
 __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
                                              size_t stride, int width,
                                              int height, int channels,
                                              int border, int e0, int y0) {
   using G = Window<R>;
+  constexpr int kChunks = G::kCols / 16;
   const int row_bytes = width * channels;
   const int c0 = e0 - G::kPad;
-  const bool interior =
-      c0 >= 0 && c0 + G::kCols <= row_bytes && y0 - R >= 0 &&
-      y0 + kTileH + R <= height &&
-      ((reinterpret_cast<uintptr_t>(src) | stride) & 15) == 0;
-  if (interior) {
-    constexpr int kChunks = G::kCols / 16;
-    const uint8_t *base = src + static_cast<size_t>(y0 - R) * stride + c0;
-    for (int i = threadIdx.x; i < G::kRows * kChunks; i += kThreads) {
-      const int r = i / kChunks, q = i - r * kChunks;
-      const uint4 v = *reinterpret_cast<const uint4 *>(
-          base + static_cast<size_t>(r) * stride + q * 16);
-      *reinterpret_cast<uint4 *>(win + r * G::kCols + q * 16) = v;
+  const bool aligned = ((reinterpret_cast<uintptr_t>(src) | stride) & 15) == 0;
+  for (int i = threadIdx.x; i < G::kRows * kChunks; i += kThreads) {
+    const int r = i / kChunks, q = i - r * kChunks;
+    const int y = map_index(y0 - R + r, height, border);
+    const uint8_t *srow = src + static_cast<size_t>(y) * stride;
+    const int e = c0 + 16 * q;
+    uint4 v;
+    if (aligned && e >= 0 && e + 16 <= row_bytes) {
+      v = *reinterpret_cast<const uint4 *>(srow + e);
+    } else {
+      uint32_t w[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (int b = 0; b < 16; ++b) {
+        const int eb = e + b;
+        const int x = eb >= 0 ? eb / channels : -((channels - 1 - eb) / channels);
+        const int c = eb - x * channels;
+        const int xm = map_index(x, width, border);
+        w[b >> 2] |= static_cast<uint32_t>(srow[xm * channels + c]) << (8 * (b & 3));
+      }
+      v = make_uint4(w[0], w[1], w[2], w[3]);
     }
-  } else {
-    for (int i = threadIdx.x; i < G::kRows * G::kCols; i += kThreads) {
-      const int r = i / G::kCols, w = i - r * G::kCols;
-      const int y = map_index(y0 - R + r, height, border);
-      const int e = c0 + w;
-      const int x = e >= 0 ? e / channels : -((channels - 1 - e) / channels);
-      const int c = e - x * channels;
-      const int xm = map_index(x, width, border);
-      win[i] = src[static_cast<size_t>(y) * stride + xm * channels + c];
-    }
+    *reinterpret_cast<uint4 *>(win + r * G::kCols + 16 * q) = v;
   }
 }
 
@@ -108,9 +111,86 @@ __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
 
 // ---------------------------------------------------------------------------
 // Separable integer filters.
+//
+// Vertical pass: each work item takes one dword (4 bytes) of K window rows
+// and accumulates in 16-bit halves (bytes 0/2 and 1/3 side by side), then
+// writes the four column sums as u16 to LDS with one 8-byte store.  Every
+// vertical sum fits 16 bits: binomial and sigma taps sum to at most 256 and
+// the saturating mode clamps at 65535, which leaves min(255, ...) unchanged
+// (a clamped column only occurs where some row product already exceeds 255,
+// and a zero horizontal tap ignores it either way).
+//
+// Horizontal pass: a thread produces 8 consecutive output bytes of one row.
+// It reads the span of column sums those outputs need with 16-byte LDS
+// loads (neighbouring lanes read neighbouring chunks: no bank conflicts) and
+// accumulates in 32 bits with compile-time tap offsets (channel count is a
+// template parameter).
+typedef unsigned short sep_us2 __attribute__((ext_vector_type(2)));
+
+template <int K>
+constexpr int separable_win_bytes() {
+  return (Window<K / 2>::kRows * Window<K / 2>::kCols + 15) / 16 * 16;
+}
+
 template <int K>
 constexpr int separable_smem_bytes() {
-  return Window<K / 2>::kRows * Window<K / 2>::kCols + 4 * kTileH * Window<K / 2>::kCols;
+  return separable_win_bytes<K>() + 2 * kTileH * Window<K / 2>::kCols;
+}
+
+template <int MODE, int K, int C>
+__device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uint16_t *vert,
+                                                     int e0, int y0) {
+  constexpr int R = K / 2;
+  using G = Window<R>;
+  constexpr int RC = R * C;
+  constexpr int D = (RC + 7) / 8;         // 8-value chunks each side
+  constexpr int NV = 8 * (2 * D + 1);      // column sums held per thread
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int row_bytes = a.width * C;
+  uint32_t w[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) w[j] = static_cast<uint32_t>(a.wx[j]);
+#pragma unroll
+  for (int pass = 0; pass < kTileH / 8; ++pass) {
+    const int r = ty + 8 * pass;
+    const int y = y0 + r;
+    if (y >= a.height) break;
+    const uint4 *row = reinterpret_cast<const uint4 *>(vert + r * G::kCols);
+    uint32_t v[NV / 2];
+#pragma unroll
+    for (int d = 0; d < 2 * D + 1; ++d) {
+      const uint4 q = row[tx + G::kPad / 8 + d - D];
+      v[4 * d] = q.x;
+      v[4 * d + 1] = q.y;
+      v[4 * d + 2] = q.z;
+      v[4 * d + 3] = q.w;
+    }
+    auto col = [&](int i) -> uint32_t { return (v[i >> 1] >> (16 * (i & 1))) & 0xFFFFu; };
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      uint32_t acc = 0;
+#pragma unroll
+      for (int j = 0; j < K; ++j) acc += w[j] * col(8 * D + b + (j - R) * C);
+      uint32_t out;
+      if (MODE == kSepBinomial)
+        out = (acc + (1u << (a.shift - 1))) >> a.shift;
+      else if (MODE == kSepSigma)
+        out = (acc + 128u) >> 8;
+      else
+        out = min(acc, 255u);
+      if (b < 4) lo |= (out & 0xFFu) << (8 * b);
+      else hi |= (out & 0xFFu) << (8 * (b - 4));
+    }
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
+    const int e = e0 + 8 * tx;
+    if (e + 7 < row_bytes && (reinterpret_cast<uintptr_t>(out + e) & 7) == 0) {
+      *reinterpret_cast<uint2 *>(out + e) = make_uint2(lo, hi);
+    } else {
+      store_quad(out, e, row_bytes, lo);
+      store_quad(out, e + 4, row_bytes, hi);
+    }
+  }
 }
 
 // One 256-byte x 16-row output tile at tile coordinates (bx, by).  `smem`
@@ -121,7 +201,7 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   constexpr int R = K / 2;
   using G = Window<R>;
   uint8_t *win = smem;
-  uint32_t *vert = reinterpret_cast<uint32_t *>(smem + G::kRows * G::kCols);
+  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<K>());
 
   const int e0 = bx * kTileW;
   const int y0 = by * kTileH;
@@ -131,55 +211,46 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
                   y0);
   __syncthreads();
 
-  // Vertical pass: four columns per work item, one LDS dword per tap row.
+  // Vertical pass: one window dword per tap row, two 16-bit lanes per op.
   constexpr int kQuads = G::kCols / 4;
+  const uint32_t *win32 = reinterpret_cast<const uint32_t *>(win);
+  sep_us2 wy[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    wy[k] = sep_us2{static_cast<uint16_t>(a.wy[k]), static_cast<uint16_t>(a.wy[k])};
   for (int i = threadIdx.x; i < kTileH * kQuads; i += kThreads) {
     const int r = i / kQuads, q = i - r * kQuads;
-    uint32_t acc[4] = {0, 0, 0, 0};
+    sep_us2 even = sep_us2{0, 0}, odd = sep_us2{0, 0};  // bytes 0,2 and 1,3
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const uint32_t v =
-          *reinterpret_cast<const uint32_t *>(win + (r + k) * G::kCols + 4 * q);
-      const uint32_t w = static_cast<uint32_t>(a.wy[k]);
-#pragma unroll
-      for (int b = 0; b < 4; ++b) acc[b] += w * ((v >> (8 * b)) & 0xFFu);
+      const uint32_t v = win32[(r + k) * kQuads + q];
+      const sep_us2 e = __builtin_bit_cast(sep_us2, v & 0x00FF00FFu);
+      const sep_us2 o = __builtin_bit_cast(sep_us2, (v >> 8) & 0x00FF00FFu);
+      if (MODE == kSepSaturate) {
+        even = __builtin_elementwise_add_sat(even, e * wy[k]);
+        odd = __builtin_elementwise_add_sat(odd, o * wy[k]);
+      } else {
+        even += e * wy[k];
+        odd += o * wy[k];
+      }
     }
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      uint32_t v = acc[b];
-      if (MODE == kSepSigma) v = (v + 128u) >> 8;
-      vert[r * G::kCols + 4 * q + b] = v;
+    if (MODE == kSepSigma) {
+      even = (even + sep_us2{128, 128}) >> 8;
+      odd = (odd + sep_us2{128, 128}) >> 8;
     }
+    const uint32_t ev = __builtin_bit_cast(uint32_t, even);
+    const uint32_t od = __builtin_bit_cast(uint32_t, odd);
+    // column order 0, 1, 2, 3 = even.x, odd.x, even.y, odd.y
+    *reinterpret_cast<uint2 *>(vert + r * G::kCols + 4 * q) =
+        make_uint2((ev & 0xFFFFu) | (od << 16), (ev >> 16) | (od & 0xFFFF0000u));
   }
   __syncthreads();
 
-  // Horizontal pass: each thread writes 4 bytes in each of 4 rows.
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int row_bytes = a.width * C;
-#pragma unroll
-  for (int rr = 0; rr < kTileH / 4; ++rr) {
-    const int r = ty + 4 * rr;
-    const int y = y0 + r;
-    if (y >= a.height) break;
-    uint32_t packed = 0;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int base = r * G::kCols + G::kPad + 4 * tx + b;
-      uint32_t acc = 0;
-#pragma unroll
-      for (int j = 0; j < K; ++j)
-        acc += static_cast<uint32_t>(a.wx[j]) * vert[base + (j - R) * C];
-      uint32_t out;
-      if (MODE == kSepBinomial)
-        out = (acc + (1u << (a.shift - 1))) >> a.shift;
-      else if (MODE == kSepSigma)
-        out = (acc + 128u) >> 8;
-      else
-        out = min(acc, 255u);
-      packed |= (out & 0xFFu) << (8 * b);
-    }
-    store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
-               row_bytes, packed);
+  switch (C) {
+    case 1: separable_horizontal<MODE, K, 1>(a, vert, e0, y0); break;
+    case 2: separable_horizontal<MODE, K, 2>(a, vert, e0, y0); break;
+    case 3: separable_horizontal<MODE, K, 3>(a, vert, e0, y0); break;
+    default: separable_horizontal<MODE, K, 4>(a, vert, e0, y0); break;
   }
 }
 

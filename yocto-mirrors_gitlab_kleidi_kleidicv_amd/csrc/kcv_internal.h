@@ -26,6 +26,19 @@ enum SepMode : int {
   kSepSaturate = 2,   // min(255, sum) with unsigned taps, no shift
 };
 
+// Tile order for 1-D grids: the hardware dispatches workgroup b to XCD
+// b % 8, so tile index = this XCD's contiguous slice of the row-major tile
+// list.  Neighbouring tiles (and their shared halo lines) then stay in one
+// XCD's L2 instead of being fetched once per XCD.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int x = b & 7, slot = b >> 3;
+  const int per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + slot;
+}
+
+// Largest 1-D grid used with xcd_tile (blocks of 256 threads).
+constexpr int64_t kMaxLinearTiles = int64_t{1} << 23;
+
 struct SepArgs {
   const uint8_t *src;
   size_t src_stride;

@@ -25,6 +25,10 @@ __global__ __launch_bounds__(kThreads) void resize_staged_kernel(
   resize_staged_tile<MODE>(a, cols, rows, band, blockIdx.x, blockIdx.y, smem);
 }
 
+__global__ __launch_bounds__(kThreads) void resize_up2x4_kernel(ResizeArgs a) {
+  for (int g = blockIdx.y; g <= a.src_height; g += gridDim.y) up2x4_group(a, blockIdx.x, g);
+}
+
 __global__ __launch_bounds__(kThreads) void resize_quarter_kernel(ResizeArgs a) {
   resize_quarter_rows(a, blockIdx.x, blockIdx.y, a.dst_height, gridDim.y);
 }
@@ -44,7 +48,7 @@ hipError_t launch_staged(const ResizeArgs &a, const int2 *cols, const int2 *rows
 
 size_t resize_workspace_bytes(const ResizeArgs &a) {
   const Path p = pick(a);
-  if (p == Path::kNone || p == Path::kQuarter) return 0;
+  if (p == Path::kNone || p == Path::kQuarter || p == Path::kUp2x4) return 0;
   return (static_cast<size_t>(a.dst_width) * a.channels + a.dst_height) *
          sizeof(int2);
 }
@@ -58,6 +62,13 @@ hipError_t launch_resize(const ResizeArgs &a, void *workspace,
     const dim3 grid(static_cast<unsigned>((n + 16 * kThreads - 1) / (16 * kThreads)),
                     static_cast<unsigned>(a.dst_height < 65535 ? a.dst_height : 65535));
     resize_quarter_kernel<<<grid, kThreads, 0, stream>>>(a);
+    return hipGetLastError();
+  }
+  if (path == Path::kUp2x4) {
+    const int groups = a.src_height + 1;
+    const dim3 grid(static_cast<unsigned>((a.src_width + 4 * kThreads - 1) / (4 * kThreads)),
+                    static_cast<unsigned>(groups < 65535 ? groups : 65535));
+    resize_up2x4_kernel<<<grid, kThreads, 0, stream>>>(a);
     return hipGetLastError();
   }
   const int64_t n = int64_t{a.dst_width} * a.channels;

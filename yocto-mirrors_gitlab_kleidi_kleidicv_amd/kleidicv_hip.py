@@ -457,14 +457,23 @@ def batch_workspace_size(descs, tuning=None):
 
 
 def batch_launch(descs, workspace, stream=None, tuning=None):
-    """One-shot batch into a caller-provided device workspace (a uint8 tensor)."""
+    """One-shot batch into a caller-provided device workspace (a uint8 tensor).
+
+    Returns 0 when launched; when the workspace is too small nothing is
+    launched and the required byte count is returned instead.
+    """
     arr = (OpDesc * len(descs))(*descs)
     failed = C.c_size_t(0)
+    need = C.c_size_t(0)
     nbytes = workspace.numel() if workspace is not None else 0
     code = lib().kleidicv_hip_batch_launch(
         arr, C.c_size_t(len(descs)), C.byref(tuning) if tuning is not None else None,
-        _ptr(workspace), C.c_size_t(nbytes), _stream_arg(stream), C.byref(failed))
+        _ptr(workspace), C.c_size_t(nbytes), _stream_arg(stream), C.byref(need),
+        C.byref(failed))
+    if code == RANGE and need.value > nbytes:
+        return need.value
     if code != OK:
         err = KleidiCVError(code, "kleidicv_hip_batch_launch (op %d)" % failed.value)
         err.failed_index = failed.value
         raise err
+    return 0

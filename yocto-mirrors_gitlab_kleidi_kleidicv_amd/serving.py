@@ -108,11 +108,9 @@ class TorchBackend:
         return self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8,
                                 device=self.device)
 
-    def workspace_size(self, descs, tuning):
-        return K.batch_workspace_size(descs, tuning)
-
     def launch(self, descs, workspace, stream, tuning, requests):
-        K.batch_launch(descs, workspace, stream, tuning)
+        """Launches, or returns the workspace bytes needed (nothing launched)."""
+        return K.batch_launch(descs, workspace, stream, tuning)
 
     def record(self, stream):
         ev = self.torch.cuda.Event()
@@ -198,25 +196,26 @@ class Server:
             reqs = []
             while self.ready and len(reqs) < self.max_batch_ops:
                 reqs.append(self.ready.popleft())
-            # A slot's workspace lives in its reserve: halve the batch until
-            # its work lists fit, putting the rest back in order.
+            # A slot's workspace lives in its reserve.  Launch into it; when
+            # the work lists need more, grow it within the reserve or halve
+            # the batch (the rest goes back to the front of the queue).
+            descs = [r.desc() for r in reqs]
             while True:
-                descs = [r.desc() for r in reqs]
-                need = self.be.workspace_size(descs, self.tuning)
-                if need <= self.reserve:
+                need = self.be.launch(descs, self.ws[i], self.streams[i], self.tuning, reqs)
+                if need == 0:
                     break
+                if need <= self.reserve:
+                    grow = min(max(need, 2 * self.ws_bytes[i]), self.reserve)
+                    self.ws[i] = self.be.workspace(grow)
+                    self.ws_bytes[i] = grow
+                    self.peak = max(self.peak, self.in_use)
+                    continue
                 if len(reqs) == 1:
                     raise ValueError("work lists of request %d exceed the workspace reserve"
                                      % reqs[0].rid)
                 keep = len(reqs) // 2
                 self.ready.extendleft(reversed(reqs[keep:]))
-                reqs = reqs[:keep]
-            if need > self.ws_bytes[i]:
-                grow = min(max(need, 2 * self.ws_bytes[i]), self.reserve)
-                self.ws[i] = self.be.workspace(grow)
-                self.ws_bytes[i] = grow
-                self.peak = max(self.peak, self.in_use)
-            self.be.launch(descs, self.ws[i], self.streams[i], self.tuning, reqs)
+                reqs, descs = reqs[:keep], descs[:keep]
             ev = self.be.record(self.streams[i])
             self.batches += 1
             for r in reqs:
