@@ -66,6 +66,38 @@ __device__ __forceinline__ uint32_t byte_of(uint32_t v, int c) {
   return (v >> (8 * c)) & 0xFFu;
 }
 
+struct __attribute__((aligned(4))) U32x2 {
+  uint32_t x, y;
+};
+
+// The two horizontal taps of one source row, pixel x0 and (when `adjacent`)
+// pixel x0 + 1, else x0 again, with one load where possible: an 8-byte load
+// of both 4-channel pixels, or for 1 channel the 8 bytes from the dword
+// holding x0.  `tail_ok` says those 8 bytes lie inside the image: within
+// this row, or (rows below this one exist and the width is at least 8)
+// running into the next row.
+template <int C>
+__device__ __forceinline__ void load_pair(const uint8_t *row, uint32_t x0, bool adjacent,
+                                          bool tail_ok, bool aligned4, uint32_t &pa,
+                                          uint32_t &pb) {
+  if (C == 4 && aligned4 && adjacent) {
+    const U32x2 v = *reinterpret_cast<const U32x2 *>(row + 4 * static_cast<size_t>(x0));
+    pa = v.x;
+    pb = v.y;
+    return;
+  }
+  if (C == 1 && aligned4 && tail_ok) {
+    const U32x2 v = *reinterpret_cast<const U32x2 *>(row + (x0 & ~3u));
+    const uint64_t q = (static_cast<uint64_t>(v.y) << 32) | v.x;
+    const uint32_t sh = 8 * (x0 & 3u);
+    pa = static_cast<uint32_t>(q >> sh) & 0xFFu;
+    pb = adjacent ? static_cast<uint32_t>(q >> (sh + 8)) & 0xFFu : pa;
+    return;
+  }
+  pa = load_px<C>(row + static_cast<size_t>(x0) * C, aligned4);
+  pb = adjacent ? load_px<C>(row + static_cast<size_t>(x0 + 1) * C, aligned4) : pa;
+}
+
 // ---------------------------------------------------------------------------
 // remap s16point5
 __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
@@ -118,13 +150,13 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
         if (mx < 0) fx[i] = 0;
         if (my < 0) fy[i] = 0;
         const int x0 = min(max(mx, 0), xmax), y0 = min(max(my, 0), ymax);
-        const int x1 = x0 + (x0 < xmax ? 1 : 0), y1 = y0 + (y0 < ymax ? 1 : 0);
+        const int y1 = y0 + (y0 < ymax ? 1 : 0);
+        const bool adj = x0 < xmax;
+        const bool tail = (x0 & ~3) + 7 <= xmax;
         const uint8_t *r0 = a.src + static_cast<size_t>(y0) * a.src_stride;
         const uint8_t *r1 = a.src + static_cast<size_t>(y1) * a.src_stride;
-        pa[i] = load_px<C>(r0 + x0 * C, src_al);
-        pb[i] = load_px<C>(r0 + x1 * C, src_al);
-        pc[i] = load_px<C>(r1 + x0 * C, src_al);
-        pd[i] = load_px<C>(r1 + x1 * C, src_al);
+        load_pair<C>(r0, x0, adj, tail || (y0 < ymax && xmax >= 7), src_al, pa[i], pb[i]);
+        load_pair<C>(r1, x0, adj, tail || (y1 < ymax && xmax >= 7), src_al, pc[i], pd[i]);
       } else {
         // RemapS16Point5ConstantBorder (:321-425) / get_coordinates_constant
         // (:860-881): each corner tests its own range with 16-bit unsigned
@@ -136,12 +168,27 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
         const bool x1in = ux1 < static_cast<uint32_t>(a.src_width);
         const bool y0in = uy0 < static_cast<uint32_t>(a.src_height);
         const bool y1in = uy1 < static_cast<uint32_t>(a.src_height);
-        const uint8_t *r0 = a.src + static_cast<size_t>(y0in ? uy0 : 0) * a.src_stride;
-        const uint8_t *r1 = a.src + static_cast<size_t>(y1in ? uy1 : 0) * a.src_stride;
-        pa[i] = (x0in && y0in) ? load_px<C>(r0 + ux0 * C, src_al) : bv;
-        pb[i] = (x1in && y0in) ? load_px<C>(r0 + ux1 * C, src_al) : bv;
-        pc[i] = (x0in && y1in) ? load_px<C>(r1 + ux0 * C, src_al) : bv;
-        pd[i] = (x1in && y1in) ? load_px<C>(r1 + ux1 * C, src_al) : bv;
+        const uint32_t ry0 = y0in ? uy0 : 0, ry1 = y1in ? uy1 : 0;
+        const uint8_t *r0 = a.src + static_cast<size_t>(ry0) * a.src_stride;
+        const uint8_t *r1 = a.src + static_cast<size_t>(ry1) * a.src_stride;
+        if (x0in && x1in) {
+          // both columns inside: one paired load per row that is inside
+          const bool tail = (ux0 & ~3u) + 7 <= static_cast<uint32_t>(xmax);
+          uint32_t t0, t1, b0, b1;
+          load_pair<C>(r0, ux0, true, tail || (ry0 < static_cast<uint32_t>(ymax) && xmax >= 7),
+                      src_al, t0, t1);
+          load_pair<C>(r1, ux0, true, tail || (ry1 < static_cast<uint32_t>(ymax) && xmax >= 7),
+                      src_al, b0, b1);
+          pa[i] = y0in ? t0 : bv;
+          pb[i] = y0in ? t1 : bv;
+          pc[i] = y1in ? b0 : bv;
+          pd[i] = y1in ? b1 : bv;
+        } else {
+          pa[i] = (x0in && y0in) ? load_px<C>(r0 + ux0 * C, src_al) : bv;
+          pb[i] = (x1in && y0in) ? load_px<C>(r0 + ux1 * C, src_al) : bv;
+          pc[i] = (x0in && y1in) ? load_px<C>(r1 + ux0 * C, src_al) : bv;
+          pd[i] = (x1in && y1in) ? load_px<C>(r1 + ux1 * C, src_al) : bv;
+        }
       }
     }
 #pragma unroll
@@ -218,22 +265,34 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
     const uint32_t y1 = y0 + ((yf >= 0.0f && y0 < s.ymax) ? 1u : 0u);
     const uint8_t *r0 = s.p + static_cast<size_t>(y0) * s.stride;
     const uint8_t *r1 = s.p + static_cast<size_t>(y1) * s.stride;
-    q.a = load_px<C>(r0 + static_cast<size_t>(x0) * C, s.aligned4);
-    q.b = load_px<C>(r0 + static_cast<size_t>(x1) * C, s.aligned4);
-    q.c = load_px<C>(r1 + static_cast<size_t>(x0) * C, s.aligned4);
-    q.d = load_px<C>(r1 + static_cast<size_t>(x1) * C, s.aligned4);
+    const bool adj = x1 != x0;
+    const bool tail = (x0 & ~3u) + 7 <= s.xmax;
+    load_pair<C>(r0, x0, adj, tail || (y0 < s.ymax && s.xmax >= 7), s.aligned4, q.a, q.b);
+    load_pair<C>(r1, x0, adj, tail || (y1 < s.ymax && s.xmax >= 7), s.aligned4, q.c, q.d);
   } else {
     const uint32_t x0 = static_cast<uint32_t>(cvtm_s32(xf));
     const uint32_t y0 = static_cast<uint32_t>(cvtm_s32(yf));
     const uint32_t x1 = x0 + 1u, y1 = y0 + 1u;
     const bool x0in = x0 <= s.xmax, x1in = x1 <= s.xmax;
     const bool y0in = y0 <= s.ymax, y1in = y1 <= s.ymax;
-    const uint8_t *r0 = s.p + static_cast<size_t>(y0in ? y0 : 0) * s.stride;
-    const uint8_t *r1 = s.p + static_cast<size_t>(y1in ? y1 : 0) * s.stride;
-    q.a = (x0in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
-    q.b = (x1in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
-    q.c = (x0in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
-    q.d = (x1in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
+    const uint32_t ry0 = y0in ? y0 : 0, ry1 = y1in ? y1 : 0;
+    const uint8_t *r0 = s.p + static_cast<size_t>(ry0) * s.stride;
+    const uint8_t *r1 = s.p + static_cast<size_t>(ry1) * s.stride;
+    if (x0in && x1in) {
+      const bool tail = (x0 & ~3u) + 7 <= s.xmax;
+      uint32_t t0, t1, b0, b1;
+      load_pair<C>(r0, x0, true, tail || (ry0 < s.ymax && s.xmax >= 7), s.aligned4, t0, t1);
+      load_pair<C>(r1, x0, true, tail || (ry1 < s.ymax && s.xmax >= 7), s.aligned4, b0, b1);
+      q.a = y0in ? t0 : s.bv;
+      q.b = y0in ? t1 : s.bv;
+      q.c = y1in ? b0 : s.bv;
+      q.d = y1in ? b1 : s.bv;
+    } else {
+      q.a = (x0in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
+      q.b = (x1in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
+      q.c = (x0in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
+      q.d = (x1in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
+    }
   }
   return q;
 }
