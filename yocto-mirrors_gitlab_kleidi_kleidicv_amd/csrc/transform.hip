@@ -7,36 +7,57 @@ namespace {
 
 using namespace xform;
 
-// The grid's y dimension is capped at 65535 rows of tiles; blocks stride
-// over the rest.
+// Tiles of this workgroup: one XCD-ordered tile of a 1-D grid (neighbouring
+// tiles, whose source footprints overlap, then share one XCD's L2), or with
+// tiles_x == 0 a 2-D grid whose y dimension is capped at 65535 rows of
+// tiles, blocks striding over the rest.  (The kernel argument is passed on
+// by value: taking its address would copy it to scratch memory.)
+#define KCV_FOR_TILES(TILES_X, DST_HEIGHT, CALL)                      \
+  do {                                                                \
+    if ((TILES_X) > 0) {                                              \
+      const int t_ = xcd_tile(blockIdx.x, gridDim.x);                 \
+      const int by = t_ / (TILES_X);                                  \
+      const int bx = t_ - by * (TILES_X);                             \
+      CALL;                                                           \
+    } else {                                                          \
+      const int bx = blockIdx.x;                                      \
+      for (int by = blockIdx.y; by * kBlockPx < (DST_HEIGHT); by += gridDim.y) CALL; \
+    }                                                                 \
+  } while (0)
+
 template <int C, int BORDER>
-__global__ __launch_bounds__(kThreads) void remap_s16point5_kernel(RemapS16Args a) {
-  for (int by = blockIdx.y; by * kBlockPx < a.dst_height; by += gridDim.y)
-    remap_s16point5_tile<C, BORDER>(a, blockIdx.x, by);
+__global__ __launch_bounds__(kThreads) void remap_s16point5_kernel(RemapS16Args a,
+                                                                   int tiles_x) {
+  KCV_FOR_TILES(tiles_x, a.dst_height, (remap_s16point5_tile<C, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void remap_f32_kernel(RemapF32Args a) {
-  for (int by = blockIdx.y; by * kBlockPx < a.dst_height; by += gridDim.y)
-    remap_f32_tile<C, INTERP, BORDER>(a, blockIdx.x, by);
+__global__ __launch_bounds__(kThreads) void remap_f32_kernel(RemapF32Args a, int tiles_x) {
+  KCV_FOR_TILES(tiles_x, a.dst_height, (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void warp_kernel(WarpArgs a) {
-  for (int by = blockIdx.y; by * kBlockPx < a.dst_height; by += gridDim.y)
-    warp_tile<C, INTERP, BORDER>(a, blockIdx.x, by);
+__global__ __launch_bounds__(kThreads) void warp_kernel(WarpArgs a, int tiles_x) {
+  KCV_FOR_TILES(tiles_x, a.dst_height, (warp_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
-dim3 tile_grid(int width, int height) {
-  const int gy = (height + kBlockPx - 1) / kBlockPx;
-  return dim3((width + kBlockPx - 1) / kBlockPx, gy < 65535 ? gy : 65535);
+struct TileGrid {
+  dim3 grid;
+  int tiles_x;
+};
+
+TileGrid tile_grid(int width, int height) {
+  const int tx = (width + kBlockPx - 1) / kBlockPx;
+  const int ty = (height + kBlockPx - 1) / kBlockPx;
+  if (int64_t{tx} * ty <= kMaxLinearTiles) return {dim3(tx * ty), tx};
+  return {dim3(tx, ty < 65535 ? ty : 65535), 0};
 }
 
-#define KCV_FLOAT_DISPATCH(KERNEL, ARGS, C, INTERP, BORDER, GRID, STREAM)      \
+#define KCV_FLOAT_DISPATCH(KERNEL, ARGS, TX, C, INTERP, BORDER, GRID, STREAM)  \
   do {                                                                          \
     auto go = [&](auto c_tag, auto i_tag, auto b_tag) {                         \
       KERNEL<decltype(c_tag)::value, decltype(i_tag)::value,                   \
-             decltype(b_tag)::value><<<GRID, kThreads, 0, STREAM>>>(ARGS);      \
+             decltype(b_tag)::value><<<GRID, kThreads, 0, STREAM>>>(ARGS, TX);  \
     };                                                                          \
     auto with_b = [&](auto c_tag, auto i_tag) {                                 \
       if ((BORDER) == kReplicate)                                               \
@@ -62,33 +83,34 @@ dim3 tile_grid(int width, int height) {
 
 hipError_t launch_remap_s16point5(const RemapS16Args &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const dim3 grid = tile_grid(a.dst_width, a.dst_height);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
   if (a.channels == 1) {
     if (a.border == kReplicate)
-      remap_s16point5_kernel<1, kReplicate><<<grid, kThreads, 0, stream>>>(a);
+      remap_s16point5_kernel<1, kReplicate><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x);
     else
-      remap_s16point5_kernel<1, kConstant><<<grid, kThreads, 0, stream>>>(a);
+      remap_s16point5_kernel<1, kConstant><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x);
   } else {
     if (a.border == kReplicate)
-      remap_s16point5_kernel<4, kReplicate><<<grid, kThreads, 0, stream>>>(a);
+      remap_s16point5_kernel<4, kReplicate><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x);
     else
-      remap_s16point5_kernel<4, kConstant><<<grid, kThreads, 0, stream>>>(a);
+      remap_s16point5_kernel<4, kConstant><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_remap_f32(const RemapF32Args &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const dim3 grid = tile_grid(a.dst_width, a.dst_height);
-  KCV_FLOAT_DISPATCH(remap_f32_kernel, a, a.channels, a.interpolation, a.border, grid,
-                     stream);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
+  KCV_FLOAT_DISPATCH(remap_f32_kernel, a, g.tiles_x, a.channels, a.interpolation, a.border,
+                     g.grid, stream);
   return hipGetLastError();
 }
 
 hipError_t launch_warp_perspective(const WarpArgs &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const dim3 grid = tile_grid(a.dst_width, a.dst_height);
-  KCV_FLOAT_DISPATCH(warp_kernel, a, a.channels, a.interpolation, a.border, grid, stream);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
+  KCV_FLOAT_DISPATCH(warp_kernel, a, g.tiles_x, a.channels, a.interpolation, a.border,
+                     g.grid, stream);
   return hipGetLastError();
 }
 

@@ -297,6 +297,16 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
   return q;
 }
 
+// FCVTAU of a lerp result: round half away from zero, NaN to 0.  A lerp of
+// bytes with a fraction in [0, 1) is NaN or lies in [0, 255] (the fused
+// steps cannot round outside the range of their inputs), so no saturation
+// test is needed; the NaN test also covers signalling NaNs.
+__device__ __forceinline__ uint32_t round_lerp_u8(float v) {
+  const float c = v == v ? v : 0.0f;
+  const float t = truncf(c);
+  return static_cast<uint32_t>(t) + (c - t >= 0.5f ? 1u : 0u);
+}
+
 // lerp_2d (transform_neon.h:331-337) per channel.
 template <int C>
 __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
@@ -310,7 +320,7 @@ __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
     const float line0 = fmaf(pb - pa, q.xfrac, pa);
     const float line1 = fmaf(pd - pc, q.xfrac, pc);
     const float res = fmaf(line1 - line0, q.yfrac, line0);
-    out |= (cvta_u32(res) & 0xFFu) << (8 * c);
+    out |= round_lerp_u8(res) << (8 * c);
   }
   return out;
 }
