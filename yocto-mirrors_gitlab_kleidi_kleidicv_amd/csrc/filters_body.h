@@ -255,9 +255,9 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
 }
 
 // ---------------------------------------------------------------------------
-// Median: exact selection by the forgetful-selection scheme on pairs of
-// outputs packed in 16-bit halves (v_pk_min_u16 / v_pk_max_u16).  Any exact
-// selection returns the reference sorting network's value.
+// Median: exact selection on pairs of horizontally adjacent outputs packed
+// in 16-bit halves (v_pk_min_u16 / v_pk_max_u16).  Any exact selection
+// returns the reference sorting network's value.
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void cmpex(us2 &lo, us2 &hi) {
