@@ -11,7 +11,7 @@ image per launch, inputs resident in HBM:
   yuv_semiplanar_to_rgb (NV12 -> RGB)
 Inputs rotate over several distinct sets (> 512 MB in total) so reads come
 from HBM rather than the 256 MB Infinity Cache.  Kernel time per op is taken
-with HIP events on the stream the kernels run on.
+inside the timed steps with HIP events on the stream the kernels run on.
 
 Prints one JSON line (metric, value in destination Mpix/s, per-op GB/s and
 % of the 8 TB/s HBM peak, the dominant kernel's roofline, a CPU baseline).
@@ -120,7 +120,7 @@ OPS = [
 OP_KERNELS = {
     "separable_7x7_1ch": ["separable_kernel"],
     "median_5x5_1ch": ["median_kernel"],
-    "resize_2x_up_4ch": ["resize_tables_kernel", "resize_staged_kernel"],
+    "resize_2x_up_4ch": ["resize_up2x4_kernel"],
     "resize_0.5x_4ch": ["resize_quarter_kernel"],
     "remap_s16point5_1ch": ["remap_s16point5_kernel"],
     "warp_perspective_4ch": ["warp_kernel"],
@@ -200,6 +200,36 @@ def run_op(K, torch, name, s, out, kx, ky, T, stream):
     elif name == "nv12_to_rgb":
         K.yuv_sp_to_rgb(s["nv12_y"], s["nv12_uv"], K.NV12_TO_RGB, dst=out[name], stream=stream,
                         device_api=True)
+
+
+def host_api(K, np, s_host, kx, ky, T, ops, reps=3):
+    """The drop-in entry points on host (pageable numpy) buffers: each call
+    copies its inputs to the device, runs, copies the result back and waits.
+    Mpix/s including the host-device copies."""
+    calls = {
+        "separable_7x7_1ch": lambda: K.separable_filter_2d(s_host["sep_src"], kx, ky,
+                                                           "REPLICATE"),
+        "median_5x5_1ch": lambda: K.median_blur(s_host["med_src"], 5, "REPLICATE"),
+        "resize_2x_up_4ch": lambda: K.resize_linear(s_host["up_src"], 2 * W, 2 * H),
+        "resize_0.5x_4ch": lambda: K.resize_linear(s_host["down_src"], W // 2, H // 2),
+        "remap_s16point5_1ch": lambda: K.remap_s16point5(
+            s_host["remap_src"], s_host["mapxy"], s_host["mapfrac"].view(np.uint16), "CONSTANT"),
+        "warp_perspective_4ch": lambda: K.warp_perspective(s_host["warp_src"], T, W, H, "LINEAR",
+                                                           "REPLICATE"),
+        "nv12_to_rgb": lambda: K.yuv_sp_to_rgb(s_host["nv12_y"], s_host["nv12_uv"], 0x02),
+    }
+    out, t_all, pix_all = {}, 0.0, 0
+    for name, dst_pix, _ in ops:
+        calls[name]()  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            calls[name]()
+        t = (time.perf_counter() - t0) / reps
+        out[name] = {"ms": round(t * 1e3, 3), "mpix_s": round(dst_pix / t / 1e6, 1)}
+        t_all += t
+        pix_all += dst_pix
+    out["all_ops_mpix_s"] = round(pix_all / t_all / 1e6, 1)
+    return out
 
 
 def cpu_baseline(np, s_host, kx, ky, T, ops):
@@ -400,7 +430,8 @@ def main():
             run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
     torch.cuda.synchronize()
 
-    # Whole-step time: every op of a step back to back on one stream.
+    # Whole-step time: every op of a step back to back on one stream, with
+    # nothing between the launches.
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -415,18 +446,23 @@ def main():
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_time(stop) / args.steps
 
-    # Per-op launch duration: the op alone, `steps` launches back to back on
-    # the same stream between two events, rotating the input sets.
+    # Per-op duration: the same steps again with an event recorded on the
+    # launch stream between consecutive ops.  A recorded event waits for the
+    # op before it to finish, so these steps run slightly longer than the
+    # ones above; each op's own time is what rocprofv3 reports for it.
+    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(len(ops) + 1)]
+             for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    for i in range(args.steps):
+        s = sets[i % len(sets)]
+        marks[i][0].record(stream)
+        for k, (name, _, _) in enumerate(ops):
+            run_op(K, torch, name, s, out, kx, ky, T, stream)
+            marks[i][k + 1].record(stream)
+    torch.cuda.synchronize()
     op_ms = {}
-    for name, _, _ in ops:
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for i in range(args.steps):
-            run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        op_ms[name] = e0.elapsed_time(e1) / args.steps
+    for k, (name, _, _) in enumerate(ops):
+        op_ms[name] = sum(m[k].elapsed_time(m[k + 1]) for m in marks) / args.steps
 
     per_op = {}
     total_pix = 0
@@ -476,8 +512,10 @@ def main():
         result["batched"] = bench_batched(K, torch, WL, dev, stream, args.steps, args.warmup)
     if not args.no_serving and not args.ops:
         result["serving"] = bench_serving(torch, dev, args.serving_requests)
+    s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
+    if not args.ops:
+        result["including_host_copies"] = host_api(K, np, s_host, kx, ky, T, ops)
     if not args.no_cpu_baseline:
-        s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
         result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)
     print(json.dumps(result))
 
