@@ -67,11 +67,11 @@ __global__ __launch_bounds__(kThreads) void batch_sep_kernel(Segment s) {
   });
 }
 
-template <int K>
+template <int K, int C>
 __global__ __launch_bounds__(kThreads) void batch_median_kernel(Segment s) {
   __shared__ __align__(16) uint8_t smem[filt::median_smem_bytes<K>()];
   for_each_item(s, [&](const Job &job, int bx, int by) {
-    filt::median_tile<K>(job.u.median, bx, by, smem);
+    filt::median_tile_c<K, C>(job.u.median, bx, by, smem);
   });
 }
 
@@ -198,10 +198,15 @@ const void *segment_kernel(Family family, int variant) {
         default: return nullptr;
       }
     case kFamMedian:
-      return variant == 3   ? KCV_K(batch_median_kernel<3>)
-             : variant == 5 ? KCV_K(batch_median_kernel<5>)
-             : variant == 7 ? KCV_K(batch_median_kernel<7>)
-                            : nullptr;
+      switch (variant) {
+#define KCV_M(K, C) \
+  case K * 8 + C: return KCV_K(batch_median_kernel<K, C>);
+        KCV_M(3, 1) KCV_M(3, 2) KCV_M(3, 3) KCV_M(3, 4)
+        KCV_M(5, 1) KCV_M(5, 2) KCV_M(5, 3) KCV_M(5, 4)
+        KCV_M(7, 1) KCV_M(7, 2) KCV_M(7, 3) KCV_M(7, 4)
+#undef KCV_M
+        default: return nullptr;
+      }
     case kFamResize:
       return variant == kRsGeneric   ? KCV_K(batch_resize_kernel<kRsGeneric>)
              : variant == kRsExact2  ? KCV_K(batch_resize_kernel<kRsExact2>)

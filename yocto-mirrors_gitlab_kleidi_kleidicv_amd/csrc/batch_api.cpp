@@ -113,7 +113,7 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
                                   d.src_height, d.channels, d.kernel_size, d.kernel_size,
                                   d.border_type, &j.u.median, &act));
       p->family = kcv::kFamMedian;
-      j.variant = j.u.median.ksize;
+      j.variant = j.u.median.ksize * 8 + j.u.median.channels;
       break;
     case KLEIDICV_HIP_OP_RESIZE_LINEAR: {
       kcv::ResizeArgs a;

@@ -28,12 +28,12 @@ __global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a, int tile
   separable_tile<MODE, K>(a, bx, by, smem);
 }
 
-template <int K>
+template <int K, int C>
 __global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a, int tiles_x) {
   __shared__ __align__(16) uint8_t smem[median_smem_bytes<K>()];
   int bx, by;
   tile_of(tiles_x, bx, by);
-  median_tile<K>(a, bx, by, smem);
+  median_tile_c<K, C>(a, bx, by, smem);
 }
 
 // Grid and tiles_x argument for an image: a 1-D XCD-ordered grid unless the
@@ -97,10 +97,13 @@ hipError_t launch_separable(const SepArgs &a, hipStream_t stream) {
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
   const TileGrid g = tile_grid(a.width, a.height, a.channels);
-  switch (a.ksize) {
-    case 3: median_kernel<3><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
-    case 5: median_kernel<5><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
-    case 7: median_kernel<7><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
+  switch (a.ksize * 8 + a.channels) {
+#define KCV_M(K, C) \
+  case K * 8 + C: median_kernel<K, C><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;
+    KCV_M(3, 1) KCV_M(3, 2) KCV_M(3, 3) KCV_M(3, 4)
+    KCV_M(5, 1) KCV_M(5, 2) KCV_M(5, 3) KCV_M(5, 4)
+    KCV_M(7, 1) KCV_M(7, 2) KCV_M(7, 3) KCV_M(7, 4)
+#undef KCV_M
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

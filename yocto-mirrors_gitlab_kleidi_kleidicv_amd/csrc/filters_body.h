@@ -318,30 +318,49 @@ __device__ __forceinline__ void batcher_sort(us2 *a) {
 // and each window's median is the (K+1)-th smallest of those K+1 sorted
 // values and its own K sorted values:
 //   min over i = 0..K of max(A[K-i], B[i-1])   (B[-1] = -infinity).
-// For 5x5 that is about 56 packed min/max operations per output instead of
-// the ~100 of a 25-input network.
+// For 5x5 that is 224 packed min/max operations per pair of vertically
+// adjacent outputs (each operation covers two horizontally adjacent bytes).
 template <int K>
 constexpr int median_smem_bytes() {
   return Window<K / 2>::kRows * Window<K / 2>::kCols;
 }
 
-// One 256-byte x 16-row median tile at tile coordinates (bx, by).
-template <int K>
-__device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
-                                            uint8_t *smem) {
+// floor(v / 4), also for negative v.
+constexpr int floor_div4(int v) { return v >= 0 ? v / 4 : -((3 - v) / 4); }
+
+// Bytes b and b + 1 of the little-endian dword run d, zero-extended into the
+// two 16-bit halves (one v_perm_b32; b is a compile-time constant).
+template <int ND>
+__device__ __forceinline__ us2 byte_pair(const uint32_t (&d)[ND], int b) {
+  const int q = b >> 2, s = b & 3;
+  const uint32_t sel = s | (0x0cu << 8) | ((s + 1u) << 16) | (0x0cu << 24);
+  const uint32_t hi = s == 3 ? d[q + 1] : d[q];
+  return __builtin_bit_cast(us2, __builtin_amdgcn_perm(hi, d[q], sel));
+}
+
+// One 256-byte x 16-row median tile of a C-channel image at tile
+// coordinates (bx, by).  Each thread owns 4 output bytes of 4 rows and works
+// on 2 rows at a time.  The taps of a window row are read from LDS as aligned
+// dwords (unaligned 8/16-bit LDS reads are several times slower) and paired
+// with v_perm_b32.
+template <int K, int C>
+__device__ __forceinline__ void median_tile_c(const MedianArgs &a, int bx, int by,
+                                              uint8_t *smem) {
   constexpr int R = K / 2;
   constexpr int NC = K * (K - 1);        // shared values
   constexpr int M = (K * K + 1) / 2;     // median rank (1-based)
   constexpr int LO = M - K - 1;          // first kept shared rank (0-based)
+  // dwords covering bytes [-R*C, 3 + R*C] around the thread's first byte
+  constexpr int DLO = floor_div4(-R * C), ND = floor_div4(3 + R * C) - DLO + 1;
   using G = Window<R>;
+  static_assert(-4 * DLO <= G::kPad && 4 * (DLO + ND) <= 4 + G::kPad,
+                "taps stay inside the staged window");
   uint8_t *win = smem;
 
   const int e0 = bx * kTileW;
   const int y0 = by * kTileH;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  const int C = a.channels;
-  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0,
-                  y0);
+  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0, y0);
   __syncthreads();
 
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -350,13 +369,19 @@ __device__ __forceinline__ void median_tile(const MedianArgs &a, int bx, int by,
   for (int rp = 0; rp < kTileH / 8; ++rp) {
     const int r = 4 * ty + 2 * rp;  // this pass: tile rows r and r + 1
     if (y0 + r >= a.height) break;
+    const uint32_t *w0 =
+        reinterpret_cast<const uint32_t *>(win + r * G::kCols + G::kPad) + tx + DLO;
+    uint32_t d[K + 1][ND];
+#pragma unroll
+    for (int i = 0; i <= K; ++i)
+#pragma unroll
+      for (int q = 0; q < ND; ++q) d[i][q] = w0[i * (G::kCols / 4) + q];
     uint32_t packed[2] = {0, 0};
 #pragma unroll
     for (int pair = 0; pair < 2; ++pair) {
-      const int base = r * G::kCols + G::kPad + 4 * tx + 2 * pair;
+      // window row i, column j of the upper output's window
       auto at = [&](int i, int j) {
-        const int o = base + i * G::kCols + (j - R) * C;
-        return us2{win[o], win[o + 1]};
+        return byte_pair(d[i], 2 * pair + (j - R) * C - 4 * DLO);
       };
       // window rows 1..K-1 of the upper output = rows 0..K-2 of the lower
       us2 v[NC];
