@@ -177,8 +177,8 @@ def test_resize_generic_sweep(rng, c):
 
 def test_resize_exact_2x_4ch(rng):
     # closed-form kernel for the generic path at exactly 2x, 4 channels
-    for sw, sh in [(4, 1), (4, 4), (5, 3), (17, 9), (255, 7), (256, 5), (1023, 3),
-                   (1025, 4), (480, 6)]:
+    for sw, sh in [(4, 1), (4, 4), (5, 3), (17, 9), (255, 7), (256, 5),
+                   (511, 2), (513, 3), (1023, 3), (1025, 4), (480, 6)]:
         resize_case(rng, sw, sh, 2 * sw, 2 * sh, 4)
     # padded rows on both sides, host and device entry points
     src = padded_rows(rng, 11, 37, 4, 5)

@@ -89,7 +89,7 @@ constexpr int kFilterTileH = 16;   // rows
 constexpr int kXformTile = 32;     // pixels (remap, warp)
 constexpr int kResizeTileE = 1024;            // destination bytes (staged)
 constexpr int kQuarterTileE = 16 * 256;       // destination bytes (quarter)
-constexpr int kUp2x4TilePx = 4 * 256;         // source pixels (exact 2x, 4 channels)
+constexpr int kUp2x4TilePx = rsz::kUp2x4ThreadPx * 256;  // source pixels (exact 2x, 4 ch)
 constexpr int kYuvTileW = 8 * 256;            // pixels
 constexpr int kCopyTileE = 16 * 256;          // bytes
 constexpr int kTablesPerItem = 256;           // table entries

@@ -66,7 +66,8 @@ hipError_t launch_resize(const ResizeArgs &a, void *workspace,
   }
   if (path == Path::kUp2x4) {
     const int groups = a.src_height + 1;
-    const dim3 grid(static_cast<unsigned>((a.src_width + 4 * kThreads - 1) / (4 * kThreads)),
+    const int px = kUp2x4ThreadPx * kThreads;
+    const dim3 grid(static_cast<unsigned>((a.src_width + px - 1) / px),
                     static_cast<unsigned>(groups < 65535 ? groups : 65535));
     resize_up2x4_kernel<<<grid, kThreads, 0, stream>>>(a);
     return hipGetLastError();

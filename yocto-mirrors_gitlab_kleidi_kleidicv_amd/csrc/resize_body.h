@@ -651,21 +651,24 @@ __device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
 //
 // Row group g in [0, h] produces output rows 2g-1 (vertical fraction 64) and
 // 2g (fraction 192) from source rows max(g-1, 0) and min(g, h-1); group 0
-// has only row 0 and group h only row 2h-1.  A thread owns 4 source pixels
-// of its group, i.e. 8 output pixels (32 bytes) of each output row.
+// has only row 0 and group h only row 2h-1.  A thread owns 2 source pixels
+// of its group, i.e. 4 output pixels (16 bytes) of each output row, so the
+// lanes of a wave store one contiguous 1 KiB run per output row.
+constexpr int kUp2x4ThreadPx = 2;
+
 __device__ __forceinline__ uint32_t lerp_q3(uint32_t a, uint32_t b) {  // (3a + b + 2) >> 2
   return (((a << 1) + a + b + 0x00020002u) >> 2) & 0x00FF00FFu;
 }
 
 __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) {
   const int w = a.src_width, h = a.src_height;
-  const int m0 = (bx * kThreads + threadIdx.x) * 4;
+  const int m0 = (bx * kThreads + threadIdx.x) * kUp2x4ThreadPx;
   if (m0 >= w || g > h) return;
   const uint8_t *top = a.src + static_cast<size_t>(g > 0 ? g - 1 : 0) * a.src_stride;
   const uint8_t *bot = a.src + static_cast<size_t>(g < h ? g : h - 1) * a.src_stride;
-  const bool al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 15) == 0;
-  // source pixels m0-1 .. m0+4 (clamped) of both rows
-  uint32_t t[6], b[6];
+  const bool al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
+  // source pixels m0-1 .. m0+2 (clamped) of both rows
+  uint32_t t[4], b[4];
   auto px = [&](const uint8_t *row, int m) -> uint32_t {
     m = min(max(m, 0), w - 1);
     const uint8_t *p = row + 4 * m;
@@ -673,48 +676,44 @@ __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) 
     return static_cast<uint32_t>(p[0]) | (static_cast<uint32_t>(p[1]) << 8) |
            (static_cast<uint32_t>(p[2]) << 16) | (static_cast<uint32_t>(p[3]) << 24);
   };
-  if (al && m0 + 4 <= w) {
-    const uint4 qt = *reinterpret_cast<const uint4 *>(top + 4 * m0);
-    const uint4 qb = *reinterpret_cast<const uint4 *>(bot + 4 * m0);
-    t[1] = qt.x; t[2] = qt.y; t[3] = qt.z; t[4] = qt.w;
-    b[1] = qb.x; b[2] = qb.y; b[3] = qb.z; b[4] = qb.w;
+  if (al && m0 + 2 <= w) {
+    const uint2 qt = *reinterpret_cast<const uint2 *>(top + 4 * m0);
+    const uint2 qb = *reinterpret_cast<const uint2 *>(bot + 4 * m0);
+    t[1] = qt.x; t[2] = qt.y;
+    b[1] = qb.x; b[2] = qb.y;
   } else {
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      t[1 + k] = px(top, m0 + k);
-      b[1 + k] = px(bot, m0 + k);
-    }
+    t[1] = px(top, m0); t[2] = px(top, m0 + 1);
+    b[1] = px(bot, m0); b[2] = px(bot, m0 + 1);
   }
   t[0] = px(top, m0 - 1);
   b[0] = px(bot, m0 - 1);
-  t[5] = px(top, m0 + 4);
-  b[5] = px(bot, m0 + 4);
-  const int n_out = min(8, 2 * (w - m0));  // output pixels of this thread per row
+  t[3] = px(top, m0 + 2);
+  b[3] = px(bot, m0 + 2);
+  const int n_out = min(4, 2 * (w - m0));  // output pixels of this thread per row
   const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int y = 2 * g - 1 + half;  // half 0: fraction 64, half 1: fraction 192
     if (y < 0 || y >= a.dst_height) continue;
-    // vertical lerp of the 6 pixels, channels {0,2} and {1,3}
-    uint32_t lo[6], hi[6];
+    // vertical lerp of the 4 pixels, channels {0,2} and {1,3}
+    uint32_t lo[4], hi[4];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 4; ++k) {
       const uint32_t tl = t[k] & 0x00FF00FFu, th = (t[k] >> 8) & 0x00FF00FFu;
       const uint32_t bl = b[k] & 0x00FF00FFu, bh = (b[k] >> 8) & 0x00FF00FFu;
       lo[k] = half == 0 ? lerp_q3(tl, bl) : lerp_q3(bl, tl);
       hi[k] = half == 0 ? lerp_q3(th, bh) : lerp_q3(bh, th);
     }
-    uint32_t o[8];
+    uint32_t o[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
       // output 2m: L(V[m-1], V[m], 192); output 2m+1: L(V[m], V[m+1], 64)
       o[2 * k] = lerp_q3(lo[k + 1], lo[k]) | (lerp_q3(hi[k + 1], hi[k]) << 8);
       o[2 * k + 1] = lerp_q3(lo[k + 1], lo[k + 2]) | (lerp_q3(hi[k + 1], hi[k + 2]) << 8);
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 8 * static_cast<size_t>(m0);
-    if (n_out == 8 && out_al) {
-      reinterpret_cast<uint4 *>(out)[0] = make_uint4(o[0], o[1], o[2], o[3]);
-      reinterpret_cast<uint4 *>(out)[1] = make_uint4(o[4], o[5], o[6], o[7]);
+    if (n_out == 4 && out_al) {
+      *reinterpret_cast<uint4 *>(out) = make_uint4(o[0], o[1], o[2], o[3]);
     } else {
       for (int k = 0; k < n_out; ++k)
 #pragma unroll
