@@ -63,7 +63,8 @@ struct Window {
 // border-extended image into `win` (row pitch Window<R>::kCols).  Rows are
 // mapped through the border first, so every 16-byte chunk that lies inside
 // the row is one 16-byte load (edge tiles included); only chunks that cross
-// the left or right edge are assembled byte by byte.
+// the left or right edge are assembled byte by byte.  Each thread issues all
+// of its loads before its first LDS store, so their latencies overlap.
 template <int R>
 __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
                                              size_t stride, int width,
@@ -71,17 +72,22 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
                                              int border, int e0, int y0) {
   using G = Window<R>;
   constexpr int kChunks = G::kCols / 16;
+  constexpr int kTotal = G::kRows * kChunks;
+  constexpr int kPerThread = (kTotal + kThreads - 1) / kThreads;
   const int row_bytes = width * channels;
   const int c0 = e0 - G::kPad;
   const bool aligned = ((reinterpret_cast<uintptr_t>(src) | stride) & 15) == 0;
-  for (int i = threadIdx.x; i < G::kRows * kChunks; i += kThreads) {
+  uint4 v[kPerThread];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int i = threadIdx.x + k * kThreads;
+    if (i >= kTotal) break;
     const int r = i / kChunks, q = i - r * kChunks;
     const int y = map_index(y0 - R + r, height, border);
     const uint8_t *srow = src + static_cast<size_t>(y) * stride;
     const int e = c0 + 16 * q;
-    uint4 v;
     if (aligned && e >= 0 && e + 16 <= row_bytes) {
-      v = *reinterpret_cast<const uint4 *>(srow + e);
+      v[k] = *reinterpret_cast<const uint4 *>(srow + e);
     } else {
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -92,9 +98,15 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
         const int xm = map_index(x, width, border);
         w[b >> 2] |= static_cast<uint32_t>(srow[xm * channels + c]) << (8 * (b & 3));
       }
-      v = make_uint4(w[0], w[1], w[2], w[3]);
+      v[k] = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    *reinterpret_cast<uint4 *>(win + r * G::kCols + 16 * q) = v;
+  }
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int i = threadIdx.x + k * kThreads;
+    if (i >= kTotal) break;
+    const int r = i / kChunks, q = i - r * kChunks;
+    *reinterpret_cast<uint4 *>(win + r * G::kCols + 16 * q) = v[k];
   }
 }
 
@@ -124,7 +136,9 @@ __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
 // It reads the span of column sums those outputs need with 16-byte LDS
 // loads (neighbouring lanes read neighbouring chunks: no bank conflicts) and
 // accumulates in 32 bits with compile-time tap offsets (channel count is a
-// template parameter).
+// template parameter).  Taps are at most 16 bits and column sums 16 bits, so
+// the products use the full-rate 24-bit multiplier (v_mad_u32_u24) rather
+// than the quarter-rate 32-bit one.
 typedef unsigned short sep_us2 __attribute__((ext_vector_type(2)));
 
 template <int K>
@@ -171,7 +185,7 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
     for (int b = 0; b < 8; ++b) {
       uint32_t acc = 0;
 #pragma unroll
-      for (int j = 0; j < K; ++j) acc += w[j] * col(8 * D + b + (j - R) * C);
+      for (int j = 0; j < K; ++j) acc += __umul24(w[j], col(8 * D + b + (j - R) * C));
       uint32_t out;
       if (MODE == kSepBinomial)
         out = (acc + (1u << (a.shift - 1))) >> a.shift;
