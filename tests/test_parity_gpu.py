@@ -249,6 +249,25 @@ def test_remap_s16point5(rng, c, border):
         assert_same(got, want, "%s c%d %dx%d" % (border, c, dw, dh))
 
 
+@pytest.mark.parametrize("c", [1, 4])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_s16point5_narrow_and_odd_stride(rng, c, border):
+    # sources one to three pixels wide (the pair loads clamp inside a row),
+    # and rows with an odd pitch (unaligned pair loads)
+    cases = [(padded_rows(rng, sh, sw, c, 0), dw, dh)
+             for (sw, sh, dw, dh) in [(1, 1, 8, 2), (1, 5, 9, 3), (2, 1, 8, 4), (3, 2, 13, 5)]]
+    cases.append((padded_rows(rng, 23, 37, c, 3), 70, 19))
+    for src, dw, dh in cases:
+        sh, sw = src.shape[0], src.shape[1]
+        mapxy = np.stack([rng.integers(-3, sw + 3, (dh, dw)), rng.integers(-3, sh + 3, (dh, dw))],
+                         axis=-1).astype(np.int16)
+        mapfrac = rng.integers(0, 1 << 16, (dh, dw)).astype(np.uint16)
+        bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+        got = K.remap_s16point5(src, mapxy, mapfrac, border, bv)
+        _, want = O.remap_s16point5(np.ascontiguousarray(src), mapxy, mapfrac, border, bv)
+        assert_same(got, want, "%s c%d %dx%d -> %dx%d" % (border, c, sw, sh, dw, dh))
+
+
 @pytest.mark.parametrize("c", [1, 2, 3, 4])
 @pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
 @pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
@@ -288,6 +307,31 @@ def test_warp(rng, c, interp, border):
             got = K.warp_perspective(src, T, dw, dh, interp, border, bv)
             _, want = O.warp_perspective(src, T, dw, dh, interp, border, bv)
             assert_same(got, want, "%s %s c%d %g" % (interp, border, c, persp))
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_float_gathers_narrow_and_odd_stride(rng, c, interp, border):
+    # sources one to three pixels wide (pair loads clamp inside a row) and
+    # rows with an odd pitch (unaligned pair loads), for remap f32 and warp
+    cases = [(padded_rows(rng, sh, sw, c, 0), dw, dh)
+             for (sw, sh, dw, dh) in [(1, 1, 8, 2), (1, 4, 9, 3), (2, 1, 8, 4), (3, 2, 13, 5)]]
+    cases.append((padded_rows(rng, 21, 35, c, 3), 67, 18))
+    for src, dw, dh in cases:
+        sh, sw = src.shape[0], src.shape[1]
+        mapx = rng.uniform(-2.5, sw + 1.5, (dh, dw)).astype(np.float32)
+        mapy = rng.uniform(-2.5, sh + 1.5, (dh, dw)).astype(np.float32)
+        bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+        what = "%s %s c%d %dx%d" % (interp, border, c, sw, sh)
+        got = K.remap_f32(src, mapx, mapy, interp, border, bv)
+        _, want = O.remap_f32(np.ascontiguousarray(src), mapx, mapy, interp, border, bv)
+        assert_same(got, want, "remap_f32 " + what)
+        T = np.array([[sw / dw, 0.01, -0.7], [0.02, sh / dh, -0.6], [1e-3, 2e-3, 1.0]],
+                     np.float32)
+        got = K.warp_perspective(src, T, dw, dh, interp, border, bv)
+        _, want = O.warp_perspective(np.ascontiguousarray(src), T, dw, dh, interp, border, bv)
+        assert_same(got, want, "warp " + what)
 
 
 def test_warp_zero_weight_row(rng):

@@ -36,6 +36,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Destination tile of the gather kernels (remap s16point5, remap f32, warp
+// perspective), per-image and batched.
+constexpr int kGatherTileW = 64, kGatherTileH = 16;
+
 // Largest 1-D grid used with xcd_tile (blocks of 256 threads).
 constexpr int64_t kMaxLinearTiles = int64_t{1} << 23;
 

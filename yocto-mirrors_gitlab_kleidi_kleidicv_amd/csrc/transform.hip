@@ -12,7 +12,7 @@ using namespace xform;
 // tiles_x == 0 a 2-D grid whose y dimension is capped at 65535 rows of
 // tiles, blocks striding over the rest.  (The kernel argument is passed on
 // by value: taking its address would copy it to scratch memory.)
-#define KCV_FOR_TILES(TILES_X, DST_HEIGHT, CALL)                      \
+#define KCV_FOR_TILES(TILES_X, DST_HEIGHT, TILE_H, CALL)              \
   do {                                                                \
     if ((TILES_X) > 0) {                                              \
       const int t_ = xcd_tile(blockIdx.x, gridDim.x);                 \
@@ -21,24 +21,24 @@ using namespace xform;
       CALL;                                                           \
     } else {                                                          \
       const int bx = blockIdx.x;                                      \
-      for (int by = blockIdx.y; by * kBlockPx < (DST_HEIGHT); by += gridDim.y) CALL; \
+      for (int by = blockIdx.y; by * (TILE_H) < (DST_HEIGHT); by += gridDim.y) CALL; \
     }                                                                 \
   } while (0)
 
 template <int C, int BORDER>
 __global__ __launch_bounds__(kThreads) void remap_s16point5_kernel(RemapS16Args a,
                                                                    int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, (remap_s16point5_tile<C, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (remap_s16point5_tile<C, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
 __global__ __launch_bounds__(kThreads) void remap_f32_kernel(RemapF32Args a, int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
 __global__ __launch_bounds__(kThreads) void warp_kernel(WarpArgs a, int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, (warp_tile<C, INTERP, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (warp_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 struct TileGrid {
@@ -47,8 +47,9 @@ struct TileGrid {
 };
 
 TileGrid tile_grid(int width, int height) {
-  const int tx = (width + kBlockPx - 1) / kBlockPx;
-  const int ty = (height + kBlockPx - 1) / kBlockPx;
+  constexpr int tile_w = kGatherTileW, tile_h = kGatherTileH;
+  const int tx = (width + tile_w - 1) / tile_w;
+  const int ty = (height + tile_h - 1) / tile_h;
   if (int64_t{tx} * ty <= kMaxLinearTiles) return {dim3(tx * ty), tx};
   return {dim3(tx, ty < 65535 ? ty : 65535), 0};
 }

@@ -2,13 +2,12 @@
 // transform.hip, batched kernels in batch.hip): remap with s16 + 5/5-bit fractional maps,
 // remap with float maps and warp-perspective.
 //
-// Work layout: each wavefront owns 8 x 8 destination pixels per step, so the
-// source footprint of one load instruction stays a compact patch (a few
-// cache lines) under rotation or displacement instead of a long slanted run.
-// A workgroup of 4 wavefronts covers 32 x 32 pixels; every thread handles
-// four pixels from four different 8 x 8 tiles and issues all of their loads
-// before using any result.  Four-channel pixels are fetched as one dword per
-// corner.
+// Work layout: a workgroup of 256 threads covers a 64 x 16 tile of
+// destination pixels and each thread owns 4 horizontally adjacent pixels of
+// one row (16 threads per tile row).  Map entries are then read with 16-byte
+// loads and results written with one 4-16 byte store per thread, and the
+// gathers of a thread are independent loads with no branch between them.
+// Each source row of a bilinear tap is one load of a pixel pair.
 //
 // Float semantics follow the reference's AArch64 build: the lerps and the
 // row/column homography updates are fused multiply-adds (vmlaq_f32 /
@@ -30,17 +29,6 @@ namespace kcv {
 namespace xform {
 
 constexpr int kThreads = 256;
-constexpr int kBlockPx = 32;  // workgroup tile edge in pixels
-constexpr int kPerThread = 4;
-
-// Pixel i (0..3) of this thread: tile (wave*4 + i) of the 4 x 4 grid of
-// 8 x 8 tiles, lane -> position inside the tile.
-__device__ __forceinline__ void pixel_of(int i, int bx, int by, int &x, int &y) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int j = wave * kPerThread + i;
-  x = bx * kBlockPx + (j & 3) * 8 + (lane & 7);
-  y = by * kBlockPx + (j >> 2) * 8 + (lane >> 3);
-}
 
 // Up to four channel bytes of one pixel packed in a dword.
 template <int C>
@@ -66,40 +54,21 @@ __device__ __forceinline__ uint32_t byte_of(uint32_t v, int c) {
   return (v >> (8 * c)) & 0xFFu;
 }
 
-struct __attribute__((aligned(4))) U32x2 {
-  uint32_t x, y;
-};
-
-// The two horizontal taps of one source row, pixel x0 and (when `adjacent`)
-// pixel x0 + 1, else x0 again, with one load where possible: an 8-byte load
-// of both 4-channel pixels, or for 1 channel the 8 bytes from the dword
-// holding x0.  `tail_ok` says those 8 bytes lie inside the image: within
-// this row, or (rows below this one exist and the width is at least 8)
-// running into the next row.
-template <int C>
-__device__ __forceinline__ void load_pair(const uint8_t *row, uint32_t x0, bool adjacent,
-                                          bool tail_ok, bool aligned4, uint32_t &pa,
-                                          uint32_t &pb) {
-  if (C == 4 && aligned4 && adjacent) {
-    const U32x2 v = *reinterpret_cast<const U32x2 *>(row + 4 * static_cast<size_t>(x0));
-    pa = v.x;
-    pb = v.y;
-    return;
-  }
-  if (C == 1 && aligned4 && tail_ok) {
-    const U32x2 v = *reinterpret_cast<const U32x2 *>(row + (x0 & ~3u));
-    const uint64_t q = (static_cast<uint64_t>(v.y) << 32) | v.x;
-    const uint32_t sh = 8 * (x0 & 3u);
-    pa = static_cast<uint32_t>(q >> sh) & 0xFFu;
-    pb = adjacent ? static_cast<uint32_t>(q >> (sh + 8)) & 0xFFu : pa;
-    return;
-  }
-  pa = load_px<C>(row + static_cast<size_t>(x0) * C, aligned4);
-  pb = adjacent ? load_px<C>(row + static_cast<size_t>(x0 + 1) * C, aligned4) : pa;
-}
-
 // ---------------------------------------------------------------------------
 // remap s16point5
+//
+// Tiles of 64 x 16 destination pixels; a thread owns 4 horizontally adjacent
+// pixels (16 threads per tile row), so the map entries of a thread are one
+// 16-byte and one 8-byte load and its results one 4- or 16-byte store.  The
+// gathers are branch-free: each source row contributes one load covering the
+// pixel pair (cx, cx + 1), with cx clamped so the pair lies inside the row,
+// and taps outside the image are then replaced by the border value.  With
+// no branches between them all eight gathers of a thread are in flight
+// together.  Source offsets fit 31 bits (stride <= 65535 / channels and
+// coordinates <= 32767 are preconditions), so the gathers are buffer loads
+// with a 32-bit offset.
+constexpr uint32_t kRawBufferWord3 = 0x00020000u;  // gfx9 raw buffer resource
+
 __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
                                               uint32_t c, uint32_t d,
                                               uint32_t fx, uint32_t fy) {
@@ -108,99 +77,165 @@ __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
   return (line0 * (32 - fy) + line1 * fy + 512) >> 10;
 }
 
-// Destination tile (bx, by) of kBlockPx x kBlockPx pixels.
+// One destination pixel from its map entries (xy: x in the low, y in the
+// high 16 bits; fr: 5-bit x and y fractions).
+template <int C, int BORDER>
+__device__ __forceinline__ uint32_t remap_s16_px(__amdgpu_buffer_rsrc_t src, uint32_t stride,
+                                                 int w, int h, uint32_t xy, uint32_t fr,
+                                                 uint32_t bv) {
+  const int mx = static_cast<int16_t>(xy & 0xFFFF);
+  const int my = static_cast<int16_t>(xy >> 16);
+  uint32_t fx = fr & 31u, fy = (fr >> 5) & 31u;
+  int cx, cy0, cy1, i0, i1;           // pair column, rows, tap index in the pair
+  bool ina = true, inb = true, inc = true, ind = true;
+  if (BORDER == kReplicate) {
+    // RemapS16Point5Replicate (:16-137) / get_coordinates_replicate
+    // (:621-640): fractions of negative coordinates are dropped, taps
+    // clamp to the image.
+    if (mx < 0) fx = 0;
+    if (my < 0) fy = 0;
+    const int x0 = min(max(mx, 0), w - 1);
+    cy0 = min(max(my, 0), h - 1);
+    cy1 = cy0 + (cy0 < h - 1 ? 1 : 0);
+    cx = max(min(x0, w - 2), 0);
+    i0 = x0 - cx;
+    i1 = min(x0 + 1, w - 1) - cx;
+  } else {
+    // RemapS16Point5ConstantBorder (:321-425) / get_coordinates_constant
+    // (:860-881): each corner tests its own range with 16-bit unsigned
+    // compares and takes the border value when outside.
+    const uint32_t ux0 = static_cast<uint16_t>(mx), uy0 = static_cast<uint16_t>(my);
+    const uint32_t ux1 = static_cast<uint16_t>(ux0 + 1);
+    const uint32_t uy1 = static_cast<uint16_t>(uy0 + 1);
+    const bool x0in = ux0 < static_cast<uint32_t>(w), x1in = ux1 < static_cast<uint32_t>(w);
+    const bool y0in = uy0 < static_cast<uint32_t>(h), y1in = uy1 < static_cast<uint32_t>(h);
+    ina = x0in && y0in;
+    inb = x1in && y0in;
+    inc = x0in && y1in;
+    ind = x1in && y1in;
+    cy0 = y0in ? static_cast<int>(uy0) : 0;
+    cy1 = y1in ? static_cast<int>(uy1) : 0;
+    cx = max(min(mx, w - 2), 0);
+    i0 = (mx - cx) & 1;  // meaningful only when the tap is inside
+    i1 = (mx + 1 - cx) & 1;
+  }
+  const uint32_t off0 = static_cast<uint32_t>(cy0) * stride + static_cast<uint32_t>(cx * C);
+  const uint32_t off1 = static_cast<uint32_t>(cy1) * stride + static_cast<uint32_t>(cx * C);
+  uint32_t pa, pb, pc, pd;
+  if (C == 1) {
+    const uint32_t t0 = __builtin_amdgcn_raw_buffer_load_b16(src, off0, 0, 0);
+    const uint32_t t1 = __builtin_amdgcn_raw_buffer_load_b16(src, off1, 0, 0);
+    pa = (t0 >> (8 * i0)) & 0xFFu;
+    pb = (t0 >> (8 * i1)) & 0xFFu;
+    pc = (t1 >> (8 * i0)) & 0xFFu;
+    pd = (t1 >> (8 * i1)) & 0xFFu;
+  } else {
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    const u32x2 t0 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(src, off0, 0, 0));
+    const u32x2 t1 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(src, off1, 0, 0));
+    pa = i0 ? t0.y : t0.x;
+    pb = i1 ? t0.y : t0.x;
+    pc = i0 ? t1.y : t1.x;
+    pd = i1 ? t1.y : t1.x;
+  }
+  if (BORDER != kReplicate) {
+    pa = ina ? pa : bv;
+    pb = inb ? pb : bv;
+    pc = inc ? pc : bv;
+    pd = ind ? pd : bv;
+  }
+  uint32_t out = 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+    out |= bilinear5(byte_of(pa, c), byte_of(pb, c), byte_of(pc, c), byte_of(pd, c), fx, fy)
+           << (8 * c);
+  return out;
+}
+
+// Same pixel for a source one pixel wide (no pair fits in a row).
+template <int C, int BORDER>
+__device__ __forceinline__ uint32_t remap_s16_px_narrow(const RemapS16Args &a, uint32_t xy,
+                                                        uint32_t fr, uint32_t bv) {
+  const int mx = static_cast<int16_t>(xy & 0xFFFF);
+  const int my = static_cast<int16_t>(xy >> 16);
+  uint32_t fx = fr & 31u, fy = (fr >> 5) & 31u;
+  const int h = a.src_height;
+  uint32_t pa, pb, pc, pd;
+  if (BORDER == kReplicate) {
+    if (mx < 0) fx = 0;
+    if (my < 0) fy = 0;
+    const int y0 = min(max(my, 0), h - 1), y1 = y0 + (y0 < h - 1 ? 1 : 0);
+    pa = pb = load_px<C>(a.src + static_cast<size_t>(y0) * a.src_stride, false);
+    pc = pd = load_px<C>(a.src + static_cast<size_t>(y1) * a.src_stride, false);
+  } else {
+    const uint32_t ux0 = static_cast<uint16_t>(mx), uy0 = static_cast<uint16_t>(my);
+    const uint32_t ux1 = static_cast<uint16_t>(ux0 + 1), uy1 = static_cast<uint16_t>(uy0 + 1);
+    const uint8_t *r0 = a.src + static_cast<size_t>(uy0 < static_cast<uint32_t>(h) ? uy0 : 0) * a.src_stride;
+    const uint8_t *r1 = a.src + static_cast<size_t>(uy1 < static_cast<uint32_t>(h) ? uy1 : 0) * a.src_stride;
+    const bool y0in = uy0 < static_cast<uint32_t>(h), y1in = uy1 < static_cast<uint32_t>(h);
+    pa = (ux0 == 0 && y0in) ? load_px<C>(r0, false) : bv;
+    pb = (ux1 == 0 && y0in) ? load_px<C>(r0, false) : bv;
+    pc = (ux0 == 0 && y1in) ? load_px<C>(r1, false) : bv;
+    pd = (ux1 == 0 && y1in) ? load_px<C>(r1, false) : bv;
+  }
+  uint32_t out = 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+    out |= bilinear5(byte_of(pa, c), byte_of(pb, c), byte_of(pc, c), byte_of(pd, c), fx, fy)
+           << (8 * c);
+  return out;
+}
+
+// Destination tile (bx, by) of kGatherTileW x kGatherTileH pixels.
 template <int C, int BORDER>
 __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int bx, int by) {
-  const bool src_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
-  const bool dst_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 3) == 0;
-  const bool map_al = ((reinterpret_cast<uintptr_t>(a.mapxy) | a.mapxy_stride) & 3) == 0;
-  const int xmax = a.src_width - 1, ymax = a.src_height - 1;
+  const int q = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
+  if (x >= a.dst_width || y >= a.dst_height) return;
+  const int n = min(4, a.dst_width - x);
   uint32_t bv = 0;
 #pragma unroll
   for (int c = 0; c < C; ++c)
     bv |= static_cast<uint32_t>(a.border_value[C == 4 ? c : 0]) << (8 * c);
-  {
-    int px[kPerThread], py[kPerThread];
-    bool ok[kPerThread];
-    uint32_t xy[kPerThread], fr[kPerThread];
+  const uint8_t *mrow = reinterpret_cast<const uint8_t *>(a.mapxy) + static_cast<size_t>(y) * a.mapxy_stride;
+  const uint8_t *frow = reinterpret_cast<const uint8_t *>(a.mapfrac) + static_cast<size_t>(y) * a.mapfrac_stride;
+  uint32_t xy[4], fr[4];
+  if (n == 4 && ((reinterpret_cast<uintptr_t>(mrow) & 15) | (reinterpret_cast<uintptr_t>(frow) & 7)) == 0) {
+    const uint4 m = *reinterpret_cast<const uint4 *>(mrow + 4 * static_cast<size_t>(x));
+    const uint2 f = *reinterpret_cast<const uint2 *>(frow + 2 * static_cast<size_t>(x));
+    xy[0] = m.x; xy[1] = m.y; xy[2] = m.z; xy[3] = m.w;
+    fr[0] = f.x & 0xFFFFu; fr[1] = f.x >> 16; fr[2] = f.y & 0xFFFFu; fr[3] = f.y >> 16;
+  } else {
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
-      pixel_of(i, bx, by, px[i], py[i]);
-      ok[i] = px[i] < a.dst_width && py[i] < a.dst_height;
-      const int x = ok[i] ? px[i] : 0, y = ok[i] ? py[i] : 0;
-      const uint16_t *m = reinterpret_cast<const uint16_t *>(
-          reinterpret_cast<const uint8_t *>(a.mapxy) + y * a.mapxy_stride + 4 * x);
-      xy[i] = map_al ? *reinterpret_cast<const uint32_t *>(m)
-                     : (static_cast<uint32_t>(m[0]) | (static_cast<uint32_t>(m[1]) << 16));
-      fr[i] = *reinterpret_cast<const uint16_t *>(
-          reinterpret_cast<const uint8_t *>(a.mapfrac) + y * a.mapfrac_stride + 2 * x);
+    for (int k = 0; k < 4; ++k) {
+      const int xx = x + (k < n ? k : 0);
+      const uint16_t *m = reinterpret_cast<const uint16_t *>(mrow + 4 * static_cast<size_t>(xx));
+      xy[k] = static_cast<uint32_t>(m[0]) | (static_cast<uint32_t>(m[1]) << 16);
+      fr[k] = *reinterpret_cast<const uint16_t *>(frow + 2 * static_cast<size_t>(xx));
     }
-    uint32_t pa[kPerThread], pb[kPerThread], pc[kPerThread], pd[kPerThread];
-    uint32_t fx[kPerThread], fy[kPerThread];
+  }
+  const int w = a.src_width, h = a.src_height;
+  uint32_t out[4];
+  if (w >= 2) {
+    const uint32_t span = static_cast<uint32_t>(h - 1) * static_cast<uint32_t>(a.src_stride) +
+                          static_cast<uint32_t>(w * C);
+    const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t *>(a.src), 0, static_cast<int>(span), kRawBufferWord3);
+    const uint32_t stride = static_cast<uint32_t>(a.src_stride);
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
-      const int mx = static_cast<int16_t>(xy[i] & 0xFFFF);
-      const int my = static_cast<int16_t>(xy[i] >> 16);
-      fx[i] = fr[i] & 31u;
-      fy[i] = (fr[i] >> 5) & 31u;
-      if (BORDER == kReplicate) {
-        // RemapS16Point5Replicate (:16-137) / get_coordinates_replicate
-        // (:621-640): fractions of negative coordinates are dropped, taps
-        // clamp to the image.
-        if (mx < 0) fx[i] = 0;
-        if (my < 0) fy[i] = 0;
-        const int x0 = min(max(mx, 0), xmax), y0 = min(max(my, 0), ymax);
-        const int y1 = y0 + (y0 < ymax ? 1 : 0);
-        const bool adj = x0 < xmax;
-        const bool tail = (x0 & ~3) + 7 <= xmax;
-        const uint8_t *r0 = a.src + static_cast<size_t>(y0) * a.src_stride;
-        const uint8_t *r1 = a.src + static_cast<size_t>(y1) * a.src_stride;
-        load_pair<C>(r0, x0, adj, tail || (y0 < ymax && xmax >= 7), src_al, pa[i], pb[i]);
-        load_pair<C>(r1, x0, adj, tail || (y1 < ymax && xmax >= 7), src_al, pc[i], pd[i]);
-      } else {
-        // RemapS16Point5ConstantBorder (:321-425) / get_coordinates_constant
-        // (:860-881): each corner tests its own range with 16-bit unsigned
-        // compares and takes the border value when outside.
-        const uint32_t ux0 = static_cast<uint16_t>(mx), uy0 = static_cast<uint16_t>(my);
-        const uint32_t ux1 = static_cast<uint16_t>(ux0 + 1);
-        const uint32_t uy1 = static_cast<uint16_t>(uy0 + 1);
-        const bool x0in = ux0 < static_cast<uint32_t>(a.src_width);
-        const bool x1in = ux1 < static_cast<uint32_t>(a.src_width);
-        const bool y0in = uy0 < static_cast<uint32_t>(a.src_height);
-        const bool y1in = uy1 < static_cast<uint32_t>(a.src_height);
-        const uint32_t ry0 = y0in ? uy0 : 0, ry1 = y1in ? uy1 : 0;
-        const uint8_t *r0 = a.src + static_cast<size_t>(ry0) * a.src_stride;
-        const uint8_t *r1 = a.src + static_cast<size_t>(ry1) * a.src_stride;
-        if (x0in && x1in) {
-          // both columns inside: one paired load per row that is inside
-          const bool tail = (ux0 & ~3u) + 7 <= static_cast<uint32_t>(xmax);
-          uint32_t t0, t1, b0, b1;
-          load_pair<C>(r0, ux0, true, tail || (ry0 < static_cast<uint32_t>(ymax) && xmax >= 7),
-                      src_al, t0, t1);
-          load_pair<C>(r1, ux0, true, tail || (ry1 < static_cast<uint32_t>(ymax) && xmax >= 7),
-                      src_al, b0, b1);
-          pa[i] = y0in ? t0 : bv;
-          pb[i] = y0in ? t1 : bv;
-          pc[i] = y1in ? b0 : bv;
-          pd[i] = y1in ? b1 : bv;
-        } else {
-          pa[i] = (x0in && y0in) ? load_px<C>(r0 + ux0 * C, src_al) : bv;
-          pb[i] = (x1in && y0in) ? load_px<C>(r0 + ux1 * C, src_al) : bv;
-          pc[i] = (x0in && y1in) ? load_px<C>(r1 + ux0 * C, src_al) : bv;
-          pd[i] = (x1in && y1in) ? load_px<C>(r1 + ux1 * C, src_al) : bv;
-        }
-      }
-    }
+    for (int k = 0; k < 4; ++k) out[k] = remap_s16_px<C, BORDER>(src, stride, w, h, xy[k], fr[k], bv);
+  } else {
 #pragma unroll
-    for (int i = 0; i < kPerThread; ++i) {
-      if (!ok[i]) continue;
-      uint32_t out = 0;
-#pragma unroll
-      for (int c = 0; c < C; ++c)
-        out |= min(bilinear5(byte_of(pa[i], c), byte_of(pb[i], c), byte_of(pc[i], c),
-                             byte_of(pd[i], c), fx[i], fy[i]), 255u) << (8 * c);
-      store_px<C>(a.dst + static_cast<size_t>(py[i]) * a.dst_stride + px[i] * C, out, dst_al);
-    }
+    for (int k = 0; k < 4; ++k) out[k] = remap_s16_px_narrow<C, BORDER>(a, xy[k], fr[k], bv);
+  }
+  uint8_t *d = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x) * C;
+  if (C == 1 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 3) == 0) {
+    *reinterpret_cast<uint32_t *>(d) = out[0] | (out[1] << 8) | (out[2] << 16) | (out[3] << 24);
+  } else if (C == 4 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 15) == 0) {
+    *reinterpret_cast<uint4 *>(d) = make_uint4(out[0], out[1], out[2], out[3]);
+  } else {
+    const bool al = (reinterpret_cast<uintptr_t>(d) & 3) == 0;
+    for (int k = 0; k < n; ++k) store_px<C>(d + k * C, out[k], al);
   }
 }
 
@@ -238,13 +273,87 @@ __device__ __forceinline__ int32_t cvta_s32(float f) {
   return static_cast<int32_t>(r);
 }
 
+// Source image of a float gather.  When the image spans less than 2 GiB its
+// bytes are read with buffer loads at 32-bit offsets (no 64-bit address
+// arithmetic per tap), else with global loads at 64-bit addresses; the
+// choice is uniform over a launch.  Loads may be unaligned (gfx950 global
+// and buffer accesses are byte-addressed in unaligned mode) and never read
+// past the last byte of the image.
 struct Src {
   const uint8_t *p;
   size_t stride;
+  uint32_t stride32;
   uint32_t xmax, ymax;
-  bool aligned4;
   uint32_t bv;  // border value bytes, one per channel
+  __amdgpu_buffer_rsrc_t rsrc;
 };
+
+template <bool BUF, int NB>
+__device__ __forceinline__ uint32_t fetch(const Src &s, uint32_t y, uint32_t xb) {
+  if (BUF) {
+    const int off = static_cast<int>(y * s.stride32 + xb);
+    if (NB == 1) return __builtin_amdgcn_raw_buffer_load_b8(s.rsrc, off, 0, 0);
+    if (NB == 2) return __builtin_amdgcn_raw_buffer_load_b16(s.rsrc, off, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(s.rsrc, off, 0, 0);
+  }
+  const uint8_t *q = s.p + static_cast<size_t>(y) * s.stride + xb;
+  typedef uint16_t u16u __attribute__((aligned(1)));
+  typedef uint32_t u32u __attribute__((aligned(1)));
+  if (NB == 1) return *q;
+  if (NB == 2) return *reinterpret_cast<const u16u *>(q);
+  return *reinterpret_cast<const u32u *>(q);
+}
+
+template <bool BUF>
+__device__ __forceinline__ uint64_t fetch64(const Src &s, uint32_t y, uint32_t xb) {
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  if (BUF) {
+    const int off = static_cast<int>(y * s.stride32 + xb);
+    const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(s.rsrc, off, 0, 0));
+    return (static_cast<uint64_t>(v.y) << 32) | v.x;
+  }
+  typedef uint64_t u64u __attribute__((aligned(1)));
+  return *reinterpret_cast<const u64u *>(s.p + static_cast<size_t>(y) * s.stride + xb);
+}
+
+// Pixel x of row y, C bytes packed in a dword.
+template <int C, bool BUF>
+__device__ __forceinline__ uint32_t fetch_px(const Src &s, uint32_t y, uint32_t x) {
+  const uint32_t xb = x * C;
+  if (C == 1) return fetch<BUF, 1>(s, y, xb);
+  if (C == 2) return fetch<BUF, 2>(s, y, xb);
+  if (C == 3) return fetch<BUF, 2>(s, y, xb) | (fetch<BUF, 1>(s, y, xb + 2) << 16);
+  return fetch<BUF, 4>(s, y, xb);
+}
+
+// Pixels cx and cx + 1 of row y (cx + 1 <= xmax), or pixel 0 twice for an
+// image one pixel wide.
+template <int C, bool BUF>
+__device__ __forceinline__ void fetch_pair(const Src &s, uint32_t y, uint32_t cx, uint32_t &p0,
+                                           uint32_t &p1) {
+  if (s.xmax == 0) {
+    p0 = p1 = fetch_px<C, BUF>(s, y, 0);
+    return;
+  }
+  const uint32_t xb = cx * C;
+  if (C == 1) {
+    const uint32_t t = fetch<BUF, 2>(s, y, xb);
+    p0 = t & 0xFFu;
+    p1 = t >> 8;
+  } else if (C == 2) {
+    const uint32_t t = fetch<BUF, 4>(s, y, xb);
+    p0 = t & 0xFFFFu;
+    p1 = t >> 16;
+  } else if (C == 3) {
+    const uint32_t lo = fetch<BUF, 4>(s, y, xb), hi = fetch<BUF, 2>(s, y, xb + 4);
+    p0 = lo & 0xFFFFFFu;
+    p1 = (lo >> 24) | (hi << 8);
+  } else {
+    const uint64_t t = fetch64<BUF>(s, y, xb);
+    p0 = static_cast<uint32_t>(t);
+    p1 = static_cast<uint32_t>(t >> 32);
+  }
+}
 
 // Gathered corners of one pixel, ready for the lerp.
 struct Quad {
@@ -252,47 +361,54 @@ struct Quad {
   float xfrac, yfrac;
 };
 
-// load_quad_pixels_{replicate,constant} (transform_neon.h:189-329).
-template <int C, int BORDER>
+// load_quad_pixels_{replicate,constant} (transform_neon.h:189-329).  Each
+// row is one pair load at cx = the left tap clamped to [0, xmax - 1]; the
+// taps are then picked from the pair (index 0 or 1) and, for the constant
+// border, replaced by the border value when outside the image.  No branch
+// depends on the pixel, so the loads of all pixels of a thread overlap.
+template <int C, int BORDER, bool BUF>
 __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) {
   Quad q;
   q.xfrac = xf - floorf(xf);
   q.yfrac = yf - floorf(yf);
+  uint32_t cx, y0, y1, i0, i1;
+  bool ina = true, inb = true, inc = true, ind = true;
   if (BORDER == kReplicate) {
     const uint32_t x0 = min(cvtm_u32(xf), s.xmax);
-    const uint32_t y0 = min(cvtm_u32(yf), s.ymax);
+    y0 = min(cvtm_u32(yf), s.ymax);
     const uint32_t x1 = x0 + ((xf >= 0.0f && x0 < s.xmax) ? 1u : 0u);
-    const uint32_t y1 = y0 + ((yf >= 0.0f && y0 < s.ymax) ? 1u : 0u);
-    const uint8_t *r0 = s.p + static_cast<size_t>(y0) * s.stride;
-    const uint8_t *r1 = s.p + static_cast<size_t>(y1) * s.stride;
-    const bool adj = x1 != x0;
-    const bool tail = (x0 & ~3u) + 7 <= s.xmax;
-    load_pair<C>(r0, x0, adj, tail || (y0 < s.ymax && s.xmax >= 7), s.aligned4, q.a, q.b);
-    load_pair<C>(r1, x0, adj, tail || (y1 < s.ymax && s.xmax >= 7), s.aligned4, q.c, q.d);
+    y1 = y0 + ((yf >= 0.0f && y0 < s.ymax) ? 1u : 0u);
+    cx = s.xmax ? min(x0, s.xmax - 1) : 0u;
+    i0 = x0 - cx;
+    i1 = x1 - cx;
   } else {
-    const uint32_t x0 = static_cast<uint32_t>(cvtm_s32(xf));
-    const uint32_t y0 = static_cast<uint32_t>(cvtm_s32(yf));
-    const uint32_t x1 = x0 + 1u, y1 = y0 + 1u;
+    const int32_t ix = cvtm_s32(xf), iy = cvtm_s32(yf);
+    const uint32_t x0 = static_cast<uint32_t>(ix), x1 = x0 + 1u;
+    const uint32_t ya = static_cast<uint32_t>(iy), yb = ya + 1u;
     const bool x0in = x0 <= s.xmax, x1in = x1 <= s.xmax;
-    const bool y0in = y0 <= s.ymax, y1in = y1 <= s.ymax;
-    const uint32_t ry0 = y0in ? y0 : 0, ry1 = y1in ? y1 : 0;
-    const uint8_t *r0 = s.p + static_cast<size_t>(ry0) * s.stride;
-    const uint8_t *r1 = s.p + static_cast<size_t>(ry1) * s.stride;
-    if (x0in && x1in) {
-      const bool tail = (x0 & ~3u) + 7 <= s.xmax;
-      uint32_t t0, t1, b0, b1;
-      load_pair<C>(r0, x0, true, tail || (ry0 < s.ymax && s.xmax >= 7), s.aligned4, t0, t1);
-      load_pair<C>(r1, x0, true, tail || (ry1 < s.ymax && s.xmax >= 7), s.aligned4, b0, b1);
-      q.a = y0in ? t0 : s.bv;
-      q.b = y0in ? t1 : s.bv;
-      q.c = y1in ? b0 : s.bv;
-      q.d = y1in ? b1 : s.bv;
-    } else {
-      q.a = (x0in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
-      q.b = (x1in && y0in) ? load_px<C>(r0 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
-      q.c = (x0in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x0) * C, s.aligned4) : s.bv;
-      q.d = (x1in && y1in) ? load_px<C>(r1 + static_cast<size_t>(x1) * C, s.aligned4) : s.bv;
-    }
+    const bool y0in = ya <= s.ymax, y1in = yb <= s.ymax;
+    ina = x0in && y0in;
+    inb = x1in && y0in;
+    inc = x0in && y1in;
+    ind = x1in && y1in;
+    y0 = y0in ? ya : 0u;
+    y1 = y1in ? yb : 0u;
+    cx = s.xmax ? static_cast<uint32_t>(max(min(ix, static_cast<int32_t>(s.xmax) - 1), 0)) : 0u;
+    i0 = (x0 - cx) & 1u;  // meaningful only when the tap is inside
+    i1 = (x1 - cx) & 1u;
+  }
+  uint32_t a0, a1, b0, b1;
+  fetch_pair<C, BUF>(s, y0, cx, a0, a1);
+  fetch_pair<C, BUF>(s, y1, cx, b0, b1);
+  q.a = i0 ? a1 : a0;
+  q.b = i1 ? a1 : a0;
+  q.c = i0 ? b1 : b0;
+  q.d = i1 ? b1 : b0;
+  if (BORDER != kReplicate) {
+    q.a = ina ? q.a : s.bv;
+    q.b = inb ? q.b : s.bv;
+    q.c = inc ? q.c : s.bv;
+    q.d = ind ? q.d : s.bv;
   }
   return q;
 }
@@ -300,9 +416,9 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
 // FCVTAU of a lerp result: round half away from zero, NaN to 0.  A lerp of
 // bytes with a fraction in [0, 1) is NaN or lies in [0, 255] (the fused
 // steps cannot round outside the range of their inputs), so no saturation
-// test is needed; the NaN test also covers signalling NaNs.
+// test is needed; fmaxf(v, 0) maps a (quiet) NaN to 0.
 __device__ __forceinline__ uint32_t round_lerp_u8(float v) {
-  const float c = v == v ? v : 0.0f;
+  const float c = fmaxf(v, 0.0f);
   const float t = truncf(c);
   return static_cast<uint32_t>(t) + (c - t >= 0.5f ? 1u : 0u);
 }
@@ -326,20 +442,18 @@ __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
 }
 
 // transform_pixels_{replicate,constant} (transform_neon.h:339-491).
-template <int C, int BORDER>
+template <int C, int BORDER, bool BUF>
 __device__ __forceinline__ uint32_t gather_nearest(const Src &s, float xf, float yf) {
   if (BORDER == kReplicate) {
     const uint32_t x = min(cvta_u32(xf), s.xmax);
     const uint32_t y = min(cvta_u32(yf), s.ymax);
-    return load_px<C>(s.p + static_cast<size_t>(y) * s.stride + static_cast<size_t>(x) * C,
-                      s.aligned4);
+    return fetch_px<C, BUF>(s, y, x);
   }
   const uint32_t x = static_cast<uint32_t>(cvta_s32(xf));
   const uint32_t y = static_cast<uint32_t>(cvta_s32(yf));
-  if (x <= s.xmax && y <= s.ymax)
-    return load_px<C>(s.p + static_cast<size_t>(y) * s.stride + static_cast<size_t>(x) * C,
-                      s.aligned4);
-  return s.bv;
+  const bool in = x <= s.xmax && y <= s.ymax;
+  const uint32_t v = fetch_px<C, BUF>(s, in ? y : 0u, in ? x : 0u);
+  return in ? v : s.bv;
 }
 
 template <int C>
@@ -348,55 +462,111 @@ __device__ __forceinline__ Src make_src(const uint8_t *p, size_t stride, int w, 
   Src s;
   s.p = p;
   s.stride = stride;
+  s.stride32 = static_cast<uint32_t>(stride);
   s.xmax = static_cast<uint32_t>(w - 1);
   s.ymax = static_cast<uint32_t>(h - 1);
-  s.aligned4 = ((reinterpret_cast<uintptr_t>(p) | stride) & 3) == 0;
   s.bv = 0;
 #pragma unroll
   for (int c = 0; c < C; ++c) s.bv |= static_cast<uint32_t>(border_value[c]) << (8 * c);
+  const uint64_t span = static_cast<uint64_t>(h - 1) * stride + static_cast<uint64_t>(w) * C;
+  s.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(p), 0,
+                                             static_cast<int>(span < (1ull << 31) ? span : 0),
+                                             kRawBufferWord3);
   return s;
 }
 
-// Shared body: `coords(x, y, xf, yf)` gives the source position of a pixel.
+__device__ __forceinline__ bool src_fits_buffer(const Src &s, int c) {
+  return static_cast<uint64_t>(s.ymax) * s.stride + (static_cast<uint64_t>(s.xmax) + 1) * c <
+         (1ull << 31);
+}
+
+// The 4 destination pixels (x .. x + n - 1) of this thread, C bytes each,
+// as one 4/8/12/16-byte store when whole and aligned.
+template <int C>
+__device__ __forceinline__ void store_quad_px(uint8_t *d, int n, const uint32_t (&o)[4]) {
+  const uintptr_t al = reinterpret_cast<uintptr_t>(d);
+  if (n == 4) {
+    if (C == 1 && (al & 3) == 0) {
+      *reinterpret_cast<uint32_t *>(d) = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+      return;
+    }
+    if (C == 2 && (al & 7) == 0) {
+      *reinterpret_cast<uint2 *>(d) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+      return;
+    }
+    if (C == 3 && (al & 3) == 0) {
+      uint32_t *w = reinterpret_cast<uint32_t *>(d);
+      w[0] = o[0] | (o[1] << 24);
+      w[1] = (o[1] >> 8) | (o[2] << 16);
+      w[2] = (o[2] >> 16) | (o[3] << 8);
+      return;
+    }
+    if (C == 4 && (al & 15) == 0) {
+      *reinterpret_cast<uint4 *>(d) = make_uint4(o[0], o[1], o[2], o[3]);
+      return;
+    }
+  }
+  for (int k = 0; k < n; ++k) store_px<C>(d + k * C, o[k], (al & 3) == 0 && C == 4);
+}
+
+// Shared body of the float gathers over a kGatherTileW x kGatherTileH tile:
+// a thread owns 4 horizontally adjacent pixels of one row; `coords(y, x, n,
+// xf, yf)` fills the source positions of those pixels.
+template <int C, int INTERP, int BORDER, bool BUF>
+__device__ __forceinline__ void gather_quad(const Src &s, const float (&xf)[4],
+                                            const float (&yf)[4], uint32_t (&out)[4]) {
+  if (INTERP == 1) {
+    Quad q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = gather_linear<C, BORDER, BUF>(s, xf[k], yf[k]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = lerp_quad<C>(q[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) out[k] = gather_nearest<C, BORDER, BUF>(s, xf[k], yf[k]);
+  }
+}
+
 template <int C, int INTERP, int BORDER, class Coords>
 __device__ __forceinline__ void gather_tiles(const Src &s, uint8_t *dst, size_t dst_stride,
                                              int dst_width, int dst_height, int bx,
                                              int by, const Coords &coords) {
-  const bool dst_al = ((reinterpret_cast<uintptr_t>(dst) | dst_stride) & 3) == 0;
-  int px[kPerThread], py[kPerThread];
-  bool ok[kPerThread];
-  float xf[kPerThread], yf[kPerThread];
-#pragma unroll
-  for (int i = 0; i < kPerThread; ++i) {
-    pixel_of(i, bx, by, px[i], py[i]);
-    ok[i] = px[i] < dst_width && py[i] < dst_height;
-    coords(ok[i] ? px[i] : 0, ok[i] ? py[i] : 0, xf[i], yf[i]);
-  }
-  uint32_t out[kPerThread];
-  if (INTERP == 1) {
-    Quad q[kPerThread];
-#pragma unroll
-    for (int i = 0; i < kPerThread; ++i) q[i] = gather_linear<C, BORDER>(s, xf[i], yf[i]);
-#pragma unroll
-    for (int i = 0; i < kPerThread; ++i) out[i] = lerp_quad<C>(q[i]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < kPerThread; ++i) out[i] = gather_nearest<C, BORDER>(s, xf[i], yf[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < kPerThread; ++i)
-    if (ok[i])
-      store_px<C>(dst + static_cast<size_t>(py[i]) * dst_stride + px[i] * C, out[i], dst_al);
+  const int q = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
+  if (x >= dst_width || y >= dst_height) return;
+  const int n = min(4, dst_width - x);
+  float xf[4], yf[4];
+  coords(y, x, n, xf, yf);
+  uint32_t out[4];
+  if (src_fits_buffer(s, C))
+    gather_quad<C, INTERP, BORDER, true>(s, xf, yf, out);
+  else
+    gather_quad<C, INTERP, BORDER, false>(s, xf, yf, out);
+  store_quad_px<C>(dst + static_cast<size_t>(y) * dst_stride + static_cast<size_t>(x) * C, n,
+                   out);
 }
 
 template <int C, int INTERP, int BORDER>
 __device__ __forceinline__ void remap_f32_tile(const RemapF32Args &a, int bx, int by) {
   const Src s = make_src<C>(a.src, a.src_stride, a.src_width, a.src_height, a.border_value);
-  auto coords = [&](int x, int y, float &xf, float &yf) {
-    xf = *reinterpret_cast<const float *>(reinterpret_cast<const uint8_t *>(a.mapx) +
-                                          y * a.mapx_stride + 4 * x);
-    yf = *reinterpret_cast<const float *>(reinterpret_cast<const uint8_t *>(a.mapy) +
-                                          y * a.mapy_stride + 4 * x);
+  auto coords = [&](int y, int x, int n, float (&xf)[4], float (&yf)[4]) {
+    const uint8_t *mx = reinterpret_cast<const uint8_t *>(a.mapx) + static_cast<size_t>(y) * a.mapx_stride;
+    const uint8_t *my = reinterpret_cast<const uint8_t *>(a.mapy) + static_cast<size_t>(y) * a.mapy_stride;
+    const float *px = reinterpret_cast<const float *>(mx) + x;
+    const float *py = reinterpret_cast<const float *>(my) + x;
+    if (n == 4 && ((reinterpret_cast<uintptr_t>(px) | reinterpret_cast<uintptr_t>(py)) & 15) == 0) {
+      const float4 vx = *reinterpret_cast<const float4 *>(px);
+      const float4 vy = *reinterpret_cast<const float4 *>(py);
+      xf[0] = vx.x; xf[1] = vx.y; xf[2] = vx.z; xf[3] = vx.w;
+      yf[0] = vy.x; yf[1] = vy.y; yf[2] = vy.z; yf[3] = vy.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int kk = k < n ? k : 0;
+        xf[k] = px[kk];
+        yf[k] = py[kk];
+      }
+    }
   };
   gather_tiles<C, INTERP, BORDER>(s, a.dst, a.dst_stride, a.dst_width, a.dst_height, bx, by,
                                   coords);
@@ -405,19 +575,23 @@ __device__ __forceinline__ void remap_f32_tile(const RemapF32Args &a, int bx, in
 template <int C, int INTERP, int BORDER>
 __device__ __forceinline__ void warp_tile(const WarpArgs &a, int bx, int by) {
   const Src s = make_src<C>(a.src, a.src_stride, a.src_width, a.src_height, a.border_value);
-  // warp_perspective_neon.cpp:55-76 and :98-102
-  auto coords = [&](int x, int y, float &xf, float &yf) {
+  // warp_perspective_neon.cpp:55-76 and :98-102: the row terms are shared by
+  // the pixels of a row, each pixel adds its column term with one fma.
+  auto coords = [&](int y, int x, int, float (&xf)[4], float (&yf)[4]) {
     const float fy = static_cast<float>(y);
     const float tx0 = fmaf(a.t[1], fy, a.t[2]);
     const float ty0 = fmaf(a.t[4], fy, a.t[5]);
     const float tw0 = fmaf(a.t[7], fy, a.t[8]);
-    const float fx = static_cast<float>(static_cast<uint32_t>(x));
-    const float tx = fmaf(fx, a.t[0], tx0);
-    const float ty = fmaf(fx, a.t[3], ty0);
-    const float tw = fmaf(fx, a.t[6], tw0);
-    const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : 1.0f / tw;
-    xf = tx * iw;
-    yf = ty * iw;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float fx = static_cast<float>(static_cast<uint32_t>(x + k));
+      const float tx = fmaf(fx, a.t[0], tx0);
+      const float ty = fmaf(fx, a.t[3], ty0);
+      const float tw = fmaf(fx, a.t[6], tw0);
+      const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : 1.0f / tw;
+      xf[k] = tx * iw;
+      yf[k] = ty * iw;
+    }
   };
   gather_tiles<C, INTERP, BORDER>(s, a.dst, a.dst_stride, a.dst_width, a.dst_height, bx, by,
                                   coords);
