@@ -54,6 +54,8 @@ def parse():
                     help="skip the batched workloads (pyramid and transform batches)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the serving-loop workload")
+    ap.add_argument("--no-host-api", action="store_true",
+                    help="skip the drop-in (host buffer) entry point timings")
     ap.add_argument("--serving-requests", type=int, default=1500)
     return ap.parse_args()
 
@@ -447,9 +449,13 @@ def main():
     step_ms = start.elapsed_time(stop) / args.steps
 
     # Per-op duration: the same steps again with an event recorded on the
-    # launch stream between consecutive ops.  A recorded event waits for the
-    # op before it to finish, so these steps run slightly longer than the
-    # ones above; each op's own time is what rocprofv3 reports for it.
+    # launch stream between consecutive ops.  Each recorded event adds a
+    # fixed delay (the next op is dispatched only after the timestamp is
+    # written), which the marked steps show as their excess over the plain
+    # steps above; that excess is shared evenly among the ops, so the per-op
+    # times add up to the plain step time.  What remains of an op's time is
+    # its kernels' duration plus the gap between two launches, the figure
+    # rocprofv3 --kernel-trace reports (less that gap).
     marks = [[torch.cuda.Event(enable_timing=True) for _ in range(len(ops) + 1)]
              for _ in range(args.steps)]
     torch.cuda.synchronize()
@@ -460,9 +466,11 @@ def main():
             run_op(K, torch, name, s, out, kx, ky, T, stream)
             marks[i][k + 1].record(stream)
     torch.cuda.synchronize()
-    op_ms = {}
+    marked = {}
     for k, (name, _, _) in enumerate(ops):
-        op_ms[name] = sum(m[k].elapsed_time(m[k + 1]) for m in marks) / args.steps
+        marked[name] = sum(m[k].elapsed_time(m[k + 1]) for m in marks) / args.steps
+    event_delay = max(0.0, (sum(marked.values()) - step_ms) / len(ops))
+    op_ms = {name: marked[name] - event_delay for name in marked}
 
     per_op = {}
     total_pix = 0
@@ -497,6 +505,9 @@ def main():
                                "(%d ops per step, %d rotating input sets)" % (len(ops), args.sets),
                    "ops": [o[0] for o in ops]},
         "per_op": per_op,
+        "per_op_note": "kernel time of each op inside the timed steps: event-bracketed time "
+                       "less the per-event delay %.4f ms (marked minus plain step time, per op)"
+                       % event_delay,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gb_s"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
                      "traffic": d.get("traffic")},
@@ -513,7 +524,7 @@ def main():
     if not args.no_serving and not args.ops:
         result["serving"] = bench_serving(torch, dev, args.serving_requests)
     s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
-    if not args.ops:
+    if not args.ops and not args.no_host_api:
         result["including_host_copies"] = host_api(K, np, s_host, kx, ky, T, ops)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)

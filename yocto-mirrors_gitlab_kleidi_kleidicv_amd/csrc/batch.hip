@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kThreads) void batch_copy_kernel(Segment s) {
       const uint8_t *sp = a.src + static_cast<size_t>(y) * a.src_stride + e0;
       uint8_t *dp = a.dst + static_cast<size_t>(y) * a.dst_stride + e0;
       if (vec) {
-        *reinterpret_cast<uint4 *>(dp) = *reinterpret_cast<const uint4 *>(sp);
+        store_nt(reinterpret_cast<uint4 *>(dp), *reinterpret_cast<const uint4 *>(sp));
       } else {
         for (int k = 0; k < 16 && e0 + k < a.row_bytes; ++k) dp[k] = sp[k];
       }

@@ -113,7 +113,7 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
 __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
                                            uint32_t packed) {
   if (e + 3 < row_bytes && (reinterpret_cast<uintptr_t>(row + e) & 3) == 0) {
-    *reinterpret_cast<uint32_t *>(row + e) = packed;
+    store_nt(reinterpret_cast<uint32_t *>(row + e), packed);
   } else {
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -199,7 +199,7 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
     const int e = e0 + 8 * tx;
     if (e + 7 < row_bytes && (reinterpret_cast<uintptr_t>(out + e) & 7) == 0) {
-      *reinterpret_cast<uint2 *>(out + e) = make_uint2(lo, hi);
+      store_nt(reinterpret_cast<uint2 *>(out + e), make_uint2(lo, hi));
     } else {
       store_quad(out, e, row_bytes, lo);
       store_quad(out, e + 4, row_bytes, hi);

@@ -36,6 +36,22 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
   return x * per + (x < rem ? x : rem) + slot;
 }
 
+// Output stores.  Each output byte is written once and read by no later
+// kernel of the same single-image call, so whole-vector output stores carry
+// the non-temporal hint: they stream through the caches instead of
+// displacing cached inputs (measured on the 4K 2x upscale: 36 -> 26 us).
+__device__ __forceinline__ void store_nt(uint32_t *p, uint32_t v) {
+  __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ void store_nt(uint2 *p, uint2 v) {
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(u32x2{v.x, v.y}, reinterpret_cast<u32x2 *>(p));
+}
+__device__ __forceinline__ void store_nt(uint4 *p, uint4 v) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4 *>(p));
+}
+
 // Destination tile of the gather kernels (remap s16point5, remap f32, warp
 // perspective), per-image and batched.
 constexpr int kGatherTileW = 64, kGatherTileH = 16;

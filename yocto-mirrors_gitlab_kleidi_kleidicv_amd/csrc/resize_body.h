@@ -578,7 +578,7 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
     if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
-      *reinterpret_cast<uint32_t *>(out + e0) = packed;
+      store_nt(reinterpret_cast<uint32_t *>(out + e0), packed);
     } else {
       for (int k = 0; k < 4 && e0 + k < n; ++k)
         out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
@@ -628,7 +628,7 @@ __device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
       o.y = quarter4(u64(t0.z, t0.w), u64(b0.z, b0.w), C);
       o.z = quarter4(u64(t1.x, t1.y), u64(b1.x, b1.y), C);
       o.w = quarter4(u64(t1.z, t1.w), u64(b1.z, b1.w), C);
-      *reinterpret_cast<uint4 *>(out + e0) = o;
+      store_nt(reinterpret_cast<uint4 *>(out + e0), o);
     } else {
       for (int k = 0; k < 16 && e0 + k < n; ++k) {
         const int e = e0 + k;
@@ -713,7 +713,7 @@ __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) 
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 8 * static_cast<size_t>(m0);
     if (n_out == 4 && out_al) {
-      *reinterpret_cast<uint4 *>(out) = make_uint4(o[0], o[1], o[2], o[3]);
+      store_nt(reinterpret_cast<uint4 *>(out), make_uint4(o[0], o[1], o[2], o[3]));
     } else {
       for (int k = 0; k < n_out; ++k)
 #pragma unroll

@@ -230,9 +230,9 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
   }
   uint8_t *d = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x) * C;
   if (C == 1 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 3) == 0) {
-    *reinterpret_cast<uint32_t *>(d) = out[0] | (out[1] << 8) | (out[2] << 16) | (out[3] << 24);
+    store_nt(reinterpret_cast<uint32_t *>(d), out[0] | (out[1] << 8) | (out[2] << 16) | (out[3] << 24));
   } else if (C == 4 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 15) == 0) {
-    *reinterpret_cast<uint4 *>(d) = make_uint4(out[0], out[1], out[2], out[3]);
+    store_nt(reinterpret_cast<uint4 *>(d), make_uint4(out[0], out[1], out[2], out[3]));
   } else {
     const bool al = (reinterpret_cast<uintptr_t>(d) & 3) == 0;
     for (int k = 0; k < n; ++k) store_px<C>(d + k * C, out[k], al);
@@ -487,22 +487,22 @@ __device__ __forceinline__ void store_quad_px(uint8_t *d, int n, const uint32_t 
   const uintptr_t al = reinterpret_cast<uintptr_t>(d);
   if (n == 4) {
     if (C == 1 && (al & 3) == 0) {
-      *reinterpret_cast<uint32_t *>(d) = o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24);
+      store_nt(reinterpret_cast<uint32_t *>(d), o[0] | (o[1] << 8) | (o[2] << 16) | (o[3] << 24));
       return;
     }
     if (C == 2 && (al & 7) == 0) {
-      *reinterpret_cast<uint2 *>(d) = make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16));
+      store_nt(reinterpret_cast<uint2 *>(d), make_uint2(o[0] | (o[1] << 16), o[2] | (o[3] << 16)));
       return;
     }
     if (C == 3 && (al & 3) == 0) {
       uint32_t *w = reinterpret_cast<uint32_t *>(d);
-      w[0] = o[0] | (o[1] << 24);
-      w[1] = (o[1] >> 8) | (o[2] << 16);
-      w[2] = (o[2] >> 16) | (o[3] << 8);
+      store_nt(w, o[0] | (o[1] << 24));
+      store_nt(w + 1, (o[1] >> 8) | (o[2] << 16));
+      store_nt(w + 2, (o[2] >> 16) | (o[3] << 8));
       return;
     }
     if (C == 4 && (al & 15) == 0) {
-      *reinterpret_cast<uint4 *>(d) = make_uint4(o[0], o[1], o[2], o[3]);
+      store_nt(reinterpret_cast<uint4 *>(d), make_uint4(o[0], o[1], o[2], o[3]));
       return;
     }
   }

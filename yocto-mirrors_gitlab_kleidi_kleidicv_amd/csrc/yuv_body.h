@@ -101,8 +101,8 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
       uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x0) * OC;
       if (full && out_al) {
         if (OC == 4) {
-          reinterpret_cast<uint4 *>(out)[0] = make_uint4(px[0], px[1], px[2], px[3]);
-          reinterpret_cast<uint4 *>(out)[1] = make_uint4(px[4], px[5], px[6], px[7]);
+          store_nt(reinterpret_cast<uint4 *>(out), make_uint4(px[0], px[1], px[2], px[3]));
+          store_nt(reinterpret_cast<uint4 *>(out) + 1, make_uint4(px[4], px[5], px[6], px[7]));
         } else {
           // 8 RGB pixels = 24 bytes = three 8-byte words
           uint32_t w[6];
@@ -116,9 +116,9 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
               w[byte >> 2] |= ((px[k] >> (8 * c)) & 0xFF) << (8 * (byte & 3));
             }
           uint2 *o = reinterpret_cast<uint2 *>(out);
-          o[0] = make_uint2(w[0], w[1]);
-          o[1] = make_uint2(w[2], w[3]);
-          o[2] = make_uint2(w[4], w[5]);
+          store_nt(o, make_uint2(w[0], w[1]));
+          store_nt(o + 1, make_uint2(w[2], w[3]));
+          store_nt(o + 2, make_uint2(w[4], w[5]));
         }
       } else {
         for (int k = 0; k < kBlockW && x0 + k < a.width; ++k)
