@@ -384,3 +384,28 @@ def test_device_api_matches_host_api(rng):
     assert_same(outs["median"].cpu().numpy(), K.median_blur(src, 5, "REPLICATE"))
     assert_same(outs["resize"].cpu().numpy(), K.resize_linear(src, 125, 60))
     assert_same(outs["resize_up"].cpu().numpy(), K.resize_linear(src, 333, 200))
+
+
+# ---------------------------------------------------------------------------
+# The warp's refined reciprocal equals the IEEE quotient over every float of
+# its range (the library's own self-check kernel; about 0.1 s).
+def test_refined_reciprocal_is_correctly_rounded():
+    import ctypes as C
+    fn = K.lib().kcv_selfcheck_reciprocal
+    fn.argtypes = [C.POINTER(C.c_ulonglong)]
+    fn.restype = C.c_int
+    bad = C.c_ulonglong(123)
+    assert fn(C.byref(bad)) == 0
+    assert bad.value == 0
+
+
+# The lerp rounding of the float gathers (a hardware conversion) equals
+# round-half-away over every float in [0, 256] (self-check kernel).
+def test_lerp_rounding_is_half_away():
+    import ctypes as C
+    fn = K.lib().kcv_selfcheck_rounding
+    fn.argtypes = [C.POINTER(C.c_ulonglong)]
+    fn.restype = C.c_int
+    bad = C.c_ulonglong(123)
+    assert fn(C.byref(bad)) == 0
+    assert bad.value == 0

@@ -115,4 +115,69 @@ hipError_t launch_warp_perspective(const WarpArgs &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
+namespace {
+
+// recip_refined against the IEEE quotient for the 2^24 float bit patterns
+// starting at `base`; lanes outside recip_refined_ok are skipped.
+__global__ __launch_bounds__(kThreads) void reciprocal_check_kernel(uint32_t base,
+                                                                    unsigned long long *bad) {
+  const uint32_t bits = base + blockIdx.x * kThreads + threadIdx.x;
+  const float w = __builtin_bit_cast(float, bits);
+  if (!recip_refined_ok(w)) return;
+  const float q = 1.0f / w;
+  if (__builtin_bit_cast(uint32_t, recip_refined(w)) != __builtin_bit_cast(uint32_t, q))
+    atomicAdd(bad, 1ull);
+}
+
+// round_half_away_u8 against truncation plus a half test for the float bit
+// patterns [base, min(base + 2^24, limit)].
+__global__ __launch_bounds__(kThreads) void rounding_check_kernel(uint32_t base, uint32_t limit,
+                                                                  unsigned long long *bad) {
+  const uint32_t bits = base + blockIdx.x * kThreads + threadIdx.x;
+  if (bits > limit) return;
+  const float x = __builtin_bit_cast(float, bits);
+  const float t = truncf(x);
+  const uint32_t want = static_cast<uint32_t>(t) + (x - t >= 0.5f ? 1u : 0u);
+  if (round_half_away_u8(x) != want) atomicAdd(bad, 1ull);
+}
+
+}  // namespace
 }  // namespace kcv
+
+// Self-check of the lerp rounding used by remap f32 and warp perspective
+// (not part of the KleidiCV API): counts the floats x in [0, 256] where the
+// hardware conversion differs from round-half-away.  Synchronous; returns 0
+// on success.
+extern "C" int kcv_selfcheck_rounding(unsigned long long *mismatches) {
+  unsigned long long *bad = nullptr;
+  if (hipMalloc(&bad, sizeof(*bad)) != hipSuccess) return 1;
+  hipError_t e = hipMemset(bad, 0, sizeof(*bad));
+  constexpr uint32_t kChunk = 1u << 24, kLimit = 0x43800000u;  // 256.0f
+  for (uint64_t base = 0; e == hipSuccess && base <= kLimit; base += kChunk) {
+    kcv::rounding_check_kernel<<<kChunk / kcv::xform::kThreads, kcv::xform::kThreads>>>(
+        static_cast<uint32_t>(base), kLimit, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(mismatches, bad, sizeof(*bad), hipMemcpyDeviceToHost);
+  hipFree(bad);
+  return e == hipSuccess ? 0 : 1;
+}
+
+// Self-check of the reciprocal used by warp perspective (not part of the
+// KleidiCV API): counts the floats w with |w| in [2^-125, 2^125] whose
+// refined hardware reciprocal differs from 1.0f / w.  Synchronous; returns
+// 0 on success.
+extern "C" int kcv_selfcheck_reciprocal(unsigned long long *mismatches) {
+  unsigned long long *bad = nullptr;
+  if (hipMalloc(&bad, sizeof(*bad)) != hipSuccess) return 1;
+  hipError_t e = hipMemset(bad, 0, sizeof(*bad));
+  constexpr uint32_t kChunk = 1u << 24;
+  for (uint64_t base = 0; e == hipSuccess && base < (uint64_t{1} << 32); base += kChunk) {
+    kcv::reciprocal_check_kernel<<<kChunk / kcv::xform::kThreads, kcv::xform::kThreads>>>(
+        static_cast<uint32_t>(base), bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(mismatches, bad, sizeof(*bad), hipMemcpyDeviceToHost);
+  hipFree(bad);
+  return e == hipSuccess ? 0 : 1;
+}

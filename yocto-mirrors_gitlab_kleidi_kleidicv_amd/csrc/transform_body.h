@@ -241,36 +241,50 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
 
 // ---------------------------------------------------------------------------
 // Float gathers (transform_neon.h:189-491).
-__device__ __forceinline__ uint32_t cvtm_u32(float f) {
-  if (!(f == f)) return 0;
-  const float r = floorf(f);
-  if (r <= 0.0f) return 0;
-  if (r >= 4294967296.0f) return 0xFFFFFFFFu;
-  return static_cast<uint32_t>(r);
+//
+// Coordinate conversions.  The reference converts with FCVTM* (floor) and
+// FCVTA* (round half away from zero), saturating, NaN -> 0, then clamps the
+// unsigned results to the image (replicate) or range-tests the signed ones
+// against it (constant).  Source sizes are below 2^24, so saturating at
+// +-2^24 instead of at the integer limits gives the same clamps and tests;
+// written this way the conversions are branch-free.
+constexpr float kIndexLimit = 16777216.0f;  // 2^24
+
+__device__ __forceinline__ uint32_t floor_u(float f) {  // fmaxf maps NaN to 0
+  return static_cast<uint32_t>(fminf(fmaxf(floorf(f), 0.0f), kIndexLimit));
 }
 
-__device__ __forceinline__ int32_t cvtm_s32(float f) {
-  if (!(f == f)) return 0;
-  const float r = floorf(f);
-  if (r >= 2147483648.0f) return 0x7FFFFFFF;
-  if (r < -2147483648.0f) return INT32_MIN;
-  return static_cast<int32_t>(r);
+__device__ __forceinline__ int32_t floor_s(float f) {
+  const float r = f == f ? floorf(f) : 0.0f;
+  return static_cast<int32_t>(fminf(fmaxf(r, -kIndexLimit), kIndexLimit));
 }
 
-__device__ __forceinline__ uint32_t cvta_u32(float f) {
-  if (!(f == f)) return 0;
-  const float r = roundf(f);  // ties away from zero
-  if (r <= 0.0f) return 0;
-  if (r >= 4294967296.0f) return 0xFFFFFFFFu;
-  return static_cast<uint32_t>(r);
+__device__ __forceinline__ uint32_t round_u(float f) {  // roundf: ties away from zero
+  return static_cast<uint32_t>(fminf(fmaxf(roundf(f), 0.0f), kIndexLimit));
 }
 
-__device__ __forceinline__ int32_t cvta_s32(float f) {
-  if (!(f == f)) return 0;
-  const float r = roundf(f);
-  if (r >= 2147483648.0f) return 0x7FFFFFFF;
-  if (r < -2147483648.0f) return INT32_MIN;
-  return static_cast<int32_t>(r);
+__device__ __forceinline__ int32_t round_s(float f) {
+  const float r = f == f ? roundf(f) : 0.0f;
+  return static_cast<int32_t>(fminf(fmaxf(r, -kIndexLimit), kIndexLimit));
+}
+
+// 1/w rounded to nearest even.  The hardware reciprocal refined by one
+// fused Newton step is bit-identical to the IEEE quotient 1.0f / w for every
+// |w| in [2^-125, 2^125] (kcv_selfcheck_reciprocal compares the two over all
+// floats); a wave with any lane outside that range divides instead.
+__device__ __forceinline__ bool recip_refined_ok(float w) {
+  const float aw = fabsf(w);
+  return aw >= 0x1p-125f && aw <= 0x1p125f;
+}
+
+__device__ __forceinline__ float recip_refined(float w) {
+  const float y0 = __builtin_amdgcn_rcpf(w);
+  return fmaf(fmaf(-w, y0, 1.0f), y0, y0);
+}
+
+__device__ __forceinline__ float recip_rn(float w) {
+  if (__builtin_amdgcn_ballot_w64(!recip_refined_ok(w)) == 0) return recip_refined(w);
+  return 1.0f / w;
 }
 
 // Source image of a float gather.  When the image spans less than 2 GiB its
@@ -374,15 +388,15 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
   uint32_t cx, y0, y1, i0, i1;
   bool ina = true, inb = true, inc = true, ind = true;
   if (BORDER == kReplicate) {
-    const uint32_t x0 = min(cvtm_u32(xf), s.xmax);
-    y0 = min(cvtm_u32(yf), s.ymax);
+    const uint32_t x0 = min(floor_u(xf), s.xmax);
+    y0 = min(floor_u(yf), s.ymax);
     const uint32_t x1 = x0 + ((xf >= 0.0f && x0 < s.xmax) ? 1u : 0u);
     y1 = y0 + ((yf >= 0.0f && y0 < s.ymax) ? 1u : 0u);
     cx = s.xmax ? min(x0, s.xmax - 1) : 0u;
     i0 = x0 - cx;
     i1 = x1 - cx;
   } else {
-    const int32_t ix = cvtm_s32(xf), iy = cvtm_s32(yf);
+    const int32_t ix = floor_s(xf), iy = floor_s(yf);
     const uint32_t x0 = static_cast<uint32_t>(ix), x1 = x0 + 1u;
     const uint32_t ya = static_cast<uint32_t>(iy), yb = ya + 1u;
     const bool x0in = x0 <= s.xmax, x1in = x1 <= s.xmax;
@@ -416,11 +430,17 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
 // FCVTAU of a lerp result: round half away from zero, NaN to 0.  A lerp of
 // bytes with a fraction in [0, 1) is NaN or lies in [0, 255] (the fused
 // steps cannot round outside the range of their inputs), so no saturation
-// test is needed; fmaxf(v, 0) maps a (quiet) NaN to 0.
+// test is needed; fmaxf(v, 0) maps a (quiet) NaN to 0.  v_cvt_rpi_i32_f32
+// (floor(x + 0.5) without an intermediate rounding) equals round-half-away
+// for every float in [0, 256] (kcv_selfcheck_rounding compares the two).
+__device__ __forceinline__ uint32_t round_half_away_u8(float c) {
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(c));
+  return static_cast<uint32_t>(r);
+}
+
 __device__ __forceinline__ uint32_t round_lerp_u8(float v) {
-  const float c = fmaxf(v, 0.0f);
-  const float t = truncf(c);
-  return static_cast<uint32_t>(t) + (c - t >= 0.5f ? 1u : 0u);
+  return round_half_away_u8(fmaxf(v, 0.0f));
 }
 
 // lerp_2d (transform_neon.h:331-337) per channel.
@@ -445,12 +465,12 @@ __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
 template <int C, int BORDER, bool BUF>
 __device__ __forceinline__ uint32_t gather_nearest(const Src &s, float xf, float yf) {
   if (BORDER == kReplicate) {
-    const uint32_t x = min(cvta_u32(xf), s.xmax);
-    const uint32_t y = min(cvta_u32(yf), s.ymax);
+    const uint32_t x = min(round_u(xf), s.xmax);
+    const uint32_t y = min(round_u(yf), s.ymax);
     return fetch_px<C, BUF>(s, y, x);
   }
-  const uint32_t x = static_cast<uint32_t>(cvta_s32(xf));
-  const uint32_t y = static_cast<uint32_t>(cvta_s32(yf));
+  const uint32_t x = static_cast<uint32_t>(round_s(xf));
+  const uint32_t y = static_cast<uint32_t>(round_s(yf));
   const bool in = x <= s.xmax && y <= s.ymax;
   const uint32_t v = fetch_px<C, BUF>(s, in ? y : 0u, in ? x : 0u);
   return in ? v : s.bv;
@@ -588,7 +608,7 @@ __device__ __forceinline__ void warp_tile(const WarpArgs &a, int bx, int by) {
       const float tx = fmaf(fx, a.t[0], tx0);
       const float ty = fmaf(fx, a.t[3], ty0);
       const float tw = fmaf(fx, a.t[6], tw0);
-      const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : 1.0f / tw;
+      const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : recip_rn(tw);
       xf[k] = tx * iw;
       yf[k] = ty * iw;
     }
