@@ -324,16 +324,90 @@ __device__ __forceinline__ void batcher_sort(us2 *a) {
     cmpex(a[BatcherNet<N>::kTable.p[c].a], a[BatcherNet<N>::kTable.p[c].b]);
 }
 
-// Two vertically adjacent K x K windows share K(K-1) values (their common
-// K-1 rows).  A shared value of rank r among those has rank r..r+K in
-// either window, so only shared ranks m-K..m (m = (K*K+1)/2) can be a
-// window's median; lower ones are always below it, higher ones above.  The
-// shared values are therefore sorted once (only those K+1 ranks are kept),
-// and each window's median is the (K+1)-th smallest of those K+1 sorted
-// values and its own K sorted values:
+// Batcher's odd-even merge of two sorted runs of N and M values, as
+// compare-exchanges on N + M registers (run A in registers 0..N-1, run B in
+// N..N+M-1).  Both runs are padded with +infinity to a power of two P and
+// the network is traced at compile time: a comparator that meets a padding
+// slot only moves a register (or nothing), so it is not emitted, and
+// rank[k] names the register holding the k-th smallest value at the end.
+template <int N, int M>
+struct MergeNet {
+  static constexpr int pow2_at_least(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+  }
+  static constexpr int P = pow2_at_least(N > M ? N : M);
+  struct Table {
+    CmpPair c[16 * P];  // a merge of 2P values has at most P * (log2(P) + 1) + 1
+    int count;
+    int rank[N + M];
+  };
+  struct Trace {
+    int pos[2 * P];  // register at each position, -1 for padding
+    Table t;
+    constexpr void cmp(int i, int j) {
+      const int a = pos[i], b = pos[j];
+      if (a >= 0 && b >= 0) {
+        t.c[t.count++] = CmpPair{a, b};
+      } else if (a < 0 && b >= 0) {
+        pos[i] = b;
+        pos[j] = -1;
+      }
+    }
+    constexpr void merge(int lo, int n, int r) {
+      const int m = 2 * r;
+      if (m < n) {
+        merge(lo, n, m);
+        merge(lo + r, n, m);
+        for (int i = lo + r; i + r < lo + n; i += m) cmp(i, i + r);
+      } else {
+        cmp(lo, lo + r);
+      }
+    }
+  };
+  static constexpr Table make() {
+    Trace tr{};
+    for (int i = 0; i < P; ++i) {
+      tr.pos[i] = i < N ? i : -1;
+      tr.pos[P + i] = i < M ? N + i : -1;
+    }
+    tr.t.count = 0;
+    tr.merge(0, 2 * P, 1);
+    for (int k = 0; k < N + M; ++k) tr.t.rank[k] = tr.pos[k];
+    return tr.t;
+  }
+  static constexpr Table kTable = make();
+};
+
+// out[0..N+M) = merge of sorted a[0..N) and sorted b[0..M).
+template <int N, int M>
+__device__ __forceinline__ void merge_runs(const us2 *a, const us2 *b, us2 *out) {
+  using Net = MergeNet<N, M>;
+  us2 v[N + M];
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = a[i];
+#pragma unroll
+  for (int i = 0; i < M; ++i) v[N + i] = b[i];
+#pragma unroll
+  for (int c = 0; c < Net::kTable.count; ++c) cmpex(v[Net::kTable.c[c].a], v[Net::kTable.c[c].b]);
+#pragma unroll
+  for (int k = 0; k < N + M; ++k) out[k] = v[Net::kTable.rank[k]];
+}
+
+// Median of each K x K window, computed for four vertically adjacent
+// outputs at once.  Windows 0 and 1 share window rows 1..K-1, windows 2 and
+// 3 share rows 3..K+1, and all four share rows 3..K-1.  A shared value of
+// rank r among the K(K-1) values two adjacent windows share has rank
+// r..r+K in either window, so only shared ranks m-K-1..m-1 (m = (K*K+1)/2,
+// 0-based) can be a window's median; lower ones are always below it, higher
+// ones above.  Each window's median is therefore the (K+1)-th smallest of
+// those K+1 shared values and its own sorted row:
 //   min over i = 0..K of max(A[K-i], B[i-1])   (B[-1] = -infinity).
-// For 5x5 that is 224 packed min/max operations per pair of vertically
-// adjacent outputs (each operation covers two horizontally adjacent bytes).
+// Every window row is sorted once (K-input network); the rows common to a
+// pair of windows are combined with merge networks, the rows common to all
+// four windows only once.  For 5x5 this is 167 packed min/max operations
+// per two outputs instead of 224 for pairs alone.
 template <int K>
 constexpr int median_smem_bytes() {
   return Window<K / 2>::kRows * Window<K / 2>::kCols;
@@ -352,18 +426,49 @@ __device__ __forceinline__ us2 byte_pair(const uint32_t (&d)[ND], int b) {
   return __builtin_bit_cast(us2, __builtin_amdgcn_perm(hi, d[q], sel));
 }
 
+// (K+1)-th smallest (0-based K) of the K + 1 shared candidates c and the
+// window's own sorted row r.
+template <int K>
+__device__ __forceinline__ us2 median_select(const us2 *c, const us2 *r) {
+  us2 m = c[K];
+#pragma unroll
+  for (int i = 1; i <= K; ++i)
+    m = __builtin_elementwise_min(m, __builtin_elementwise_max(c[K - i], r[i - 1]));
+  return m;
+}
+
+// Sorted union of rows [first, first + n) of the sorted rows s (n >= 1),
+// merged pairwise.
+template <int K, int N>
+struct RowMerge {
+  static __device__ __forceinline__ void run(const us2 (*s)[K], us2 *out) {
+    constexpr int H = N / 2;
+    us2 lo[H * K], hi[(N - H) * K];
+    RowMerge<K, H>::run(s, lo);
+    RowMerge<K, N - H>::run(s + H, hi);
+    merge_runs<H * K, (N - H) * K>(lo, hi, out);
+  }
+};
+template <int K>
+struct RowMerge<K, 1> {
+  static __device__ __forceinline__ void run(const us2 (*s)[K], us2 *out) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) out[j] = s[0][j];
+  }
+};
+
 // One 256-byte x 16-row median tile of a C-channel image at tile
-// coordinates (bx, by).  Each thread owns 4 output bytes of 4 rows and works
-// on 2 rows at a time.  The taps of a window row are read from LDS as aligned
-// dwords (unaligned 8/16-bit LDS reads are several times slower) and paired
-// with v_perm_b32.
+// coordinates (bx, by).  Each thread owns 4 output bytes (two us2 lanes) of
+// 4 consecutive rows.  The taps of a window row are read from LDS as
+// aligned dwords (unaligned 8/16-bit LDS reads are several times slower)
+// and paired with v_perm_b32.
 template <int K, int C>
 __device__ __forceinline__ void median_tile_c(const MedianArgs &a, int bx, int by,
                                               uint8_t *smem) {
   constexpr int R = K / 2;
-  constexpr int NC = K * (K - 1);        // shared values
   constexpr int M = (K * K + 1) / 2;     // median rank (1-based)
   constexpr int LO = M - K - 1;          // first kept shared rank (0-based)
+  constexpr int NR = K + 3;              // window rows of four outputs
   // dwords covering bytes [-R*C, 3 + R*C] around the thread's first byte
   constexpr int DLO = floor_div4(-R * C), ND = floor_div4(3 + R * C) - DLO + 1;
   using G = Window<R>;
@@ -378,58 +483,51 @@ __device__ __forceinline__ void median_tile_c(const MedianArgs &a, int bx, int b
   __syncthreads();
 
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int r = 4 * ty;  // tile rows r .. r + 3
+  if (y0 + r >= a.height) return;
+  const uint32_t *w0 =
+      reinterpret_cast<const uint32_t *>(win + r * G::kCols + G::kPad) + tx + DLO;
+  uint32_t packed[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int pair = 0; pair < 2; ++pair) {
+    // sorted window rows 0 .. K+2 (row i of output 0's window is tile row r + i - R)
+    us2 s[NR][K];
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      uint32_t d[ND];
+#pragma unroll
+      for (int q = 0; q < ND; ++q) d[q] = w0[i * (G::kCols / 4) + q];
+#pragma unroll
+      for (int j = 0; j < K; ++j) s[i][j] = byte_pair(d, 2 * pair + (j - R) * C - 4 * DLO);
+      batcher_sort<K>(s[i]);
+    }
+    // rows 1..K-1 (outputs 0, 1) and 3..K+1 (outputs 2, 3)
+    us2 ca[K * (K - 1)], cb[K * (K - 1)];
+    if constexpr (K == 3) {
+      merge_runs<K, K>(s[1], s[2], ca);
+      merge_runs<K, K>(s[3], s[4], cb);
+    } else {
+      us2 mid[K * (K - 3)], e1[2 * K];
+      RowMerge<K, K - 3>::run(s + 3, mid);
+      merge_runs<K, K>(s[1], s[2], e1);
+      merge_runs<2 * K, K * (K - 3)>(e1, mid, ca);
+      merge_runs<K, K>(s[K], s[K + 1], e1);
+      merge_runs<K * (K - 3), 2 * K>(mid, e1, cb);
+    }
+    const us2 out[4] = {median_select<K>(ca + LO, s[0]), median_select<K>(ca + LO, s[K]),
+                        median_select<K>(cb + LO, s[2]), median_select<K>(cb + LO, s[K + 2])};
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      packed[h] |= (static_cast<uint32_t>(out[h].x) | (static_cast<uint32_t>(out[h].y) << 8))
+                   << (16 * pair);
+  }
   const int row_bytes = a.width * C;
-#pragma unroll 1
-  for (int rp = 0; rp < kTileH / 8; ++rp) {
-    const int r = 4 * ty + 2 * rp;  // this pass: tile rows r and r + 1
-    if (y0 + r >= a.height) break;
-    const uint32_t *w0 =
-        reinterpret_cast<const uint32_t *>(win + r * G::kCols + G::kPad) + tx + DLO;
-    uint32_t d[K + 1][ND];
 #pragma unroll
-    for (int i = 0; i <= K; ++i)
-#pragma unroll
-      for (int q = 0; q < ND; ++q) d[i][q] = w0[i * (G::kCols / 4) + q];
-    uint32_t packed[2] = {0, 0};
-#pragma unroll
-    for (int pair = 0; pair < 2; ++pair) {
-      // window row i, column j of the upper output's window
-      auto at = [&](int i, int j) {
-        return byte_pair(d[i], 2 * pair + (j - R) * C - 4 * DLO);
-      };
-      // window rows 1..K-1 of the upper output = rows 0..K-2 of the lower
-      us2 v[NC];
-#pragma unroll
-      for (int i = 0; i < K - 1; ++i)
-#pragma unroll
-        for (int j = 0; j < K; ++j) v[i * K + j] = at(i + 1, j);
-      batcher_sort<NC>(v);
-      us2 up[K], dn[K];
-#pragma unroll
-      for (int j = 0; j < K; ++j) {
-        up[j] = at(0, j);
-        dn[j] = at(K, j);
-      }
-      batcher_sort<K>(up);
-      batcher_sort<K>(dn);
-      us2 mu = v[LO + K], md = v[LO + K];
-#pragma unroll
-      for (int i = 1; i <= K; ++i) {
-        mu = __builtin_elementwise_min(mu, __builtin_elementwise_max(v[LO + K - i], up[i - 1]));
-        md = __builtin_elementwise_min(md, __builtin_elementwise_max(v[LO + K - i], dn[i - 1]));
-      }
-      packed[0] |= (static_cast<uint32_t>(mu.x) | (static_cast<uint32_t>(mu.y) << 8))
-                   << (16 * pair);
-      packed[1] |= (static_cast<uint32_t>(md.x) | (static_cast<uint32_t>(md.y) << 8))
-                   << (16 * pair);
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int y = y0 + r + h;
-      if (y < a.height)
-        store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx,
-                   row_bytes, packed[h]);
-    }
+  for (int h = 0; h < 4; ++h) {
+    const int y = y0 + r + h;
+    if (y < a.height)
+      store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx, row_bytes,
+                 packed[h]);
   }
 }
 
