@@ -349,8 +349,9 @@ def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 1e9)):
     budget = 64 << 30
     stream = torch.cuda.Stream()
     # warm-up of both paths (kernel loading, allocator caches)
-    slots, max_ops = 4, 64
-    srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops)
+    slots, max_ops, large_px = 2, 16, 1 << 18
+    srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
+                   large_px=large_px)
     S.run_served(srv, WL.serving_requests(corpus, 100, 1e6, seed=1))
     S.run_plain(be, WL.serving_requests(corpus, 100, 1e6, seed=2), stream)
     torch.cuda.synchronize()
@@ -358,10 +359,12 @@ def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 1e9)):
                        "5x5, resize 0.5x, warp_perspective, NV12->RGB; frames from a resident "
                        "corpus of 48, W x H log-uniform 64x64..7680x4320, C in {1,3,4}; "
                        "latency = arrival to host-observed completion" % n,
-           "hbm_budget_gb": budget / 2 ** 30, "slots": slots, "max_batch_ops": max_ops}
+           "hbm_budget_gb": budget / 2 ** 30, "slots": slots, "max_batch_ops": max_ops,
+           "large_px": large_px}
     for qps in qps_list:
         reqs = WL.serving_requests(corpus, n, qps, seed=11)
-        srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops)
+        srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
+                       large_px=large_px)
         t0 = S.run_served(srv, reqs)
         torch.cuda.synchronize()
         served = S.latency_stats(reqs, t0)

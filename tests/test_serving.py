@@ -38,6 +38,9 @@ class FakeBackend:
         self.launches.append([r.rid for r in requests])
         return 0
 
+    def launch_one(self, request, stream):
+        self.launches.append(["alone", request.rid])
+
     def record(self, stream):
         return FakeEvent(self.polls)
 
@@ -158,6 +161,27 @@ def test_single_slot_serialises_batches():
     assert len(be.launches) == 1      # the only slot is busy
     srv.drain()
     assert be.launches == [[0, 1, 2], [9]]
+
+
+def test_large_requests_run_alone_on_slot_zero_and_small_ones_pass_them():
+    be = FakeBackend(polls=6)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=8, clock=Clock(),
+                   large_px=64 * 64)
+    big = [req(i, h=64, w=64) for i in range(3)]           # 4096 px: large
+    small = [req(10 + i, h=16, w=16) for i in range(5)]    # small
+    for r in big[:2] + small + big[2:]:
+        srv.submit(r)
+    srv.pump()
+    # slot 0 took the first large request alone, slot 1 the small batch
+    assert be.launches[0] == ["alone", 0]
+    assert be.launches[1] == [10, 11, 12, 13, 14]
+    srv.drain()
+    alone = [b[1] for b in be.launches if b[0] == "alone"]
+    assert alone == [0, 1, 2]                              # large: arrival order, one at a time
+    batched = [rid for b in be.launches if b[0] != "alone" for rid in b]
+    assert sorted(batched) == [10, 11, 12, 13, 14]
+    assert max(r.done for r in small) < min(r.done for r in big[1:])
+    assert sorted(r.rid for r in srv.completed) == [0, 1, 2, 10, 11, 12, 13, 14]
 
 
 def test_poisson_arrivals_rate():

@@ -2,13 +2,17 @@
 
 Requests (an op on a frame that is already in device memory) arrive at any
 time.  The server keeps every buffer resident on the device and admits a
-request once its output fits the HBM budget.  Each tick it packs all admitted
-requests that are not yet running into one batch (kleidicv_hip_batch_launch,
-one persistent launch per kernel variant) on a free stream slot.  With two or
-more slots, the next batch is formed and launched while earlier ones still
-run, so requests that arrive during a batch overlap it on the device.  A
-request completes when the host observes its batch's event; its output is
-then handed to the caller and its budget released.
+request once its output fits the HBM budget.  Admitted requests are split by
+size.  A large request (at least `large_px` output pixels) fills the GPU on
+its own, so it is launched alone through its single-image device entry point
+on the large-request slot, in arrival order.  Small requests are packed into
+batches (kleidicv_hip_batch_launch: one persistent launch per kernel variant)
+on the other stream slots, so they never wait behind a large one and the
+launch cost is shared; the large-request slot also takes small batches when
+no large request waits.  Batches on different slots overlap on the device,
+so requests that arrive during a batch start before it ends.  A request
+completes when the host observes its launch's event; its output is then
+handed to the caller and its budget released.
 
 Ops served: gaussian blur 5x5 (sigma 0, REVERSE), median blur 5x5
 (REPLICATE), linear resize to ceil(w/2) x ceil(h/2), warp perspective
@@ -112,6 +116,10 @@ class TorchBackend:
         """Launches, or returns the workspace bytes needed (nothing launched)."""
         return K.batch_launch(descs, workspace, stream, tuning)
 
+    def launch_one(self, request, stream):
+        """One request through its single-image device entry point."""
+        request.run_alone(request.dst, stream)
+
     def record(self, stream):
         ev = self.torch.cuda.Event()
         ev.record(stream)
@@ -127,11 +135,13 @@ class Server:
     hbm_budget bounds the bytes of outputs and workspaces held at once (the
     inputs are resident already).  Each slot's work-list workspace comes out
     of a fixed reserve of workspace_reserve bytes; outputs are admitted
-    against the rest.  max_batch_ops bounds one batch.
+    against the rest.  max_batch_ops bounds one batch.  Slot 0 is the
+    large-request slot; with one slot, everything runs on it.
     """
 
     def __init__(self, backend, hbm_budget=64 << 30, slots=2, max_batch_ops=512,
-                 workspace_reserve=64 << 20, tuning=None, clock=time.perf_counter):
+                 workspace_reserve=64 << 20, tuning=None, clock=time.perf_counter,
+                 large_px=1 << 18):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         if slots * workspace_reserve >= hbm_budget:
@@ -146,11 +156,13 @@ class Server:
         self.tuning = tuning
         self.clock = clock
         self.streams = [backend.stream() for _ in range(slots)]
-        self.inflight = [None] * slots  # (event, requests, workspace bytes)
+        self.inflight = [None] * slots  # (event, requests)
         self.ws = [None] * slots
         self.ws_bytes = [0] * slots
+        self.large_px = large_px
         self.queue = collections.deque()   # arrived, waiting for budget
-        self.ready = collections.deque()   # admitted, waiting for a slot
+        self.ready = collections.deque()   # admitted small requests, waiting for a slot
+        self.ready_large = collections.deque()  # admitted large requests
         self.completed = []
         self.batches = 0
 
@@ -172,14 +184,17 @@ class Server:
             self.peak = max(self.peak, self.in_use)
             req.dst = self.be.alloc(req.out_shape())
             req.admitted = now
-            self.ready.append(req)
+            if req.out_pixels() >= self.large_px:
+                self.ready_large.append(req)
+            else:
+                self.ready.append(req)
 
     # -- completion -----------------------------------------------------------
     def _retire(self, now):
         for i, slot in enumerate(self.inflight):
             if slot is None or not self.be.query(slot[0]):
                 continue
-            _, reqs, _ = slot
+            _, reqs = slot
             for r in reqs:
                 r.done = now
                 self.out_in_use -= r.out_bytes()
@@ -189,9 +204,14 @@ class Server:
     # -- batching -----------------------------------------------------------
     def _launch(self, now):
         for i in range(len(self.streams)):
-            if not self.ready:
-                return
             if self.inflight[i] is not None:
+                continue
+            if i == 0 and self.ready_large:
+                req = self.ready_large.popleft()
+                self.be.launch_one(req, self.streams[0])
+                self._started(0, [req], now)
+                continue
+            if not self.ready:
                 continue
             reqs = []
             while self.ready and len(reqs) < self.max_batch_ops:
@@ -216,12 +236,15 @@ class Server:
                 keep = len(reqs) // 2
                 self.ready.extendleft(reversed(reqs[keep:]))
                 reqs, descs = reqs[:keep], descs[:keep]
-            ev = self.be.record(self.streams[i])
-            self.batches += 1
-            for r in reqs:
-                r.launched = now
-                r.batch = self.batches
-            self.inflight[i] = (ev, reqs, need)
+            self._started(i, reqs, now)
+
+    def _started(self, i, reqs, now):
+        ev = self.be.record(self.streams[i])
+        self.batches += 1
+        for r in reqs:
+            r.launched = now
+            r.batch = self.batches
+        self.inflight[i] = (ev, reqs)
 
     def pump(self):
         """One scheduling tick; returns the number of requests still pending."""
@@ -237,7 +260,7 @@ class Server:
         return self.out_in_use + sum(self.ws_bytes)
 
     def pending(self):
-        return (len(self.queue) + len(self.ready) +
+        return (len(self.queue) + len(self.ready) + len(self.ready_large) +
                 sum(len(s[1]) for s in self.inflight if s is not None))
 
     def drain(self):
