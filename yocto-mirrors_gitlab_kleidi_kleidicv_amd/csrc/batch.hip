@@ -59,11 +59,11 @@ __device__ __forceinline__ void for_each_item(const Segment &s, const Body &body
   }
 }
 
-template <int MODE, int K>
+template <int MODE, int K, int C>
 __global__ __launch_bounds__(kThreads) void batch_sep_kernel(Segment s) {
   __shared__ __align__(16) uint8_t smem[filt::separable_smem_bytes<K>()];
   for_each_item(s, [&](const Job &job, int bx, int by) {
-    filt::separable_tile<MODE, K>(job.u.sep, bx, by, smem);
+    filt::separable_tile<MODE, K, C>(job.u.sep, bx, by, smem);
   });
 }
 
@@ -187,8 +187,11 @@ const void *segment_kernel(Family family, int variant) {
       return KCV_K(batch_copy_kernel);
     case kFamSep:
       switch (variant) {
-#define KCV_S(MODE, K) \
-  case MODE * 32 + K: return KCV_K(batch_sep_kernel<MODE, K>);
+#define KCV_S(MODE, K)                                                  \
+  case (MODE * 32 + K) * 8 + 1: return KCV_K(batch_sep_kernel<MODE, K, 1>); \
+  case (MODE * 32 + K) * 8 + 2: return KCV_K(batch_sep_kernel<MODE, K, 2>); \
+  case (MODE * 32 + K) * 8 + 3: return KCV_K(batch_sep_kernel<MODE, K, 3>); \
+  case (MODE * 32 + K) * 8 + 4: return KCV_K(batch_sep_kernel<MODE, K, 4>);
         KCV_S(kSepBinomial, 3) KCV_S(kSepBinomial, 5) KCV_S(kSepBinomial, 7)
         KCV_S(kSepBinomial, 9) KCV_S(kSepSigma, 3) KCV_S(kSepSigma, 5)
         KCV_S(kSepSigma, 7) KCV_S(kSepSigma, 9) KCV_S(kSepSigma, 15)

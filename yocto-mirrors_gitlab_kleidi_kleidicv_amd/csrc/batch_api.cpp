@@ -96,7 +96,7 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
         j.u.copy = c;
       } else {
         p->family = kcv::kFamSep;
-        j.variant = j.u.sep.mode * 32 + j.u.sep.ksize;
+        j.variant = (j.u.sep.mode * 32 + j.u.sep.ksize) * 8 + j.u.sep.channels;
       }
       break;
     }
@@ -106,7 +106,7 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
                                      d.kernel_y, d.kernel_size, d.border_type, &j.u.sep,
                                      &act));
       p->family = kcv::kFamSep;
-      j.variant = j.u.sep.mode * 32 + j.u.sep.ksize;
+      j.variant = (j.u.sep.mode * 32 + j.u.sep.ksize) * 8 + j.u.sep.channels;
       break;
     case KLEIDICV_HIP_OP_MEDIAN_BLUR:
       KCV_TRY(kcv::prepare_median(d.src, d.src_stride, d.dst, d.dst_stride, d.src_width,

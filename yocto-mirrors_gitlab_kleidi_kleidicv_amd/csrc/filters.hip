@@ -20,12 +20,23 @@ __device__ __forceinline__ void tile_of(int tiles_x, int &bx, int &by) {
   bx = t - by * tiles_x;
 }
 
-template <int MODE, int K>
+template <int MODE, int K, int C>
 __global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a, int tiles_x) {
   __shared__ __align__(16) uint8_t smem[separable_smem_bytes<K>()];
   int bx, by;
   tile_of(tiles_x, bx, by);
-  separable_tile<MODE, K>(a, bx, by, smem);
+  separable_tile<MODE, K, C>(a, bx, by, smem);
+}
+
+// One kernel per channel count: the tile body is specialised on it.
+template <int MODE, int K>
+void launch_channels(const SepArgs &a, dim3 grid, int tx, hipStream_t s) {
+  switch (a.channels) {
+    case 1: separable_kernel<MODE, K, 1><<<grid, kThreads, 0, s>>>(a, tx); break;
+    case 2: separable_kernel<MODE, K, 2><<<grid, kThreads, 0, s>>>(a, tx); break;
+    case 3: separable_kernel<MODE, K, 3><<<grid, kThreads, 0, s>>>(a, tx); break;
+    default: separable_kernel<MODE, K, 4><<<grid, kThreads, 0, s>>>(a, tx); break;
+  }
 }
 
 template <int K, int C>
@@ -56,24 +67,24 @@ hipError_t launch_mode(const SepArgs &a, hipStream_t s) {
   const dim3 grid = g.grid;
   const int tx = g.tiles_x;
   switch (a.ksize) {
-    case 3: separable_kernel<MODE, 3><<<grid, kThreads, 0, s>>>(a, tx); break;
-    case 5: separable_kernel<MODE, 5><<<grid, kThreads, 0, s>>>(a, tx); break;
-    case 7: separable_kernel<MODE, 7><<<grid, kThreads, 0, s>>>(a, tx); break;
+    case 3: launch_channels<MODE, 3>(a, grid, tx, s); break;
+    case 5: launch_channels<MODE, 5>(a, grid, tx, s); break;
+    case 7: launch_channels<MODE, 7>(a, grid, tx, s); break;
     case 9:
       if constexpr (MODE != kSepSaturate) {
-        separable_kernel<MODE, 9><<<grid, kThreads, 0, s>>>(a, tx);
+        launch_channels<MODE, 9>(a, grid, tx, s);
         break;
       }
       return hipErrorInvalidValue;
     case 15:
       if constexpr (MODE == kSepSigma) {
-        separable_kernel<MODE, 15><<<grid, kThreads, 0, s>>>(a, tx);
+        launch_channels<MODE, 15>(a, grid, tx, s);
         break;
       }
       return hipErrorInvalidValue;
     case 21:
       if constexpr (MODE == kSepSigma) {
-        separable_kernel<MODE, 21><<<grid, kThreads, 0, s>>>(a, tx);
+        launch_channels<MODE, 21>(a, grid, tx, s);
         break;
       }
       return hipErrorInvalidValue;

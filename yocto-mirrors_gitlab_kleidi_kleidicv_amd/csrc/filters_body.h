@@ -164,6 +164,10 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
   uint32_t w[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) w[j] = static_cast<uint32_t>(a.wx[j]);
+  sep_us2 wpair[K / 2];  // taps (2i, 2i + 1) for the dot2 form
+#pragma unroll
+  for (int i = 0; i < K / 2; ++i)
+    wpair[i] = sep_us2{static_cast<uint16_t>(w[2 * i]), static_cast<uint16_t>(w[2 * i + 1])};
 #pragma unroll
   for (int pass = 0; pass < kTileH / 8; ++pass) {
     const int r = ty + 8 * pass;
@@ -180,12 +184,26 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
       v[4 * d + 3] = q.w;
     }
     auto col = [&](int i) -> uint32_t { return (v[i >> 1] >> (16 * (i & 1))) & 0xFFFFu; };
+    // column pair (i, i + 1) as one dword: held in v when i is even, else
+    // assembled from two neighbouring dwords (one v_alignbit each)
+    auto col_pair = [&](int i) -> uint32_t {
+      return (i & 1) ? __builtin_amdgcn_alignbit(v[(i >> 1) + 1], v[i >> 1], 16) : v[i >> 1];
+    };
     uint32_t lo = 0, hi = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       uint32_t acc = 0;
+      if (C == 1) {
+        // adjacent taps two at a time: v_dot2_u32_u16 (exact, no clamp)
 #pragma unroll
-      for (int j = 0; j < K; ++j) acc += __umul24(w[j], col(8 * D + b + (j - R) * C));
+        for (int j = 0; j + 1 < K; j += 2)
+          acc = __builtin_amdgcn_udot2(__builtin_bit_cast(sep_us2, col_pair(8 * D + b + j - R)),
+                                       wpair[j / 2], acc, false);
+        acc += __umul24(w[K - 1], col(8 * D + b + K - 1 - R));
+      } else {
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc += __umul24(w[j], col(8 * D + b + (j - R) * C));
+      }
       uint32_t out;
       if (MODE == kSepBinomial)
         out = (acc + (1u << (a.shift - 1))) >> a.shift;
@@ -209,7 +227,7 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
 
 // One 256-byte x 16-row output tile at tile coordinates (bx, by).  `smem`
 // holds separable_smem_bytes<K>() bytes, 16-byte aligned.
-template <int MODE, int K>
+template <int MODE, int K, int C>
 __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
                                                uint8_t *smem) {
   constexpr int R = K / 2;
@@ -220,52 +238,66 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   const int e0 = bx * kTileW;
   const int y0 = by * kTileH;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  const int C = a.channels;
   stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0,
                   y0);
   __syncthreads();
 
-  // Vertical pass: one window dword per tap row, two 16-bit lanes per op.
+  // Vertical pass: each item is one window dword column over two output
+  // rows, so its K + 1 window dwords are read and split into 16-bit lanes
+  // once for both rows; v_pk_mad_u16 does two lanes per op (with clamp in
+  // the saturating mode).
   constexpr int kQuads = G::kCols / 4;
+  constexpr int kPairs = kTileH / 2;
   const uint32_t *win32 = reinterpret_cast<const uint32_t *>(win);
   sep_us2 wy[K];
 #pragma unroll
   for (int k = 0; k < K; ++k)
     wy[k] = sep_us2{static_cast<uint16_t>(a.wy[k]), static_cast<uint16_t>(a.wy[k])};
-  for (int i = threadIdx.x; i < kTileH * kQuads; i += kThreads) {
-    const int r = i / kQuads, q = i - r * kQuads;
-    sep_us2 even = sep_us2{0, 0}, odd = sep_us2{0, 0};  // bytes 0,2 and 1,3
+  auto mac = [](sep_us2 acc, sep_us2 x, sep_us2 w) {
+    if (MODE == kSepSaturate) {
+      sep_us2 r;
+      asm("v_pk_mad_u16 %0, %1, %2, %3 clamp" : "=v"(r) : "v"(x), "v"(w), "v"(acc));
+      return r;
+    }
+    return acc + x * w;
+  };
+  for (int i = threadIdx.x; i < kPairs * kQuads; i += kThreads) {
+    const int p = i / kQuads, q = i - p * kQuads;  // rows 2p and 2p + 1
+    // bytes 0,2 (even) and 1,3 (odd) of each row
+    sep_us2 e0 = sep_us2{0, 0}, o0 = sep_us2{0, 0}, e1 = sep_us2{0, 0}, o1 = sep_us2{0, 0};
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const uint32_t v = win32[(r + k) * kQuads + q];
+    for (int k = 0; k <= K; ++k) {
+      const uint32_t v = win32[(2 * p + k) * kQuads + q];
       const sep_us2 e = __builtin_bit_cast(sep_us2, v & 0x00FF00FFu);
       const sep_us2 o = __builtin_bit_cast(sep_us2, (v >> 8) & 0x00FF00FFu);
-      if (MODE == kSepSaturate) {
-        even = __builtin_elementwise_add_sat(even, e * wy[k]);
-        odd = __builtin_elementwise_add_sat(odd, o * wy[k]);
-      } else {
-        even += e * wy[k];
-        odd += o * wy[k];
+      if (k < K) {
+        e0 = mac(e0, e, wy[k]);
+        o0 = mac(o0, o, wy[k]);
+      }
+      if (k > 0) {
+        e1 = mac(e1, e, wy[k - 1]);
+        o1 = mac(o1, o, wy[k - 1]);
       }
     }
     if (MODE == kSepSigma) {
-      even = (even + sep_us2{128, 128}) >> 8;
-      odd = (odd + sep_us2{128, 128}) >> 8;
+      e0 = (e0 + sep_us2{128, 128}) >> 8;
+      o0 = (o0 + sep_us2{128, 128}) >> 8;
+      e1 = (e1 + sep_us2{128, 128}) >> 8;
+      o1 = (o1 + sep_us2{128, 128}) >> 8;
     }
-    const uint32_t ev = __builtin_bit_cast(uint32_t, even);
-    const uint32_t od = __builtin_bit_cast(uint32_t, odd);
     // column order 0, 1, 2, 3 = even.x, odd.x, even.y, odd.y
-    *reinterpret_cast<uint2 *>(vert + r * G::kCols + 4 * q) =
-        make_uint2((ev & 0xFFFFu) | (od << 16), (ev >> 16) | (od & 0xFFFF0000u));
+    auto put = [&](int r, sep_us2 even, sep_us2 odd) {
+      const uint32_t ev = __builtin_bit_cast(uint32_t, even);
+      const uint32_t od = __builtin_bit_cast(uint32_t, odd);
+      *reinterpret_cast<uint2 *>(vert + r * G::kCols + 4 * q) =
+          make_uint2((ev & 0xFFFFu) | (od << 16), (ev >> 16) | (od & 0xFFFF0000u));
+    };
+    put(2 * p, e0, o0);
+    put(2 * p + 1, e1, o1);
   }
   __syncthreads();
 
-  switch (C) {
-    case 1: separable_horizontal<MODE, K, 1>(a, vert, e0, y0); break;
-    case 2: separable_horizontal<MODE, K, 2>(a, vert, e0, y0); break;
-    case 3: separable_horizontal<MODE, K, 3>(a, vert, e0, y0); break;
-    default: separable_horizontal<MODE, K, 4>(a, vert, e0, y0); break;
-  }
+  separable_horizontal<MODE, K, C>(a, vert, e0, y0);
 }
 
 // ---------------------------------------------------------------------------
