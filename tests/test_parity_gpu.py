@@ -142,8 +142,17 @@ def resize_case(rng, sw, sh, dw, dh, c):
 
 @pytest.mark.parametrize("c", [1, 2, 3, 4])
 def test_resize_quarter(rng, c):
-    for sw, sh in [(2, 2), (34, 10), (640, 96), (1026, 18)]:
+    for sw, sh in [(2, 2), (34, 10), (50, 4), (98, 6), (514, 8), (640, 96), (1026, 18)]:
         resize_case(rng, sw, sh, sw // 2, sh // 2, c)
+    # rows with odd pitch on both sides (unaligned vector loads and stores)
+    src = padded_rows(rng, 14, 402, c, 3)
+    err, want = O.resize_linear(np.ascontiguousarray(src), 201, 7)
+    assert err == 0
+    dst = np.full((7, 201 * c + 5), 0xEE, np.uint8)
+    view = dst[:, :201 * c].reshape(7, 201, c) if c > 1 else dst[:, :201]
+    K.resize_linear(src, 201, 7, dst=view)
+    assert_same(view, want.reshape(view.shape), "padded quarter c%d" % c)
+    assert np.all(dst[:, 201 * c:] == 0xEE)
 
 
 def test_resize_exact_1ch_2x_4x(rng):

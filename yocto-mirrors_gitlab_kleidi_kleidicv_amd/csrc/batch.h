@@ -87,7 +87,6 @@ struct Segment {
 constexpr int kFilterTileW = 256;  // bytes
 constexpr int kFilterTileH = 16;   // rows
 constexpr int kResizeTileE = 1024;            // destination bytes (staged)
-constexpr int kQuarterTileE = 16 * 256;       // destination bytes (quarter)
 constexpr int kUp2x4TilePx = rsz::kUp2x4ThreadPx * 256;  // source pixels (exact 2x, 4 ch)
 constexpr int kYuvTileW = 8 * 256;            // pixels
 constexpr int kCopyTileE = 16 * 256;          // bytes

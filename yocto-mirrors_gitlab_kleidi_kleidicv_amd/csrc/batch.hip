@@ -88,15 +88,21 @@ __global__ __launch_bounds__(kThreads) void batch_tables_kernel(Segment s) {
   });
 }
 
+template <int C>
+__global__ __launch_bounds__(kThreads) void batch_quarter_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    const ResizeJob &rj = job.u.resize;
+    const int y0 = by * job.band;
+    rsz::resize_quarter_rows<C>(rj.a, bx, y0, min(y0 + job.band, rj.a.dst_height), 1);
+  });
+}
+
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void batch_resize_kernel(Segment s) {
   __shared__ __align__(16) uint8_t smem[KIND >= kRsQuarter ? 16 : rsz::kStagedSmem];
   for_each_item(s, [&](const Job &job, int bx, int by) {
     const ResizeJob &rj = job.u.resize;
-    if (KIND == kRsQuarter) {
-      const int y0 = by * job.band;
-      rsz::resize_quarter_rows(rj.a, bx, y0, min(y0 + job.band, rj.a.dst_height), 1);
-    } else if (KIND == kRsUp2x4) {
+    if (KIND == kRsUp2x4) {
       rsz::up2x4_group(rj.a, bx, by);
     } else {
       constexpr int kMode = KIND == kRsGeneric  ? rsz::kLerpGeneric
@@ -214,7 +220,10 @@ const void *segment_kernel(Family family, int variant) {
       return variant == kRsGeneric   ? KCV_K(batch_resize_kernel<kRsGeneric>)
              : variant == kRsExact2  ? KCV_K(batch_resize_kernel<kRsExact2>)
              : variant == kRsExact4  ? KCV_K(batch_resize_kernel<kRsExact4>)
-             : variant == kRsQuarter ? KCV_K(batch_resize_kernel<kRsQuarter>)
+             : variant == kRsQuarter + 8 * 1 ? KCV_K(batch_quarter_kernel<1>)
+             : variant == kRsQuarter + 8 * 2 ? KCV_K(batch_quarter_kernel<2>)
+             : variant == kRsQuarter + 8 * 3 ? KCV_K(batch_quarter_kernel<3>)
+             : variant == kRsQuarter + 8 * 4 ? KCV_K(batch_quarter_kernel<4>)
              : variant == kRsUp2x4   ? KCV_K(batch_resize_kernel<kRsUp2x4>)
                                      : nullptr;
     case kFamRemapS16:

@@ -135,7 +135,9 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
       }
       j.u.resize.a = a;
       j.u.resize.kind = p->resize_kind;
-      j.variant = p->resize_kind;
+      // the quarter kernels are specialised per channel count
+      j.variant = p->resize_kind == kcv::kRsQuarter ? kcv::kRsQuarter + 8 * a.channels
+                                                     : p->resize_kind;
       p->table_bytes = kcv::resize_workspace_bytes(a);
       break;
     }
@@ -221,7 +223,8 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
       const int64_t n = int64_t{a.dst_width} * a.channels;
       if (p.resize_kind == kRsQuarter) {
         j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows) : 2;
-        j->tiles_x = static_cast<int>(div_up(n, kQuarterTileE));
+        j->tiles_x = static_cast<int>(
+            div_up(n, int64_t{rsz::quarter_thread_bytes(a.channels)} * rsz::kThreads));
       } else if (p.resize_kind == kRsUp2x4) {
         // one item per row group (two output rows) of 1024 source pixels
         j->band = 1;

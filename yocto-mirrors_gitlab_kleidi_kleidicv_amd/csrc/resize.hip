@@ -29,8 +29,9 @@ __global__ __launch_bounds__(kThreads) void resize_up2x4_kernel(ResizeArgs a) {
   for (int g = blockIdx.y; g <= a.src_height; g += gridDim.y) up2x4_group(a, blockIdx.x, g);
 }
 
+template <int C>
 __global__ __launch_bounds__(kThreads) void resize_quarter_kernel(ResizeArgs a) {
-  resize_quarter_rows(a, blockIdx.x, blockIdx.y, a.dst_height, gridDim.y);
+  resize_quarter_rows<C>(a, blockIdx.x, blockIdx.y, a.dst_height, gridDim.y);
 }
 
 template <int MODE>
@@ -59,9 +60,15 @@ hipError_t launch_resize(const ResizeArgs &a, void *workspace,
   if (path == Path::kNone || a.dst_width == 0 || a.dst_height == 0) return hipSuccess;
   if (path == Path::kQuarter) {
     const int64_t n = int64_t{a.dst_width} * a.channels;
-    const dim3 grid(static_cast<unsigned>((n + 16 * kThreads - 1) / (16 * kThreads)),
+    const int64_t per = int64_t{quarter_thread_bytes(a.channels)} * kThreads;
+    const dim3 grid(static_cast<unsigned>((n + per - 1) / per),
                     static_cast<unsigned>(a.dst_height < 65535 ? a.dst_height : 65535));
-    resize_quarter_kernel<<<grid, kThreads, 0, stream>>>(a);
+    switch (a.channels) {
+      case 1: resize_quarter_kernel<1><<<grid, kThreads, 0, stream>>>(a); break;
+      case 2: resize_quarter_kernel<2><<<grid, kThreads, 0, stream>>>(a); break;
+      case 3: resize_quarter_kernel<3><<<grid, kThreads, 0, stream>>>(a); break;
+      default: resize_quarter_kernel<4><<<grid, kThreads, 0, stream>>>(a); break;
+    }
     return hipGetLastError();
   }
   if (path == Path::kUp2x4) {

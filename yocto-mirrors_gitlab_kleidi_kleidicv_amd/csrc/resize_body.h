@@ -587,54 +587,69 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
 }
 
 // ---------------------------------------------------------------------------
-// Exact 0.5 x 0.5: 16 output bytes per thread per row for 1, 2 and 4
-// channels (two 16-byte loads from each source row), bytewise for 3.
-__device__ __forceinline__ uint32_t quarter4(uint64_t t, uint64_t b, int C) {
-  uint32_t packed = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int i0 = 2 * k - k % C, i1 = i0 + C;
-    const uint32_t s = ((t >> (8 * i0)) & 0xFF) + ((t >> (8 * i1)) & 0xFF) +
-                       ((b >> (8 * i0)) & 0xFF) + ((b >> (8 * i1)) & 0xFF);
-    packed |= ((s + 2) >> 2) << (8 * k);
-  }
-  return packed;
+// Exact 0.5 x 0.5, one kernel per channel count.  Destination bytes per thread: 16 for 1, 2 and 4 channels, 24 (eight
+// pixels) for 3, so that a thread's source bytes are whole 16-byte chunks.
+__host__ __device__ constexpr int quarter_thread_bytes(int channels) {
+  return channels == 3 ? 24 : 16;
 }
 
-// Bytes [bx*16*kThreads, +16*kThreads) of rows y_begin, y_begin + y_step, ...
+// Destination bytes [e0, e0 + OB) of rows y_begin, y_begin + y_step, ...
+// with e0 = (bx * kThreads + thread) * OB.  Each output byte is
+// (a + b + c + d + 2) >> 2 of its 2 x 2 source bytes.  Rows are read with
+// 16-byte loads at any alignment (gfx950 vector memory is byte-addressed in
+// unaligned mode); the last partial group of a row goes byte by byte.
+template <int C>
 __device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
                                                     int y_begin, int y_end, int y_step) {
-  const int C = a.channels;
+  constexpr int OB = quarter_thread_bytes(C), IB = 2 * OB;
+  typedef uint4 __attribute__((aligned(1))) u4u;
   const int n = a.dst_width * C;
-  const int e0 = (bx * kThreads + threadIdx.x) * 16;
+  const int e0 = (bx * kThreads + threadIdx.x) * OB;
   if (e0 >= n) return;
-  const bool fast = C != 3 && e0 + 15 < n &&
-                    ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
-                      reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+  const bool whole = e0 + OB <= n;
   for (int y = y_begin; y < y_end; y += y_step) {
-    const uint8_t *r0 = a.src + static_cast<size_t>(2 * y) * a.src_stride;
+    const uint8_t *r0 = a.src + static_cast<size_t>(2 * y) * a.src_stride + 2 * e0;
     const uint8_t *r1 = r0 + a.src_stride;
-    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
-    if (fast) {
-      const uint4 t0 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0);
-      const uint4 t1 = *reinterpret_cast<const uint4 *>(r0 + 2 * e0 + 16);
-      const uint4 b0 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0);
-      const uint4 b1 = *reinterpret_cast<const uint4 *>(r1 + 2 * e0 + 16);
-      auto u64 = [](uint32_t lo, uint32_t hi) {
-        return (static_cast<uint64_t>(hi) << 32) | lo;
-      };
-      uint4 o;
-      o.x = quarter4(u64(t0.x, t0.y), u64(b0.x, b0.y), C);
-      o.y = quarter4(u64(t0.z, t0.w), u64(b0.z, b0.w), C);
-      o.z = quarter4(u64(t1.x, t1.y), u64(b1.x, b1.y), C);
-      o.w = quarter4(u64(t1.z, t1.w), u64(b1.z, b1.w), C);
-      store_nt(reinterpret_cast<uint4 *>(out + e0), o);
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + e0;
+    if (whole) {
+      uint32_t t[IB / 4], b[IB / 4];
+#pragma unroll
+      for (int k = 0; k < IB / 16; ++k) {
+        const uint4 qt = *reinterpret_cast<const u4u *>(r0 + 16 * k);
+        const uint4 qb = *reinterpret_cast<const u4u *>(r1 + 16 * k);
+        t[4 * k] = qt.x; t[4 * k + 1] = qt.y; t[4 * k + 2] = qt.z; t[4 * k + 3] = qt.w;
+        b[4 * k] = qb.x; b[4 * k + 1] = qb.y; b[4 * k + 2] = qb.z; b[4 * k + 3] = qb.w;
+      }
+      auto byte = [](const uint32_t *v, int i) { return (v[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
+      uint32_t o[OB / 4];
+#pragma unroll
+      for (int w = 0; w < OB / 4; ++w) {
+        uint32_t packed = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int j = 4 * w + k;                  // output byte in the group
+          const int i0 = 2 * (j - j % C) + j % C;   // its left source byte
+          const uint32_t sum = byte(t, i0) + byte(t, i0 + C) + byte(b, i0) + byte(b, i0 + C);
+          packed |= ((sum + 2) >> 2) << (8 * k);
+        }
+        o[w] = packed;
+      }
+      if ((reinterpret_cast<uintptr_t>(out) & 15) == 0 && OB == 16) {
+        store_nt(reinterpret_cast<uint4 *>(out), make_uint4(o[0], o[1], o[2], o[3]));
+      } else if ((reinterpret_cast<uintptr_t>(out) & 7) == 0) {
+#pragma unroll
+        for (int w = 0; w < OB / 8; ++w)
+          store_nt(reinterpret_cast<uint2 *>(out) + w, make_uint2(o[2 * w], o[2 * w + 1]));
+      } else {
+        typedef uint32_t __attribute__((aligned(1))) u32u;
+#pragma unroll
+        for (int w = 0; w < OB / 4; ++w) reinterpret_cast<u32u *>(out)[w] = o[w];
+      }
     } else {
-      for (int k = 0; k < 16 && e0 + k < n; ++k) {
-        const int e = e0 + k;
-        const int i0 = 2 * e - e % C, i1 = i0 + C;
-        const uint32_t s = r0[i0] + r0[i1] + r1[i0] + r1[i1];
-        out[e] = static_cast<uint8_t>((s + 2) >> 2);
+      for (int k = 0; e0 + k < n; ++k) {
+        const int i0 = 2 * (k - k % C) + k % C;
+        const uint32_t sum = r0[i0] + r0[i0 + C] + r1[i0] + r1[i0 + C];
+        out[k] = static_cast<uint8_t>((sum + 2) >> 2);
       }
     }
   }
