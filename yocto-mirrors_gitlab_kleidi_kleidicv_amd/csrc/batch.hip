@@ -18,19 +18,6 @@ namespace kcv {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kNoItem = 0xFFFFFFFFu;
-
-// Item handled by this workgroup at loop step t, or >= nitems when done.
-__device__ __forceinline__ uint32_t item_at(const Segment &s, uint32_t t) {
-  if (!s.xcd_order) return blockIdx.x + t * gridDim.x;
-  // Grid is a multiple of 8: XCD x owns items [x*chunk, (x+1)*chunk) and its
-  // gridDim.x/8 workgroups take consecutive items of that slice together.
-  const uint32_t per_xcd = gridDim.x >> 3;
-  const uint32_t x = blockIdx.x & 7, lb = blockIdx.x >> 3;
-  const uint32_t chunk = (s.nitems + 7) >> 3;
-  const uint32_t o = t * per_xcd + lb;
-  return o < chunk ? x * chunk + o : kNoItem;
-}
 
 // Last job whose first item is <= item (first[0] == 0).
 __device__ __forceinline__ int find_job(const Segment &s, uint32_t item) {
@@ -42,20 +29,44 @@ __device__ __forceinline__ int find_job(const Segment &s, uint32_t item) {
   return lo;
 }
 
-// Calls body(job, bx, by) for each item of the segment owned by this
-// workgroup.  Jobs are read through the constant address space: they are
-// never written while a segment runs, so their fields load as scalars.
+// Calls body(job, bx, by) for each item of this workgroup's share of the
+// segment: a contiguous run of ceil(nitems / grid) items, so that consecutive
+// tiles of an image stay on one CU.  With XCD order the runs of the
+// workgroups on one XCD (workgroup b runs on XCD b mod 8) are adjacent, so
+// neighbouring tiles and their halos share that XCD's L2.  The job is found
+// by one binary search, then followed item by item; jobs are read through
+// the constant address space: they are never written while a segment runs,
+// so their fields load as scalars.
 template <class Body>
 __device__ __forceinline__ void for_each_item(const Segment &s, const Body &body) {
   typedef const __attribute__((address_space(4))) Job ConstJob;
-  for (uint32_t t = 0;; ++t) {
-    const uint32_t item = item_at(s, t);
-    if (item >= s.nitems) break;
-    const int j = find_job(s, item);
+  const uint32_t grid = gridDim.x;
+  const uint32_t per = (s.nitems + grid - 1) / grid;
+  const uint32_t w = s.xcd_order ? static_cast<uint32_t>(xcd_tile(blockIdx.x, grid)) : blockIdx.x;
+  uint32_t item = w * per;
+  const uint32_t end = min(item + per, s.nitems);
+  if (item >= end) return;
+  int j = find_job(s, item);
+  int bx = 0, by = 0, tiles_x = 1;
+  bool fresh = true;
+  for (; item < end; ++item) {
+    if (j + 1 < s.njobs && s.first[j + 1] <= item) {
+      ++j;
+      fresh = true;
+    }
     const Job &job = *(const Job *)((ConstJob *)s.jobs + j);
-    const int local = static_cast<int>(item - s.first[j]);
-    const int by = local / job.tiles_x;
-    body(job, local - by * job.tiles_x, by);
+    if (fresh) {
+      tiles_x = job.tiles_x;
+      const int local = static_cast<int>(item - s.first[j]);
+      by = local / tiles_x;
+      bx = local - by * tiles_x;
+      fresh = false;
+    }
+    body(job, bx, by);
+    if (++bx == tiles_x) {
+      bx = 0;
+      ++by;
+    }
   }
 }
 
