@@ -459,9 +459,81 @@ StagingRing &staging() {
   return *ring;
 }
 
+// Side streams for running the segments of one stage side by side: they
+// are independent (an op only reads outputs of earlier stages), so while one
+// segment's last workgroups drain, the next segment's fill the idle CUs.
+// The caller's stream forks to the side streams through an event and joins
+// them through one event each.  Streams and events are created once per
+// device and kept until the process ends; an event may be re-recorded as
+// soon as the waits on its previous recording have been enqueued.
+class Fork {
+ public:
+  static constexpr int kSide = 3;
+
+  kleidicv_error_t run(const SegmentPlan *segs, size_t n, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(mutex_);
+    if (!ready_) {
+      for (int k = 0; k < kSide; ++k) {
+        KCV_TRY(from_hip(hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking)));
+        KCV_TRY(from_hip(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming)));
+      }
+      KCV_TRY(from_hip(hipEventCreateWithFlags(&fork_, hipEventDisableTiming)));
+      ready_ = true;
+    }
+    const int used = static_cast<int>(std::min<size_t>(n - 1, kSide));
+    KCV_TRY(from_hip(hipEventRecord(fork_, s)));
+    for (int k = 0; k < used; ++k) KCV_TRY(from_hip(hipStreamWaitEvent(side_[k], fork_, 0)));
+    for (size_t i = 0; i < n; ++i) {
+      const SegmentPlan &sp = segs[i];
+      const hipStream_t st = i % (used + 1) == 0 ? s : side_[i % (used + 1) - 1];
+      KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, st)));
+    }
+    for (int k = 0; k < used; ++k) {
+      KCV_TRY(from_hip(hipEventRecord(join_[k], side_[k])));
+      KCV_TRY(from_hip(hipStreamWaitEvent(s, join_[k], 0)));
+    }
+    return KLEIDICV_OK;
+  }
+
+ private:
+  std::mutex mutex_;
+  bool ready_ = false;
+  hipStream_t side_[kSide] = {};
+  hipEvent_t join_[kSide] = {};
+  hipEvent_t fork_ = nullptr;
+};
+
+Fork &fork_for_device() {
+  static std::mutex mutex;
+  static std::vector<Fork *> forks;  // per device, live until the process ends
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lock(mutex);
+  if (static_cast<size_t>(dev) >= forks.size()) forks.resize(dev + 1, nullptr);
+  if (forks[dev] == nullptr) forks[dev] = new Fork;
+  return *forks[dev];
+}
+
+bool same_group(const SegmentPlan &a, const SegmentPlan &b) {
+  const bool ta = a.family == kcv::kFamTables, tb = b.family == kcv::kFamTables;
+  return ta == tb && (ta || a.stage == b.stage);
+}
+
+// Segments in plan order; the segments of one stage (or the tables) run
+// side by side.
 kleidicv_error_t enqueue(const Plan &plan, hipStream_t s) {
-  for (const SegmentPlan &sp : plan.segs)
-    KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
+  const size_t n = plan.segs.size();
+  for (size_t i = 0; i < n;) {
+    size_t j = i + 1;
+    while (j < n && same_group(plan.segs[i], plan.segs[j])) ++j;
+    if (j - i == 1) {
+      const SegmentPlan &sp = plan.segs[i];
+      KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
+    } else {
+      KCV_TRY(fork_for_device().run(&plan.segs[i], j - i, s));
+    }
+    i = j;
+  }
   return KLEIDICV_OK;
 }
 
