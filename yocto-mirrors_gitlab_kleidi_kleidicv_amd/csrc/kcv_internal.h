@@ -54,7 +54,7 @@ __device__ __forceinline__ void store_nt(uint4 *p, uint4 v) {
 
 // Destination tile of the gather kernels (remap s16point5, remap f32, warp
 // perspective), per-image and batched.
-constexpr int kGatherTileW = 64, kGatherTileH = 16;
+constexpr int kGatherTileW = 64, kGatherTileH = 16;  // 4 pixels per thread
 
 // Largest 1-D grid used with xcd_tile (blocks of 256 threads).
 constexpr int64_t kMaxLinearTiles = int64_t{1} << 23;

@@ -189,7 +189,7 @@ __device__ __forceinline__ uint32_t remap_s16_px_narrow(const RemapS16Args &a, u
 // Destination tile (bx, by) of kGatherTileW x kGatherTileH pixels.
 template <int C, int BORDER>
 __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int bx, int by) {
-  const int q = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const int q = threadIdx.x % (kGatherTileW / 4), r = threadIdx.x / (kGatherTileW / 4);
   const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
   if (x >= a.dst_width || y >= a.dst_height) return;
   const int n = min(4, a.dst_width - x);
@@ -551,7 +551,7 @@ template <int C, int INTERP, int BORDER, class Coords>
 __device__ __forceinline__ void gather_tiles(const Src &s, uint8_t *dst, size_t dst_stride,
                                              int dst_width, int dst_height, int bx,
                                              int by, const Coords &coords) {
-  const int q = threadIdx.x & 15, r = threadIdx.x >> 4;
+  const int q = threadIdx.x % (kGatherTileW / 4), r = threadIdx.x / (kGatherTileW / 4);
   const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
   if (x >= dst_width || y >= dst_height) return;
   const int n = min(4, dst_width - x);
