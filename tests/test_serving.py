@@ -150,9 +150,9 @@ def test_arrivals_during_a_batch_overlap_it_on_another_slot():
     assert max(r.launched for r in late) < min(r.done for r in first)
 
 
-def test_single_slot_serialises_batches():
+def test_single_slot_of_depth_one_serialises_batches():
     be = FakeBackend(polls=4)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=1, clock=Clock())
+    srv = S.Server(be, hbm_budget=1 << 30, slots=1, clock=Clock(), depth=1)
     for i in range(3):
         srv.submit(req(i))
     srv.pump()
@@ -160,13 +160,13 @@ def test_single_slot_serialises_batches():
     srv.pump()
     assert len(be.launches) == 1      # the only slot is busy
     srv.drain()
-    assert be.launches == [[0, 1, 2], [9]]
+    assert be.launches == [[0, 1, 2], ["alone", 9]]   # a lone request runs alone
 
 
 def test_large_requests_run_alone_on_slot_zero_and_small_ones_pass_them():
     be = FakeBackend(polls=6)
     srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=8, clock=Clock(),
-                   large_px=64 * 64)
+                   large_px=64 * 64, depth=1)
     big = [req(i, h=64, w=64) for i in range(3)]           # 4096 px: large
     small = [req(10 + i, h=16, w=16) for i in range(5)]    # small
     for r in big[:2] + small + big[2:]:
@@ -182,6 +182,64 @@ def test_large_requests_run_alone_on_slot_zero_and_small_ones_pass_them():
     assert sorted(batched) == [10, 11, 12, 13, 14]
     assert max(r.done for r in small) < min(r.done for r in big[1:])
     assert sorted(r.rid for r in srv.completed) == [0, 1, 2, 10, 11, 12, 13, 14]
+
+
+def test_a_slot_queues_up_to_depth_launches():
+    be = FakeBackend(polls=50)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=1, max_batch_ops=2, clock=Clock(), depth=3)
+    for i in range(10):
+        srv.submit(req(i))
+    srv.pump()
+    assert be.launches == [[0, 1], [2, 3], [4, 5]]     # three queued, none retired yet
+    assert srv.pending() == 10
+    srv.drain()
+    assert [rid for b in be.launches for rid in b] == list(range(10))
+    assert [r.rid for r in srv.completed] == list(range(10))   # retired in launch order
+
+
+def test_device_arena_ring():
+    a = S.DeviceArena(1024, align=64)
+    b1, b2, b3 = a.take(300), a.take(300), a.take(300)   # 320 bytes each
+    assert [b1[0], b2[0], b3[0]] == [0, 320, 640]
+    assert a.take(200) is None                          # 64 bytes left at the end
+    a.release(b2)                                       # out of order: nothing returns yet
+    assert a.take(200) is None
+    a.release(b1)                                       # [0, 640) returns
+    b4 = a.take(500)                                    # wraps to the start
+    assert b4[0] == 0
+    assert a.take(200) is None                          # [512, 640) too small
+    a.release(b3)
+    a.release(b4)
+    assert a.take(1024)[0] == 0                         # empty ring starts over
+    with pytest.raises(ValueError):
+        S.DeviceArena(128).take(129)
+
+
+def test_admission_waits_for_output_buffers():
+    class Limited(FakeBackend):
+        def __init__(self):
+            super().__init__(polls=3)
+            self.live = 0
+            self.peak = 0
+
+        def alloc(self, shape):
+            if self.live == 2:
+                return None
+            self.live += 1
+            self.peak = max(self.peak, self.live)
+            return np.zeros(shape, np.uint8)
+
+        def free(self, t):
+            self.live -= 1
+
+    be = Limited()
+    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=4, clock=Clock(),
+                   release_done=True)
+    for i in range(7):
+        srv.submit(req(i))
+    srv.drain()
+    assert be.peak == 2 and be.live == 0
+    assert sorted(r.rid for r in srv.completed) == list(range(7))
 
 
 def test_poisson_arrivals_rate():

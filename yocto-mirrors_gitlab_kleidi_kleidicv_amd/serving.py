@@ -8,11 +8,13 @@ its own, so it is launched alone through its single-image device entry point
 on the large-request slot, in arrival order.  Small requests are packed into
 batches (kleidicv_hip_batch_launch: one persistent launch per kernel variant)
 on the other stream slots, so they never wait behind a large one and the
-launch cost is shared; the large-request slot also takes small batches when
-no large request waits.  Batches on different slots overlap on the device,
-so requests that arrive during a batch start before it ends.  A request
-completes when the host observes its launch's event; its output is then
-handed to the caller and its budget released.
+launch cost is shared (a lone small request also runs through its own entry
+point); the large-request slot also takes small batches when no large
+request waits.  Each slot keeps up to `depth` launches queued on
+its stream, so the device never waits for the host between them, and
+batches on different slots overlap on the device.  A request completes
+when the host observes its launch's event; its output is then handed to the
+caller and its budget released.
 
 Ops served: gaussian blur 5x5 (sigma 0, REVERSE), median blur 5x5
 (REPLICATE), linear resize to ceil(w/2) x ceil(h/2), warp perspective
@@ -96,17 +98,88 @@ class Request:
         return dst
 
 
-class TorchBackend:
-    """Device side through torch tensors, streams and events."""
+class DeviceArena:
+    """Byte ranges of one device allocation handed out as a ring.
 
-    def __init__(self, torch, device):
+    Blocks are taken at the head in allocation order and may be released in
+    any order; the space of the oldest blocks returns once they and every
+    block before them are released.  Requests complete in roughly arrival
+    order, so the ring stays compact, and taking a block costs no device
+    allocation.
+    """
+
+    def __init__(self, size, align=256):
+        self.size, self.align = int(size), align
+        self.head = 0
+        self.blocks = collections.deque()   # [offset, nbytes, released]
+
+    def take(self, nbytes):
+        """Offset of a new block, or None while the ring is full."""
+        n = -(-int(nbytes) // self.align) * self.align
+        if n > self.size:
+            raise ValueError("block of %d bytes exceeds the arena (%d bytes)" % (n, self.size))
+        if not self.blocks:
+            self.head = 0
+        tail = self.blocks[0][0] if self.blocks else 0
+        if not self.blocks or self.head > tail:     # free: [head, size) and [0, tail)
+            if self.size - self.head >= n:
+                off = self.head
+            elif tail > n:                           # wrap; the end of the ring idles
+                off = 0
+            else:
+                return None
+        elif tail - self.head > n:                   # wrapped; free: [head, tail)
+            off = self.head
+        else:
+            return None
+        block = [off, n, False]
+        self.blocks.append(block)
+        self.head = off + n
+        return block
+
+    def release(self, block):
+        block[2] = True
+        while self.blocks and self.blocks[0][2]:
+            self.blocks.popleft()
+
+
+class TorchBackend:
+    """Device side through torch tensors, streams and events.
+
+    With arena_bytes, output buffers are views into one device allocation
+    managed as a DeviceArena (no allocator call per request); alloc returns
+    None while it is full and free returns a buffer to it.  Without, each
+    output is a torch allocation and free does nothing.
+    """
+
+    def __init__(self, torch, device, arena_bytes=None):
         self.torch, self.device = torch, device
+        self.arena = DeviceArena(arena_bytes) if arena_bytes else None
+        self.arena_buf = None
+        self.blocks = {}
 
     def stream(self):
         return self.torch.cuda.Stream(device=self.device)
 
     def alloc(self, shape):
-        return self.torch.empty(shape, dtype=self.torch.uint8, device=self.device)
+        if self.arena is None:
+            return self.torch.empty(shape, dtype=self.torch.uint8, device=self.device)
+        if self.arena_buf is None:
+            self.arena_buf = self.torch.empty(self.arena.size, dtype=self.torch.uint8,
+                                              device=self.device)
+        n = 1
+        for d in shape:
+            n *= d
+        block = self.arena.take(n)
+        if block is None:
+            return None
+        t = self.arena_buf[block[0]:block[0] + n].view(shape)
+        self.blocks[t.data_ptr()] = block
+        return t
+
+    def free(self, tensor):
+        if self.arena is not None:
+            self.arena.release(self.blocks.pop(tensor.data_ptr()))
 
     def workspace(self, nbytes):
         return self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8,
@@ -136,12 +209,16 @@ class Server:
     inputs are resident already).  Each slot's work-list workspace comes out
     of a fixed reserve of workspace_reserve bytes; outputs are admitted
     against the rest.  max_batch_ops bounds one batch.  Slot 0 is the
-    large-request slot; with one slot, everything runs on it.
+    large-request slot; with one slot, everything runs on it.  depth bounds
+    the launches queued on one slot's stream.  A slot's work lists are
+    uploaded on its own stream, so a queued batch can reuse the workspace;
+    a replaced workspace stays referenced until the launches that read it
+    retire.
     """
 
     def __init__(self, backend, hbm_budget=64 << 30, slots=2, max_batch_ops=512,
                  workspace_reserve=64 << 20, tuning=None, clock=time.perf_counter,
-                 large_px=1 << 18):
+                 large_px=1 << 18, depth=4, release_done=False):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         if slots * workspace_reserve >= hbm_budget:
@@ -155,8 +232,13 @@ class Server:
         self.max_batch_ops = max_batch_ops
         self.tuning = tuning
         self.clock = clock
+        if depth < 1:
+            raise ValueError("depth must be >= 1")
+        self.depth = depth
+        self.release_done = release_done
         self.streams = [backend.stream() for _ in range(slots)]
-        self.inflight = [None] * slots  # (event, requests)
+        # per slot, in launch order: (event, requests, workspace read)
+        self.inflight = [collections.deque() for _ in range(slots)]
         self.ws = [None] * slots
         self.ws_bytes = [0] * slots
         self.large_px = large_px
@@ -171,7 +253,10 @@ class Server:
         self.queue.append(req)
 
     def _admit(self, now):
-        while self.queue:
+        # Admitted requests hold their output buffers; admit no more than the
+        # slots can queue, so allocation interleaves with launches.
+        cap = self.depth * self.max_batch_ops * len(self.streams)
+        while self.queue and len(self.ready) + len(self.ready_large) < cap:
             req = self.queue[0]
             need = req.out_bytes()
             if need > self.out_budget:
@@ -179,10 +264,13 @@ class Server:
                                  % (req.rid, need))
             if self.out_in_use + need > self.out_budget:
                 return  # FIFO: later requests wait behind the head
+            dst = self.be.alloc(req.out_shape())
+            if dst is None:
+                return  # output buffers exhausted until earlier requests complete
             self.queue.popleft()
             self.out_in_use += need
             self.peak = max(self.peak, self.in_use)
-            req.dst = self.be.alloc(req.out_shape())
+            req.dst = dst
             req.admitted = now
             if req.out_pixels() >= self.large_px:
                 self.ready_large.append(req)
@@ -191,60 +279,70 @@ class Server:
 
     # -- completion -----------------------------------------------------------
     def _retire(self, now):
-        for i, slot in enumerate(self.inflight):
-            if slot is None or not self.be.query(slot[0]):
-                continue
-            _, reqs = slot
-            for r in reqs:
-                r.done = now
-                self.out_in_use -= r.out_bytes()
-                self.completed.append(r)
-            self.inflight[i] = None
+        for q in self.inflight:
+            while q and self.be.query(q[0][0]):
+                _, reqs, _ = q.popleft()
+                for r in reqs:
+                    r.done = now
+                    self.out_in_use -= r.out_bytes()
+                    self.completed.append(r)
+                    if self.release_done:
+                        self.be.free(r.dst)
 
     # -- batching -----------------------------------------------------------
     def _launch(self, now):
         for i in range(len(self.streams)):
-            if self.inflight[i] is not None:
-                continue
-            if i == 0 and self.ready_large:
-                req = self.ready_large.popleft()
-                self.be.launch_one(req, self.streams[0])
-                self._started(0, [req], now)
-                continue
-            if not self.ready:
-                continue
-            reqs = []
-            while self.ready and len(reqs) < self.max_batch_ops:
-                reqs.append(self.ready.popleft())
-            # A slot's workspace lives in its reserve.  Launch into it; when
-            # the work lists need more, grow it within the reserve or halve
-            # the batch (the rest goes back to the front of the queue).
-            descs = [r.desc() for r in reqs]
-            while True:
-                need = self.be.launch(descs, self.ws[i], self.streams[i], self.tuning, reqs)
-                if need == 0:
-                    break
-                if need <= self.reserve:
-                    grow = min(max(need, 2 * self.ws_bytes[i]), self.reserve)
-                    self.ws[i] = self.be.workspace(grow)
-                    self.ws_bytes[i] = grow
-                    self.peak = max(self.peak, self.in_use)
-                    continue
-                if len(reqs) == 1:
-                    raise ValueError("work lists of request %d exceed the workspace reserve"
-                                     % reqs[0].rid)
-                keep = len(reqs) // 2
-                self.ready.extendleft(reversed(reqs[keep:]))
-                reqs, descs = reqs[:keep], descs[:keep]
-            self._started(i, reqs, now)
+            while len(self.inflight[i]) < self.depth and self._launch_one(i, now):
+                pass
 
-    def _started(self, i, reqs, now):
+    def _launch_one(self, i, now):
+        """Launches one request or batch on slot i; False when nothing fits."""
+        if i == 0 and self.ready_large:
+            req = self.ready_large.popleft()
+            self.be.launch_one(req, self.streams[0])
+            self._started(0, [req], None, now)
+            return True
+        if not self.ready:
+            return False
+        if len(self.ready) == 1:
+            # a lone small request: its own entry point is cheaper than a batch
+            req = self.ready.popleft()
+            self.be.launch_one(req, self.streams[i])
+            self._started(i, [req], None, now)
+            return True
+        reqs = []
+        while self.ready and len(reqs) < self.max_batch_ops:
+            reqs.append(self.ready.popleft())
+        # A slot's workspace lives in its reserve.  Launch into it; when
+        # the work lists need more, grow it within the reserve or halve
+        # the batch (the rest goes back to the front of the queue).
+        descs = [r.desc() for r in reqs]
+        while True:
+            need = self.be.launch(descs, self.ws[i], self.streams[i], self.tuning, reqs)
+            if need == 0:
+                break
+            if need <= self.reserve:
+                grow = min(max(need, 2 * self.ws_bytes[i]), self.reserve)
+                self.ws[i] = self.be.workspace(grow)
+                self.ws_bytes[i] = grow
+                self.peak = max(self.peak, self.in_use)
+                continue
+            if len(reqs) == 1:
+                raise ValueError("work lists of request %d exceed the workspace reserve"
+                                 % reqs[0].rid)
+            keep = len(reqs) // 2
+            self.ready.extendleft(reversed(reqs[keep:]))
+            reqs, descs = reqs[:keep], descs[:keep]
+        self._started(i, reqs, self.ws[i], now)
+        return True
+
+    def _started(self, i, reqs, workspace, now):
         ev = self.be.record(self.streams[i])
         self.batches += 1
         for r in reqs:
             r.launched = now
             r.batch = self.batches
-        self.inflight[i] = (ev, reqs)
+        self.inflight[i].append((ev, reqs, workspace))
 
     def pump(self):
         """One scheduling tick; returns the number of requests still pending."""
@@ -261,7 +359,7 @@ class Server:
 
     def pending(self):
         return (len(self.queue) + len(self.ready) + len(self.ready_large) +
-                sum(len(s[1]) for s in self.inflight if s is not None))
+                sum(len(e[1]) for q in self.inflight for e in q))
 
     def drain(self):
         while self.pending():
@@ -286,8 +384,9 @@ def run_served(server, requests, clock=time.perf_counter):
     return t0
 
 
-def run_plain(backend, requests, stream, clock=time.perf_counter):
-    """Baseline: each request launched alone at its arrival on one stream."""
+def run_plain(backend, requests, stream, clock=time.perf_counter, release_done=False):
+    """Baseline: each request launched alone at its arrival on one stream.
+    A request whose output buffer is not available waits (arena backend)."""
     t0 = clock()
     order = sorted(requests, key=lambda r: r.arrival)
     inflight = collections.deque()
@@ -297,12 +396,17 @@ def run_plain(backend, requests, stream, clock=time.perf_counter):
         while i < len(order) and order[i].arrival <= now:
             r = order[i]
             r.dst = backend.alloc(r.out_shape())
+            if r.dst is None:
+                break
             r.admitted = r.launched = clock()
             r.run_alone(r.dst, stream)
             inflight.append((backend.record(stream), r))
             i += 1
         while inflight and backend.query(inflight[0][0]):
-            inflight.popleft()[1].done = clock()
+            r = inflight.popleft()[1]
+            r.done = clock()
+            if release_done:
+                backend.free(r.dst)
     return t0
 
 
