@@ -272,12 +272,14 @@ struct Plan {
   size_t bytes = 0;
   std::vector<SegmentPlan> segs;
   size_t items = 0;
+  int blocks_per_cu = 1;
 };
 
 kleidicv_error_t make_plan(const std::vector<Prepared> &prep,
                            const kleidicv_hip_batch_tuning &t, char *base, Plan *out) {
   const int cus = kcv::runtime().compute_units > 0 ? kcv::runtime().compute_units : 256;
   *out = Plan{};
+  out->blocks_per_cu = static_cast<int>(t.blocks_per_cu);
 
   // Jobs in (stage, family, variant) order; resize ops add a tables job.
   struct Entry {
@@ -503,15 +505,18 @@ class Fork {
   hipEvent_t fork_ = nullptr;
 };
 
-Fork &fork_for_device() {
+// Side streams per (device, caller stream): independent callers never wait
+// on each other's side work.
+Fork &fork_for(hipStream_t s) {
   static std::mutex mutex;
-  static std::vector<Fork *> forks;  // per device, live until the process ends
+  static std::vector<std::pair<std::pair<int, hipStream_t>, Fork *>> forks;  // live until exit
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lock(mutex);
-  if (static_cast<size_t>(dev) >= forks.size()) forks.resize(dev + 1, nullptr);
-  if (forks[dev] == nullptr) forks[dev] = new Fork;
-  return *forks[dev];
+  for (auto &f : forks)
+    if (f.first.first == dev && f.first.second == s) return *f.second;
+  forks.push_back({{dev, s}, new Fork});
+  return *forks.back().second;
 }
 
 bool same_group(const SegmentPlan &a, const SegmentPlan &b) {
@@ -519,18 +524,24 @@ bool same_group(const SegmentPlan &a, const SegmentPlan &b) {
   return ta == tb && (ta || a.stage == b.stage);
 }
 
-// Segments in plan order; the segments of one stage (or the tables) run
-// side by side.
+// Segments in plan order.  The segments of one stage (or the tables) run
+// side by side when they hold enough work to pay for the fork and join: at
+// least four waves of workgroups over the whole GPU.
 kleidicv_error_t enqueue(const Plan &plan, hipStream_t s) {
   const size_t n = plan.segs.size();
+  const int64_t full = int64_t{4} * std::max(kcv::runtime().compute_units, 1) *
+                       std::max<int64_t>(plan.blocks_per_cu, 1);
   for (size_t i = 0; i < n;) {
     size_t j = i + 1;
-    while (j < n && same_group(plan.segs[i], plan.segs[j])) ++j;
-    if (j - i == 1) {
-      const SegmentPlan &sp = plan.segs[i];
-      KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
+    int64_t items = plan.segs[i].seg.nitems;
+    while (j < n && same_group(plan.segs[i], plan.segs[j])) items += plan.segs[j++].seg.nitems;
+    if (j - i == 1 || items < full) {
+      for (size_t k = i; k < j; ++k) {
+        const SegmentPlan &sp = plan.segs[k];
+        KCV_TRY(from_hip(kcv::launch_segment(sp.family, sp.variant, sp.seg, sp.grid, s)));
+      }
     } else {
-      KCV_TRY(fork_for_device().run(&plan.segs[i], j - i, s));
+      KCV_TRY(fork_for(s).run(&plan.segs[i], j - i, s));
     }
     i = j;
   }

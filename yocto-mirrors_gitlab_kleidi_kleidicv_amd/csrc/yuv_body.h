@@ -41,20 +41,37 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
                        (OC == 4 ? 15 : 7)) == 0;
   for (int yp = yp_begin; yp < yp_end && 2 * yp < a.height; yp += yp_step) {
     const uint8_t *uvrow = a.uv + static_cast<size_t>(yp) * a.uv_stride;
-    // chroma terms of the 4 pairs
-    int ru[4], gu[4], bu[4];
-    uint32_t uv_lo = 0, uv_hi = 0;
+    const bool two = 2 * yp + 1 < a.height;
+    // All three loads (chroma row and both luma rows) are issued before any
+    // arithmetic, so their latencies overlap.
+    uint32_t uv_lo = 0, uv_hi = 0, yw_lo[2] = {0, 0}, yw_hi[2] = {0, 0};
     if (full && in_al) {
+      const uint8_t *y0row = a.y + static_cast<size_t>(2 * yp) * a.y_stride + x0;
       const uint2 q = *reinterpret_cast<const uint2 *>(uvrow + x0);
+      const uint2 q0 = *reinterpret_cast<const uint2 *>(y0row);
+      const uint2 q1 = two ? *reinterpret_cast<const uint2 *>(y0row + a.y_stride) : q0;
       uv_lo = q.x;
       uv_hi = q.y;
+      yw_lo[0] = q0.x;
+      yw_hi[0] = q0.y;
+      yw_lo[1] = q1.x;
+      yw_hi[1] = q1.y;
     } else {
       for (int k = 0; k < kBlockW; ++k) {
         const int x = min(x0 + k, ((a.width + 1) & ~1) - 1);
         const uint32_t v = uvrow[x];
         if (k < 4) uv_lo |= v << (8 * k); else uv_hi |= v << (8 * (k - 4));
       }
+      for (int r = 0; r < (two ? 2 : 1); ++r) {
+        const uint8_t *yrow = a.y + static_cast<size_t>(2 * yp + r) * a.y_stride;
+        for (int k = 0; k < kBlockW && x0 + k < a.width; ++k) {
+          const uint32_t v = yrow[x0 + k];
+          if (k < 4) yw_lo[r] |= v << (8 * k); else yw_hi[r] |= v << (8 * (k - 4));
+        }
+      }
     }
+    // chroma terms of the 4 pairs
+    int ru[4], gu[4], bu[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const uint32_t w = p < 2 ? uv_lo : uv_hi;
@@ -70,18 +87,7 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
     for (int r = 0; r < 2; ++r) {
       const int y = 2 * yp + r;
       if (y >= a.height) break;
-      const uint8_t *yrow = a.y + static_cast<size_t>(y) * a.y_stride;
-      uint32_t ylo = 0, yhi = 0;
-      if (full && in_al) {
-        const uint2 q = *reinterpret_cast<const uint2 *>(yrow + x0);
-        ylo = q.x;
-        yhi = q.y;
-      } else {
-        for (int k = 0; k < kBlockW && x0 + k < a.width; ++k) {
-          const uint32_t v = yrow[x0 + k];
-          if (k < 4) ylo |= v << (8 * k); else yhi |= v << (8 * (k - 4));
-        }
-      }
+      const uint32_t ylo = yw_lo[r], yhi = yw_hi[r];
       uint32_t px[kBlockW];  // packed output pixels (R, G, B, A bytes)
 #pragma unroll
       for (int k = 0; k < kBlockW; ++k) {
