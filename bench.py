@@ -130,6 +130,54 @@ OP_KERNELS = {
 }
 
 
+# Entry points beyond the headline step, timed one op at a time on the same
+# 3840x2160 inputs: name, dst pixels, algorithmic bytes, call.
+def extended_ops(K, s, out, stream):
+    def bd(src, name):
+        return lambda: K.blur_and_downsample(src, "REVERSE", dst=out[name], stream=stream,
+                                             device_api=True)
+    return [
+        ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
+         bd(s["sep_src"], "blur_and_downsample_1ch")),
+        ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
+         bd(s["down_src"], "blur_and_downsample_4ch")),
+    ]
+
+
+def extended_outputs(torch, dev):
+    return {
+        "blur_and_downsample_1ch": torch.empty(H // 2, W // 2, dtype=torch.uint8, device=dev),
+        "blur_and_downsample_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
+    }
+
+
+def bench_extended(K, torch, sets, dev, stream, steps, warmup):
+    """Each extended op alone: `steps` launches over the rotating input sets
+    between two events on the launch stream; ms per launch."""
+    out = extended_outputs(torch, dev)
+    res = {}
+    names = [o[0] for o in extended_ops(K, sets[0], out, stream)]
+    for k, name in enumerate(names):
+        calls = [extended_ops(K, s, out, stream)[k] for s in sets]
+        _, pix, nbytes, _ = calls[0]
+        for i in range(warmup):
+            calls[i % len(calls)][3]()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for i in range(steps):
+            calls[i % len(calls)][3]()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "mpix_s": round(pix / (ms * 1e-3) / 1e6, 1),
+                     "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "bytes": nbytes}
+    return res
+
+
 def _pmc_pass(exe, counter, extra, td, tag):
     """One rocprofv3 --pmc run of this script; mean counter value per kernel name."""
     out = os.path.join(td, counter + "_" + tag)
@@ -537,6 +585,11 @@ def main():
         result["roofline"]["traffic_note"] = (
             "HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, each divided by its "
             "reading per byte of a plain 512 MiB copy (factors %s)" % json.dumps(traffic["_factor"]))
+    if not args.ops:
+        result["extended"] = bench_extended(K, torch, sets, dev, stream, args.steps,
+                                            args.warmup)
+        result["extended_note"] = ("entry points outside the headline step, each timed alone "
+                                   "on the same 3840x2160 inputs (launch-to-launch time)")
     if not args.no_batched and not args.ops:
         import workloads as WL
         result["batched"] = bench_batched(K, torch, WL, dev, stream, args.steps, args.warmup)

@@ -104,6 +104,14 @@ kleidicv_error_t kleidicv_separable_filter_2d_u8(
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type);
 
+/* Replaces kleidicv.h:2215-2218 (src/filters/blur_and_downsample_api.cpp:
+ * 37-45): the binomial 5x5 Gaussian at even rows and columns (cv::pyrDown),
+ * destination ((src_width+1)/2) x ((src_height+1)/2). */
+kleidicv_error_t kleidicv_blur_and_downsample_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t channels,
+    kleidicv_border_type_t border_type);
+
 /* Replaces kleidicv.h:2271 (src/filters/median_blur_api.cpp:121-130).
  * Kernel sizes 3, 5 and 7 (the reference's sorting-network sizes). */
 kleidicv_error_t kleidicv_median_blur_u8(
@@ -181,6 +189,11 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:482-485 */
+kleidicv_error_t kleidicv_thread_blur_and_downsample_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t channels,
+    kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
 /* kleidicv_thread.h:496 */
 kleidicv_error_t kleidicv_thread_median_blur_u8(
     const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
@@ -239,6 +252,11 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
     const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_blur_and_downsample_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t channels,
     kleidicv_border_type_t border_type, void *stream);
 
 kleidicv_error_t kleidicv_hip_median_blur_u8(

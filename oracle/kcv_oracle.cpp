@@ -197,6 +197,54 @@ extern "C" int orc_gaussian_blur_u8(const uint8_t *src, size_t src_stride,
 }
 
 // ---------------------------------------------------------------------------
+// Blur and downsample (cv::pyrDown): the 5x5 binomial Gaussian evaluated at
+// even source rows and columns only.  src/filters/blur_and_downsample_neon.cpp
+// sums [1,4,6,4,1] vertically into u16 (max 4080) and horizontally into u16
+// (max 65280), then rounds once: (acc + 128) >> 8.  Output is
+// ((w+1)/2) x ((h+1)/2).  Validation order: blur_and_downsample.h:28-32
+// (w, h >= 4, 1 <= channels <= 4), then the fixed border types
+// (blur_and_downsample_api.cpp:25-28), then pointers and image size
+// (blur_and_downsample_neon.cpp:358-360).
+// Parity: pinned by the reference's Minimal_u8 KAT
+// (test_blur_and_downsample.cpp:179-206) and by a numpy padded brute force
+// in tests/, and equal to the binomial Gaussian 5x5 at even positions.
+extern "C" int orc_blur_and_downsample_u8_stripe(
+    const uint8_t *src, size_t src_stride, size_t width, size_t height,
+    uint8_t *dst, size_t dst_stride, size_t channels, int border,
+    size_t y_begin, size_t y_end) {
+  if (width < 4 || height < 4 || channels < 1 || channels > 4)
+    return kErrNotImplemented;
+  if (!is_filter_border(border)) return kErrNotImplemented;
+  if (!src || !dst) return kErrNull;
+  static const int kTap[5] = {1, 4, 6, 4, 1};
+  const long W = static_cast<long>(width), H = static_cast<long>(height);
+  const long C = static_cast<long>(channels);
+  const long dw = (W + 1) / 2;
+  // y_begin / y_end are source rows, as the reference's stripes are
+  for (long y = (static_cast<long>(y_begin) + 1) & ~1L; y < static_cast<long>(y_end); y += 2)
+    for (long x = 0; x < dw; ++x)
+      for (long c = 0; c < C; ++c) {
+        uint32_t acc = 0;
+        for (int j = 0; j < 5; ++j) {
+          uint32_t col = 0;
+          const long xs = border_index(2 * x + j - 2, W, border);
+          for (int i = 0; i < 5; ++i)
+            col += kTap[i] * src[border_index(y + i - 2, H, border) * src_stride + xs * C + c];
+          acc += kTap[j] * col;
+        }
+        dst[(y / 2) * dst_stride + x * C + c] = static_cast<uint8_t>((acc + 128) >> 8);
+      }
+  return kErrOk;
+}
+
+extern "C" int orc_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,
+                                          size_t width, size_t height, uint8_t *dst,
+                                          size_t dst_stride, size_t channels, int border) {
+  return orc_blur_and_downsample_u8_stripe(src, src_stride, width, height, dst, dst_stride,
+                                           channels, border, 0, height);
+}
+
+// ---------------------------------------------------------------------------
 // a5: separable filter.  src/filters/separable_filter_2d_neon.cpp:19-119:
 // vertical u16 saturating sum, horizontal u32 sum, saturating narrow to u8.
 // With unsigned taps an inner saturation (>= 65535) forces the output to 255

@@ -44,6 +44,16 @@ int orc_separable_filter_2d_u8(const uint8_t *src, size_t src_stride,
                                const uint8_t *kernel_y, size_t kernel_height,
                                int border);
 
+/* Blur and downsample (pyrDown): binomial 5x5 at even rows and columns,
+ * output ((w+1)/2) x ((h+1)/2).  The stripe variant takes source rows. */
+int orc_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,
+                               size_t width, size_t height, uint8_t *dst,
+                               size_t dst_stride, size_t channels, int border);
+int orc_blur_and_downsample_u8_stripe(const uint8_t *src, size_t src_stride,
+                                      size_t width, size_t height, uint8_t *dst,
+                                      size_t dst_stride, size_t channels,
+                                      int border, size_t y_begin, size_t y_end);
+
 /* a6: median blur by brute-force sort. */
 int orc_median_blur_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
                        size_t dst_stride, size_t width, size_t height,

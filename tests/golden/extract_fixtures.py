@@ -15,6 +15,7 @@ Sources (paths relative to the reference root):
   test/api/test_yuv420sp_to_rgb.cpp:42-53         fixed NV12 inputs
   test/api/test_resize_linear.cpp:1016-1187       ResizeLinearU8Params tables
   test/unit_neon/test_resize_linear_generic_u8.cpp:251-389  generator sizes
+  test/api/test_blur_and_downsample.cpp:179-206   BlurAndDownsample.Minimal_u8
 """
 
 import json
@@ -139,6 +140,14 @@ def generator_sizes(ref):
     return out
 
 
+def blur_and_downsample(ref):
+    text = open(os.path.join(ref, "test/api/test_blur_and_downsample.cpp")).read()
+    block = _block(text, "TEST(BlurAndDownsample, Minimal_u8)")
+    src = [r[2] for r in sorted(_sets(block, "input"))]
+    exp = [r[2] for r in sorted(_sets(block, "expected"))]
+    return [{"border": "REPLICATE", "channels": 1, "src": src, "expected": exp}]
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     data = {
@@ -148,6 +157,7 @@ def main():
         "nv12_fixed_inputs": nv12(ref),
         "resize_u8_params": resize_params(ref),
         "resize_generator_sizes": generator_sizes(ref),
+        "blur_and_downsample_minimal": blur_and_downsample(ref),
     }
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "reference_kats.json"), "w") as f:

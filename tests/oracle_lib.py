@@ -64,6 +64,17 @@ def gaussian_blur(src, kernel, sigma=0.0, border="REVERSE", y_range=None):
     return err, dst.reshape(src.shape)
 
 
+def blur_and_downsample(src, border="REVERSE", y_range=None):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros(((h + 1) // 2, (w + 1) // 2, c), np.uint8)
+    y0, y1 = (0, h) if y_range is None else y_range
+    err = lib().orc_blur_and_downsample_u8_stripe(
+        _p(s), *_sz(w * c, w, h), _p(dst), *_sz(dst.strides[0], c), C.c_int(BORDER[border]),
+        *_sz(y0, y1))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
 def separable_filter_2d(src, kx, ky, border="REPLICATE"):
     s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
     h, w, c = s.shape

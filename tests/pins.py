@@ -41,6 +41,13 @@ def binomial_blur(src, k, border):
     return out.reshape(src.shape)
 
 
+def blur_and_downsample(src, border):
+    """cv::pyrDown as the reference's test computes it
+    (test_blur_and_downsample.cpp:86-103): the 5x5 binomial at even rows and
+    columns, (acc + 128) / 256."""
+    return binomial_blur(src, 5, border)[::2, ::2]
+
+
 def separable_two_stage(src, kx, ky, border):
     """The reference's vector path: u16-saturated vertical sum, then a
     horizontal sum saturated to u8 (separable_filter_2d_neon.cpp:37-69)."""

@@ -94,6 +94,29 @@ def test_median_error_codes():
     assert med(p, p + 1, 16, 16, 1, 4, 1) == K.NOT_IMPLEMENTED
 
 
+def test_blur_and_downsample_error_codes():
+    # test_blur_and_downsample.cpp:208-287, in the reference's check order
+    lib = K.lib()
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+
+    def bd(src, w, h, dst, c, b):
+        return lib.kleidicv_blur_and_downsample_u8(
+            C.c_void_p(src), C.c_size_t(1), C.c_size_t(w), C.c_size_t(h), C.c_void_p(dst),
+            C.c_size_t(1), C.c_size_t(c), C.c_int(b))
+    for b in (0, 5, 6):
+        assert bd(p, 4, 4, p + 1, 1, b) == K.NOT_IMPLEMENTED
+    assert bd(p, 3, 3, p + 1, 1, 1) == K.NOT_IMPLEMENTED
+    assert bd(p, 3, 4, p + 1, 1, 1) == K.NOT_IMPLEMENTED
+    assert bd(p, 4, 3, p + 1, 1, 1) == K.NOT_IMPLEMENTED
+    assert bd(p, 4, 4, p + 1, 5, 2) == K.NOT_IMPLEMENTED
+    assert bd(p, 4, 4, p + 1, 0, 2) == K.NOT_IMPLEMENTED
+    assert bd(0, 4, 4, p + 1, 1, 1) == K.NULL_POINTER
+    assert bd(p, 4, 4, 0, 1, 1) == K.NULL_POINTER
+    assert bd(p, (1 << 48) // 4 + 1, 4, p + 1, 1, 2) == K.RANGE
+    assert bd(p, 1 << 48, 1 << 48, p + 1, 1, 2) == K.RANGE
+
+
 def test_resize_error_codes():
     lib = K.lib()
     buf = np.zeros(4096, np.uint8)

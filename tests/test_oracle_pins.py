@@ -64,6 +64,43 @@ def test_gaussian_sigma_rounds_twice(rng):
     np.testing.assert_array_equal(out, ((hor + 128) >> 8).astype(np.uint8))
 
 
+@pytest.mark.parametrize("kat", KATS["blur_and_downsample_minimal"])
+def test_blur_and_downsample_kat(kat):
+    err, out = O.blur_and_downsample(np.array(kat["src"], np.uint8), kat["border"])
+    assert err == 0
+    np.testing.assert_array_equal(out, np.array(kat["expected"]))
+
+
+@pytest.mark.parametrize("border", BORDERS)
+@pytest.mark.parametrize("channels", [1, 2, 3, 4])
+def test_blur_and_downsample_vs_padded_convolution(rng, border, channels):
+    for h, w in [(4, 4), (5, 5), (7, 9), (12, 161), (9, 30)]:
+        src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+        err, out = O.blur_and_downsample(src, border)
+        assert err == 0
+        np.testing.assert_array_equal(out, pins.blur_and_downsample(src, border))
+        # the even-position samples of the full Gaussian 5x5
+        _, full = O.gaussian_blur(src, 5, 0.0, border)
+        np.testing.assert_array_equal(out, full[::2, ::2])
+
+
+def test_blur_and_downsample_stripes_and_errors(rng):
+    src = rng.integers(0, 256, (23, 17, 3), dtype=np.uint8)
+    _, whole = O.blur_and_downsample(src, "WRAP")
+    parts = np.zeros_like(whole)
+    for y0, y1 in [(0, 5), (5, 6), (6, 15), (15, 23)]:
+        err, part = O.blur_and_downsample(src, "WRAP", (y0, y1))
+        assert err == 0
+        rows = slice((y0 + 1) // 2, (y1 + 1) // 2)
+        parts[rows] = part[rows]
+    np.testing.assert_array_equal(parts, whole)
+    # test_blur_and_downsample.cpp:208-287: borders, undersize, channel count
+    for border in ("CONSTANT", "TRANSPARENT", "NONE"):
+        assert O.blur_and_downsample(src, border)[0] == 1
+    assert O.blur_and_downsample(src[:3], "REPLICATE")[0] == 1
+    assert O.blur_and_downsample(src[:, :3], "REPLICATE")[0] == 1
+
+
 @pytest.mark.parametrize("kat", KATS["separable_overflow"], ids=lambda k: k["name"])
 def test_separable_kats(kat):
     err, out = O.separable_filter_2d(np.array(kat["src"], np.uint8), kat["kernel_x"],

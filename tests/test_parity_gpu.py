@@ -110,6 +110,44 @@ def test_separable_kats(kat):
 
 # ---------------------------------------------------------------------------
 # Median
+# ---------------------------------------------------------------------------
+# Blur and downsample (pyrDown)
+@pytest.mark.parametrize("border", BORDERS)
+def test_blur_and_downsample(rng, border):
+    # test_blur_and_downsample.cpp:153-177 sizes (161 px wide), the minimum,
+    # odd sizes and widths across several 128-byte output tiles
+    for (w, h) in [(4, 4), (5, 5), (160, 120), (161, 121), (161, 123), (257, 33),
+                   (300, 17), (1023, 19)]:
+        for c in (1, 2, 3, 4):
+            src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+            got = K.blur_and_downsample(src, border)
+            _, want = O.blur_and_downsample(src, border)
+            assert_same(got, want, "%s %dx%dx%d" % (border, w, h, c))
+
+
+@pytest.mark.parametrize("kat", KATS["blur_and_downsample_minimal"])
+def test_blur_and_downsample_kat(kat):
+    src = np.array(kat["src"], np.uint8)
+    assert_same(K.blur_and_downsample(src, kat["border"]), np.array(kat["expected"]))
+
+
+def test_blur_and_downsample_padded_and_device(rng):
+    import torch
+    for c in (1, 3, 4):
+        src = padded_rows(rng, 45, 77, c, 13)
+        h, w = 45, 77
+        dbuf = np.full((23, 39 * c + 7), 0xA5, np.uint8)
+        dst = dbuf[:, :39 * c].reshape(23, 39, c) if c > 1 else dbuf[:, :39]
+        K.blur_and_downsample(src, "REVERSE", dst=dst)
+        _, want = O.blur_and_downsample(np.ascontiguousarray(src), "REVERSE")
+        assert_same(dst, want, "padded c%d" % c)
+        assert (dbuf[:, 39 * c:] == 0xA5).all()
+        s = torch.from_numpy(np.ascontiguousarray(src)).cuda()
+        d = K.blur_and_downsample(s, "REVERSE", device_api=True)
+        torch.cuda.synchronize()
+        assert_same(d.cpu().numpy(), want, "device c%d" % c)
+
+
 @pytest.mark.parametrize("k", [3, 5, 7])
 @pytest.mark.parametrize("border", BORDERS)
 def test_median(rng, k, border):

@@ -144,6 +144,24 @@ kleidicv_error_t gaussian_impl(const uint8_t *src, size_t src_stride, uint8_t *d
   return ss.finish();
 }
 
+kleidicv_error_t blur_down_impl(const uint8_t *src, size_t src_stride, size_t width,
+                                size_t height, uint8_t *dst, size_t dst_stride,
+                                size_t channels, kleidicv_border_type_t border_type,
+                                void *stream, bool sync) {
+  kcv::SepArgs a;
+  Action act;
+  KCV_TRY(kcv::prepare_blur_down(src, src_stride, width, height, dst, dst_stride, channels,
+                                 border_type, &a, &act));
+  if (act == Action::kNone) return KLEIDICV_OK;
+  hipStream_t s = pick_stream(stream, sync);
+  if (!sync) return from_hip(kcv::launch_blur_down(a, s));
+  Session ss(s);
+  KCV_TRY(ss.input(src, extent(src_stride, width * channels, height), &a.src));
+  KCV_TRY(ss.output(dst, dst_stride, (width + 1) / 2 * channels, (height + 1) / 2, &a.dst));
+  KCV_TRY(from_hip(kcv::launch_blur_down(a, s)));
+  return ss.finish();
+}
+
 kleidicv_error_t separable_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,
                                 size_t channels, const uint8_t *kernel_x,
@@ -385,6 +403,15 @@ kleidicv_error_t kleidicv_separable_filter_2d_u8(
                         nullptr, true);
 }
 
+kleidicv_error_t kleidicv_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,
+                                                 size_t src_width, size_t src_height,
+                                                 uint8_t *dst, size_t dst_stride,
+                                                 size_t channels,
+                                                 kleidicv_border_type_t border_type) {
+  return blur_down_impl(src, src_stride, src_width, src_height, dst, dst_stride, channels,
+                        border_type, nullptr, true);
+}
+
 kleidicv_error_t kleidicv_median_blur_u8(const uint8_t *src, size_t src_stride,
                                          uint8_t *dst, size_t dst_stride, size_t width,
                                          size_t height, size_t channels,
@@ -435,6 +462,15 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
   return kleidicv_separable_filter_2d_u8(src, src_stride, dst, dst_stride, width,
                                          height, channels, kernel_x, kernel_width,
                                          kernel_y, kernel_height, border_type);
+}
+
+kleidicv_error_t kleidicv_thread_blur_and_downsample_u8(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height, uint8_t *dst,
+    size_t dst_stride, size_t channels, kleidicv_border_type_t border_type,
+    kleidicv_thread_multithreading mt) {
+  KCV_TRY(check_mt(mt));
+  return blur_down_impl(src, src_stride, src_width, src_height, dst, dst_stride, channels,
+                        border_type, nullptr, true);
 }
 
 kleidicv_error_t kleidicv_thread_median_blur_u8(
@@ -526,6 +562,16 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
   return separable_impl(src, src_stride, dst, dst_stride, width, height, channels,
                         kernel_x, kernel_width, kernel_y, kernel_height, border_type,
                         stream, false);
+}
+
+kleidicv_error_t kleidicv_hip_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,
+                                                     size_t src_width, size_t src_height,
+                                                     uint8_t *dst, size_t dst_stride,
+                                                     size_t channels,
+                                                     kleidicv_border_type_t border_type,
+                                                     void *stream) {
+  return blur_down_impl(src, src_stride, src_width, src_height, dst, dst_stride, channels,
+                        border_type, stream, false);
 }
 
 kleidicv_error_t kleidicv_hip_median_blur_u8(

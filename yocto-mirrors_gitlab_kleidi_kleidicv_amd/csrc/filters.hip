@@ -47,6 +47,14 @@ __global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a, int tile
   median_tile_c<K, C>(a, bx, by, smem);
 }
 
+template <int C>
+__global__ __launch_bounds__(kThreads) void blur_down_kernel(SepArgs a, int tiles_x) {
+  __shared__ __align__(16) uint8_t smem[blur_down_smem_bytes()];
+  int bx, by;
+  tile_of(tiles_x, bx, by);
+  blur_down_tile<C>(a, bx, by, smem);
+}
+
 // Grid and tiles_x argument for an image: a 1-D XCD-ordered grid unless the
 // tile count is very large.
 struct TileGrid {
@@ -103,6 +111,28 @@ hipError_t launch_separable(const SepArgs &a, hipStream_t stream) {
     case kSepSigma: return launch_mode<kSepSigma>(a, stream);
     default: return launch_mode<kSepSaturate>(a, stream);
   }
+}
+
+hipError_t launch_blur_down(const SepArgs &a, hipStream_t stream) {
+  if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
+  const int out_w = (a.width + 1) / 2, out_h = (a.height + 1) / 2;
+  const int per_tile = a.channels == 3 ? Down<3>::kOutW : Down<4>::kOutW;
+  const int tx = (out_w * a.channels + per_tile - 1) / per_tile;
+  const int ty = (out_h + kDownRows - 1) / kDownRows;
+  dim3 grid(tx, ty);
+  int tiles_x = 0;
+  if (int64_t{tx} * ty <= kMaxLinearTiles) {
+    grid = dim3(tx * ty);
+    tiles_x = tx;
+  }
+  switch (a.channels) {
+    case 1: blur_down_kernel<1><<<grid, kThreads, 0, stream>>>(a, tiles_x); break;
+    case 2: blur_down_kernel<2><<<grid, kThreads, 0, stream>>>(a, tiles_x); break;
+    case 3: blur_down_kernel<3><<<grid, kThreads, 0, stream>>>(a, tiles_x); break;
+    case 4: blur_down_kernel<4><<<grid, kThreads, 0, stream>>>(a, tiles_x); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {

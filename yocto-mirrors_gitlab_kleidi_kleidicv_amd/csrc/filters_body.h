@@ -301,6 +301,118 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
 }
 
 // ---------------------------------------------------------------------------
+// Blur and downsample (pyrDown): the binomial 5x5 Gaussian at even source
+// rows and columns, output ((w+1)/2) x ((h+1)/2).
+//
+// An output tile is kDownRows rows x 32 * OB bytes, OB = 4 output bytes per
+// thread (3 when C == 3), so OB * thread and the tile start are multiples of
+// C and every output byte's channel is a compile-time constant.  The source
+// window is the radius-2 window of the separable filter over source rows
+// [2 y0' - 2, 2 y0' + 18) and source bytes from twice the tile's first output
+// byte, staged once; a vertical pass writes [1,4,6,4,1] column sums (u16,
+// at most 4080) for the even source rows only, and each thread then reads
+// the column sums of its outputs with aligned LDS loads.  One rounding:
+// (sum + 128) >> 8, as src/filters/blur_and_downsample_neon.cpp:193-225.
+constexpr int kDownRows = 8;
+
+template <int C>
+struct Down {
+  static constexpr int kOB = C == 3 ? 3 : 4;
+  static constexpr int kOutW = 32 * kOB;  // output bytes per tile row
+};
+
+constexpr int blur_down_smem_bytes() {
+  return separable_win_bytes<5>() + 2 * kDownRows * Window<2>::kCols;
+}
+
+template <int C>
+__device__ __forceinline__ void blur_down_tile(const SepArgs &a, int bx, int by,
+                                               uint8_t *smem) {
+  using G = Window<2>;
+  constexpr int OB = Down<C>::kOB;
+  uint8_t *win = smem;
+  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<5>());
+  const int o0 = bx * Down<C>::kOutW;  // first output byte of the tile
+  const int y0 = 2 * kDownRows * by;   // first source row of the tile
+  __syncthreads();  // LDS may still hold the previous tile of a batched loop
+  stage_window<2>(win, a.src, a.src_stride, a.width, a.height, C, a.border, 2 * o0, y0);
+  __syncthreads();
+
+  // vertical pass: output row r uses window rows 2r .. 2r + 4
+  constexpr int kQuads = G::kCols / 4;
+  const uint32_t *win32 = reinterpret_cast<const uint32_t *>(win);
+  for (int i = threadIdx.x; i < kDownRows * kQuads; i += kThreads) {
+    const int r = i / kQuads, q = i - r * kQuads;
+    uint32_t v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = win32[(2 * r + k) * kQuads + q];
+    auto lanes = [&](int shift) {
+      const uint32_t m = 0x00FF00FFu;
+      const sep_us2 x0 = __builtin_bit_cast(sep_us2, (v[0] >> shift) & m);
+      const sep_us2 x1 = __builtin_bit_cast(sep_us2, (v[1] >> shift) & m);
+      const sep_us2 x2 = __builtin_bit_cast(sep_us2, (v[2] >> shift) & m);
+      const sep_us2 x3 = __builtin_bit_cast(sep_us2, (v[3] >> shift) & m);
+      const sep_us2 x4 = __builtin_bit_cast(sep_us2, (v[4] >> shift) & m);
+      return (x0 + x4) + ((x1 + x3) << 2) + x2 * sep_us2{6, 6};
+    };
+    const uint32_t ev = __builtin_bit_cast(uint32_t, lanes(0));
+    const uint32_t od = __builtin_bit_cast(uint32_t, lanes(8));
+    *reinterpret_cast<uint2 *>(vert + r * G::kCols + 4 * q) =
+        make_uint2((ev & 0xFFFFu) | (od << 16), (ev >> 16) | (od & 0xFFFF0000u));
+  }
+  __syncthreads();
+
+  // horizontal pass: thread tx of row r makes output bytes o0 + OB tx + b.
+  // Output byte b with channel c = b % C takes the column sums at window
+  // u16 index kPad + 2 (OB tx + b) - c + (j - 2) C, j = 0..4; relative to
+  // the thread's first loaded index (8 tx for OB = 4, 6 tx + 8 for OB = 3)
+  // that is a compile-time offset.
+  const int tx = threadIdx.x & 31, r = threadIdx.x >> 5;
+  const int yo = kDownRows * by + r;
+  const int out_rows = (a.height + 1) / 2;
+  const int out_bytes = ((a.width + 1) / 2) * C;
+  const int eo = o0 + OB * tx;
+  if (yo >= out_rows || eo >= out_bytes) return;
+  const uint16_t *row = vert + r * G::kCols;
+  constexpr int kWords = OB == 4 ? 16 : 9;
+  constexpr int kBase = OB == 4 ? 0 : 8;  // first loaded u16 index - 6 tx
+  uint32_t w[kWords];
+  if constexpr (OB == 4) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(row + 8 * tx);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint4 q = p[m];
+      w[4 * m] = q.x;
+      w[4 * m + 1] = q.y;
+      w[4 * m + 2] = q.z;
+      w[4 * m + 3] = q.w;
+    }
+  } else {
+    const uint32_t *p = reinterpret_cast<const uint32_t *>(row + 6 * tx + kBase);
+#pragma unroll
+    for (int m = 0; m < kWords; ++m) w[m] = p[m];
+  }
+  auto col = [&](int li) -> uint32_t { return (w[li >> 1] >> (16 * (li & 1))) & 0xFFFFu; };
+  uint32_t packed = 0;
+#pragma unroll
+  for (int b = 0; b < OB; ++b) {
+    const int c = b % C;
+    const int li = G::kPad + 2 * b - c - 2 * C - kBase;
+    const uint32_t acc = col(li) + col(li + 4 * C) + ((col(li + C) + col(li + 3 * C)) << 2) +
+                         6 * col(li + 2 * C);
+    packed |= ((acc + 128u) >> 8) << (8 * b);
+  }
+  uint8_t *out = a.dst + static_cast<size_t>(yo) * a.dst_stride + eo;
+  if (OB == 4 && eo + 4 <= out_bytes && (reinterpret_cast<uintptr_t>(out) & 3) == 0) {
+    store_nt(reinterpret_cast<uint32_t *>(out), packed);
+  } else {
+#pragma unroll
+    for (int b = 0; b < OB; ++b)
+      if (eo + b < out_bytes) out[b] = static_cast<uint8_t>(packed >> (8 * b));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Median: exact selection on pairs of horizontally adjacent outputs packed
 // in 16-bit halves (v_pk_min_u16 / v_pk_max_u16).  Any exact selection
 // returns the reference sorting network's value.

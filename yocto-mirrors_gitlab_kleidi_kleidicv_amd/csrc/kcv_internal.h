@@ -73,6 +73,9 @@ struct SepArgs {
 };
 
 hipError_t launch_separable(const SepArgs &a, hipStream_t stream);
+// Blur and downsample: SepArgs with the source size in width/height; the
+// destination is ((width+1)/2) x ((height+1)/2).
+hipError_t launch_blur_down(const SepArgs &a, hipStream_t stream);
 
 struct MedianArgs {
   const uint8_t *src;

@@ -199,6 +199,38 @@ kleidicv_error_t prepare_gaussian(const uint8_t *src, size_t src_stride, uint8_t
 }
 
 // ---------------------------------------------------------------------------
+// Blur and downsample.  src/filters/blur_and_downsample_api.cpp:18-30
+// (size and channels, then the fixed border types), then
+// src/filters/blur_and_downsample_neon.cpp:358-360 (pointers with the
+// source and destination row counts, then the image size).
+kleidicv_error_t prepare_blur_down(const uint8_t *src, size_t src_stride, size_t width,
+                                   size_t height, uint8_t *dst, size_t dst_stride,
+                                   size_t channels, kleidicv_border_type_t border_type,
+                                   SepArgs *a, Action *act) {
+  *act = Action::kNone;
+  if (width < 4 || height < 4 || channels < 1 || channels > KLEIDICV_MAXIMUM_CHANNEL_COUNT)
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (!is_filter_border(border_type)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, (height + 1) / 2));
+  KCV_TRY(check_image_size(width, height));
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  *a = SepArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->channels = static_cast<int>(channels);
+  a->border = static_cast<int>(border_type);
+  a->ksize = 5;
+  a->mode = kSepBinomial;
+  a->shift = 8;
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
 // Separable filter.  src/filters/separable_filter_2d_api.cpp:12-31,
 // src/filters/separable_filter_2d_neon.cpp:223-236.
 kleidicv_error_t prepare_separable(const uint8_t *src, size_t src_stride, uint8_t *dst,

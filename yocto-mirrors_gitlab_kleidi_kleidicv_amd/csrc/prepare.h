@@ -53,6 +53,11 @@ kleidicv_error_t prepare_separable(const uint8_t *src, size_t src_stride, uint8_
                                    size_t kernel_height, kleidicv_border_type_t border_type,
                                    SepArgs *a, Action *act);
 
+kleidicv_error_t prepare_blur_down(const uint8_t *src, size_t src_stride, size_t width,
+                                   size_t height, uint8_t *dst, size_t dst_stride,
+                                   size_t channels, kleidicv_border_type_t border_type,
+                                   SepArgs *a, Action *act);
+
 kleidicv_error_t prepare_median(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,
                                 size_t channels, size_t kernel_width, size_t kernel_height,

@@ -151,6 +151,22 @@ def separable_filter_2d(src, kernel_x, kernel_y, border="REPLICATE", dst=None,
     return dst
 
 
+def blur_and_downsample(src, border="REVERSE", dst=None, stream=None, device_api=False):
+    """cv::pyrDown: binomial 5x5 at even rows and columns, ((w+1)/2, (h+1)/2)."""
+    h, w, c = _shape(src)
+    shape = ((h + 1) // 2, (w + 1) // 2) + ((c,) if len(src.shape) == 3 else ())
+    dst = _empty_like_shape(src, shape) if dst is None else dst
+    args = [_ptr(src), *_sz(_stride(src), w, h), _ptr(dst), *_sz(_stride(dst), c),
+            C.c_int(BORDER[border])]
+    if device_api:
+        _check(lib().kleidicv_hip_blur_and_downsample_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_blur_and_downsample_u8")
+    else:
+        _check(lib().kleidicv_blur_and_downsample_u8(*args),
+               "kleidicv_blur_and_downsample_u8")
+    return dst
+
+
 def median_blur(src, kernel_size, border="REPLICATE", dst=None, stream=None,
                 device_api=False):
     h, w, c = _shape(src)
