@@ -86,6 +86,10 @@ def make_inputs(torch, gen, dev):
     iy = torch.floor((yy + fy) * 32).to(torch.int64)
     s["mapxy"] = torch.stack([ix >> 5, iy >> 5], dim=-1).to(torch.int16).contiguous()
     s["mapfrac"] = ((ix & 31) | ((iy & 31) << 5)).to(torch.int16).contiguous()
+    # inputs of the extended (non-step) ops, drawn last
+    s["i420"] = u8((3 * H) // 2, W)
+    s["yuyv"] = u8(H, 2 * W)
+    s["yuv444"] = u8(H, 3 * W)
     return s
 
 
@@ -136,7 +140,14 @@ def extended_ops(K, s, out, stream):
     def bd(src, name):
         return lambda: K.blur_and_downsample(src, "REVERSE", dst=out[name], stream=stream,
                                              device_api=True)
+    def yuv(src, name, fmt):
+        return lambda: K.yuv_to_rgb(src, W, H, fmt, dst=out[name], stream=stream,
+                                    device_api=True)
     return [
+        ("i420_to_rgb", W * H, (W * H * 3) // 2 + 3 * W * H, yuv(s["i420"], "i420_to_rgb", 0x03)),
+        ("yuyv_to_rgb", W * H, 2 * W * H + 3 * W * H, yuv(s["yuyv"], "yuyv_to_rgb", 0x04)),
+        ("yuv444_to_rgba", W * H, 3 * W * H + 4 * W * H,
+         yuv(s["yuv444"], "yuv444_to_rgba", 0x41)),
         ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
          bd(s["sep_src"], "blur_and_downsample_1ch")),
         ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
@@ -146,6 +157,9 @@ def extended_ops(K, s, out, stream):
 
 def extended_outputs(torch, dev):
     return {
+        "i420_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
+        "yuyv_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
+        "yuv444_to_rgba": torch.empty(H, W, 4, dtype=torch.uint8, device=dev),
         "blur_and_downsample_1ch": torch.empty(H // 2, W // 2, dtype=torch.uint8, device=dev),
         "blur_and_downsample_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
     }

@@ -55,8 +55,7 @@ typedef enum {
   KLEIDICV_INTERPOLATION_LINEAR,
 } kleidicv_interpolation_type_t;
 
-/* ctypes.h:149-360: base format in the low nibble, modifier flags above.
- * Only the semi-planar (NV12/NV21) conversions are on this path. */
+/* ctypes.h:149-360: base format in the low nibble, modifier flags above. */
 typedef enum {
   KLEIDICV_COLOR_CONVERSION_FMT_YUV444 = 0x01,
   KLEIDICV_COLOR_CONVERSION_FMT_YUV420SP = 0x02,
@@ -75,6 +74,30 @@ typedef enum {
   KLEIDICV_NV21_TO_BGR = 0x32,
   KLEIDICV_NV21_TO_RGBA = 0x52,
   KLEIDICV_NV21_TO_BGRA = 0x72,
+  KLEIDICV_YUV444_TO_RGB = 0x01,
+  KLEIDICV_YUV444_TO_BGR = 0x21,
+  KLEIDICV_YUV444_TO_RGBA = 0x41,
+  KLEIDICV_YUV444_TO_BGRA = 0x61,
+  KLEIDICV_IYUV_TO_RGB = 0x03,
+  KLEIDICV_IYUV_TO_BGR = 0x23,
+  KLEIDICV_IYUV_TO_RGBA = 0x43,
+  KLEIDICV_IYUV_TO_BGRA = 0x63,
+  KLEIDICV_YV12_TO_RGB = 0x13,
+  KLEIDICV_YV12_TO_BGR = 0x33,
+  KLEIDICV_YV12_TO_RGBA = 0x53,
+  KLEIDICV_YV12_TO_BGRA = 0x73,
+  KLEIDICV_YUYV_TO_RGB = 0x04,
+  KLEIDICV_YUYV_TO_BGR = 0x24,
+  KLEIDICV_YUYV_TO_RGBA = 0x44,
+  KLEIDICV_YUYV_TO_BGRA = 0x64,
+  KLEIDICV_YVYU_TO_RGB = 0x14,
+  KLEIDICV_YVYU_TO_BGR = 0x34,
+  KLEIDICV_YVYU_TO_RGBA = 0x54,
+  KLEIDICV_YVYU_TO_BGRA = 0x74,
+  KLEIDICV_UYVY_TO_RGB = 0x84,
+  KLEIDICV_UYVY_TO_BGR = 0xA4,
+  KLEIDICV_UYVY_TO_RGBA = 0xC4,
+  KLEIDICV_UYVY_TO_BGRA = 0xE4,
 } kleidicv_color_conversion_t;
 
 /* kleidicv.h:103-113 */
@@ -103,6 +126,16 @@ kleidicv_error_t kleidicv_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type);
+
+/* Replaces kleidicv.h:1579-1581 (src/conversions/yuv_to_rgb_api.cpp:
+ * 22-66): one-buffer YUV to RGB/BGR/RGBA/BGRA.  Planar 4:2:0 (I420/IYUV,
+ * YV12: luma plane then two chroma planes in OpenCV's packed layout, (3h+1)/2
+ * rows of src_stride), interleaved 4:2:2 (YUYV, UYVY, YVYU; even width) and
+ * interleaved 4:4:4 (full-range BT.601). */
+kleidicv_error_t kleidicv_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,
+                                        uint8_t *dst, size_t dst_stride,
+                                        size_t width, size_t height,
+                                        kleidicv_color_conversion_t color_format);
 
 /* Replaces kleidicv.h:2215-2218 (src/filters/blur_and_downsample_api.cpp:
  * 37-45): the binomial 5x5 Gaussian at even rows and columns (cv::pyrDown),
@@ -189,6 +222,11 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:208-211 */
+kleidicv_error_t kleidicv_thread_yuv_to_rgb_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    kleidicv_thread_multithreading mt);
 /* kleidicv_thread.h:482-485 */
 kleidicv_error_t kleidicv_thread_blur_and_downsample_u8(
     const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
@@ -253,6 +291,11 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_yuv_to_rgb_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    void *stream);
 
 kleidicv_error_t kleidicv_hip_blur_and_downsample_u8(
     const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,

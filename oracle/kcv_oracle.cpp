@@ -1073,3 +1073,98 @@ extern "C" int orc_yuv_semiplanar_to_rgb_u8(const uint8_t *src_y,
     }
   return kErrOk;
 }
+
+// ---------------------------------------------------------------------------
+// One-buffer YUV to RGB (src/conversions/yuv_to_rgb_api.cpp:22-66).
+//  - 4:2:0 planar (I420 / YV12): the chroma addressing of
+//    test_yuv420p_to_rgb.cpp:161-210 (and yuv420p_to_rgb_sc.h:46-110): the
+//    first chroma plane at height rows, the second at height + height/4 rows
+//    plus width/2 bytes when height % 4 >= 2, each chroma row advancing by
+//    width/2 and stride - width/2 in turn (the second plane taking the
+//    longer step first when height % 4 == 2); pixel x reads element x >> 1.
+//  - 4:2:2 interleaved (YUYV / UYVY / YVYU), even width:
+//    test_yuv422_to_rgb.cpp:145-190 with yuv422_to_rgb_sc.h:398-470.
+//  - 4:4:4 interleaved: full-range 14-bit constants of
+//    yuv444_coefficients.h, test_yuv444_to_rgb.cpp:101-125.
+// 4:2:0 and 4:2:2 use the BT.601 limited-range arithmetic of NV12 above.
+// Parity: 4:2:0 and 4:2:2 are pinned in tests/ by equivalence with the
+// NV12 oracle (itself pinned by the reference's fixed inputs) on the same
+// chroma, 4:4:4 by the reference's fixed inputs (:43-48) and by the vector
+// form of yuv444_to_rgb_sc.h (pre-shift by 2, top 16 bits of 32), written
+// independently in tests/.
+namespace {
+void put_rgb(uint8_t *o, int r, int g, int b, bool bgr, bool alpha) {
+  if (bgr) std::swap(r, b);
+  o[0] = static_cast<uint8_t>(std::clamp(r, 0, 255));
+  o[1] = static_cast<uint8_t>(std::clamp(g, 0, 255));
+  o[2] = static_cast<uint8_t>(std::clamp(b, 0, 255));
+  if (alpha) o[3] = 0xFF;
+}
+
+void bt601_limited(int Y, int u, int v, int *r, int *g, int *b) {
+  const int yw = 1220542 * std::max(Y - 16, 0);
+  *r = (yw + 1673527 * v + (1 << 19)) >> 20;
+  *g = (yw - 409993 * u - 852492 * v + (1 << 19)) >> 20;
+  *b = (yw + 2116026 * u + (1 << 19)) >> 20;
+}
+}  // namespace
+
+extern "C" int orc_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                 size_t dst_stride, size_t width, size_t height, int fmt) {
+  const int base = fmt & 0x0F, flags = fmt & ~0x0F;
+  const bool bgr = fmt & 0x20, alpha = fmt & 0x40;
+  const size_t oc = alpha ? 4 : 3;
+  if (base == 0x01) {
+    if (flags & ~0x60) return kErrNotImplemented;
+    if (!src || !dst) return kErrNull;
+    for (size_t y = 0; y < height; ++y)
+      for (size_t x = 0; x < width; ++x) {
+        const uint8_t *p = src + y * src_stride + 3 * x;
+        const int Y = p[0], u = p[1] - 128, v = p[2] - 128;
+        put_rgb(dst + y * dst_stride + x * oc, Y + ((18678 * v + (1 << 13)) >> 14),
+                Y + ((-6472 * u - 9519 * v + (1 << 13)) >> 14),
+                Y + ((33292 * u + (1 << 13)) >> 14), bgr, alpha);
+      }
+    return kErrOk;
+  }
+  if (base == 0x03) {
+    if (flags & ~0x70) return kErrNotImplemented;
+    if (!src || !dst) return kErrNull;
+    const bool yv12 = fmt & 0x10;
+    const uint8_t *p1 = src + src_stride * height;
+    const uint8_t *p2 = src + src_stride * (height + height / 4) + (width / 2) * ((height % 4) / 2);
+    const size_t steps[2] = {width / 2, src_stride - width / 2};
+    size_t i1 = 0, i2 = height % 4 == 2 ? 1 : 0;
+    for (size_t y = 0; y < height; ++y) {
+      for (size_t x = 0; x < width; ++x) {
+        int u = p1[x >> 1] - 128, v = p2[x >> 1] - 128;
+        if (yv12) std::swap(u, v);
+        int r, g, b;
+        bt601_limited(src[y * src_stride + x], u, v, &r, &g, &b);
+        put_rgb(dst + y * dst_stride + x * oc, r, g, b, bgr, alpha);
+      }
+      if (y % 2 == 1) {
+        p1 += steps[(i1++) & 1];
+        p2 += steps[(i2++) & 1];
+      }
+    }
+    return kErrOk;
+  }
+  // everything else is routed to 4:2:2, which checks pointers, then the
+  // width, then the format
+  if (!src || !dst) return kErrNull;
+  if (width < 2 || width % 2) return kErrNotImplemented;
+  const bool vu = fmt & 0x10, cf = fmt & 0x80;
+  if (base != 0x04 || (vu && cf)) return kErrNotImplemented;
+  const int yo = cf ? 1 : 0, uo = cf ? 0 : (vu ? 3 : 1), vo = cf ? 2 : (vu ? 1 : 3);
+  for (size_t y = 0; y < height; ++y)
+    for (size_t x = 0; x < width; x += 2) {
+      const uint8_t *p = src + y * src_stride + 2 * x;
+      for (int k = 0; k < 2; ++k) {
+        int r, g, b;
+        bt601_limited(p[yo + 2 * k], p[uo] - 128, p[vo] - 128, &r, &g, &b);
+        put_rgb(dst + y * dst_stride + (x + k) * oc, r, g, b, bgr, alpha);
+      }
+    }
+  return kErrOk;
+}

@@ -111,6 +111,11 @@ int orc_yuv_semiplanar_to_rgb_u8(const uint8_t *src_y, size_t src_y_stride,
                                  uint8_t *dst, size_t dst_stride, size_t width,
                                  size_t height, int color_format);
 
+/* One-buffer YUV to RGB: planar 4:2:0 (I420/YV12), interleaved 4:2:2
+ * (YUYV/UYVY/YVYU) and 4:4:4, with the reference's routing and checks. */
+int orc_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                      size_t dst_stride, size_t width, size_t height, int fmt);
+
 /* Stripe variants: destination rows [y_begin, y_end) only, the rest of the
  * image is read as in the full call.  The CPU baseline runs these on
  * std::thread-style row stripes (kleidicv_thread.cpp:60-75). */

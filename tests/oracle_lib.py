@@ -75,6 +75,16 @@ def blur_and_downsample(src, border="REVERSE", y_range=None):
     return err, (dst if src.ndim == 3 else dst[:, :, 0])
 
 
+def yuv_to_rgb(src, width, height, fmt):
+    """src: 2-D uint8 buffer whose rows are the source rows (any padding)."""
+    s = np.ascontiguousarray(src, dtype=np.uint8)
+    oc = 4 if fmt & 0x40 else 3
+    dst = np.zeros((height, width, oc), np.uint8)
+    err = lib().orc_yuv_to_rgb_u8(_p(s), *_sz(s.strides[0]), _p(dst),
+                                  *_sz(dst.strides[0], width, height), C.c_int(fmt))
+    return err, dst
+
+
 def separable_filter_2d(src, kx, ky, border="REPLICATE"):
     s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
     h, w, c = s.shape

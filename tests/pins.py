@@ -214,3 +214,45 @@ def resize_generator_direct(C, sw, dw, R, up):
             frac[di] = (sx & 0xFFFF) >> 8
             di += 1
     return np.array(left), np.array(right), np.array(frac)
+
+
+def i420_buffer(yplane, uplane, vplane, stride, yv12=False):
+    """OpenCV's packed I420/YV12 buffer for an even height h: the luma plane,
+    then the chroma area, a sequence of h chroma rows of w/2 bytes stored two
+    to a source row (slot j at (j // 2) * stride + (j % 2) * w/2): the first
+    plane's h/2 rows, then the second plane's."""
+    h, w = yplane.shape
+    assert h % 2 == 0 and w % 2 == 0
+    buf = np.zeros(((3 * h + 1) // 2, stride), np.uint8)
+    buf[:h, :w] = yplane
+    chroma = buf[h:].reshape(-1)
+    half = w // 2
+    first, second = (vplane, uplane) if yv12 else (uplane, vplane)
+    for j, row in enumerate(list(first) + list(second)):
+        pos = (j // 2) * stride + (j % 2) * half
+        chroma[pos:pos + half] = row
+    return buf
+
+
+def yuv444_vector_form(src3, bgr=False, alpha=False):
+    """yuv444_to_rgb_sc.h's vector arithmetic: U and V pre-shifted by 2 as
+    u16, multiplied into 32 bits with the 14-bit weights and the 4x deltas,
+    top 16 bits kept, added to Y, saturated."""
+    Y = src3[..., 0].astype(np.int64)
+    U4 = src3[..., 1].astype(np.int64) << 2
+    V4 = src3[..., 2].astype(np.int64) << 2
+    b_delta = (-128 * 33292 + (1 << 13)) * 4
+    g_delta = (-128 * -6472 - 128 * -9519 + (1 << 13)) * 4
+    r_delta = (-128 * 18678 + (1 << 13)) * 4
+
+    def top16(x):
+        x = x.astype(np.int64) & 0xFFFFFFFF
+        x = np.where(x >= 1 << 31, x - (1 << 32), x)
+        return x >> 16
+    b = Y + top16(U4 * 33292 + b_delta)
+    g = Y + top16(U4 * -6472 + V4 * -9519 + g_delta)
+    r = Y + top16(V4 * 18678 + r_delta)
+    if bgr:
+        r, b = b, r
+    ch = [r, g, b] + ([np.full_like(Y, 255)] if alpha else [])
+    return np.clip(np.stack(ch, axis=-1), 0, 255).astype(np.uint8)

@@ -117,6 +117,34 @@ def test_blur_and_downsample_error_codes():
     assert bd(p, 1 << 48, 1 << 48, p + 1, 1, 2) == K.RANGE
 
 
+def test_yuv_to_rgb_error_codes():
+    # test_yuv420p_to_rgb.cpp:120-153 and the 4:2:2 / 4:4:4 equivalents
+    lib = K.lib()
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+
+    def yuv(src, dst, w, h, fmt):
+        return lib.kleidicv_yuv_to_rgb_u8(C.c_void_p(src), C.c_size_t(64), C.c_void_p(dst),
+                                          C.c_size_t(256), C.c_size_t(w), C.c_size_t(h),
+                                          C.c_int(fmt))
+    for fmt in (0x03, 0x73, 0x04, 0x84, 0x14, 0x01, 0x61):
+        assert yuv(0, p, 20, 10, fmt) == K.NULL_POINTER
+        assert yuv(p, 0, 20, 10, fmt) == K.NULL_POINTER
+        assert yuv(p, p, (1 << 48), 1, fmt) == K.RANGE
+        assert yuv(p, p, 1 << 48, 1 << 48, fmt) == K.RANGE
+    for fmt in (0x03, 0x01):
+        assert yuv(p, p, 0, 1, fmt) == K.OK
+        assert yuv(p, p, 1, 0, fmt) == K.OK
+    assert yuv(p, p, 20, 10, 0x00) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 20, 10, 0x83) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 20, 10, 0x11) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 20, 10, 0x94) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 20, 10, 0x02) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 19, 10, 0x04) == K.NOT_IMPLEMENTED
+    assert yuv(p, p, 0, 10, 0x04) == K.NOT_IMPLEMENTED
+    assert yuv(0, p, 20, 10, 0x83) == K.NOT_IMPLEMENTED  # format before pointers
+
+
 def test_resize_error_codes():
     lib = K.lib()
     buf = np.zeros(4096, np.uint8)

@@ -309,3 +309,73 @@ def test_nv12_gray_axis_exact(rng):
     g = np.clip(g, 0, 255)
     for c in range(3):
         np.testing.assert_array_equal(out[..., c], g)
+
+
+# ---------------------------------------------------------------------------
+# One-buffer YUV layouts
+def _nv12_ref(yplane, uplane, vplane, fmt_flags):
+    h, w = yplane.shape
+    uv = np.stack([uplane, vplane], axis=-1).reshape(uplane.shape[0], -1)
+    err, out = O.yuv_sp_to_rgb(yplane, uv, 0x02 | fmt_flags)
+    assert err == 0
+    return out
+
+
+@pytest.mark.parametrize("h,w", [(2, 2), (4, 6), (6, 18), (16, 34), (10, 8)])
+@pytest.mark.parametrize("flags", [0x00, 0x20, 0x40, 0x60])
+def test_i420_yv12_equal_nv12_on_the_same_planes(rng, h, w, flags):
+    for yv12 in (False, True):
+        yplane = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        up = rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8)
+        vp = rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8)
+        buf = pins.i420_buffer(yplane, up, vp, w + 4, yv12)
+        err, got = O.yuv_to_rgb(buf, w, h, 0x03 | flags | (0x10 if yv12 else 0))
+        assert err == 0
+        np.testing.assert_array_equal(got, _nv12_ref(yplane, up, vp, flags))
+
+
+@pytest.mark.parametrize("order", ["YUYV", "UYVY", "YVYU"])
+@pytest.mark.parametrize("flags", [0x00, 0x20, 0x40, 0x60])
+def test_yuv422_equals_nv12_with_chroma_on_both_rows(rng, order, flags):
+    h, w = 6, 20
+    yplane = rng.integers(0, 256, (h, w), dtype=np.uint8)
+    up = rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8)
+    vp = rng.integers(0, 256, (h // 2, w // 2), dtype=np.uint8)
+    src = np.zeros((h, 2 * w + 6), np.uint8)
+    for y in range(h):
+        for p in range(w // 2):
+            y0, y1 = yplane[y, 2 * p], yplane[y, 2 * p + 1]
+            u, v = up[y // 2, p], vp[y // 2, p]
+            src[y, 4 * p:4 * p + 4] = {"YUYV": (y0, u, y1, v), "UYVY": (u, y0, v, y1),
+                                       "YVYU": (y0, v, y1, u)}[order]
+    base = {"YUYV": 0x04, "UYVY": 0x84, "YVYU": 0x14}[order]
+    err, got = O.yuv_to_rgb(src, w, h, base | flags)
+    assert err == 0
+    np.testing.assert_array_equal(got, _nv12_ref(yplane, up, vp, flags))
+
+
+@pytest.mark.parametrize("flags", [0x00, 0x20, 0x40, 0x60])
+def test_yuv444_vector_form(rng, flags):
+    h, w = 7, 33
+    src = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    # the reference's fixed rows (test_yuv444_to_rgb.cpp:45-48) in front
+    fixed = np.array([[10, 100, 130, 20, 100, 130, 255, 255, 255, 199, 255, 255],
+                      [1, 100, 130, 120, 100, 130, 0, 255, 255, 17, 255, 255],
+                      [2, 0, 1, 3, 0, 1, 240, 3, 4, 228, 3, 4],
+                      [7, 7, 8, 11, 7, 8, 128, 9, 10, 129, 9, 10]], np.uint8)
+    src[:4, :4] = fixed.reshape(4, 4, 3)
+    err, got = O.yuv_to_rgb(src.reshape(h, -1), w, h, 0x01 | flags)
+    assert err == 0
+    np.testing.assert_array_equal(got, pins.yuv444_vector_form(src, bool(flags & 0x20),
+                                                               bool(flags & 0x40)))
+
+
+def test_yuv_routing_errors():
+    buf = np.zeros((16, 64), np.uint8)
+    assert O.yuv_to_rgb(buf, 10, 10, 0x83)[0] == 1     # 4:2:0 with chroma-first
+    assert O.yuv_to_rgb(buf, 10, 10, 0x11)[0] == 1     # 4:4:4 with VU
+    assert O.yuv_to_rgb(buf, 9, 4, 0x04)[0] == 1       # 4:2:2 odd width
+    assert O.yuv_to_rgb(buf, 10, 4, 0x94)[0] == 1      # VU and chroma-first
+    assert O.yuv_to_rgb(buf, 10, 4, 0x02)[0] == 1      # NV12 is not one-buffer
+    assert O.yuv_to_rgb(buf, 10, 4, 0x00)[0] == 1
+    assert O.yuv_to_rgb(buf, 0, 1, 0x03)[0] == 0

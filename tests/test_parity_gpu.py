@@ -413,6 +413,64 @@ def test_nv12(rng, fmt):
 
 # ---------------------------------------------------------------------------
 # Device-resident API on a torch stream gives the same bytes as the host API.
+# ---------------------------------------------------------------------------
+# One-buffer YUV layouts (test_yuv420p_to_rgb.cpp:58-69, test_yuv422_to_rgb.cpp,
+# test_yuv444_to_rgb.cpp sizes, plus widths across several thread blocks)
+FMT_420P = [0x03, 0x23, 0x43, 0x63, 0x13, 0x33, 0x53, 0x73]
+FMT_422 = [b | f for b in (0x04, 0x84, 0x14) for f in (0x00, 0x20, 0x40, 0x60)]
+FMT_444 = [0x01, 0x21, 0x41, 0x61]
+
+
+@pytest.mark.parametrize("fmt", FMT_420P)
+def test_yuv420p(rng, fmt):
+    for w in (2, 4, 6, 18, 33, 2049, 4100):
+        for h in (1, 2, 3, 5, 11, 16, 22):
+            src = rng.integers(0, 256, ((3 * h + 1) // 2, w + 4), dtype=np.uint8)
+            got = K.yuv_to_rgb(src, w, h, fmt)
+            _, want = O.yuv_to_rgb(src, w, h, fmt)
+            assert_same(got, want, "fmt %#x %dx%d" % (fmt, w, h))
+
+
+@pytest.mark.parametrize("fmt", FMT_422)
+def test_yuv422(rng, fmt):
+    for w in (2, 4, 18, 64, 2050, 4098):
+        for h in (1, 3, 10):
+            src = rng.integers(0, 256, (h, 2 * w + 6), dtype=np.uint8)
+            got = K.yuv_to_rgb(src, w, h, fmt)
+            _, want = O.yuv_to_rgb(src, w, h, fmt)
+            assert_same(got, want, "fmt %#x %dx%d" % (fmt, w, h))
+
+
+@pytest.mark.parametrize("fmt", FMT_444)
+def test_yuv444(rng, fmt):
+    for w in (1, 2, 5, 17, 2049, 4100):
+        for h in (1, 4, 9):
+            for pad in (0, 5):
+                src = rng.integers(0, 256, (h, 3 * w + pad), dtype=np.uint8)
+                got = K.yuv_to_rgb(src, w, h, fmt)
+                _, want = O.yuv_to_rgb(src, w, h, fmt)
+                assert_same(got, want, "fmt %#x %dx%d pad %d" % (fmt, w, h, pad))
+
+
+def test_yuv_layouts_device_api_and_padded_dst(rng):
+    import torch
+    for fmt, rows, row_bytes in ((0x63, lambda h: (3 * h + 1) // 2, lambda w: w),
+                                 (0x84, lambda h: h, lambda w: 2 * w),
+                                 (0x21, lambda h: h, lambda w: 3 * w)):
+        w, h = 1030, 26
+        oc = 4 if fmt & 0x40 else 3
+        src = rng.integers(0, 256, (rows(h), row_bytes(w) + 16), dtype=np.uint8)
+        _, want = O.yuv_to_rgb(src, w, h, fmt)
+        dbuf = np.full((h, w * oc + 9), 0x5A, np.uint8)
+        dst = dbuf[:, :w * oc].reshape(h, w, oc)
+        K.yuv_to_rgb(src, w, h, fmt, dst=dst)
+        assert_same(dst, want, "padded %#x" % fmt)
+        assert (dbuf[:, w * oc:] == 0x5A).all()
+        d = K.yuv_to_rgb(torch.from_numpy(src).cuda(), w, h, fmt, device_api=True)
+        torch.cuda.synchronize()
+        assert_same(d.cpu().numpy(), want, "device %#x" % fmt)
+
+
 def test_device_api_matches_host_api(rng):
     import torch
     dev = torch.device("cuda:0")

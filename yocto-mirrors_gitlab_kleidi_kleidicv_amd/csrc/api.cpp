@@ -162,6 +162,28 @@ kleidicv_error_t blur_down_impl(const uint8_t *src, size_t src_stride, size_t wi
   return ss.finish();
 }
 
+kleidicv_error_t yuv_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                          size_t dst_stride, size_t width, size_t height,
+                          kleidicv_color_conversion_t color_format, void *stream, bool sync) {
+  kcv::YuvXArgs a;
+  Action act;
+  KCV_TRY(kcv::prepare_yuv(src, src_stride, dst, dst_stride, width, height, color_format, &a,
+                           &act));
+  if (act == Action::kNone) return KLEIDICV_OK;
+  hipStream_t s = pick_stream(stream, sync);
+  if (!sync) return from_hip(kcv::launch_yuv_to_rgb(a, s));
+  Session ss(s);
+  const uint8_t *dsrc = nullptr;
+  KCV_TRY(ss.input(src, a.src_extent, &dsrc));
+  // the chroma plane pointers move with the luma plane
+  a.u = dsrc + (a.u - a.src);
+  a.v = dsrc + (a.v - a.src);
+  a.src = dsrc;
+  KCV_TRY(ss.output(dst, dst_stride, width * (a.alpha ? 4 : 3), height, &a.dst));
+  KCV_TRY(from_hip(kcv::launch_yuv_to_rgb(a, s)));
+  return ss.finish();
+}
+
 kleidicv_error_t separable_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,
                                 size_t channels, const uint8_t *kernel_x,
@@ -403,6 +425,12 @@ kleidicv_error_t kleidicv_separable_filter_2d_u8(
                         nullptr, true);
 }
 
+kleidicv_error_t kleidicv_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                        size_t dst_stride, size_t width, size_t height,
+                                        kleidicv_color_conversion_t color_format) {
+  return yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format, nullptr, true);
+}
+
 kleidicv_error_t kleidicv_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,
                                                  size_t src_width, size_t src_height,
                                                  uint8_t *dst, size_t dst_stride,
@@ -462,6 +490,15 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
   return kleidicv_separable_filter_2d_u8(src, src_stride, dst, dst_stride, width,
                                          height, channels, kernel_x, kernel_width,
                                          kernel_y, kernel_height, border_type);
+}
+
+kleidicv_error_t kleidicv_thread_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,
+                                               uint8_t *dst, size_t dst_stride, size_t width,
+                                               size_t height,
+                                               kleidicv_color_conversion_t color_format,
+                                               kleidicv_thread_multithreading mt) {
+  KCV_TRY(check_mt(mt));
+  return yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format, nullptr, true);
 }
 
 kleidicv_error_t kleidicv_thread_blur_and_downsample_u8(
@@ -562,6 +599,14 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
   return separable_impl(src, src_stride, dst, dst_stride, width, height, channels,
                         kernel_x, kernel_width, kernel_y, kernel_height, border_type,
                         stream, false);
+}
+
+kleidicv_error_t kleidicv_hip_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride, size_t width,
+                                            size_t height,
+                                            kleidicv_color_conversion_t color_format,
+                                            void *stream) {
+  return yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format, stream, false);
 }
 
 kleidicv_error_t kleidicv_hip_blur_and_downsample_u8(const uint8_t *src, size_t src_stride,

@@ -165,6 +165,25 @@ struct YuvArgs {
 
 hipError_t launch_yuv_sp_to_rgb(const YuvArgs &a, hipStream_t stream);
 
+// One-buffer YUV layouts: planar 4:2:0 (I420 / YV12), interleaved 4:2:2 and
+// interleaved 4:4:4.
+enum { kYuv420P = 0, kYuv422 = 1, kYuv444 = 2 };
+
+struct YuvXArgs {
+  const uint8_t *src;  // luma plane (4:2:0) or the interleaved image
+  size_t src_stride;
+  size_t src_extent;   // bytes from src that reads may touch
+  const uint8_t *u, *v;  // 4:2:0 chroma plane starts
+  int u_phase, v_phase;  // 4:2:0: 1 when a plane starts half-way along a row
+  uint8_t *dst;
+  size_t dst_stride;
+  int width, height;
+  int layout, bgr, alpha;
+  int y_off, u_off, v_off;  // 4:2:2 byte positions within a pixel pair
+};
+
+hipError_t launch_yuv_to_rgb(const YuvXArgs &a, hipStream_t stream);
+
 }  // namespace kcv
 
 #endif  // KCV_INTERNAL_H

@@ -549,4 +549,86 @@ kleidicv_error_t prepare_yuv_sp(const uint8_t *src_y, size_t src_y_stride,
   return KLEIDICV_OK;
 }
 
+// One-buffer YUV to RGB.  src/conversions/yuv_to_rgb_api.cpp:22-66 routes on
+// the base format: 4:4:4 (format first, then pointers and size,
+// yuv444_to_rgb_sc.h:167-185), 4:2:0 planar (format first, then the source
+// with (3h+1)/2 rows, yuv420p_to_rgb_sc.h:46-50,226-290), anything else to
+// 4:2:2 (pointers and size, then an even width >= 2, then the format,
+// yuv422_to_rgb_sc.h:398-470).
+kleidicv_error_t prepare_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                             size_t dst_stride, size_t width, size_t height,
+                             kleidicv_color_conversion_t color_format, YuvXArgs *a,
+                             Action *act) {
+  *act = Action::kNone;
+  const unsigned fmt = static_cast<unsigned>(color_format);
+  const unsigned base = fmt & KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK;
+  const unsigned flags = fmt & ~KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK;
+  const unsigned bgr_alpha = KLEIDICV_COLOR_CONVERSION_FLAG_BGR | KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA;
+  int layout;
+  size_t src_rows = height;
+  if (base == KLEIDICV_COLOR_CONVERSION_FMT_YUV444) {
+    if (flags & ~bgr_alpha) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+    layout = kYuv444;
+  } else if (base == KLEIDICV_COLOR_CONVERSION_FMT_YUV420P) {
+    if (flags & ~(bgr_alpha | KLEIDICV_COLOR_CONVERSION_FLAG_VU))
+      return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+    layout = kYuv420P;
+    src_rows = (height * 3 + 1) / 2;
+  } else {
+    layout = kYuv422;
+  }
+  KCV_TRY(check_ptr_stride(src, src_stride, src_rows));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, height));
+  KCV_TRY(check_image_size(width, height));
+  if (layout == kYuv422) {
+    if (width < 2 || (width % 2) != 0) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+    const bool vu = flags & KLEIDICV_COLOR_CONVERSION_FLAG_VU;
+    const bool cf = flags & KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST;
+    if (base != KLEIDICV_COLOR_CONVERSION_FMT_YUV422 || (vu && cf) ||
+        (flags & ~(bgr_alpha | KLEIDICV_COLOR_CONVERSION_FLAG_VU |
+                   KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST)))
+      return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  }
+  if (width == 0 || height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  *a = YuvXArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->layout = layout;
+  a->bgr = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_BGR) ? 1 : 0;
+  a->alpha = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA) ? 1 : 0;
+  const size_t row_bytes = width * (layout == kYuv420P ? 1 : layout == kYuv422 ? 2 : 3);
+  a->src_extent = (src_rows - 1) * src_stride + row_bytes;
+  if (layout == kYuv420P) {
+    // yuv420p_to_rgb_sc.h:52-75: the first chroma plane follows the luma
+    // plane; the second starts height/4 source rows later, plus half a row
+    // when height % 4 is 2 or 3, and then steps by the other half first
+    // when height % 4 == 2.  YV12 stores V first.
+    const uint8_t *p1 = src + src_stride * height;
+    const uint8_t *p2 = p1 + src_stride * (height / 4) + (width / 2) * ((height % 4) / 2);
+    const int ph2 = height % 4 == 2 ? 1 : 0;
+    const bool yv12 = fmt & KLEIDICV_COLOR_CONVERSION_FLAG_VU;
+    a->u = yv12 ? p2 : p1;
+    a->v = yv12 ? p1 : p2;
+    a->u_phase = yv12 ? ph2 : 0;
+    a->v_phase = yv12 ? 0 : ph2;
+  } else if (layout == kYuv422) {
+    // pixel pair bytes: YUYV = Y0 U Y1 V, UYVY = U Y0 V Y1, YVYU = Y0 V Y1 U
+    if (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST) {
+      a->y_off = 1; a->u_off = 0; a->v_off = 2;
+    } else if (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_VU) {
+      a->y_off = 0; a->u_off = 3; a->v_off = 1;
+    } else {
+      a->y_off = 0; a->u_off = 1; a->v_off = 3;
+    }
+  }
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
 }  // namespace kcv

@@ -94,6 +94,11 @@ kleidicv_error_t prepare_warp(const uint8_t *src, size_t src_stride, size_t src_
                               kleidicv_border_type_t border_type,
                               const uint8_t *border_value, WarpArgs *a, Action *act);
 
+kleidicv_error_t prepare_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                             size_t dst_stride, size_t width, size_t height,
+                             kleidicv_color_conversion_t color_format, YuvXArgs *a,
+                             Action *act);
+
 kleidicv_error_t prepare_yuv_sp(const uint8_t *src_y, size_t src_y_stride,
                                 const uint8_t *src_uv, size_t src_uv_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,

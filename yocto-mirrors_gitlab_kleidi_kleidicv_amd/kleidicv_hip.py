@@ -271,6 +271,21 @@ def yuv_sp_to_rgb(y, uv, color_format=NV12_TO_RGB, dst=None, stream=None,
     return dst
 
 
+def yuv_to_rgb(src, width, height, color_format, dst=None, stream=None, device_api=False):
+    """One-buffer YUV (planar 4:2:0, 4:2:2, 4:4:4) to RGB/BGR/RGBA/BGRA.
+    `src` holds the source rows (2-D, rows may be padded)."""
+    oc = 4 if color_format & 0x40 else 3
+    dst = _empty_like_shape(src, (height, width, oc)) if dst is None else dst
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(dst), *_sz(_stride(dst), width, height),
+            C.c_int(color_format)]
+    if device_api:
+        _check(lib().kleidicv_hip_yuv_to_rgb_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_yuv_to_rgb_u8")
+    else:
+        _check(lib().kleidicv_yuv_to_rgb_u8(*args), "kleidicv_yuv_to_rgb_u8")
+    return dst
+
+
 def synchronize(stream=None):
     _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
 
