@@ -90,6 +90,7 @@ def make_inputs(torch, gen, dev):
     s["i420"] = u8((3 * H) // 2, W)
     s["yuyv"] = u8(H, 2 * W)
     s["yuv444"] = u8(H, 3 * W)
+    s["rgb"] = u8(H, W, 3)
     return s
 
 
@@ -148,6 +149,10 @@ def extended_ops(K, s, out, stream):
         ("yuyv_to_rgb", W * H, 2 * W * H + 3 * W * H, yuv(s["yuyv"], "yuyv_to_rgb", 0x04)),
         ("yuv444_to_rgba", W * H, 3 * W * H + 4 * W * H,
          yuv(s["yuv444"], "yuv444_to_rgba", 0x41)),
+        ("rgb_to_nv12", W * H, 3 * W * H + (W * H * 3) // 2,
+         lambda: K.rgb_to_yuv_semiplanar(s["rgb"], 0x02, y=out["rgb_to_nv12"][0],
+                                         uv=out["rgb_to_nv12"][1], stream=stream,
+                                         device_api=True)),
         ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
          bd(s["sep_src"], "blur_and_downsample_1ch")),
         ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
@@ -160,6 +165,8 @@ def extended_outputs(torch, dev):
         "i420_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
         "yuyv_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
         "yuv444_to_rgba": torch.empty(H, W, 4, dtype=torch.uint8, device=dev),
+        "rgb_to_nv12": (torch.empty(H, W, dtype=torch.uint8, device=dev),
+                        torch.empty(H // 2, W, dtype=torch.uint8, device=dev)),
         "blur_and_downsample_1ch": torch.empty(H // 2, W // 2, dtype=torch.uint8, device=dev),
         "blur_and_downsample_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
     }

@@ -98,6 +98,39 @@ typedef enum {
   KLEIDICV_UYVY_TO_BGR = 0xA4,
   KLEIDICV_UYVY_TO_RGBA = 0xC4,
   KLEIDICV_UYVY_TO_BGRA = 0xE4,
+  /* RGB -> YUV names: the same values, the flags describing the source */
+  KLEIDICV_RGB_TO_YUV444 = 0x01,
+  KLEIDICV_RGB_TO_NV12 = 0x02,
+  KLEIDICV_RGB_TO_NV21 = 0x12,
+  KLEIDICV_RGB_TO_IYUV = 0x03,
+  KLEIDICV_RGB_TO_YV12 = 0x13,
+  KLEIDICV_RGB_TO_YUYV = 0x04,
+  KLEIDICV_RGB_TO_YVYU = 0x14,
+  KLEIDICV_RGB_TO_UYVY = 0x84,
+  KLEIDICV_BGR_TO_YUV444 = 0x21,
+  KLEIDICV_BGR_TO_NV12 = 0x22,
+  KLEIDICV_BGR_TO_NV21 = 0x32,
+  KLEIDICV_BGR_TO_IYUV = 0x23,
+  KLEIDICV_BGR_TO_YV12 = 0x33,
+  KLEIDICV_BGR_TO_YUYV = 0x24,
+  KLEIDICV_BGR_TO_YVYU = 0x34,
+  KLEIDICV_BGR_TO_UYVY = 0xA4,
+  KLEIDICV_RGBA_TO_YUV444 = 0x41,
+  KLEIDICV_RGBA_TO_NV12 = 0x42,
+  KLEIDICV_RGBA_TO_NV21 = 0x52,
+  KLEIDICV_RGBA_TO_IYUV = 0x43,
+  KLEIDICV_RGBA_TO_YV12 = 0x53,
+  KLEIDICV_RGBA_TO_YUYV = 0x44,
+  KLEIDICV_RGBA_TO_YVYU = 0x54,
+  KLEIDICV_RGBA_TO_UYVY = 0xC4,
+  KLEIDICV_BGRA_TO_YUV444 = 0x61,
+  KLEIDICV_BGRA_TO_NV12 = 0x62,
+  KLEIDICV_BGRA_TO_NV21 = 0x72,
+  KLEIDICV_BGRA_TO_IYUV = 0x63,
+  KLEIDICV_BGRA_TO_YV12 = 0x73,
+  KLEIDICV_BGRA_TO_YUYV = 0x64,
+  KLEIDICV_BGRA_TO_YVYU = 0x74,
+  KLEIDICV_BGRA_TO_UYVY = 0xE4,
 } kleidicv_color_conversion_t;
 
 /* kleidicv.h:103-113 */
@@ -136,6 +169,21 @@ kleidicv_error_t kleidicv_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,
                                         uint8_t *dst, size_t dst_stride,
                                         size_t width, size_t height,
                                         kleidicv_color_conversion_t color_format);
+
+/* Replaces kleidicv.h:1740-1742 (src/conversions/rgb_to_yuv_api.cpp:
+ * 65-70): RGB/BGR(A) to planar 4:2:0 (I420/YV12 in OpenCV's packed layout,
+ * (3h+1)/2 destination rows), interleaved 4:2:2 and 4:4:4. */
+kleidicv_error_t kleidicv_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride,
+                                        uint8_t *dst, size_t dst_stride,
+                                        size_t width, size_t height,
+                                        kleidicv_color_conversion_t color_format);
+
+/* Replaces kleidicv.h:1811-1814 (rgb_to_yuv_api.cpp:47-55): RGB/BGR(A) to
+ * NV12/NV21, chroma rows of (width+1)/2 pairs. */
+kleidicv_error_t kleidicv_rgb_to_yuv_semiplanar_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride,
+    uint8_t *uv_dst, size_t uv_stride, size_t width, size_t height,
+    kleidicv_color_conversion_t color_format);
 
 /* Replaces kleidicv.h:2215-2218 (src/filters/blur_and_downsample_api.cpp:
  * 37-45): the binomial 5x5 Gaussian at even rows and columns (cv::pyrDown),
@@ -222,6 +270,16 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:223-227 */
+kleidicv_error_t kleidicv_thread_rgb_to_yuv_semiplanar_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride,
+    uint8_t *uv_dst, size_t uv_stride, size_t width, size_t height,
+    kleidicv_color_conversion_t color_format, kleidicv_thread_multithreading mt);
+/* kleidicv_thread.h:231-234 */
+kleidicv_error_t kleidicv_thread_rgb_to_yuv_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    kleidicv_thread_multithreading mt);
 /* kleidicv_thread.h:208-211 */
 kleidicv_error_t kleidicv_thread_yuv_to_rgb_u8(
     const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
@@ -291,6 +349,16 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
     size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     kleidicv_border_type_t border_type, void *stream);
+
+kleidicv_error_t kleidicv_hip_rgb_to_yuv_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    void *stream);
+
+kleidicv_error_t kleidicv_hip_rgb_to_yuv_semiplanar_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride,
+    uint8_t *uv_dst, size_t uv_stride, size_t width, size_t height,
+    kleidicv_color_conversion_t color_format, void *stream);
 
 kleidicv_error_t kleidicv_hip_yuv_to_rgb_u8(
     const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,

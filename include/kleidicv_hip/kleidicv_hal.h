@@ -39,6 +39,20 @@ using uchar = unsigned char;
 /* kleidicv_hal.h:24, kleidicv_hal.cpp:191-200 */
 int yuv422_to_bgr(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
                   int width, int height, int dcn, bool swapBlue, int uIdx, int ycn);
+/* kleidicv_hal.h:28, kleidicv_hal.cpp:180-189 (dcn is the source's channel
+ * count) */
+int bgr_to_yuv422(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+                  int width, int height, int dcn, bool swapBlue, int uIdx, int ycn);
+/* kleidicv_hal.h:39, kleidicv_hal.cpp:342-354 */
+int bgr_to_yuv420_p(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+                    int width, int height, int scn, bool swapBlue, int uIdx);
+/* kleidicv_hal.h:43, kleidicv_hal.cpp:356-368 */
+int bgr_to_yuv420_sp(const uchar *src_data, size_t src_step, uchar *y_data, size_t y_step,
+                     uchar *uv_data, size_t uv_step, int width, int height, int scn,
+                     bool swapBlue, int uIdx);
+/* kleidicv_hal.h:64, kleidicv_hal.cpp:370-385 */
+int bgr_to_yuv(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+               int width, int height, int depth, int scn, bool swapBlue, bool isCbCr);
 /* kleidicv_hal.h:47, kleidicv_hal.cpp:286-293 */
 int yuv_to_bgr_sp(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
                   int dst_width, int dst_height, int dcn, bool swapBlue, int uIdx);
@@ -106,6 +120,19 @@ extern "C" {
 #endif
 
 /* C twins of the functions above (same arguments, same return codes). */
+int kleidicv_hip_hal_bgr_to_yuv422(const unsigned char *src_data, size_t src_step,
+                                   unsigned char *dst_data, size_t dst_step, int width,
+                                   int height, int dcn, bool swapBlue, int uIdx, int ycn);
+int kleidicv_hip_hal_bgr_to_yuv420_p(const unsigned char *src_data, size_t src_step,
+                                     unsigned char *dst_data, size_t dst_step, int width,
+                                     int height, int scn, bool swapBlue, int uIdx);
+int kleidicv_hip_hal_bgr_to_yuv420_sp(const unsigned char *src_data, size_t src_step,
+                                      unsigned char *y_data, size_t y_step,
+                                      unsigned char *uv_data, size_t uv_step, int width,
+                                      int height, int scn, bool swapBlue, int uIdx);
+int kleidicv_hip_hal_bgr_to_yuv(const unsigned char *src_data, size_t src_step,
+                                unsigned char *dst_data, size_t dst_step, int width,
+                                int height, int depth, int scn, bool swapBlue, bool isCbCr);
 int kleidicv_hip_hal_yuv422_to_bgr(const unsigned char *src_data, size_t src_step,
                                    unsigned char *dst_data, size_t dst_step, int width,
                                    int height, int dcn, bool swapBlue, int uIdx, int ycn);

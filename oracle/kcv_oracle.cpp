@@ -1168,3 +1168,124 @@ extern "C" int orc_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t 
     }
   return kErrOk;
 }
+
+// ---------------------------------------------------------------------------
+// RGB to YUV (src/conversions/rgb_to_yuv_api.cpp:20-45 routes 4:4:4 and
+// 4:2:2 by base format, everything else to planar 4:2:0).  The source has 4
+// channels with the alpha flag, B first with the BGR flag.
+//  - 4:2:0 (planar and semi-planar): 20-bit limited-range weights
+//    (yuv42x_coefficients.h:17-29); Y = (sum + 2^19 + 16 2^20) >> 20; U and V
+//    from the top-left pixel of each 2x2 block, (sum + 2^19 + 128 2^20) >> 20
+//    (rgb_to_yuv420_sc.h:160-230).  Planar chroma rows: U at row h/4 of the
+//    chroma area plus (w+1)/2 bytes when (h/2) is odd, V at row
+//    (h + height + 1)/4 plus (w+1)/2 bytes when (h + height + 1)/2 is odd
+//    (test_rgb_to_yuv420p.cpp:196-236).
+//  - 4:2:2: 14-bit weights (rgb_to_yuv422_sc.h:8-19); Y per pixel, U and V
+//    from the pair sums, each (sum + bias) >> 14 keeping the low byte.
+//  - 4:4:4: Y = (4899 R + 9617 G + 1868 B + 2^13) >> 14, then
+//    U = sat((128 2^14 + 8061 (B - Y) + 2^13) >> 14),
+//    V = sat((128 2^14 + 14369 (R - Y) + 2^13) >> 14)
+//    (rgb_to_yuv444_coefficients.h, rgb_to_yuv444_sc.h:18-110).
+// Parity: pinned in tests/ by the reference tests' own scalar formulas for
+// 4:2:0 (test_rgb_to_yuv420p.cpp:158-236, test_rgb_to_yuv420sp.cpp) and by
+// round trips through the YUV to RGB oracle within the conversions' error.
+namespace {
+struct Px {
+  int r, g, b;
+};
+
+Px px_at(const uint8_t *row, size_t x, int scn, bool bgr) {
+  const uint8_t *p = row + x * scn;
+  return bgr ? Px{p[2], p[1], p[0]} : Px{p[0], p[1], p[2]};
+}
+
+int y420(Px p) { return (269484 * p.r + 528482 * p.g + 102760 * p.b + (1 << 19) + (16 << 20)) >> 20; }
+int u420(Px p) { return (-155188 * p.r - 305135 * p.g + 460324 * p.b + (1 << 19) + (128 << 20)) >> 20; }
+int v420(Px p) { return (460324 * p.r - 385875 * p.g - 74448 * p.b + (1 << 19) + (128 << 20)) >> 20; }
+
+bool rgb_format_420(int fmt, int base) {
+  return (fmt & 0x0F) == base && (fmt & ~0x7F) == 0;
+}
+}  // namespace
+
+extern "C" int orc_rgb_to_yuv_semiplanar_u8(const uint8_t *src, size_t src_stride,
+                                            uint8_t *y_dst, size_t y_stride, uint8_t *uv_dst,
+                                            size_t uv_stride, size_t width, size_t height,
+                                            int fmt) {
+  if (!src || !y_dst || !uv_dst) return kErrNull;
+  if (!rgb_format_420(fmt, 0x02)) return kErrNotImplemented;
+  const bool vu = fmt & 0x10, bgr = fmt & 0x20;
+  const int scn = (fmt & 0x40) ? 4 : 3;
+  for (size_t y = 0; y < height; ++y)
+    for (size_t x = 0; x < width; ++x) {
+      const Px p = px_at(src + y * src_stride, x, scn, bgr);
+      y_dst[y * y_stride + x] = static_cast<uint8_t>(y420(p));
+      if (y % 2 == 0 && x % 2 == 0) {
+        uint8_t *uv = uv_dst + (y / 2) * uv_stride + x;
+        uv[vu ? 1 : 0] = static_cast<uint8_t>(u420(p));
+        uv[vu ? 0 : 1] = static_cast<uint8_t>(v420(p));
+      }
+    }
+  return kErrOk;
+}
+
+extern "C" int orc_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                 size_t dst_stride, size_t width, size_t height, int fmt) {
+  if (!src || !dst) return kErrNull;
+  const int base = fmt & 0x0F;
+  const bool bgr = fmt & 0x20;
+  const int scn = (fmt & 0x40) ? 4 : 3;
+  if (base == 0x01) {
+    if (fmt & ~0x61) return kErrNotImplemented;
+    for (size_t y = 0; y < height; ++y)
+      for (size_t x = 0; x < width; ++x) {
+        const Px p = px_at(src + y * src_stride, x, scn, bgr);
+        const int Y = (4899 * p.r + 9617 * p.g + 1868 * p.b + (1 << 13)) >> 14;
+        const int U = ((128 << 14) + 8061 * (p.b - Y) + (1 << 13)) >> 14;
+        const int V = ((128 << 14) + 14369 * (p.r - Y) + (1 << 13)) >> 14;
+        uint8_t *o = dst + y * dst_stride + 3 * x;
+        o[0] = static_cast<uint8_t>(std::clamp(Y, 0, 255));
+        o[1] = static_cast<uint8_t>(std::clamp(U, 0, 255));
+        o[2] = static_cast<uint8_t>(std::clamp(V, 0, 255));
+      }
+    return kErrOk;
+  }
+  if (base == 0x04) {
+    if (width < 2 || width % 2) return kErrNotImplemented;
+    const bool vu = fmt & 0x10, cf = fmt & 0x80;
+    if (vu && cf) return kErrNotImplemented;
+    const int yo = cf ? 1 : 0, uo = cf ? 0 : (vu ? 3 : 1), vo = cf ? 2 : (vu ? 1 : 3);
+    for (size_t y = 0; y < height; ++y)
+      for (size_t x = 0; x < width; x += 2) {
+        const Px p0 = px_at(src + y * src_stride, x, scn, bgr);
+        const Px p1 = px_at(src + y * src_stride, x + 1, scn, bgr);
+        const int yb = (1 << 13) + (16 << 14), cb = (1 << 13) + (256 << 13);
+        const int r = p0.r + p1.r, g = p0.g + p1.g, b = p0.b + p1.b;
+        uint8_t *o = dst + y * dst_stride + 2 * x;
+        o[yo] = static_cast<uint8_t>((yb + 4211 * p0.r + 8258 * p0.g + 1606 * p0.b) >> 14);
+        o[yo + 2] = static_cast<uint8_t>((yb + 4211 * p1.r + 8258 * p1.g + 1606 * p1.b) >> 14);
+        o[uo] = static_cast<uint8_t>((cb - 1212 * r - 2384 * g + 3596 * b) >> 14);
+        o[vo] = static_cast<uint8_t>((cb + 3596 * r - 3015 * g - 582 * b) >> 14);
+      }
+    return kErrOk;
+  }
+  if (!rgb_format_420(fmt, 0x03)) return kErrNotImplemented;
+  const bool yv12 = fmt & 0x10;
+  const size_t cw = (width + 1) / 2;
+  uint8_t *area = dst + dst_stride * height;
+  for (size_t y = 0; y < height; ++y)
+    for (size_t x = 0; x < width; ++x) {
+      const Px p = px_at(src + y * src_stride, x, scn, bgr);
+      dst[y * dst_stride + x] = static_cast<uint8_t>(y420(p));
+      if (y % 2 == 0 && x % 2 == 0) {
+        const size_t hv = y + height + 1;
+        uint8_t *ur = area + dst_stride * (y / 4) + ((y / 2) % 2) * cw;
+        uint8_t *vr = area + dst_stride * (hv / 4) + ((hv / 2) % 2) * cw;
+        int u = u420(p), v = v420(p);
+        if (yv12) std::swap(u, v);
+        ur[x / 2] = static_cast<uint8_t>(u);
+        vr[x / 2] = static_cast<uint8_t>(v);
+      }
+    }
+  return kErrOk;
+}

@@ -116,6 +116,15 @@ int orc_yuv_semiplanar_to_rgb_u8(const uint8_t *src_y, size_t src_y_stride,
 int orc_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
                       size_t dst_stride, size_t width, size_t height, int fmt);
 
+/* RGB/BGR(A) to YUV: planar 4:2:0, 4:2:2, 4:4:4 (one buffer) and the
+ * semi-planar NV12/NV21 form. */
+int orc_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                      size_t dst_stride, size_t width, size_t height, int fmt);
+int orc_rgb_to_yuv_semiplanar_u8(const uint8_t *src, size_t src_stride,
+                                 uint8_t *y_dst, size_t y_stride,
+                                 uint8_t *uv_dst, size_t uv_stride,
+                                 size_t width, size_t height, int fmt);
+
 /* Stripe variants: destination rows [y_begin, y_end) only, the rest of the
  * image is read as in the full call.  The CPU baseline runs these on
  * std::thread-style row stripes (kleidicv_thread.cpp:60-75). */

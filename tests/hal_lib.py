@@ -13,6 +13,10 @@ INTER_NEAREST, INTER_LINEAR, INTER_CUBIC, INTER_AREA = 0, 1, 2, 3
 
 _P, _S, _I, _D, _B = C.c_void_p, C.c_size_t, C.c_int, C.c_double, C.c_bool
 ARGTYPES = {
+    "bgr_to_yuv422": [_P, _S, _P, _S, _I, _I, _I, _B, _I, _I],
+    "bgr_to_yuv420_p": [_P, _S, _P, _S, _I, _I, _I, _B, _I],
+    "bgr_to_yuv420_sp": [_P, _S, _P, _S, _P, _S, _I, _I, _I, _B, _I],
+    "bgr_to_yuv": [_P, _S, _P, _S, _I, _I, _I, _I, _B, _B],
     "yuv422_to_bgr": [_P, _S, _P, _S, _I, _I, _I, _B, _I, _I],
     "yuv_to_bgr_sp": [_P, _S, _P, _S, _I, _I, _I, _B, _I],
     "yuv_to_bgr_sp_ex": [_P, _S, _P, _S, _P, _S, _I, _I, _I, _B, _I],
@@ -33,6 +37,10 @@ ARGTYPES = {
 # Itanium-mangled names of the C++ entry points (namespace kleidicv::hal),
 # as an OpenCV build patched for the reference adapter references them.
 MANGLED = {
+    "bgr_to_yuv422": "_ZN8kleidicv3hal13bgr_to_yuv422EPKhmPhmiiibii",
+    "bgr_to_yuv420_p": "_ZN8kleidicv3hal15bgr_to_yuv420_pEPKhmPhmiiibi",
+    "bgr_to_yuv420_sp": "_ZN8kleidicv3hal16bgr_to_yuv420_spEPKhmPhmS3_miiibi",
+    "bgr_to_yuv": "_ZN8kleidicv3hal10bgr_to_yuvEPKhmPhmiiiibb",
     "yuv422_to_bgr": "_ZN8kleidicv3hal13yuv422_to_bgrEPKhmPhmiiibii",
     "yuv_to_bgr_sp": "_ZN8kleidicv3hal13yuv_to_bgr_spEPKhmPhmiiibi",
     "yuv_to_bgr_sp_ex": "_ZN8kleidicv3hal16yuv_to_bgr_sp_exEPKhmS2_mPhmiiibi",

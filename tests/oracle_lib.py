@@ -85,6 +85,29 @@ def yuv_to_rgb(src, width, height, fmt):
     return err, dst
 
 
+def rgb_to_yuv(src, fmt, dst_shape=None, dst_stride=None):
+    """src (h, w, c); returns (err, dst) with dst rows of dst_stride bytes."""
+    s = np.ascontiguousarray(src, dtype=np.uint8)
+    h, w = s.shape[:2]
+    base = fmt & 0x0F
+    rows, row_bytes = {0x01: (h, 3 * w), 0x04: (h, 2 * w)}.get(base, ((3 * h + 1) // 2, w))
+    stride = dst_stride or row_bytes
+    dst = np.zeros((rows, stride), np.uint8)
+    err = lib().orc_rgb_to_yuv_u8(_p(s), *_sz(s.strides[0]), _p(dst), *_sz(stride, w, h),
+                                  C.c_int(fmt))
+    return err, dst
+
+
+def rgb_to_yuv_semiplanar(src, fmt):
+    s = np.ascontiguousarray(src, dtype=np.uint8)
+    h, w = s.shape[:2]
+    y = np.zeros((h, w), np.uint8)
+    uv = np.zeros(((h + 1) // 2, 2 * ((w + 1) // 2)), np.uint8)
+    err = lib().orc_rgb_to_yuv_semiplanar_u8(_p(s), *_sz(s.strides[0]), _p(y), *_sz(w),
+                                             _p(uv), *_sz(uv.shape[1], w, h), C.c_int(fmt))
+    return err, y, uv
+
+
 def separable_filter_2d(src, kx, ky, border="REPLICATE"):
     s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
     h, w, c = s.shape

@@ -256,3 +256,16 @@ def yuv444_vector_form(src3, bgr=False, alpha=False):
         r, b = b, r
     ch = [r, g, b] + ([np.full_like(Y, 255)] if alpha else [])
     return np.clip(np.stack(ch, axis=-1), 0, 255).astype(np.uint8)
+
+
+def rgb_to_yuv420_reference_formula(rgb):
+    """Per-pixel Y and per-2x2-block (top-left) U, V as the reference's tests
+    compute them (test_rgb_to_yuv420p.cpp:158-195): 20-bit weights, clamp."""
+    r, g, b = (rgb[..., i].astype(np.int64) for i in range(3))
+    y = np.clip((269484 * r + 528482 * g + 102760 * b + (1 << 19) + (16 << 20)) >> 20, 0, 255)
+    r0, g0, b0 = r[::2, ::2], g[::2, ::2], b[::2, ::2]
+    u = np.clip((-155188 * r0 - 305135 * g0 + 460324 * b0 + (1 << 19) + (128 << 20)) >> 20,
+                0, 255)
+    v = np.clip((460324 * r0 - 385875 * g0 - 74448 * b0 + (1 << 19) + (128 << 20)) >> 20,
+                0, 255)
+    return y.astype(np.uint8), u.astype(np.uint8), v.astype(np.uint8)

@@ -145,6 +145,41 @@ def test_yuv_to_rgb_error_codes():
     assert yuv(0, p, 20, 10, 0x83) == K.NOT_IMPLEMENTED  # format before pointers
 
 
+def test_rgb_to_yuv_error_codes():
+    # test_rgb_to_yuv420p.cpp, test_rgb_to_yuv422.cpp, test_rgb_to_yuv444.cpp
+    # unsupported-argument cases
+    lib = K.lib()
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+
+    def ryuv(src, dst, w, h, fmt):
+        return lib.kleidicv_rgb_to_yuv_u8(C.c_void_p(src), C.c_size_t(256), C.c_void_p(dst),
+                                          C.c_size_t(64), C.c_size_t(w), C.c_size_t(h),
+                                          C.c_int(fmt))
+
+    def rsp(src, y, uv, w, h, fmt):
+        return lib.kleidicv_rgb_to_yuv_semiplanar_u8(
+            C.c_void_p(src), C.c_size_t(256), C.c_void_p(y), C.c_size_t(64), C.c_void_p(uv),
+            C.c_size_t(64), C.c_size_t(w), C.c_size_t(h), C.c_int(fmt))
+    for fmt in (0x03, 0x73, 0x04, 0x84, 0x01, 0x61):
+        assert ryuv(0, p, 20, 10, fmt) == K.NULL_POINTER
+        assert ryuv(p, 0, 20, 10, fmt) == K.NULL_POINTER
+        assert ryuv(p, p, 1 << 48, 1 << 48, fmt) == K.RANGE
+    assert ryuv(p, p, 19, 10, 0x04) == K.NOT_IMPLEMENTED
+    assert ryuv(p, p, 20, 10, 0x94) == K.NOT_IMPLEMENTED
+    assert ryuv(p, p, 20, 10, 0x11) == K.NOT_IMPLEMENTED
+    assert ryuv(p, p, 20, 10, 0x02) == K.NOT_IMPLEMENTED
+    assert ryuv(p, p, 20, 10, 0x83) == K.NOT_IMPLEMENTED
+    assert ryuv(0, p, 20, 10, 0x83) == K.NULL_POINTER  # pointers before the format
+    assert ryuv(p, p, 0, 1, 0x03) == K.OK
+    for fmt in (0x02, 0x72):
+        assert rsp(0, p, p, 20, 10, fmt) == K.NULL_POINTER
+        assert rsp(p, 0, p, 20, 10, fmt) == K.NULL_POINTER
+        assert rsp(p, p, 0, 20, 10, fmt) == K.NULL_POINTER
+    assert rsp(p, p, p, 20, 10, 0x03) == K.NOT_IMPLEMENTED
+    assert rsp(p, p, p, 20, 10, 0x82) == K.NOT_IMPLEMENTED
+
+
 def test_resize_error_codes():
     lib = K.lib()
     buf = np.zeros(4096, np.uint8)

@@ -148,5 +148,10 @@ def test_yuv_policy():
     assert f(None, 64, Q, 192, 16, 16, H.CV_8U, 3, False, False) == H.UNKNOWN
     assert f(P, 64, Q, 192, 16, 16, H.CV_8U, 3, False, True) == H.NOT_IMPLEMENTED  # CbCr
     assert f(P, 64, Q, 192, 16, 16, H.CV_16U, 3, False, False) == H.NOT_IMPLEMENTED
+    assert H.fn("bgr_to_yuv420_p")(None, 64, Q, 64, 16, 16, 3, False, 2) == H.UNKNOWN
+    assert H.fn("bgr_to_yuv420_sp")(None, 64, Q, 64, Q, 64, 16, 16, 3, False, 1) == H.UNKNOWN
+    assert H.fn("bgr_to_yuv422")(None, 64, Q, 64, 16, 16, 3, False, 0, 0) == H.UNKNOWN
+    assert H.fn("bgr_to_yuv")(P, 64, Q, 64, 16, 16, H.CV_8U, 3, False, True) == H.NOT_IMPLEMENTED
+    assert H.fn("bgr_to_yuv")(None, 64, Q, 64, 16, 16, H.CV_8U, 3, False, False) == H.UNKNOWN
     # 4:2:2 with an odd width: the library declines
     assert H.fn("yuv422_to_bgr")(P, 64, Q, 192, 15, 16, 3, False, 0, 0) == H.NOT_IMPLEMENTED

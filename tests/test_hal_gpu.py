@@ -119,3 +119,27 @@ def test_hal_yuv(rng):
     assert H.fn("yuv_to_bgr")(ptr(i444), 3 * w, ptr(dst), 3 * w, w, h, H.CV_8U, 3, False,
                               False) == H.OK
     np.testing.assert_array_equal(dst, O.yuv_to_rgb(i444, w, h, 0x21)[1])  # -> BGR
+
+
+def test_hal_bgr_to_yuv(rng):
+    h, w = 22, 34
+    src = rng.integers(0, 256, (h, w, 3), dtype=np.uint8)
+    # swapBlue false: the source is BGR (kleidicv_hal.cpp:342-385)
+    dst = np.zeros(((3 * h + 1) // 2, w), np.uint8)
+    assert H.fn("bgr_to_yuv420_p")(ptr(src), 3 * w, ptr(dst), w, w, h, 3, False, 2) == H.OK
+    np.testing.assert_array_equal(dst, O.rgb_to_yuv(src, 0x33)[1])   # BGR -> YV12
+    y = np.zeros((h, w), np.uint8)
+    uv = np.zeros((h // 2, w), np.uint8)
+    assert H.fn("bgr_to_yuv420_sp")(ptr(src), 3 * w, ptr(y), w, ptr(uv), w, w, h, 3, True,
+                                    1) == H.OK
+    _, wy, wuv = O.rgb_to_yuv_semiplanar(src, 0x02)                   # RGB -> NV12
+    np.testing.assert_array_equal(y, wy)
+    np.testing.assert_array_equal(uv, wuv)
+    dst = np.zeros((h, 3 * w), np.uint8)
+    assert H.fn("bgr_to_yuv")(ptr(src), 3 * w, ptr(dst), 3 * w, w, h, H.CV_8U, 3, True,
+                              False) == H.OK
+    np.testing.assert_array_equal(dst, O.rgb_to_yuv(src, 0x01)[1])    # RGB -> YUV444
+    dst = np.zeros((h, 2 * w), np.uint8)
+    assert H.fn("bgr_to_yuv422")(ptr(src), 3 * w, ptr(dst), 2 * w, w, h, 3, False, 1,
+                                 0) == H.OK
+    np.testing.assert_array_equal(dst, O.rgb_to_yuv(src, 0x34)[1])    # BGR -> YVYU

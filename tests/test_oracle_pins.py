@@ -379,3 +379,90 @@ def test_yuv_routing_errors():
     assert O.yuv_to_rgb(buf, 10, 4, 0x02)[0] == 1      # NV12 is not one-buffer
     assert O.yuv_to_rgb(buf, 10, 4, 0x00)[0] == 1
     assert O.yuv_to_rgb(buf, 0, 1, 0x03)[0] == 0
+
+
+# ---------------------------------------------------------------------------
+# RGB to YUV
+@pytest.mark.parametrize("h,w", [(16, 18), (11, 32), (10, 770), (2, 2)])
+@pytest.mark.parametrize("flags", [0x00, 0x20, 0x40, 0x60])
+def test_rgb_to_i420_yv12_and_nv12_vs_reference_formula(rng, h, w, flags):
+    scn = 4 if flags & 0x40 else 3
+    src = rng.integers(0, 256, (h, w, scn), dtype=np.uint8)
+    rgb = src[..., [2, 1, 0]] if flags & 0x20 else src[..., :3]
+    y, u, v = pins.rgb_to_yuv420_reference_formula(rgb)
+    for vu in (False, True):
+        err, buf = O.rgb_to_yuv(src, 0x03 | flags | (0x10 if vu else 0), dst_stride=w + 2)
+        assert err == 0
+        np.testing.assert_array_equal(buf[:h, :w], y)
+        if h % 2 == 0:   # pins.i420_buffer lays out the even-height chroma area
+            want = pins.i420_buffer(y, u, v, w + 2, vu)
+            np.testing.assert_array_equal(buf[h:, :w], want[h:, :w])
+        err, yy, uv = O.rgb_to_yuv_semiplanar(src, 0x02 | flags | (0x10 if vu else 0))
+        assert err == 0
+        np.testing.assert_array_equal(yy, y)
+        np.testing.assert_array_equal(uv[:, 0::2], v if vu else u)
+        np.testing.assert_array_equal(uv[:, 1::2], u if vu else v)
+
+
+def test_rgb_to_yuv_gray_axis():
+    # gray maps to U = V = 128 in every layout; 4:4:4 keeps Y = gray level,
+    # the limited-range forms put 0 at 16 and 255 at 235
+    levels = np.arange(256, dtype=np.uint8)
+    src = np.repeat(np.repeat(levels[None, :, None], 2, axis=0), 3, axis=2)
+    err, b444 = O.rgb_to_yuv(src, 0x01)
+    assert err == 0
+    t = b444[0].reshape(-1, 3)
+    assert (t[:, 0] == levels).all() and (t[:, 1] == 128).all() and (t[:, 2] == 128).all()
+    err, y, uv = O.rgb_to_yuv_semiplanar(src, 0x02)
+    assert err == 0 and (uv == 128).all() and y[0, 0] == 16 and y[0, 255] == 235
+    assert (np.diff(y[0].astype(int)) >= 0).all()
+    err, b422 = O.rgb_to_yuv(src, 0x04)
+    assert err == 0
+    q = b422[0].reshape(-1, 4)
+    assert (q[:, 1] == 128).all() and (q[:, 3] == 128).all()
+    # the same limited-range luma, up to the 14- vs 20-bit weights' rounding
+    assert np.abs(q[:, 0].astype(int) - y[0, 0::2].astype(int)).max() <= 1
+    assert q[0, 0] == 16 and q[-1, 2] == 235
+
+
+@pytest.mark.parametrize("fmt", [0x01, 0x21, 0x41, 0x61])
+def test_rgb_to_yuv444_round_trip(rng, fmt):
+    # 4:4:4 there and back stays within 2 of the source for colours whose
+    # chroma does not clip (V = 0.877 (R - Y) leaves 8 bits for saturated
+    # reds and cyans, so channels are kept within 48..208)
+    scn = 4 if fmt & 0x40 else 3
+    src = rng.integers(48, 209, (9, 31, scn), dtype=np.uint8)
+    err, buf = O.rgb_to_yuv(src, fmt)
+    assert err == 0
+    err, back = O.yuv_to_rgb(buf, 31, 9, 0x01 | (fmt & 0x20))
+    assert err == 0
+    d = np.abs(back.astype(int) - src[..., :3].astype(int))
+    assert d.max() <= 2
+
+
+@pytest.mark.parametrize("order,base", [("YUYV", 0x04), ("UYVY", 0x84), ("YVYU", 0x14)])
+def test_rgb_to_yuv422_pairs(rng, order, base):
+    # luma equals the 4:2:0 luma up to the 14- vs 20-bit weights; on pixel
+    # pairs of one colour the chroma equals that colour's 4:2:0 chroma
+    src = rng.integers(0, 256, (5, 16, 3), dtype=np.uint8)
+    src[:, 1::2] = src[:, 0::2]
+    err, buf = O.rgb_to_yuv(src, base)
+    assert err == 0
+    q = buf.reshape(5, 8, 4)
+    yo, uo, vo = {"YUYV": (0, 1, 3), "UYVY": (1, 0, 2), "YVYU": (0, 3, 1)}[order]
+    r, g, b = (src[:, 0::2, i].astype(np.int64) for i in range(3))
+    u420 = (-155188 * r - 305135 * g + 460324 * b + (1 << 19) + (128 << 20)) >> 20
+    v420 = (460324 * r - 385875 * g - 74448 * b + (1 << 19) + (128 << 20)) >> 20
+    assert np.abs(q[..., uo].astype(int) - u420).max() <= 1
+    assert np.abs(q[..., vo].astype(int) - v420).max() <= 1
+    assert (q[..., yo] == q[..., yo + 2]).all()
+
+
+def test_rgb_to_yuv_routing_errors():
+    src = np.zeros((4, 10, 3), np.uint8)
+    assert O.rgb_to_yuv(src[:, :9], 0x04)[0] == 1     # 4:2:2 odd width
+    assert O.rgb_to_yuv(src, 0x94)[0] == 1            # VU and chroma-first
+    assert O.rgb_to_yuv(src, 0x11)[0] == 1            # 4:4:4 with VU
+    assert O.rgb_to_yuv(src, 0x02)[0] == 1            # NV12 through the one-buffer call
+    assert O.rgb_to_yuv(src, 0x83)[0] == 1
+    assert O.rgb_to_yuv_semiplanar(src, 0x03)[0] == 1

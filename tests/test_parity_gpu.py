@@ -471,6 +471,60 @@ def test_yuv_layouts_device_api_and_padded_dst(rng):
         assert_same(d.cpu().numpy(), want, "device %#x" % fmt)
 
 
+# ---------------------------------------------------------------------------
+# RGB/BGR(A) to YUV (test_rgb_to_yuv420p.cpp:80-86 sizes, odd sizes, widths
+# across thread blocks)
+SRC_FLAGS = [0x00, 0x20, 0x40, 0x60]
+
+
+@pytest.mark.parametrize("flags", SRC_FLAGS)
+def test_rgb_to_yuv420(rng, flags):
+    scn = 4 if flags & 0x40 else 3
+    for w in (2, 18, 32, 33, 770, 2051):
+        for h in (1, 2, 11, 16, 21):
+            src = rng.integers(0, 256, (h, w, scn), dtype=np.uint8)
+            for vu in (0, 0x10):
+                fmt = 0x03 | flags | vu
+                _, want = O.rgb_to_yuv(src, fmt, dst_stride=w + 2)
+                dst = np.zeros(((3 * h + 1) // 2, w + 2), np.uint8)
+                K.rgb_to_yuv(src, fmt, dst=dst)
+                assert_same(dst[:, :w], want[:, :w], "planar %#x %dx%d" % (fmt, w, h))
+                fmt = 0x02 | flags | vu
+                _, wy, wuv = O.rgb_to_yuv_semiplanar(src, fmt)
+                gy, guv = K.rgb_to_yuv_semiplanar(src, fmt)
+                assert_same(gy, wy, "nv luma %#x %dx%d" % (fmt, w, h))
+                assert_same(guv, wuv, "nv chroma %#x %dx%d" % (fmt, w, h))
+
+
+@pytest.mark.parametrize("flags", SRC_FLAGS)
+def test_rgb_to_yuv422_444(rng, flags):
+    scn = 4 if flags & 0x40 else 3
+    for w in (2, 6, 18, 64, 2050):
+        for h in (1, 3, 10):
+            src = rng.integers(0, 256, (h, w, scn), dtype=np.uint8)
+            for base in (0x04, 0x84, 0x14, 0x01):
+                fmt = base | flags
+                _, want = O.rgb_to_yuv(src, fmt)
+                assert_same(K.rgb_to_yuv(src, fmt), want, "%#x %dx%d" % (fmt, w, h))
+    src = rng.integers(0, 256, (4, 9, scn), dtype=np.uint8)  # odd width: 4:4:4 only
+    assert_same(K.rgb_to_yuv(src, 0x01 | flags), O.rgb_to_yuv(src, 0x01 | flags)[1], "444 odd")
+
+
+def test_rgb_to_yuv_device_api(rng):
+    import torch
+    src = rng.integers(0, 256, (26, 1030, 4), dtype=np.uint8)
+    s = torch.from_numpy(src).cuda()
+    for fmt in (0x43, 0x64, 0x41):
+        d = K.rgb_to_yuv(s, fmt, device_api=True)
+        torch.cuda.synchronize()
+        assert_same(d.cpu().numpy(), O.rgb_to_yuv(src, fmt)[1], "device %#x" % fmt)
+    y, uv = K.rgb_to_yuv_semiplanar(s, 0x52, device_api=True)
+    torch.cuda.synchronize()
+    _, wy, wuv = O.rgb_to_yuv_semiplanar(src, 0x52)
+    assert_same(y.cpu().numpy(), wy, "device nv21 luma")
+    assert_same(uv.cpu().numpy(), wuv, "device nv21 chroma")
+
+
 def test_device_api_matches_host_api(rng):
     import torch
     dev = torch.device("cuda:0")

@@ -184,6 +184,51 @@ kleidicv_error_t yuv_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
   return ss.finish();
 }
 
+kleidicv_error_t rgb_to_yuv_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                 size_t dst_stride, size_t width, size_t height,
+                                 kleidicv_color_conversion_t color_format, void *stream,
+                                 bool sync) {
+  kcv::RgbYuvArgs a;
+  Action act;
+  KCV_TRY(kcv::prepare_rgb_to_yuv(src, src_stride, dst, dst_stride, width, height,
+                                  color_format, &a, &act));
+  if (act == Action::kNone) return KLEIDICV_OK;
+  hipStream_t s = pick_stream(stream, sync);
+  if (!sync) return from_hip(kcv::launch_rgb_to_yuv(a, s));
+  Session ss(s);
+  KCV_TRY(ss.input(src, extent(src_stride, width * a.scn, height), &a.src));
+  const size_t out_bytes = a.layout == kcv::kYuv444 ? 3 * width
+                           : a.layout == kcv::kYuv422 ? 2 * width : width;
+  const size_t out_rows = a.layout == kcv::kYuv420P ? (height * 3 + 1) / 2 : height;
+  uint8_t *ddst = nullptr;
+  KCV_TRY(ss.output(dst, dst_stride, out_bytes, out_rows, &ddst));
+  a.uv = ddst + (a.uv - a.dst);  // planar chroma area moves with the buffer
+  a.dst = ddst;
+  KCV_TRY(from_hip(kcv::launch_rgb_to_yuv(a, s)));
+  return ss.finish();
+}
+
+kleidicv_error_t rgb_to_yuv_sp_impl(const uint8_t *src, size_t src_stride, uint8_t *y_dst,
+                                    size_t y_stride, uint8_t *uv_dst, size_t uv_stride,
+                                    size_t width, size_t height,
+                                    kleidicv_color_conversion_t color_format, void *stream,
+                                    bool sync) {
+  kcv::RgbYuvArgs a;
+  Action act;
+  KCV_TRY(kcv::prepare_rgb_to_yuv_sp(src, src_stride, y_dst, y_stride, uv_dst, uv_stride,
+                                     width, height, color_format, &a, &act));
+  if (act == Action::kNone) return KLEIDICV_OK;
+  hipStream_t s = pick_stream(stream, sync);
+  if (!sync) return from_hip(kcv::launch_rgb_to_yuv(a, s));
+  Session ss(s);
+  KCV_TRY(ss.input(src, extent(src_stride, width * a.scn, height), &a.src));
+  KCV_TRY(ss.output(y_dst, y_stride, width, height, &a.dst));
+  // (w+1)/2 interleaved pairs per chroma row
+  KCV_TRY(ss.output(uv_dst, uv_stride, 2 * ((width + 1) / 2), (height + 1) / 2, &a.uv));
+  KCV_TRY(from_hip(kcv::launch_rgb_to_yuv(a, s)));
+  return ss.finish();
+}
+
 kleidicv_error_t separable_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,
                                 size_t channels, const uint8_t *kernel_x,
@@ -425,6 +470,22 @@ kleidicv_error_t kleidicv_separable_filter_2d_u8(
                         nullptr, true);
 }
 
+kleidicv_error_t kleidicv_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                        size_t dst_stride, size_t width, size_t height,
+                                        kleidicv_color_conversion_t color_format) {
+  return rgb_to_yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format,
+                         nullptr, true);
+}
+
+kleidicv_error_t kleidicv_rgb_to_yuv_semiplanar_u8(const uint8_t *src, size_t src_stride,
+                                                   uint8_t *y_dst, size_t y_stride,
+                                                   uint8_t *uv_dst, size_t uv_stride,
+                                                   size_t width, size_t height,
+                                                   kleidicv_color_conversion_t color_format) {
+  return rgb_to_yuv_sp_impl(src, src_stride, y_dst, y_stride, uv_dst, uv_stride, width, height,
+                            color_format, nullptr, true);
+}
+
 kleidicv_error_t kleidicv_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                         size_t dst_stride, size_t width, size_t height,
                                         kleidicv_color_conversion_t color_format) {
@@ -490,6 +551,25 @@ kleidicv_error_t kleidicv_thread_separable_filter_2d_u8(
   return kleidicv_separable_filter_2d_u8(src, src_stride, dst, dst_stride, width,
                                          height, channels, kernel_x, kernel_width,
                                          kernel_y, kernel_height, border_type);
+}
+
+kleidicv_error_t kleidicv_thread_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride,
+                                               uint8_t *dst, size_t dst_stride, size_t width,
+                                               size_t height,
+                                               kleidicv_color_conversion_t color_format,
+                                               kleidicv_thread_multithreading mt) {
+  KCV_TRY(check_mt(mt));
+  return rgb_to_yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format,
+                         nullptr, true);
+}
+
+kleidicv_error_t kleidicv_thread_rgb_to_yuv_semiplanar_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride, uint8_t *uv_dst,
+    size_t uv_stride, size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    kleidicv_thread_multithreading mt) {
+  KCV_TRY(check_mt(mt));
+  return rgb_to_yuv_sp_impl(src, src_stride, y_dst, y_stride, uv_dst, uv_stride, width, height,
+                            color_format, nullptr, true);
 }
 
 kleidicv_error_t kleidicv_thread_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,
@@ -599,6 +679,23 @@ kleidicv_error_t kleidicv_hip_separable_filter_2d_u8(
   return separable_impl(src, src_stride, dst, dst_stride, width, height, channels,
                         kernel_x, kernel_width, kernel_y, kernel_height, border_type,
                         stream, false);
+}
+
+kleidicv_error_t kleidicv_hip_rgb_to_yuv_u8(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride, size_t width,
+                                            size_t height,
+                                            kleidicv_color_conversion_t color_format,
+                                            void *stream) {
+  return rgb_to_yuv_impl(src, src_stride, dst, dst_stride, width, height, color_format, stream,
+                         false);
+}
+
+kleidicv_error_t kleidicv_hip_rgb_to_yuv_semiplanar_u8(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride, uint8_t *uv_dst,
+    size_t uv_stride, size_t width, size_t height, kleidicv_color_conversion_t color_format,
+    void *stream) {
+  return rgb_to_yuv_sp_impl(src, src_stride, y_dst, y_stride, uv_dst, uv_stride, width, height,
+                            color_format, stream, false);
 }
 
 kleidicv_error_t kleidicv_hip_yuv_to_rgb_u8(const uint8_t *src, size_t src_stride,

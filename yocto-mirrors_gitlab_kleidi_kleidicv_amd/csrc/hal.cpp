@@ -114,6 +114,41 @@ int yuv422_to_bgr(const uchar *src_data, size_t src_step, uchar *dst_data, size_
       color_type(dcn, !swapBlue, uIdx != 0, ycn != 0, KLEIDICV_COLOR_CONVERSION_FMT_YUV422)));
 }
 
+int bgr_to_yuv422(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+                  int width, int height, int dcn, bool swapBlue, int uIdx, int ycn) {
+  return convert_error(kleidicv_rgb_to_yuv_u8(
+      src_data, src_step, dst_data, dst_step, static_cast<size_t>(width),
+      static_cast<size_t>(height),
+      color_type(dcn, !swapBlue, uIdx != 0, ycn != 0, KLEIDICV_COLOR_CONVERSION_FMT_YUV422)));
+}
+
+int bgr_to_yuv420_p(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+                    int width, int height, int scn, bool swapBlue, int uIdx) {
+  // OpenCV's uIdx 2 is YV12 here (kleidicv_hal.cpp:345)
+  return convert_error(kleidicv_rgb_to_yuv_u8(
+      src_data, src_step, dst_data, dst_step, static_cast<size_t>(width),
+      static_cast<size_t>(height),
+      color_type(scn, !swapBlue, uIdx == 2, false, KLEIDICV_COLOR_CONVERSION_FMT_YUV420P)));
+}
+
+int bgr_to_yuv420_sp(const uchar *src_data, size_t src_step, uchar *y_data, size_t y_step,
+                     uchar *uv_data, size_t uv_step, int width, int height, int scn,
+                     bool swapBlue, int uIdx) {
+  return convert_error(kleidicv_rgb_to_yuv_semiplanar_u8(
+      src_data, src_step, y_data, y_step, uv_data, uv_step, static_cast<size_t>(width),
+      static_cast<size_t>(height),
+      color_type(scn, !swapBlue, uIdx == 2, false, KLEIDICV_COLOR_CONVERSION_FMT_YUV420SP)));
+}
+
+int bgr_to_yuv(const uchar *src_data, size_t src_step, uchar *dst_data, size_t dst_step,
+               int width, int height, int depth, int scn, bool swapBlue, bool isCbCr) {
+  if (depth != kCv8U || isCbCr) return kHalNotImplemented;
+  return convert_error(kleidicv_rgb_to_yuv_u8(
+      src_data, src_step, dst_data, dst_step, static_cast<size_t>(width),
+      static_cast<size_t>(height),
+      color_type(scn, !swapBlue, false, false, KLEIDICV_COLOR_CONVERSION_FMT_YUV444)));
+}
+
 int yuv_to_bgr_sp_ex(const uchar *y_data, size_t y_step, const uchar *uv_data, size_t uv_step,
                      uchar *dst_data, size_t dst_step, int dst_width, int dst_height, int dcn,
                      bool swapBlue, int uIdx) {
@@ -322,6 +357,27 @@ namespace H = kleidicv::hal;
 int kleidicv_hip_hal_yuv422_to_bgr(const unsigned char *a, size_t b, unsigned char *c,
                                    size_t d, int e, int f, int g, bool h, int i, int j) {
   return H::yuv422_to_bgr(a, b, c, d, e, f, g, h, i, j);
+}
+
+int kleidicv_hip_hal_bgr_to_yuv422(const unsigned char *a, size_t b, unsigned char *c,
+                                   size_t d, int e, int f, int g, bool h, int i, int j) {
+  return H::bgr_to_yuv422(a, b, c, d, e, f, g, h, i, j);
+}
+
+int kleidicv_hip_hal_bgr_to_yuv420_p(const unsigned char *a, size_t b, unsigned char *c,
+                                     size_t d, int e, int f, int g, bool h, int i) {
+  return H::bgr_to_yuv420_p(a, b, c, d, e, f, g, h, i);
+}
+
+int kleidicv_hip_hal_bgr_to_yuv420_sp(const unsigned char *a, size_t b, unsigned char *c,
+                                      size_t d, unsigned char *e, size_t f, int g, int h,
+                                      int i, bool j, int k) {
+  return H::bgr_to_yuv420_sp(a, b, c, d, e, f, g, h, i, j, k);
+}
+
+int kleidicv_hip_hal_bgr_to_yuv(const unsigned char *a, size_t b, unsigned char *c, size_t d,
+                                int e, int f, int g, int h, bool i, bool j) {
+  return H::bgr_to_yuv(a, b, c, d, e, f, g, h, i, j);
 }
 
 int kleidicv_hip_hal_yuv_to_bgr_sp(const unsigned char *a, size_t b, unsigned char *c,

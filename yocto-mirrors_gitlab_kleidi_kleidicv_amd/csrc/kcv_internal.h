@@ -184,6 +184,26 @@ struct YuvXArgs {
 
 hipError_t launch_yuv_to_rgb(const YuvXArgs &a, hipStream_t stream);
 
+// RGB/BGR(A) to YUV.  4:2:0 writes luma to dst and chroma either to the
+// interleaved plane uv (semi-planar) or into the chroma area that follows
+// the luma plane (planar, uv = dst + height * dst_stride).
+enum { kRgb420SP = 3 };
+
+struct RgbYuvArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  uint8_t *dst;
+  size_t dst_stride;
+  uint8_t *uv;
+  size_t uv_stride;
+  int width, height;
+  int layout;                 // kYuv420P, kYuv422, kYuv444, kRgb420SP
+  int scn, bgr, vu;           // source channels, B first, V first
+  int y_off, u_off, v_off;    // 4:2:2 byte positions within a pixel pair
+};
+
+hipError_t launch_rgb_to_yuv(const RgbYuvArgs &a, hipStream_t stream);
+
 }  // namespace kcv
 
 #endif  // KCV_INTERNAL_H

@@ -631,4 +631,109 @@ kleidicv_error_t prepare_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst
   return KLEIDICV_OK;
 }
 
+// RGB to YUV, one buffer.  src/conversions/rgb_to_yuv_api.cpp:20-45 routes
+// 4:4:4 (pointers and size, then the format: rgb_to_yuv444_sc.h:273-321),
+// 4:2:2 (pointers and size, an even width >= 2, then the format:
+// rgb_to_yuv422_sc.h:240-304), and everything else to planar 4:2:0
+// (pointers with (3h+1)/2 destination rows and size, then the format:
+// rgb_to_yuv420p_neon.cpp:17-74).
+kleidicv_error_t prepare_rgb_to_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                    size_t dst_stride, size_t width, size_t height,
+                                    kleidicv_color_conversion_t color_format, RgbYuvArgs *a,
+                                    Action *act) {
+  *act = Action::kNone;
+  const unsigned fmt = static_cast<unsigned>(color_format);
+  const unsigned base = fmt & KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK;
+  const unsigned flags = fmt & ~KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK;
+  const unsigned bgr_alpha = KLEIDICV_COLOR_CONVERSION_FLAG_BGR | KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA;
+  int layout;
+  size_t dst_rows = height;
+  if (base == KLEIDICV_COLOR_CONVERSION_FMT_YUV444) {
+    layout = kYuv444;
+  } else if (base == KLEIDICV_COLOR_CONVERSION_FMT_YUV422) {
+    layout = kYuv422;
+  } else {
+    layout = kYuv420P;
+    dst_rows = (height * 3 + 1) / 2;
+  }
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(dst, dst_stride, dst_rows));
+  KCV_TRY(check_image_size(width, height));
+  if (layout == kYuv444) {
+    if (flags & ~bgr_alpha) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  } else if (layout == kYuv422) {
+    if (width < 2 || (width % 2) != 0) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+    const bool vu = flags & KLEIDICV_COLOR_CONVERSION_FLAG_VU;
+    const bool cf = flags & KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST;
+    if (vu && cf) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  } else {
+    if (base != KLEIDICV_COLOR_CONVERSION_FMT_YUV420P ||
+        (flags & ~(bgr_alpha | KLEIDICV_COLOR_CONVERSION_FLAG_VU)))
+      return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  }
+  if (width == 0 || height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  *a = RgbYuvArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = dst;
+  a->dst_stride = dst_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->layout = layout;
+  a->scn = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA) ? 4 : 3;
+  a->bgr = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_BGR) ? 1 : 0;
+  a->vu = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_VU) ? 1 : 0;
+  if (layout == kYuv420P) {
+    a->uv = dst + dst_stride * height;  // the chroma area follows the luma plane
+    a->uv_stride = dst_stride;
+  } else if (layout == kYuv422) {
+    if (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST) {
+      a->y_off = 1; a->u_off = 0; a->v_off = 2;
+    } else if (a->vu) {
+      a->y_off = 0; a->u_off = 3; a->v_off = 1;
+    } else {
+      a->y_off = 0; a->u_off = 1; a->v_off = 3;
+    }
+    a->vu = 0;
+  }
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
+// RGB to NV12 / NV21.  src/conversions/rgb_to_yuv420sp_neon.cpp:12-75:
+// pointers (luma rows h, chroma rows (h+1)/2), size, then the format.
+kleidicv_error_t prepare_rgb_to_yuv_sp(const uint8_t *src, size_t src_stride, uint8_t *y_dst,
+                                       size_t y_stride, uint8_t *uv_dst, size_t uv_stride,
+                                       size_t width, size_t height,
+                                       kleidicv_color_conversion_t color_format,
+                                       RgbYuvArgs *a, Action *act) {
+  *act = Action::kNone;
+  KCV_TRY(check_ptr_stride(src, src_stride, height));
+  KCV_TRY(check_ptr_stride(y_dst, y_stride, height));
+  KCV_TRY(check_ptr_stride(uv_dst, uv_stride, (height + 1) / 2));
+  KCV_TRY(check_image_size(width, height));
+  const unsigned fmt = static_cast<unsigned>(color_format);
+  if ((fmt & KLEIDICV_COLOR_CONVERSION_YUV_FMT_MASK) != KLEIDICV_COLOR_CONVERSION_FMT_YUV420SP ||
+      (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_CHROMA_FIRST))
+    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  if (width == 0 || height == 0) return KLEIDICV_OK;
+  if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  *a = RgbYuvArgs{};
+  a->src = src;
+  a->src_stride = src_stride;
+  a->dst = y_dst;
+  a->dst_stride = y_stride;
+  a->uv = uv_dst;
+  a->uv_stride = uv_stride;
+  a->width = static_cast<int>(width);
+  a->height = static_cast<int>(height);
+  a->layout = kRgb420SP;
+  a->scn = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_ALPHA) ? 4 : 3;
+  a->bgr = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_BGR) ? 1 : 0;
+  a->vu = (fmt & KLEIDICV_COLOR_CONVERSION_FLAG_VU) ? 1 : 0;
+  *act = Action::kLaunch;
+  return KLEIDICV_OK;
+}
+
 }  // namespace kcv

@@ -99,6 +99,17 @@ kleidicv_error_t prepare_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst
                              kleidicv_color_conversion_t color_format, YuvXArgs *a,
                              Action *act);
 
+kleidicv_error_t prepare_rgb_to_yuv(const uint8_t *src, size_t src_stride, uint8_t *dst,
+                                    size_t dst_stride, size_t width, size_t height,
+                                    kleidicv_color_conversion_t color_format, RgbYuvArgs *a,
+                                    Action *act);
+
+kleidicv_error_t prepare_rgb_to_yuv_sp(const uint8_t *src, size_t src_stride, uint8_t *y_dst,
+                                       size_t y_stride, uint8_t *uv_dst, size_t uv_stride,
+                                       size_t width, size_t height,
+                                       kleidicv_color_conversion_t color_format,
+                                       RgbYuvArgs *a, Action *act);
+
 kleidicv_error_t prepare_yuv_sp(const uint8_t *src_y, size_t src_y_stride,
                                 const uint8_t *src_uv, size_t src_uv_stride, uint8_t *dst,
                                 size_t dst_stride, size_t width, size_t height,

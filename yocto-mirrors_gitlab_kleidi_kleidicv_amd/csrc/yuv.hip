@@ -28,7 +28,44 @@ __global__ __launch_bounds__(kThreads) void yuv444_kernel(YuvXArgs a) {
   yuv444_rows<OC>(a, blockIdx.x, blockIdx.y, gridDim.y);
 }
 
+template <int SCN>
+__global__ __launch_bounds__(kThreads) void rgb_to_yuv420_kernel(RgbYuvArgs a) {
+  rgb_to_yuv420_rows<SCN>(a, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+template <int SCN>
+__global__ __launch_bounds__(kThreads) void rgb_to_yuv422_kernel(RgbYuvArgs a) {
+  rgb_to_yuv422_rows<SCN>(a, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
+template <int SCN>
+__global__ __launch_bounds__(kThreads) void rgb_to_yuv444_kernel(RgbYuvArgs a) {
+  rgb_to_yuv444_rows<SCN>(a, blockIdx.x, blockIdx.y, gridDim.y);
+}
+
 }  // namespace
+
+hipError_t launch_rgb_to_yuv(const RgbYuvArgs &a, hipStream_t stream) {
+  if (a.width <= 0 || a.height <= 0) return hipSuccess;
+  const int per_block = kThreads * kBlockW;
+  const bool is420 = a.layout == kYuv420P || a.layout == kRgb420SP;
+  const int rows = is420 ? (a.height + 1) / 2 : a.height;
+  const dim3 grid((a.width + per_block - 1) / per_block, min(rows, 65535));
+  const bool four = a.scn == 4;
+  if (is420) {
+    if (four) rgb_to_yuv420_kernel<4><<<grid, kThreads, 0, stream>>>(a);
+    else rgb_to_yuv420_kernel<3><<<grid, kThreads, 0, stream>>>(a);
+  } else if (a.layout == kYuv422) {
+    if (four) rgb_to_yuv422_kernel<4><<<grid, kThreads, 0, stream>>>(a);
+    else rgb_to_yuv422_kernel<3><<<grid, kThreads, 0, stream>>>(a);
+  } else if (a.layout == kYuv444) {
+    if (four) rgb_to_yuv444_kernel<4><<<grid, kThreads, 0, stream>>>(a);
+    else rgb_to_yuv444_kernel<3><<<grid, kThreads, 0, stream>>>(a);
+  } else {
+    return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_yuv_to_rgb(const YuvXArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0) return hipSuccess;

@@ -343,6 +343,211 @@ __device__ __forceinline__ void yuv444_rows(const YuvXArgs &a, int bx, int y_beg
   }
 }
 
+// ---------------------------------------------------------------------------
+// RGB/BGR(A) to YUV, 8 source pixels per thread.
+//  4:2:0: 20-bit limited-range weights (yuv42x_coefficients.h:17-29), chroma
+//         from the top-left pixel of each 2x2 block (rgb_to_yuv420_sc.h).
+//  4:2:2: 14-bit weights, chroma from the pair sums (rgb_to_yuv422_sc.h).
+//  4:4:4: 14-bit full-range weights, chroma from B - Y and R - Y of the
+//         rounded Y (rgb_to_yuv444_coefficients.h, rgb_to_yuv444_sc.h).
+
+// Loads 8 pixels of SCN bytes at p (the first `count` when not `vec`).
+template <int SCN>
+__device__ __forceinline__ void load_px8(const uint8_t *p, bool vec, int count,
+                                         uint32_t w[2 * SCN]) {
+#pragma unroll
+  for (int i = 0; i < 2 * SCN; ++i) w[i] = 0;
+  if (vec) {
+    const uint2 *q = reinterpret_cast<const uint2 *>(p);
+#pragma unroll
+    for (int i = 0; i < SCN; ++i) {
+      const uint2 v = q[i];
+      w[2 * i] = v.x;
+      w[2 * i + 1] = v.y;
+    }
+  } else {
+    for (int b = 0; b < SCN * count; ++b) w[b >> 2] |= static_cast<uint32_t>(p[b]) << (8 * (b & 3));
+  }
+}
+
+struct Rgb8 {
+  int r[kBlockW], g[kBlockW], b[kBlockW];
+};
+
+template <int SCN>
+__device__ __forceinline__ Rgb8 split_px8(const uint32_t w[2 * SCN], bool bgr) {
+  Rgb8 p;
+  auto byte = [&](int i) -> int { return static_cast<int>((w[i >> 2] >> (8 * (i & 3))) & 0xFF); };
+#pragma unroll
+  for (int k = 0; k < kBlockW; ++k) {
+    const int c0 = byte(SCN * k), c1 = byte(SCN * k + 1), c2 = byte(SCN * k + 2);
+    p.r[k] = bgr ? c2 : c0;
+    p.g[k] = c1;
+    p.b[k] = bgr ? c0 : c2;
+  }
+  return p;
+}
+
+// Stores `n` bytes of the words w (byte i of the run is byte i & 3 of w[i >> 2]).
+__device__ __forceinline__ void store_bytes(uint8_t *out, const uint32_t *w, int n, bool vec8) {
+  if (vec8) {
+    store_nt(reinterpret_cast<uint2 *>(out), make_uint2(w[0], w[1]));
+    return;
+  }
+  for (int i = 0; i < n; ++i) out[i] = static_cast<uint8_t>(w[i >> 2] >> (8 * (i & 3)));
+}
+
+template <int SCN>
+__device__ __forceinline__ void rgb_to_yuv420_rows(const RgbYuvArgs &a, int bx, int yp_begin,
+                                                   int yp_step) {
+  const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
+  if (x0 >= a.width) return;
+  const int count = min(kBlockW, a.width - x0);
+  const bool full = count == kBlockW;
+  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
+  const bool y_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 7) == 0;
+  const int pairs = (a.height + 1) / 2;
+  const int nchroma = (count + 1) / 2;
+  const size_t cw = static_cast<size_t>((a.width + 1) / 2);
+  for (int yp = yp_begin; yp < pairs; yp += yp_step) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int y = 2 * yp + r;
+      if (y >= a.height) break;
+      uint32_t w[2 * SCN];
+      load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
+                    full && in_al, count, w);
+      const Rgb8 p = split_px8<SCN>(w, a.bgr);
+      uint32_t yw[2] = {0, 0};
+#pragma unroll
+      for (int k = 0; k < kBlockW; ++k) {
+        const uint32_t Y = static_cast<uint32_t>(269484 * p.r[k] + 528482 * p.g[k] +
+                                                 102760 * p.b[k] + (1 << 19) + (16 << 20)) >> 20;
+        yw[k >> 2] |= (Y & 0xFF) << (8 * (k & 3));
+      }
+      store_bytes(a.dst + static_cast<size_t>(y) * a.dst_stride + x0, yw, count, full && y_al);
+      if (r == 1) continue;
+      // chroma of the block's top-left pixels 0, 2, 4, 6
+      uint32_t uw = 0, vw = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int k = 2 * c;
+        const int U = (-155188 * p.r[k] - 305135 * p.g[k] + 460324 * p.b[k] + (1 << 19) +
+                       (128 << 20)) >> 20;
+        const int V = (460324 * p.r[k] - 385875 * p.g[k] - 74448 * p.b[k] + (1 << 19) +
+                       (128 << 20)) >> 20;
+        uw |= (static_cast<uint32_t>(U) & 0xFF) << (8 * c);
+        vw |= (static_cast<uint32_t>(V) & 0xFF) << (8 * c);
+      }
+      if (a.vu) {
+        const uint32_t t = uw;
+        uw = vw;
+        vw = t;
+      }
+      if (a.layout == kRgb420SP) {
+        // U0 V0 U1 V1 | U2 V2 U3 V3
+        const uint32_t lo = (uw & 0xFFu) | ((vw & 0xFFu) << 8) | ((uw & 0xFF00u) << 8) |
+                            ((vw & 0xFF00u) << 16);
+        const uint32_t hi = ((uw >> 16) & 0xFFu) | ((vw >> 8) & 0xFF00u) |
+                            ((uw >> 8) & 0xFF0000u) | (vw & 0xFF000000u);
+        const uint32_t uv[2] = {lo, hi};
+        uint8_t *out = a.uv + static_cast<size_t>(yp) * a.uv_stride + x0;
+        const bool al = ((reinterpret_cast<uintptr_t>(a.uv) | a.uv_stride) & 7) == 0;
+        store_bytes(out, uv, 2 * nchroma, nchroma == 4 && al);
+      } else {
+        // planar: U row h/4 of the chroma area (+ cw when h/2 is odd), V row
+        // (h + height + 1)/4 (+ cw when (h + height + 1)/2 is odd), h = 2 yp
+        const size_t h0 = static_cast<size_t>(2 * yp);
+        const size_t hv = h0 + a.height + 1;
+        // Two half rows of cw bytes share a row; for odd widths the second
+        // one's last byte lies past column width - 1 (in the row padding, or
+        // on the next row's first byte, which that row's own chroma then
+        // holds).  Bytes at column >= width are not written.
+        const int uhalf = static_cast<int>((h0 / 2) % 2), vhalf = static_cast<int>((hv / 2) % 2);
+        uint8_t *urow = a.uv + a.dst_stride * (h0 / 4) + uhalf * cw;
+        uint8_t *vrow = a.uv + a.dst_stride * (hv / 4) + vhalf * cw;
+        for (int c = 0; c < nchroma; ++c) {
+          const size_t j = static_cast<size_t>(x0 / 2 + c);
+          if (uhalf * cw + j < static_cast<size_t>(a.width)) urow[j] = static_cast<uint8_t>(uw >> (8 * c));
+          if (vhalf * cw + j < static_cast<size_t>(a.width)) vrow[j] = static_cast<uint8_t>(vw >> (8 * c));
+        }
+      }
+    }
+  }
+}
+
+template <int SCN>
+__device__ __forceinline__ void rgb_to_yuv422_rows(const RgbYuvArgs &a, int bx, int y_begin,
+                                                   int y_step) {
+  const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
+  if (x0 >= a.width) return;
+  const int count = min(kBlockW, a.width - x0);  // even: width is even
+  const bool full = count == kBlockW;
+  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
+  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+  for (int y = y_begin; y < a.height; y += y_step) {
+    uint32_t w[2 * SCN];
+    load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
+                  full && in_al, count, w);
+    const Rgb8 p = split_px8<SCN>(w, a.bgr);
+    uint32_t q[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int k = 2 * c;
+      const int yb = (1 << 13) + (16 << 14), cb = (1 << 13) + (256 << 13);
+      const uint32_t y0 = static_cast<uint32_t>(yb + 4211 * p.r[k] + 8258 * p.g[k] + 1606 * p.b[k]) >> 14;
+      const uint32_t y1 = static_cast<uint32_t>(yb + 4211 * p.r[k + 1] + 8258 * p.g[k + 1] +
+                                                1606 * p.b[k + 1]) >> 14;
+      const int r = p.r[k] + p.r[k + 1], g = p.g[k] + p.g[k + 1], b = p.b[k] + p.b[k + 1];
+      const uint32_t u = static_cast<uint32_t>(cb - 1212 * r - 2384 * g + 3596 * b) >> 14;
+      const uint32_t v = static_cast<uint32_t>(cb + 3596 * r - 3015 * g - 582 * b) >> 14;
+      q[c] = ((y0 & 0xFF) << (8 * a.y_off)) | ((y1 & 0xFF) << (8 * a.y_off + 16)) |
+             ((u & 0xFF) << (8 * a.u_off)) | ((v & 0xFF) << (8 * a.v_off));
+    }
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 2 * x0;
+    if (full && out_al)
+      store_nt(reinterpret_cast<uint4 *>(out), make_uint4(q[0], q[1], q[2], q[3]));
+    else
+      store_bytes(out, q, 2 * count, false);
+  }
+}
+
+template <int SCN>
+__device__ __forceinline__ void rgb_to_yuv444_rows(const RgbYuvArgs &a, int bx, int y_begin,
+                                                   int y_step) {
+  const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
+  if (x0 >= a.width) return;
+  const int count = min(kBlockW, a.width - x0);
+  const bool full = count == kBlockW;
+  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
+  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 7) == 0;
+  for (int y = y_begin; y < a.height; y += y_step) {
+    uint32_t w[2 * SCN];
+    load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
+                  full && in_al, count, w);
+    const Rgb8 p = split_px8<SCN>(w, a.bgr);
+    uint32_t o[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kBlockW; ++k) {
+      const int Y = (4899 * p.r[k] + 9617 * p.g[k] + 1868 * p.b[k] + (1 << 13)) >> 14;
+      const int U = ((128 << 14) + 8061 * (p.b[k] - Y) + (1 << 13)) >> 14;
+      const int V = ((128 << 14) + 14369 * (p.r[k] - Y) + (1 << 13)) >> 14;
+      const uint32_t c[3] = {sat_u8(Y), sat_u8(U), sat_u8(V)};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) o[(3 * k + j) >> 2] |= c[j] << (8 * ((3 * k + j) & 3));
+    }
+    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 3 * x0;
+    if (full && out_al) {
+      uint2 *q = reinterpret_cast<uint2 *>(out);
+      store_nt(q, make_uint2(o[0], o[1]));
+      store_nt(q + 1, make_uint2(o[2], o[3]));
+      store_nt(q + 2, make_uint2(o[4], o[5]));
+    } else {
+      store_bytes(out, o, 3 * count, false);
+    }
+  }
+}
+
 }  // namespace yuv
 }  // namespace kcv
 

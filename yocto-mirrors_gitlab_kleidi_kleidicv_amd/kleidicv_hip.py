@@ -286,6 +286,40 @@ def yuv_to_rgb(src, width, height, color_format, dst=None, stream=None, device_a
     return dst
 
 
+def rgb_to_yuv(src, color_format, dst=None, stream=None, device_api=False):
+    """RGB/BGR(A) (h, w, 3|4) to one-buffer YUV: planar 4:2:0 ((3h+1)/2 rows
+    of w bytes), 4:2:2 (h rows of 2w) or 4:4:4 (h rows of 3w)."""
+    h, w, _ = _shape(src)
+    base = color_format & 0x0F
+    shape = {0x01: (h, 3 * w), 0x04: (h, 2 * w)}.get(base, ((3 * h + 1) // 2, w))
+    dst = _empty_like_shape(src, shape) if dst is None else dst
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(dst), *_sz(_stride(dst), w, h),
+            C.c_int(color_format)]
+    if device_api:
+        _check(lib().kleidicv_hip_rgb_to_yuv_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_rgb_to_yuv_u8")
+    else:
+        _check(lib().kleidicv_rgb_to_yuv_u8(*args), "kleidicv_rgb_to_yuv_u8")
+    return dst
+
+
+def rgb_to_yuv_semiplanar(src, color_format, y=None, uv=None, stream=None, device_api=False):
+    """RGB/BGR(A) (h, w, 3|4) to NV12/NV21: luma (h, w) and chroma
+    ((h+1)/2, 2*((w+1)/2))."""
+    h, w, _ = _shape(src)
+    y = _empty_like_shape(src, (h, w)) if y is None else y
+    uv = _empty_like_shape(src, ((h + 1) // 2, 2 * ((w + 1) // 2))) if uv is None else uv
+    args = [_ptr(src), *_sz(_stride(src)), _ptr(y), *_sz(_stride(y)), _ptr(uv),
+            *_sz(_stride(uv), w, h), C.c_int(color_format)]
+    if device_api:
+        _check(lib().kleidicv_hip_rgb_to_yuv_semiplanar_u8(*args, _stream_arg(stream)),
+               "kleidicv_hip_rgb_to_yuv_semiplanar_u8")
+    else:
+        _check(lib().kleidicv_rgb_to_yuv_semiplanar_u8(*args),
+               "kleidicv_rgb_to_yuv_semiplanar_u8")
+    return y, uv
+
+
 def synchronize(stream=None):
     _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
 
