@@ -120,7 +120,13 @@ int gaussian_validate(const uint8_t *src, const uint8_t *dst, size_t width,
   if (k < 3 || k > 255 || (k & 1) == 0) return kErrNotImplemented;
   if (width < k - 1 || height < k - 1) return kErrNotImplemented;
   if (channels > 4) return kErrNotImplemented;
-  if (!(k <= 9 || k == 15 || k == 21)) return kErrNotImplemented;
+  // sizes without a fixed path: REPLICATE only, and wide enough for the
+  // vector border blocks (include/kleidicv/filters/gaussian_blur.h:66-77)
+  if (!(k <= 9 || k == 15 || k == 21)) {
+    if (border != 1) return kErrNotImplemented;
+    const size_t margin = k / 2;
+    if (width < (margin + 7) / 8 * 8 + margin) return kErrNotImplemented;
+  }
   if (!src || !dst) return kErrNull;
   return kErrOk;
 }

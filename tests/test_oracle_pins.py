@@ -4,6 +4,7 @@ generator's direct formula from the reference's unit test, and independent
 numpy formulations (numpy.pad borders, window sums, numpy.median, double
 precision geometry).  CPU only."""
 
+import ctypes as C
 import json
 import os
 
@@ -62,6 +63,29 @@ def test_gaussian_sigma_rounds_twice(rng):
     vert = (vert + 128) >> 8
     hor = sum(w[j] * vert[:, j:j + 12] for j in range(5))
     np.testing.assert_array_equal(out, ((hor + 128) >> 8).astype(np.uint8))
+
+
+@pytest.mark.parametrize("k,sigma", [(11, 0.0), (31, 4.0), (63, 0.0)])
+def test_gaussian_arbitrary_size_rounds_twice(rng, k, sigma):
+    # sizes without a fixed path (gaussian_blur_arbitrary_neon.cpp): REPLICATE,
+    # vertical pass rounded to u8, then the horizontal pass
+    r = k // 2
+    half = np.zeros(r + 1, np.uint8)
+    assert O.lib().orc_gaussian_half_kernel(half.ctypes.data, r + 1, C.c_float(sigma))
+    w = np.concatenate([half, half[-2::-1]]).astype(np.int64)
+    assert w.sum() == 256 and len(w) == k
+    h, wd, c = k + 3, (r + 7) // 8 * 8 + r + 5, 3
+    src = rng.integers(0, 256, (h, wd, c), dtype=np.uint8)
+    err, out = O.gaussian_blur(src, k, sigma, "REPLICATE")
+    assert err == 0
+    p = np.pad(src, ((r, r), (r, r), (0, 0)), mode="edge").astype(np.int64)
+    vert = (sum(w[i] * p[i:i + h] for i in range(k)) + 128) >> 8
+    hor = (sum(w[j] * vert[:, j:j + wd] for j in range(k)) + 128) >> 8
+    np.testing.assert_array_equal(out, hor.astype(np.uint8))
+    # only REPLICATE, and only images at least aligned_margin + margin wide
+    assert O.gaussian_blur(src, k, sigma, "REFLECT")[0] == 1
+    narrow = src[:, :(r + 7) // 8 * 8 + r - 1]
+    assert O.gaussian_blur(narrow, k, sigma, "REPLICATE")[0] == 1
 
 
 @pytest.mark.parametrize("kat", KATS["blur_and_downsample_minimal"])

@@ -69,6 +69,32 @@ def test_gaussian_sigma(rng, k, sigma):
             assert_same(got, want, "k%d s%.2f %s %dx%dx%d" % (k, sigma, border, w, h, c))
 
 
+@pytest.mark.parametrize("k,sigma", [(11, 0.0), (13, 2.0), (17, 0.0), (25, 3.5), (31, 0.0),
+                                     (63, 9.0), (127, 0.0), (255, 0.0), (255, 0.3)])
+def test_gaussian_arbitrary_size(rng, k, sigma):
+    # sizes without a fixed path: REPLICATE only, width >= aligned margin + margin
+    r = k // 2
+    min_w = (r + 7) // 8 * 8 + r
+    for (w, h) in [(min_w, k - 1), (min_w + 37, k + 20), (max(min_w, 1100), k + 3)]:
+        c = int(rng.integers(1, 5))
+        src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+        got = K.gaussian_blur(src, k, sigma, "REPLICATE")
+        _, want = O.gaussian_blur(src, k, sigma, "REPLICATE")
+        assert_same(got, want, "k%d s%.1f %dx%dx%d" % (k, sigma, w, h, c))
+
+
+def test_gaussian_arbitrary_size_device_api_and_declines(rng):
+    import torch
+    src = rng.integers(0, 256, (70, 301, 4), dtype=np.uint8)
+    d = K.gaussian_blur(torch.from_numpy(src).cuda(), 41, 0.0, "REPLICATE", device_api=True)
+    torch.cuda.synchronize()
+    assert_same(d.cpu().numpy(), O.gaussian_blur(src, 41, 0.0, "REPLICATE")[1], "device k41")
+    with pytest.raises(K.KleidiCVError):
+        K.gaussian_blur(src, 41, 0.0, "REFLECT")
+    with pytest.raises(K.KleidiCVError):
+        K.gaussian_blur(src[:, :40], 41, 0.0, "REPLICATE")
+
+
 @pytest.mark.parametrize("kat", KATS["gaussian_valid_image_size"], ids=lambda k: "k%d" % k["kernel"])
 def test_gaussian_kats(kat):
     src = np.array(kat["src"], np.uint8)

@@ -120,13 +120,31 @@ kleidicv_error_t gaussian_impl(const uint8_t *src, size_t src_stride, uint8_t *d
                                kleidicv_border_type_t border_type, void *stream,
                                bool sync) {
   kcv::SepArgs a;
+  kcv::GaussArbArgs arb;
   Action act;
   KCV_TRY(kcv::prepare_gaussian(src, src_stride, dst, dst_stride, width, height, channels,
                                 kernel_width, kernel_height, sigma_x, sigma_y, border_type,
-                                &a, &act));
+                                &a, &act, &arb));
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   const size_t row_bytes = width * channels;
+  if (act == Action::kLaunchArbitrary) {
+    // the vertical pass's u8 intermediate, with the destination's stride
+    const size_t tmp_bytes = extent(dst_stride, row_bytes, height);
+    Session ss(s);
+    void *tmp = nullptr;
+    KCV_TRY(ss.scratch(tmp_bytes, &tmp));
+    arb.tmp = static_cast<uint8_t *>(tmp);
+    if (!sync) {
+      // scratch is released on the stream after the launch, without a wait
+      KCV_TRY(from_hip(kcv::launch_gaussian_arbitrary(arb, s)));
+      return KLEIDICV_OK;
+    }
+    KCV_TRY(ss.input(src, extent(src_stride, row_bytes, height), &arb.src));
+    KCV_TRY(ss.output(dst, dst_stride, row_bytes, height, &arb.dst));
+    KCV_TRY(from_hip(kcv::launch_gaussian_arbitrary(arb, s)));
+    return ss.finish();
+  }
   if (!sync) {
     if (act == Action::kCopy)
       return from_hip(hipMemcpy2DAsync(dst, dst_stride, src, src_stride, row_bytes,

@@ -55,6 +55,101 @@ __global__ __launch_bounds__(kThreads) void blur_down_kernel(SepArgs a, int tile
   blur_down_tile<C>(a, bx, by, smem);
 }
 
+// ---------------------------------------------------------------------------
+// Gaussian blur of any odd size (src/filters/gaussian_blur_arbitrary_neon.cpp):
+// REPLICATE border, symmetric u8 taps summing to 256, vertical pass rounded
+// to u8, then the horizontal pass rounded again.
+constexpr int kArbRows = 16;        // output rows per thread in the vertical pass
+constexpr int kArbRowBytes = 1024;  // output bytes per workgroup in the horizontal pass
+
+// Vertical pass: thread = one dword column x kArbRows output rows.  Each
+// source row of the span is loaded once and added to every output row it
+// reaches, in 16-bit lanes (sums stay below 2^16: the taps sum to 256).
+__global__ __launch_bounds__(kThreads) void gauss_arb_vertical_kernel(GaussArbArgs a) {
+  __shared__ uint8_t taps[256];
+  taps[threadIdx.x] = a.taps[threadIdx.x];
+  __syncthreads();
+  const int row_bytes = a.width * a.channels;
+  const int e = 4 * (blockIdx.x * kThreads + threadIdx.x);
+  if (e >= row_bytes) return;
+  const int R = a.ksize / 2;
+  const bool vec = e + 4 <= row_bytes &&
+                   ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
+  for (int y0 = blockIdx.y * kArbRows; y0 < a.height; y0 += gridDim.y * kArbRows) {
+    sep_us2 lo[kArbRows], hi[kArbRows];
+#pragma unroll
+    for (int r = 0; r < kArbRows; ++r) lo[r] = hi[r] = sep_us2{0, 0};
+    for (int j = y0 - R; j < y0 + kArbRows + R; ++j) {
+      const uint8_t *row = a.src + static_cast<size_t>(min(max(j, 0), a.height - 1)) * a.src_stride;
+      uint32_t v = 0;
+      if (vec) {
+        v = *reinterpret_cast<const uint32_t *>(row + e);
+      } else {
+        for (int b = 0; b < 4 && e + b < row_bytes; ++b)
+          v |= static_cast<uint32_t>(row[e + b]) << (8 * b);
+      }
+      const sep_us2 ev = __builtin_bit_cast(sep_us2, v & 0x00FF00FFu);
+      const sep_us2 od = __builtin_bit_cast(sep_us2, (v >> 8) & 0x00FF00FFu);
+#pragma unroll
+      for (int r = 0; r < kArbRows; ++r) {
+        const int i = j - y0 - r + R;  // tap index for output row y0 + r
+        if (i >= 0 && i < a.ksize) {
+          const uint16_t t = taps[i];
+          lo[r] += ev * sep_us2{t, t};
+          hi[r] += od * sep_us2{t, t};
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < kArbRows; ++r) {
+      const int y = y0 + r;
+      if (y >= a.height) break;
+      const sep_us2 l = (lo[r] + sep_us2{128, 128}) >> 8;
+      const sep_us2 h = (hi[r] + sep_us2{128, 128}) >> 8;
+      const uint32_t lw = __builtin_bit_cast(uint32_t, l), hw = __builtin_bit_cast(uint32_t, h);
+      const uint32_t out = (lw & 0x00FF00FFu) | ((hw & 0x00FF00FFu) << 8);
+      uint8_t *o = a.tmp + static_cast<size_t>(y) * a.dst_stride + e;
+      if (vec && ((reinterpret_cast<uintptr_t>(o) & 3) == 0)) {
+        *reinterpret_cast<uint32_t *>(o) = out;
+      } else {
+        for (int b = 0; b < 4 && e + b < row_bytes; ++b) o[b] = static_cast<uint8_t>(out >> (8 * b));
+      }
+    }
+  }
+}
+
+// Horizontal pass: a workgroup takes kArbRowBytes output bytes of one row,
+// stages the intermediate span they read (border pixels replicated) in LDS
+// and each thread sums the taps of 4 output bytes.
+__global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArbArgs a) {
+  __shared__ uint8_t taps[256];
+  __shared__ uint8_t span[kArbRowBytes + 2 * 127 * 4];
+  taps[threadIdx.x] = a.taps[threadIdx.x];
+  const int C = a.channels, R = a.ksize / 2;
+  const int row_bytes = a.width * C;
+  const int e0 = blockIdx.x * kArbRowBytes;
+  const int n = min(kArbRowBytes, row_bytes - e0) + 2 * R * C;  // span bytes
+  for (int y = blockIdx.y; y < a.height; y += gridDim.y) {
+    const uint8_t *row = a.tmp + static_cast<size_t>(y) * a.dst_stride;
+    __syncthreads();  // the previous row's span is no longer read
+    for (int b = threadIdx.x; b < n; b += kThreads) {
+      const int eb = e0 - R * C + b;  // byte position, may lie outside the row
+      const int x = eb >= 0 ? eb / C : -((C - 1 - eb) / C);
+      const int c = eb - x * C;
+      span[b] = row[min(max(x, 0), a.width - 1) * C + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int o = 4 * threadIdx.x + k;  // output byte within the segment
+      if (e0 + o >= row_bytes) break;
+      uint32_t acc = 0;
+      for (int i = 0; i < a.ksize; ++i) acc += taps[i] * static_cast<uint32_t>(span[o + i * C]);
+      a.dst[static_cast<size_t>(y) * a.dst_stride + e0 + o] = static_cast<uint8_t>((acc + 128) >> 8);
+    }
+  }
+}
+
 // Grid and tiles_x argument for an image: a 1-D XCD-ordered grid unless the
 // tile count is very large.
 struct TileGrid {
@@ -132,6 +227,21 @@ hipError_t launch_blur_down(const SepArgs &a, hipStream_t stream) {
     case 4: blur_down_kernel<4><<<grid, kThreads, 0, stream>>>(a, tiles_x); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream) {
+  if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
+  if (a.ksize < 3 || a.ksize > 255 || (a.ksize & 1) == 0) return hipErrorInvalidValue;
+  const int row_bytes = a.width * a.channels;
+  const int vy = min((a.height + kArbRows - 1) / kArbRows, 65535);
+  gauss_arb_vertical_kernel<<<dim3((row_bytes + 4 * kThreads - 1) / (4 * kThreads), vy),
+                              kThreads, 0, stream>>>(a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  gauss_arb_horizontal_kernel<<<dim3((row_bytes + kArbRowBytes - 1) / kArbRowBytes,
+                                     min(a.height, 65535)),
+                                kThreads, 0, stream>>>(a);
   return hipGetLastError();
 }
 

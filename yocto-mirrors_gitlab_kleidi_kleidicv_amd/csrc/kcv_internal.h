@@ -73,6 +73,21 @@ struct SepArgs {
 };
 
 hipError_t launch_separable(const SepArgs &a, hipStream_t stream);
+// Gaussian blur of any odd size up to 255 (REPLICATE border): a vertical
+// pass into an intermediate u8 image `tmp` (same size and stride as dst),
+// then a horizontal pass, each (sum + 128) >> 8 with the same symmetric taps.
+struct GaussArbArgs {
+  const uint8_t *src;
+  size_t src_stride;
+  uint8_t *dst;
+  size_t dst_stride;
+  uint8_t *tmp;
+  int width, height, channels, ksize;
+  uint8_t taps[256];
+};
+
+hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream);
+
 // Blur and downsample: SepArgs with the source size in width/height; the
 // destination is ((width+1)/2) x ((height+1)/2).
 hipError_t launch_blur_down(const SepArgs &a, hipStream_t stream);

@@ -147,20 +147,45 @@ kleidicv_error_t prepare_gaussian(const uint8_t *src, size_t src_stride, uint8_t
                                   size_t channels, size_t kernel_width,
                                   size_t kernel_height, float sigma_x, float sigma_y,
                                   kleidicv_border_type_t border_type, SepArgs *a,
-                                  Action *act) {
+                                  Action *act, GaussArbArgs *arb) {
   *act = Action::kNone;
   if (!is_filter_border(border_type)) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   if (!gaussian_is_implemented(width, height, kernel_width, kernel_height, sigma_x,
                                sigma_y, channels, border_type))
     return KLEIDICV_ERROR_NOT_IMPLEMENTED;
-  // The arbitrary-size path (gaussian_blur_arbitrary_neon.cpp) is not built.
-  if (!(kernel_width <= 9 || kernel_width == 15 || kernel_width == 21))
-    return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  const bool arbitrary = !(kernel_width <= 9 || kernel_width == 15 || kernel_width == 21);
+  // the batch engine has no arbitrary-size stage
+  if (arbitrary && arb == nullptr) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   KCV_TRY(check_ptr_stride(src, src_stride, height));
   KCV_TRY(check_ptr_stride(dst, dst_stride, height));
   KCV_TRY(check_image_size(width, height));
   if (width == 0 || height == 0 || channels == 0) return KLEIDICV_OK;
   if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+
+  if (arbitrary) {
+    // src/filters/gaussian_blur_arbitrary_neon.cpp:395-430: the sigma
+    // half-kernel (sigma 0 derived from the size), or a copy when the
+    // centre tap would be 256
+    uint8_t half[128];
+    const size_t half_size = kernel_width / 2 + 1;
+    if (!gaussian_half_kernel(half, half_size, sigma_x)) {
+      *act = Action::kCopy;
+      return KLEIDICV_OK;
+    }
+    *arb = GaussArbArgs{};
+    arb->src = src;
+    arb->src_stride = src_stride;
+    arb->dst = dst;
+    arb->dst_stride = dst_stride;
+    arb->width = static_cast<int>(width);
+    arb->height = static_cast<int>(height);
+    arb->channels = static_cast<int>(channels);
+    arb->ksize = static_cast<int>(kernel_width);
+    for (size_t i = 0; i < half_size; ++i)
+      arb->taps[i] = arb->taps[kernel_width - 1 - i] = half[i];
+    *act = Action::kLaunchArbitrary;
+    return KLEIDICV_OK;
+  }
 
   *a = SepArgs{};
   a->src = src;

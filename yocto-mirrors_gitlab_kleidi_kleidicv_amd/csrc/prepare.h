@@ -18,7 +18,7 @@
 
 namespace kcv {
 
-enum class Action { kNone, kLaunch, kCopy };
+enum class Action { kNone, kLaunch, kCopy, kLaunchArbitrary };
 
 // Process-wide runtime state: the library's own non-blocking stream on the
 // device current at first use, and that device's compute-unit count.
@@ -44,7 +44,7 @@ kleidicv_error_t prepare_gaussian(const uint8_t *src, size_t src_stride, uint8_t
                                   size_t channels, size_t kernel_width,
                                   size_t kernel_height, float sigma_x, float sigma_y,
                                   kleidicv_border_type_t border_type, SepArgs *a,
-                                  Action *act);
+                                  Action *act, GaussArbArgs *arb = nullptr);
 
 kleidicv_error_t prepare_separable(const uint8_t *src, size_t src_stride, uint8_t *dst,
                                    size_t dst_stride, size_t width, size_t height,
