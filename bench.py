@@ -153,6 +153,12 @@ def extended_ops(K, s, out, stream):
          lambda: K.rgb_to_yuv_semiplanar(s["rgb"], 0x02, y=out["rgb_to_nv12"][0],
                                          uv=out["rgb_to_nv12"][1], stream=stream,
                                          device_api=True)),
+        ("gaussian_31x31_1ch", W * H, 2 * W * H,
+         lambda: K.gaussian_blur(s["sep_src"], 31, 0.0, "REPLICATE", dst=out["gaussian_31x31_1ch"],
+                                 stream=stream, device_api=True)),
+        ("median_9x9_1ch", W * H, 2 * W * H,
+         lambda: K.median_blur(s["med_src"], 9, "REPLICATE", dst=out["median_9x9_1ch"],
+                               stream=stream, device_api=True)),
         ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
          bd(s["sep_src"], "blur_and_downsample_1ch")),
         ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
@@ -167,6 +173,8 @@ def extended_outputs(torch, dev):
         "yuv444_to_rgba": torch.empty(H, W, 4, dtype=torch.uint8, device=dev),
         "rgb_to_nv12": (torch.empty(H, W, dtype=torch.uint8, device=dev),
                         torch.empty(H // 2, W, dtype=torch.uint8, device=dev)),
+        "gaussian_31x31_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
+        "median_9x9_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
         "blur_and_downsample_1ch": torch.empty(H // 2, W // 2, dtype=torch.uint8, device=dev),
         "blur_and_downsample_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
     }

@@ -185,6 +185,49 @@ def test_median(rng, k, border):
             assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
 
 
+@pytest.mark.parametrize("k", [9, 11, 15, 31, 63])
+def test_median_large(rng, k):
+    # window sizes the reference serves with histograms; every border
+    for border in BORDERS:
+        for (w, h) in [(k - 1, k - 1), (k + 45, k + 6)]:
+            c = int(rng.integers(1, 5))
+            src = rng.integers(0, 256, (h, w, c), dtype=np.uint8)
+            got = K.median_blur(src, k, border)
+            _, want = O.median_blur(src, k, border)
+            assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
+
+
+def _median_by_box_counts(src, k, border):
+    """Exact median of every k x k window from box counts of (src <= v) for
+    each level v (integral images), for sizes where sorting every window
+    would take minutes."""
+    import pins
+    r = k // 2
+    p = pins.padded(src, r, border)[:, :, 0]
+    h, w = src.shape[:2]
+    rank = (k * k - 1) // 2
+    out = np.full((h, w), 255, np.uint8)
+    done = np.zeros((h, w), bool)
+    for v in range(256):
+        ii = np.pad(np.cumsum(np.cumsum(p <= v, axis=0), axis=1), ((1, 0), (1, 0)))
+        cnt = ii[k:k + h, k:k + w] - ii[:h, k:k + w] - ii[k:k + h, :w] + ii[:h, :w]
+        hit = (cnt > rank) & ~done
+        out[hit] = v
+        done |= hit
+    return out
+
+
+def test_median_large_extremes(rng):
+    # the largest windows, and a low-entropy image where many values tie
+    src = rng.integers(0, 256, (130, 140), dtype=np.uint8)
+    assert_same(K.median_blur(src, 127, "REFLECT"), O.median_blur(src, 127, "REFLECT")[1], "k127")
+    src = rng.integers(0, 256, (254, 260), dtype=np.uint8)
+    assert_same(K.median_blur(src, 255, "REVERSE"), _median_by_box_counts(src, 255, "REVERSE"),
+                "k255")
+    src = (rng.integers(0, 3, (60, 90, 3)) * 100).astype(np.uint8)
+    assert_same(K.median_blur(src, 13, "WRAP"), O.median_blur(src, 13, "WRAP")[1], "ties")
+
+
 def test_median_low_entropy(rng):
     # many ties: values drawn from {0, 1, 255}
     src = rng.choice(np.array([0, 1, 255], np.uint8), (40, 90))

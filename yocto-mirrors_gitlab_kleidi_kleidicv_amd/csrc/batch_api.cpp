@@ -112,6 +112,8 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
       KCV_TRY(kcv::prepare_median(d.src, d.src_stride, d.dst, d.dst_stride, d.src_width,
                                   d.src_height, d.channels, d.kernel_size, d.kernel_size,
                                   d.border_type, &j.u.median, &act));
+      // batched stages hold the sorting-network sizes only
+      if (act == Action::kLaunch && j.u.median.ksize > 7) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
       p->family = kcv::kFamMedian;
       j.variant = j.u.median.ksize * 8 + j.u.median.channels;
       break;

@@ -150,6 +150,45 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArb
   }
 }
 
+// ---------------------------------------------------------------------------
+// Median of larger windows (9x9 .. 255x255), the sizes the reference serves
+// with histograms (src/filters/median_blur_{small,large}_hist_neon.cpp).  Any
+// exact selection gives the same value; here one thread finds the median of
+// one output byte bit by bit: the largest t with #(window < t) <= (K*K-1)/2,
+// eight counting sweeps over the window, which neighbouring threads read from
+// the same cache lines.  Interior columns index the row directly; columns
+// within K/2 of an edge go through the border map.
+template <int C>
+__global__ __launch_bounds__(kThreads) void median_large_kernel(MedianArgs a) {
+  const int row_bytes = a.width * C;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= row_bytes) return;
+  const int x = e / C, c = e - x * C;
+  const int K = a.ksize, R = K / 2;
+  const int rank = (K * K - 1) / 2;
+  const bool interior = x - R >= 0 && x + R < a.width;
+  for (int y = blockIdx.y; y < a.height; y += gridDim.y) {
+    uint32_t prefix = 0;
+    for (int bit = 7; bit >= 0; --bit) {
+      const uint32_t t = prefix | (1u << bit);
+      int count = 0;
+      for (int i = 0; i < K; ++i) {
+        const uint8_t *row =
+            a.src + static_cast<size_t>(map_index(y + i - R, a.height, a.border)) * a.src_stride;
+        if (interior) {
+          const uint8_t *p = row + (x - R) * C + c;
+          for (int j = 0; j < K; ++j) count += p[j * C] < t;
+        } else {
+          for (int j = 0; j < K; ++j)
+            count += row[map_index(x + j - R, a.width, a.border) * C + c] < t;
+        }
+      }
+      if (count <= rank) prefix = t;
+    }
+    a.dst[static_cast<size_t>(y) * a.dst_stride + e] = static_cast<uint8_t>(prefix);
+  }
+}
+
 // Grid and tiles_x argument for an image: a 1-D XCD-ordered grid unless the
 // tile count is very large.
 struct TileGrid {
@@ -247,6 +286,18 @@ hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream) 
 
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
+  if (a.ksize > 7) {
+    if (a.ksize > 255 || (a.ksize & 1) == 0) return hipErrorInvalidValue;
+    const dim3 grid((a.width * a.channels + kThreads - 1) / kThreads, min(a.height, 65535));
+    switch (a.channels) {
+      case 1: median_large_kernel<1><<<grid, kThreads, 0, stream>>>(a); break;
+      case 2: median_large_kernel<2><<<grid, kThreads, 0, stream>>>(a); break;
+      case 3: median_large_kernel<3><<<grid, kThreads, 0, stream>>>(a); break;
+      case 4: median_large_kernel<4><<<grid, kThreads, 0, stream>>>(a); break;
+      default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   const TileGrid g = tile_grid(a.width, a.height, a.channels);
   switch (a.ksize * 8 + a.channels) {
 #define KCV_M(K, C) \

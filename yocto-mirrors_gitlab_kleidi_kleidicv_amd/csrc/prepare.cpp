@@ -320,8 +320,6 @@ kleidicv_error_t prepare_median(const uint8_t *src, size_t src_stride, uint8_t *
         height >= kernel_height - 1 && width >= kernel_width - 1 && size_ok &&
         is_filter_border(border_type)))
     return KLEIDICV_ERROR_NOT_IMPLEMENTED;
-  // Histogram medians for kernels above 7x7 are not built.
-  if (kernel_width > 7) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   if (width == 0 || height == 0 || channels == 0) return KLEIDICV_OK;
   if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
 
