@@ -151,7 +151,8 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArb
 }
 
 // ---------------------------------------------------------------------------
-// Median of larger windows (9x9 .. 255x255), the sizes the reference serves
+// Median of larger windows (13x13 .. 255x255; 9x9 and 11x11 still fit the
+// merge-network tile above in registers), the sizes the reference serves
 // with histograms (src/filters/median_blur_{small,large}_hist_neon.cpp).  Any
 // exact selection gives the same value; here one thread finds the median of
 // one output byte bit by bit: the largest t with #(window < t) <= (K*K-1)/2,
@@ -286,7 +287,7 @@ hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream) 
 
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
-  if (a.ksize > 7) {
+  if (a.ksize > 11) {
     if (a.ksize > 255 || (a.ksize & 1) == 0) return hipErrorInvalidValue;
     const dim3 grid((a.width * a.channels + kThreads - 1) / kThreads, min(a.height, 65535));
     switch (a.channels) {
@@ -305,6 +306,8 @@ hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
     KCV_M(3, 1) KCV_M(3, 2) KCV_M(3, 3) KCV_M(3, 4)
     KCV_M(5, 1) KCV_M(5, 2) KCV_M(5, 3) KCV_M(5, 4)
     KCV_M(7, 1) KCV_M(7, 2) KCV_M(7, 3) KCV_M(7, 4)
+    KCV_M(9, 1) KCV_M(9, 2) KCV_M(9, 3) KCV_M(9, 4)
+    KCV_M(11, 1) KCV_M(11, 2) KCV_M(11, 3) KCV_M(11, 4)
 #undef KCV_M
     default: return hipErrorInvalidValue;
   }
