@@ -354,107 +354,294 @@ extern "C" {
 
 namespace H = kleidicv::hal;
 
-int kleidicv_hip_hal_yuv422_to_bgr(const unsigned char *a, size_t b, unsigned char *c,
-                                   size_t d, int e, int f, int g, bool h, int i, int j) {
-  return H::yuv422_to_bgr(a, b, c, d, e, f, g, h, i, j);
+int kleidicv_hip_hal_bgr_to_yuv422(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int dcn,
+    bool swapBlue,
+    int uIdx,
+    int ycn) {
+  return H::bgr_to_yuv422(src_data, src_step, dst_data, dst_step, width, height, dcn, swapBlue, uIdx, ycn);
 }
 
-int kleidicv_hip_hal_bgr_to_yuv422(const unsigned char *a, size_t b, unsigned char *c,
-                                   size_t d, int e, int f, int g, bool h, int i, int j) {
-  return H::bgr_to_yuv422(a, b, c, d, e, f, g, h, i, j);
+int kleidicv_hip_hal_bgr_to_yuv420_p(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int scn,
+    bool swapBlue,
+    int uIdx) {
+  return H::bgr_to_yuv420_p(src_data, src_step, dst_data, dst_step, width, height, scn, swapBlue, uIdx);
 }
 
-int kleidicv_hip_hal_bgr_to_yuv420_p(const unsigned char *a, size_t b, unsigned char *c,
-                                     size_t d, int e, int f, int g, bool h, int i) {
-  return H::bgr_to_yuv420_p(a, b, c, d, e, f, g, h, i);
+int kleidicv_hip_hal_bgr_to_yuv420_sp(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *y_data,
+    size_t y_step,
+    unsigned char *uv_data,
+    size_t uv_step,
+    int width,
+    int height,
+    int scn,
+    bool swapBlue,
+    int uIdx) {
+  return H::bgr_to_yuv420_sp(src_data, src_step, y_data, y_step, uv_data, uv_step, width, height, scn, swapBlue, uIdx);
 }
 
-int kleidicv_hip_hal_bgr_to_yuv420_sp(const unsigned char *a, size_t b, unsigned char *c,
-                                      size_t d, unsigned char *e, size_t f, int g, int h,
-                                      int i, bool j, int k) {
-  return H::bgr_to_yuv420_sp(a, b, c, d, e, f, g, h, i, j, k);
+int kleidicv_hip_hal_bgr_to_yuv(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int depth,
+    int scn,
+    bool swapBlue,
+    bool isCbCr) {
+  return H::bgr_to_yuv(src_data, src_step, dst_data, dst_step, width, height, depth, scn, swapBlue, isCbCr);
 }
 
-int kleidicv_hip_hal_bgr_to_yuv(const unsigned char *a, size_t b, unsigned char *c, size_t d,
-                                int e, int f, int g, int h, bool i, bool j) {
-  return H::bgr_to_yuv(a, b, c, d, e, f, g, h, i, j);
+int kleidicv_hip_hal_yuv422_to_bgr(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int dcn,
+    bool swapBlue,
+    int uIdx,
+    int ycn) {
+  return H::yuv422_to_bgr(src_data, src_step, dst_data, dst_step, width, height, dcn, swapBlue, uIdx, ycn);
 }
 
-int kleidicv_hip_hal_yuv_to_bgr_sp(const unsigned char *a, size_t b, unsigned char *c,
-                                   size_t d, int e, int f, int g, bool h, int i) {
-  return H::yuv_to_bgr_sp(a, b, c, d, e, f, g, h, i);
+int kleidicv_hip_hal_yuv_to_bgr_sp(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    int dcn,
+    bool swapBlue,
+    int uIdx) {
+  return H::yuv_to_bgr_sp(src_data, src_step, dst_data, dst_step, dst_width, dst_height, dcn, swapBlue, uIdx);
 }
 
-int kleidicv_hip_hal_yuv_to_bgr_sp_ex(const unsigned char *a, size_t b,
-                                      const unsigned char *c, size_t d, unsigned char *e,
-                                      size_t f, int g, int h, int i, bool j, int k) {
-  return H::yuv_to_bgr_sp_ex(a, b, c, d, e, f, g, h, i, j, k);
+int kleidicv_hip_hal_yuv_to_bgr_sp_ex(
+    const unsigned char *y_data,
+    size_t y_step,
+    const unsigned char *uv_data,
+    size_t uv_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    int dcn,
+    bool swapBlue,
+    int uIdx) {
+  return H::yuv_to_bgr_sp_ex(y_data, y_step, uv_data, uv_step, dst_data, dst_step, dst_width, dst_height, dcn, swapBlue, uIdx);
 }
 
-int kleidicv_hip_hal_yuv_to_bgr_p(const unsigned char *a, size_t b, unsigned char *c,
-                                  size_t d, int e, int f, int g, bool h, int i) {
-  return H::yuv_to_bgr_p(a, b, c, d, e, f, g, h, i);
+int kleidicv_hip_hal_yuv_to_bgr_p(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    int dcn,
+    bool swapBlue,
+    int uIdx) {
+  return H::yuv_to_bgr_p(src_data, src_step, dst_data, dst_step, dst_width, dst_height, dcn, swapBlue, uIdx);
 }
 
-int kleidicv_hip_hal_yuv_to_bgr(const unsigned char *a, size_t b, unsigned char *c, size_t d,
-                                int e, int f, int g, int h, bool i, bool j) {
-  return H::yuv_to_bgr(a, b, c, d, e, f, g, h, i, j);
+int kleidicv_hip_hal_yuv_to_bgr(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int depth,
+    int dcn,
+    bool swapBlue,
+    bool isCbCr) {
+  return H::yuv_to_bgr(src_data, src_step, dst_data, dst_step, width, height, depth, dcn, swapBlue, isCbCr);
 }
 
-int kleidicv_hip_hal_separable_filter_2d(const unsigned char *a, size_t b, int c,
-                                         unsigned char *d, size_t e, int f, int g, int h,
-                                         int i, int j, int k, int l, const unsigned char *m,
-                                         int n, const unsigned char *o, int p, int q, int r,
-                                         int s, double t, int u) {
-  return H::separable_filter_2d(a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p, q, r, s, t, u);
+int kleidicv_hip_hal_separable_filter_2d(
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_type,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_type,
+    int width,
+    int height,
+    int full_width,
+    int full_height,
+    int offset_x,
+    int offset_y,
+    const unsigned char *kernelx_data,
+    int kernelx_len,
+    const unsigned char *kernely_data,
+    int kernely_len,
+    int kernel_type,
+    int anchor_x,
+    int anchor_y,
+    double delta,
+    int borderType) {
+  return H::separable_filter_2d(src_data, src_step, src_type, dst_data, dst_step, dst_type, width, height, full_width, full_height, offset_x, offset_y, kernelx_data, kernelx_len, kernely_data, kernely_len, kernel_type, anchor_x, anchor_y, delta, borderType);
 }
 
-int kleidicv_hip_hal_gaussian_blur_binomial(const unsigned char *a, size_t b, unsigned char *c,
-                                            size_t d, int e, int f, int g, int h, size_t i,
-                                            size_t j, size_t k, size_t l, size_t m, int n) {
-  return H::gaussian_blur_binomial(a, b, c, d, e, f, g, h, i, j, k, l, m, n);
+int kleidicv_hip_hal_gaussian_blur_binomial(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int depth,
+    int cn,
+    size_t margin_left,
+    size_t margin_top,
+    size_t margin_right,
+    size_t margin_bottom,
+    size_t kernel_size,
+    int border_type) {
+  return H::gaussian_blur_binomial(src_data, src_step, dst_data, dst_step, width, height, depth, cn, margin_left, margin_top, margin_right, margin_bottom, kernel_size, border_type);
 }
 
-int kleidicv_hip_hal_gaussian_blur(const unsigned char *a, size_t b, unsigned char *c,
-                                   size_t d, int e, int f, int g, int h, size_t i, size_t j,
-                                   size_t k, size_t l, size_t m, size_t n, double o, double p,
-                                   int q) {
-  return H::gaussian_blur(a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p, q);
+int kleidicv_hip_hal_gaussian_blur(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int depth,
+    int cn,
+    size_t margin_left,
+    size_t margin_top,
+    size_t margin_right,
+    size_t margin_bottom,
+    size_t kernel_width,
+    size_t kernel_height,
+    double sigma_x,
+    double sigma_y,
+    int border_type) {
+  return H::gaussian_blur(src_data, src_step, dst_data, dst_step, width, height, depth, cn, margin_left, margin_top, margin_right, margin_bottom, kernel_width, kernel_height, sigma_x, sigma_y, border_type);
 }
 
-int kleidicv_hip_hal_resize(int a, const unsigned char *b, size_t c, int d, int e,
-                            unsigned char *f, size_t g, int h, int i, double j, double k,
-                            int l) {
-  return H::resize(a, b, c, d, e, f, g, h, i, j, k, l);
+int kleidicv_hip_hal_resize(
+    int src_type,
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_width,
+    int src_height,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    double inv_scale_x,
+    double inv_scale_y,
+    int interpolation) {
+  return H::resize(src_type, src_data, src_step, src_width, src_height, dst_data, dst_step, dst_width, dst_height, inv_scale_x, inv_scale_y, interpolation);
 }
 
-int kleidicv_hip_hal_medianBlur(const unsigned char *a, size_t b, unsigned char *c, size_t d,
-                                int e, int f, int g, int h, int i) {
-  return H::medianBlur(a, b, c, d, e, f, g, h, i);
+int kleidicv_hip_hal_medianBlur(
+    const unsigned char *src_data,
+    size_t src_step,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int width,
+    int height,
+    int depth,
+    int cn,
+    int ksize) {
+  return H::medianBlur(src_data, src_step, dst_data, dst_step, width, height, depth, cn, ksize);
 }
 
-int kleidicv_hip_hal_pyrdown(const unsigned char *a, size_t b, int c, int d, unsigned char *e,
-                             size_t f, int g, int h, int i, int j, int k) {
-  return H::pyrdown(a, b, c, d, e, f, g, h, i, j, k);
+int kleidicv_hip_hal_pyrdown(
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_width,
+    int src_height,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    int depth,
+    int cn,
+    int border_type) {
+  return H::pyrdown(src_data, src_step, src_width, src_height, dst_data, dst_step, dst_width, dst_height, depth, cn, border_type);
 }
 
-int kleidicv_hip_hal_remap_s16(int a, const unsigned char *b, size_t c, int d, int e,
-                               unsigned char *f, size_t g, int h, int i, int16_t *j, size_t k,
-                               uint16_t *l, size_t m, int n, int o, const double p[4]) {
-  return H::remap_s16(a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p);
+int kleidicv_hip_hal_remap_s16(
+    int src_type,
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_width,
+    int src_height,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    int16_t *mapx,
+    size_t mapx_step,
+    uint16_t *mapy,
+    size_t mapy_step,
+    int interpolation,
+    int border_type,
+    const double border_value_f64[4]) {
+  return H::remap_s16(src_type, src_data, src_step, src_width, src_height, dst_data, dst_step, dst_width, dst_height, mapx, mapx_step, mapy, mapy_step, interpolation, border_type, border_value_f64);
 }
 
-int kleidicv_hip_hal_remap_f32(int a, const unsigned char *b, size_t c, int d, int e,
-                               unsigned char *f, size_t g, int h, int i, const float *j,
-                               size_t k, const float *l, size_t m, int n, int o,
-                               const double p[4]) {
-  return H::remap_f32(a, b, c, d, e, f, g, h, i, j, k, l, m, n, o, p);
+int kleidicv_hip_hal_remap_f32(
+    int src_type,
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_width,
+    int src_height,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    const float *mapx,
+    size_t mapx_step,
+    const float *mapy,
+    size_t mapy_step,
+    int interpolation,
+    int border_type,
+    const double border_value[4]) {
+  return H::remap_f32(src_type, src_data, src_step, src_width, src_height, dst_data, dst_step, dst_width, dst_height, mapx, mapx_step, mapy, mapy_step, interpolation, border_type, border_value);
 }
 
-int kleidicv_hip_hal_warp_perspective(int a, const unsigned char *b, size_t c, int d, int e,
-                                      unsigned char *f, size_t g, int h, int i,
-                                      const double j[9], int k, int l, const double m[4]) {
-  return H::warp_perspective(a, b, c, d, e, f, g, h, i, j, k, l, m);
+int kleidicv_hip_hal_warp_perspective(
+    int src_type,
+    const unsigned char *src_data,
+    size_t src_step,
+    int src_width,
+    int src_height,
+    unsigned char *dst_data,
+    size_t dst_step,
+    int dst_width,
+    int dst_height,
+    const double transformation[9],
+    int interpolation,
+    int borderType,
+    const double borderValue[4]) {
+  return H::warp_perspective(src_type, src_data, src_step, src_width, src_height, dst_data, dst_step, dst_width, dst_height, transformation, interpolation, borderType, borderValue);
 }
 
 }  // extern "C"
