@@ -104,7 +104,16 @@ __global__ __launch_bounds__(kThreads) void batch_quarter_kernel(Segment s) {
   for_each_item(s, [&](const Job &job, int bx, int by) {
     const ResizeJob &rj = job.u.resize;
     const int y0 = by * job.band;
-    rsz::resize_quarter_rows<C>(rj.a, bx, y0, min(y0 + job.band, rj.a.dst_height), 1);
+    // Rows narrower than the workgroup's span take fewer threads each (a
+    // power of two), and the other threads take the band's other rows.
+    constexpr int OB = rsz::quarter_thread_bytes(C);
+    const int groups = (rj.a.dst_width * C + OB - 1) / OB;
+    int lanes = kThreads;
+    while (lanes > 1 && lanes / 2 >= groups) lanes /= 2;
+    const int rows = kThreads / lanes;
+    rsz::resize_quarter_rows<C>(rj.a, bx, threadIdx.x % lanes, lanes,
+                                y0 + static_cast<int>(threadIdx.x) / lanes,
+                                min(y0 + job.band, rj.a.dst_height), rows);
   });
 }
 

@@ -136,9 +136,10 @@ __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
 // It reads the span of column sums those outputs need with 16-byte LDS
 // loads (neighbouring lanes read neighbouring chunks: no bank conflicts) and
 // accumulates in 32 bits with compile-time tap offsets (channel count is a
-// template parameter).  Taps are at most 16 bits and column sums 16 bits, so
-// the products use the full-rate 24-bit multiplier (v_mad_u32_u24) rather
-// than the quarter-rate 32-bit one.
+// template parameter).  Taps are at most 16 bits and column sums 16 bits:
+// adjacent taps go two at a time through v_dot2_u32_u16 (their column sums
+// packed into one dword, by v_perm_b32 when more than one channel puts them
+// apart), and the last odd tap through the full-rate 24-bit multiplier.
 typedef unsigned short sep_us2 __attribute__((ext_vector_type(2)));
 
 template <int K>
@@ -201,8 +202,16 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
                                        wpair[j / 2], acc, false);
         acc += __umul24(w[K - 1], col(8 * D + b + K - 1 - R));
       } else {
+        // taps C columns apart, two at a time: one v_perm_b32 packs the two
+        // column sums (low half first), then v_dot2_u32_u16 as above
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc += __umul24(w[j], col(8 * D + b + (j - R) * C));
+        for (int j = 0; j + 1 < K; j += 2) {
+          const int i0 = 8 * D + b + (j - R) * C, i1 = i0 + C;
+          const uint32_t sel = ((i0 & 1) ? 0x0302u : 0x0100u) | (((i1 & 1) ? 0x0706u : 0x0504u) << 16);
+          const uint32_t pair = __builtin_amdgcn_perm(v[i1 >> 1], v[i0 >> 1], sel);
+          acc = __builtin_amdgcn_udot2(__builtin_bit_cast(sep_us2, pair), wpair[j / 2], acc, false);
+        }
+        acc += __umul24(w[K - 1], col(8 * D + b + (K - 1 - R) * C));
       }
       uint32_t out;
       if (MODE == kSepBinomial)

@@ -31,7 +31,8 @@ __global__ __launch_bounds__(kThreads) void resize_up2x4_kernel(ResizeArgs a) {
 
 template <int C>
 __global__ __launch_bounds__(kThreads) void resize_quarter_kernel(ResizeArgs a) {
-  resize_quarter_rows<C>(a, blockIdx.x, blockIdx.y, a.dst_height, gridDim.y);
+  resize_quarter_rows<C>(a, blockIdx.x, threadIdx.x, kThreads, blockIdx.y, a.dst_height,
+                         gridDim.y);
 }
 
 template <int MODE>

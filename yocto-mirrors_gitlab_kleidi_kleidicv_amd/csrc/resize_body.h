@@ -594,17 +594,20 @@ __host__ __device__ constexpr int quarter_thread_bytes(int channels) {
 }
 
 // Destination bytes [e0, e0 + OB) of rows y_begin, y_begin + y_step, ...
-// with e0 = (bx * kThreads + thread) * OB.  Each output byte is
-// (a + b + c + d + 2) >> 2 of its 2 x 2 source bytes.  Rows are read with
-// 16-byte loads at any alignment (gfx950 vector memory is byte-addressed in
-// unaligned mode); the last partial group of a row goes byte by byte.
+// with e0 = (bx * lanes + lane) * OB: `lanes` threads share a row (all of the
+// workgroup for wide images; fewer, with the rest on other rows, for narrow
+// ones).  Each output byte is (a + b + c + d + 2) >> 2 of its 2 x 2 source
+// bytes.  Rows are read with 16-byte loads at any alignment (gfx950 vector
+// memory is byte-addressed in unaligned mode); the last partial group of a
+// row goes byte by byte.
 template <int C>
-__device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
-                                                    int y_begin, int y_end, int y_step) {
+__device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx, int lane,
+                                                    int lanes, int y_begin, int y_end,
+                                                    int y_step) {
   constexpr int OB = quarter_thread_bytes(C), IB = 2 * OB;
   typedef uint4 __attribute__((aligned(1))) u4u;
   const int n = a.dst_width * C;
-  const int e0 = (bx * kThreads + threadIdx.x) * OB;
+  const int e0 = (bx * lanes + lane) * OB;
   if (e0 >= n) return;
   const bool whole = e0 + OB <= n;
   for (int y = y_begin; y < y_end; y += y_step) {
