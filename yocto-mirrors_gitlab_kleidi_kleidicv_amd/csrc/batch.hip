@@ -128,7 +128,11 @@ __global__ __launch_bounds__(kThreads) void batch_resize_kernel(Segment s) {
       constexpr int kMode = KIND == kRsGeneric  ? rsz::kLerpGeneric
                             : KIND == kRsExact2 ? rsz::kExact2
                                                 : rsz::kExact4;
-      rsz::resize_staged_tile<kMode>(rj.a, rj.cols, rj.rows, job.band, bx, by, smem);
+      // narrow rows: fewer threads per row, the rest on the band's other rows
+      const int quads = (rj.a.dst_width * rj.a.channels + 3) / 4;
+      int lanes = kThreads;
+      while (lanes > 1 && lanes / 2 >= quads) lanes /= 2;
+      rsz::resize_staged_tile<kMode>(rj.a, rj.cols, rj.rows, job.band, bx, by, smem, lanes);
     }
   });
 }

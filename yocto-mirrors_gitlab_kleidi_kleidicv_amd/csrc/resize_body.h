@@ -440,16 +440,20 @@ constexpr int kStagedSmem = kStageBytes + 16 + 4 * (kThreads / 64);
 
 // Destination bytes [bx*kTileE, +kTileE) of rows [by*band, +band).
 template <int MODE>
+// `lanes` threads (a power of two) share each destination row: kThreads for
+// rows of a whole tile or more; for narrower rows fewer, and the band's
+// rows are dealt to the kThreads / lanes groups of them.
 __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
                                                    const int2 *__restrict__ cols,
                                                    const int2 *__restrict__ rows,
                                                    int band, int bx, int by,
-                                                   uint8_t *smem) {
+                                                   uint8_t *smem, int lanes = kThreads) {
   uint8_t *win = smem;
   int *scratch = reinterpret_cast<int *>(smem + kStageBytes + 16);
   const int n = a.dst_width * a.channels;
   const int row_bytes = a.src_width * a.channels;
-  const int e0 = bx * kTileE + threadIdx.x * 4;
+  const int e0 = bx * kTileE + static_cast<int>(threadIdx.x % lanes) * 4;
+  const int row_lane = static_cast<int>(threadIdx.x / lanes), row_step = kThreads / lanes;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
 
   int left[4], right[4], wx[4], zero[4];
@@ -533,7 +537,7 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   const us2 wx01 = us2{static_cast<uint16_t>(wx[0]), static_cast<uint16_t>(wx[1])};
   const us2 wx23 = us2{static_cast<uint16_t>(wx[2]), static_cast<uint16_t>(wx[3])};
 
-  for (int y = y_begin; y < y_end; ++y) {
+  for (int y = y_begin + row_lane; y < y_end; y += row_step) {
     const int2 r = rows[y];
     const int top = r.x, bot = r.y & 0xFFFFFF;
     const int wy = (r.y >> 24) & 0xFF;
