@@ -62,8 +62,8 @@ struct Window {
 // Stages rows [y0-R, y0+kTileH+R) x bytes [e0-pad, e0+kTileW+pad) of the
 // border-extended image into `win` (row pitch Window<R>::kCols).  Rows are
 // mapped through the border first, so every 16-byte chunk that lies inside
-// the row is one 16-byte load (edge tiles included); only chunks that cross
-// the left or right edge are assembled byte by byte.  Each thread issues all
+// the row is one 16-byte load (edge tiles and unaligned rows included); only
+// chunks that cross the left or right edge are assembled byte by byte.  Each thread issues all
 // of its loads before its first LDS store, so their latencies overlap.
 template <int R>
 __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
@@ -74,9 +74,11 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
   constexpr int kChunks = G::kCols / 16;
   constexpr int kTotal = G::kRows * kChunks;
   constexpr int kPerThread = (kTotal + kThreads - 1) / kThreads;
+  // rows and strides of any alignment: 16-byte loads at byte addresses
+  // (gfx950 vector memory runs in unaligned mode)
+  typedef uint4 __attribute__((aligned(1))) u4u;
   const int row_bytes = width * channels;
   const int c0 = e0 - G::kPad;
-  const bool aligned = ((reinterpret_cast<uintptr_t>(src) | stride) & 15) == 0;
   uint4 v[kPerThread];
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
@@ -86,8 +88,8 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
     const int y = map_index(y0 - R + r, height, border);
     const uint8_t *srow = src + static_cast<size_t>(y) * stride;
     const int e = c0 + 16 * q;
-    if (aligned && e >= 0 && e + 16 <= row_bytes) {
-      v[k] = *reinterpret_cast<const uint4 *>(srow + e);
+    if (e >= 0 && e + 16 <= row_bytes) {
+      v[k] = *reinterpret_cast<const u4u *>(srow + e);
     } else {
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -112,8 +114,11 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
 
 __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
                                            uint32_t packed) {
+  typedef uint32_t __attribute__((aligned(1))) u32u;
   if (e + 3 < row_bytes && (reinterpret_cast<uintptr_t>(row + e) & 3) == 0) {
     store_nt(reinterpret_cast<uint32_t *>(row + e), packed);
+  } else if (e + 3 < row_bytes) {
+    *reinterpret_cast<u32u *>(row + e) = packed;  // any alignment (unaligned mode)
   } else {
 #pragma unroll
     for (int b = 0; b < 4; ++b)
@@ -225,8 +230,11 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
     const int e = e0 + 8 * tx;
+    typedef uint2 __attribute__((aligned(1))) u2u;
     if (e + 7 < row_bytes && (reinterpret_cast<uintptr_t>(out + e) & 7) == 0) {
       store_nt(reinterpret_cast<uint2 *>(out + e), make_uint2(lo, hi));
+    } else if (e + 7 < row_bytes) {
+      *reinterpret_cast<u2u *>(out + e) = make_uint2(lo, hi);  // unaligned rows
     } else {
       store_quad(out, e, row_bytes, lo);
       store_quad(out, e + 4, row_bytes, hi);

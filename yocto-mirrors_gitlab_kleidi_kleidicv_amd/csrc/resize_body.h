@@ -473,7 +473,7 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   hi = block_reduce(hi, true, scratch);
   if (hi < lo) { lo = 0; hi = 0; }
   const int lo4 = lo & ~3;
-  const int pitch = ((hi + 1 - lo4) + 3) & ~3;
+  const int pitch = ((hi + 1 - lo4) + 15) & ~15;  // whole 16-byte chunks
 
   const int y_begin = by * band;
   const int y_end = min(y_begin + band, a.dst_height);
@@ -483,23 +483,24 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   const bool staged = nrows * pitch <= kStageBytes;
 
   if (staged) {
-    const bool aligned =
-      ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
-    const int quads = pitch >> 2;
-    for (int i = threadIdx.x; i < nrows * quads; i += kThreads) {
-      const int r = i / quads, q = i - r * quads;
+    // 16-byte loads at any alignment (unaligned mode); the row's last
+    // partial chunk byte by byte
+    typedef uint4 __attribute__((aligned(1))) u4u;
+    const int chunks = pitch >> 4;
+    for (int i = threadIdx.x; i < nrows * chunks; i += kThreads) {
+      const int r = i / chunks, q = i - r * chunks;
       const uint8_t *srow = a.src + static_cast<size_t>(r_lo + r) * a.src_stride;
-      const int off = lo4 + 4 * q;
-      uint32_t v;
-      if (aligned && off + 4 <= row_bytes) {
-        v = *reinterpret_cast<const uint32_t *>(srow + off);
+      const int off = lo4 + 16 * q;
+      uint4 v;
+      if (off + 16 <= row_bytes) {
+        v = *reinterpret_cast<const u4u *>(srow + off);
       } else {
-        v = 0;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-          if (off + b < row_bytes) v |= static_cast<uint32_t>(srow[off + b]) << (8 * b);
+        uint32_t w[4] = {0, 0, 0, 0};
+        for (int b = 0; b < 16 && off + b < row_bytes; ++b)
+          w[b >> 2] |= static_cast<uint32_t>(srow[off + b]) << (8 * (b & 3));
+        v = make_uint4(w[0], w[1], w[2], w[3]);
       }
-      *reinterpret_cast<uint32_t *>(win + r * pitch + 4 * q) = v;
+      *reinterpret_cast<uint4 *>(win + r * pitch + 16 * q) = v;
     }
     __syncthreads();
   }
