@@ -173,14 +173,16 @@ __global__ __launch_bounds__(kThreads) void batch_copy_kernel(Segment s) {
     const int64_t e0 = static_cast<int64_t>(bx) * kCopyTileE + threadIdx.x * 16;
     const int y0 = by * job.band, y1 = min(y0 + job.band, a.height);
     if (e0 >= a.row_bytes) return;
-    const bool vec = e0 + 16 <= a.row_bytes &&
-                     ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride |
-                       reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+    const bool vec = e0 + 16 <= a.row_bytes;
     for (int y = y0; y < y1; ++y) {
       const uint8_t *sp = a.src + static_cast<size_t>(y) * a.src_stride + e0;
       uint8_t *dp = a.dst + static_cast<size_t>(y) * a.dst_stride + e0;
       if (vec) {
-        store_nt(reinterpret_cast<uint4 *>(dp), *reinterpret_cast<const uint4 *>(sp));
+        const uint4 v = *reinterpret_cast<const u4_any *>(sp);
+        if ((reinterpret_cast<uintptr_t>(dp) & 15) == 0)
+          store_nt(reinterpret_cast<uint4 *>(dp), v);
+        else
+          *reinterpret_cast<u4_any *>(dp) = v;
       } else {
         for (int k = 0; k < 16 && e0 + k < a.row_bytes; ++k) dp[k] = sp[k];
       }

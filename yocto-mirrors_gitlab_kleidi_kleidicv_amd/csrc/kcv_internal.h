@@ -52,6 +52,13 @@ __device__ __forceinline__ void store_nt(uint4 *p, uint4 v) {
   __builtin_nontemporal_store(u32x4{v.x, v.y, v.z, v.w}, reinterpret_cast<u32x4 *>(p));
 }
 
+// Vector accesses at any byte address (gfx950 vector memory runs in
+// unaligned mode): rows whose stride is not a multiple of the vector size
+// still move in 4-, 8- or 16-byte accesses instead of bytes.
+typedef uint32_t __attribute__((aligned(1))) u32_any;
+typedef uint2 __attribute__((aligned(1))) u2_any;
+typedef uint4 __attribute__((aligned(1))) u4_any;
+
 // Destination tile of the gather kernels (remap s16point5, remap f32, warp
 // perspective), per-image and batched.
 constexpr int kGatherTileW = 64, kGatherTileH = 16;  // 4 pixels per thread

@@ -80,7 +80,16 @@ __device__ __forceinline__ void bt601_row8(uint32_t ylo, uint32_t yhi, const Chr
 template <int OC>
 __device__ __forceinline__ void store_px8(uint8_t *out, const uint32_t px[kBlockW], bool vec,
                                           int count) {
-  if (vec) {
+  const bool al = (reinterpret_cast<uintptr_t>(out) & (OC == 4 ? 15 : 7)) == 0;
+  if (vec && !al) {
+    // any alignment: plain (not non-temporal) vector stores
+    if (OC == 4) {
+      reinterpret_cast<u4_any *>(out)[0] = make_uint4(px[0], px[1], px[2], px[3]);
+      reinterpret_cast<u4_any *>(out)[1] = make_uint4(px[4], px[5], px[6], px[7]);
+      return;
+    }
+  }
+  if (vec && (al || OC == 3)) {
     if (OC == 4) {
       store_nt(reinterpret_cast<uint4 *>(out), make_uint4(px[0], px[1], px[2], px[3]));
       store_nt(reinterpret_cast<uint4 *>(out) + 1, make_uint4(px[4], px[5], px[6], px[7]));
@@ -96,10 +105,17 @@ __device__ __forceinline__ void store_px8(uint8_t *out, const uint32_t px[kBlock
           const int byte = 3 * k + c;
           w[byte >> 2] |= ((px[k] >> (8 * c)) & 0xFF) << (8 * (byte & 3));
         }
-      uint2 *o = reinterpret_cast<uint2 *>(out);
-      store_nt(o, make_uint2(w[0], w[1]));
-      store_nt(o + 1, make_uint2(w[2], w[3]));
-      store_nt(o + 2, make_uint2(w[4], w[5]));
+      if (al) {
+        uint2 *o = reinterpret_cast<uint2 *>(out);
+        store_nt(o, make_uint2(w[0], w[1]));
+        store_nt(o + 1, make_uint2(w[2], w[3]));
+        store_nt(o + 2, make_uint2(w[4], w[5]));
+      } else {
+        u2_any *o = reinterpret_cast<u2_any *>(out);
+        o[0] = make_uint2(w[0], w[1]);
+        o[1] = make_uint2(w[2], w[3]);
+        o[2] = make_uint2(w[4], w[5]);
+      }
     }
   } else {
     for (int k = 0; k < count; ++k)
@@ -116,21 +132,17 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
   const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
   if (x0 >= a.width) return;
   const bool full = x0 + kBlockW <= a.width;
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.y) | a.y_stride |
-                       reinterpret_cast<uintptr_t>(a.uv) | a.uv_stride) & 7) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) &
-                       (OC == 4 ? 15 : 7)) == 0;
   const int pairs = min(yp_end, (a.height + 1) / 2);
   for (int yp = yp_begin; yp < pairs; yp += yp_step) {
     const uint8_t *uvrow = a.uv + static_cast<size_t>(yp) * a.uv_stride;
     const bool two = 2 * yp + 1 < a.height;
     // all loads first: the 4 chroma pairs and 8 luma bytes of both rows
     uint32_t uv_lo = 0, uv_hi = 0, yw_lo[2] = {0, 0}, yw_hi[2] = {0, 0};
-    if (full && in_al) {
+    if (full) {
       const uint8_t *y0row = a.y + static_cast<size_t>(2 * yp) * a.y_stride + x0;
-      const uint2 q = *reinterpret_cast<const uint2 *>(uvrow + x0);
-      const uint2 q0 = *reinterpret_cast<const uint2 *>(y0row);
-      const uint2 q1 = two ? *reinterpret_cast<const uint2 *>(y0row + a.y_stride) : q0;
+      const uint2 q = *reinterpret_cast<const u2_any *>(uvrow + x0);
+      const uint2 q0 = *reinterpret_cast<const u2_any *>(y0row);
+      const uint2 q1 = two ? *reinterpret_cast<const u2_any *>(y0row + a.y_stride) : q0;
       uv_lo = q.x;
       uv_hi = q.y;
       yw_lo[0] = q0.x;
@@ -159,7 +171,7 @@ __device__ __forceinline__ void nv12_rows(const YuvArgs &a, int bx, int yp_begin
       uint32_t px[kBlockW];
       bt601_row8<OC>(yw_lo[r], yw_hi[r], t, a.bgr, px);
       uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x0) * OC;
-      store_px8<OC>(out, px, full && out_al, min(kBlockW, a.width - x0));
+      store_px8<OC>(out, px, full, min(kBlockW, a.width - x0));
     }
   }
 }
@@ -204,9 +216,6 @@ __device__ __forceinline__ void yuv420p_rows(const YuvXArgs &a, int bx, int yp_b
   const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
   if (x0 >= a.width) return;
   const bool full = x0 + kBlockW <= a.width;
-  const bool y_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) &
-                       (OC == 4 ? 15 : 7)) == 0;
   const int pairs = (a.height + 1) / 2;
   const int nchroma = (min(a.width - x0, kBlockW) + 1) / 2;
   for (int yp = yp_begin; yp < pairs; yp += yp_step) {
@@ -217,9 +226,9 @@ __device__ __forceinline__ void yuv420p_rows(const YuvXArgs &a, int bx, int yp_b
     const uint32_t vw = chroma_word(a.src, vrow, a.src_extent, nchroma);
     uint32_t yw_lo[2] = {0, 0}, yw_hi[2] = {0, 0};
     const uint8_t *y0row = a.src + static_cast<size_t>(2 * yp) * a.src_stride + x0;
-    if (full && y_al) {
-      const uint2 q0 = *reinterpret_cast<const uint2 *>(y0row);
-      const uint2 q1 = two ? *reinterpret_cast<const uint2 *>(y0row + a.src_stride) : q0;
+    if (full) {
+      const uint2 q0 = *reinterpret_cast<const u2_any *>(y0row);
+      const uint2 q1 = two ? *reinterpret_cast<const u2_any *>(y0row + a.src_stride) : q0;
       yw_lo[0] = q0.x;
       yw_hi[0] = q0.y;
       yw_lo[1] = q1.x;
@@ -244,7 +253,7 @@ __device__ __forceinline__ void yuv420p_rows(const YuvXArgs &a, int bx, int yp_b
       uint32_t px[kBlockW];
       bt601_row8<OC>(yw_lo[r], yw_hi[r], t, a.bgr, px);
       uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x0) * OC;
-      store_px8<OC>(out, px, full && out_al, min(kBlockW, a.width - x0));
+      store_px8<OC>(out, px, full, min(kBlockW, a.width - x0));
     }
   }
 }
@@ -258,14 +267,11 @@ __device__ __forceinline__ void yuv422_rows(const YuvXArgs &a, int bx, int y_beg
   const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
   if (x0 >= a.width) return;
   const bool full = x0 + kBlockW <= a.width;  // width is even: pairs are whole
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 15) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) &
-                       (OC == 4 ? 15 : 7)) == 0;
   for (int y = y_begin; y < a.height; y += y_step) {
     const uint8_t *row = a.src + static_cast<size_t>(y) * a.src_stride + 2 * x0;
     uint32_t q[4] = {0, 0, 0, 0};
-    if (full && in_al) {
-      const uint4 v = *reinterpret_cast<const uint4 *>(row);
+    if (full) {
+      const uint4 v = *reinterpret_cast<const u4_any *>(row);
       q[0] = v.x;
       q[1] = v.y;
       q[2] = v.z;
@@ -294,7 +300,7 @@ __device__ __forceinline__ void yuv422_rows(const YuvXArgs &a, int bx, int y_beg
     uint32_t px[kBlockW];
     bt601_row8<OC>(ylo, yhi, t, a.bgr, px);
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x0) * OC;
-    store_px8<OC>(out, px, full && out_al, min(kBlockW, a.width - x0));
+    store_px8<OC>(out, px, full, min(kBlockW, a.width - x0));
   }
 }
 
@@ -311,14 +317,11 @@ __device__ __forceinline__ void yuv444_rows(const YuvXArgs &a, int bx, int y_beg
   const int x0 = (bx * kThreads + threadIdx.x) * kBlockW;
   if (x0 >= a.width) return;
   const bool full = x0 + kBlockW <= a.width;
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) &
-                       (OC == 4 ? 15 : 7)) == 0;
   for (int y = y_begin; y < a.height; y += y_step) {
     const uint8_t *row = a.src + static_cast<size_t>(y) * a.src_stride + 3 * x0;
     uint32_t w[6] = {0, 0, 0, 0, 0, 0};
-    if (full && in_al) {
-      const uint2 *p = reinterpret_cast<const uint2 *>(row);
+    if (full) {
+      const u2_any *p = reinterpret_cast<const u2_any *>(row);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
         const uint2 v = p[m];
@@ -339,7 +342,7 @@ __device__ __forceinline__ void yuv444_rows(const YuvXArgs &a, int bx, int y_beg
                       sat_u8(yv + ((33292 * u + (1 << 13)) >> 14)), a.bgr, OC == 4);
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x0) * OC;
-    store_px8<OC>(out, px, full && out_al, min(kBlockW, a.width - x0));
+    store_px8<OC>(out, px, full, min(kBlockW, a.width - x0));
   }
 }
 
@@ -358,7 +361,7 @@ __device__ __forceinline__ void load_px8(const uint8_t *p, bool vec, int count,
 #pragma unroll
   for (int i = 0; i < 2 * SCN; ++i) w[i] = 0;
   if (vec) {
-    const uint2 *q = reinterpret_cast<const uint2 *>(p);
+    const u2_any *q = reinterpret_cast<const u2_any *>(p);
 #pragma unroll
     for (int i = 0; i < SCN; ++i) {
       const uint2 v = q[i];
@@ -391,7 +394,10 @@ __device__ __forceinline__ Rgb8 split_px8(const uint32_t w[2 * SCN], bool bgr) {
 // Stores `n` bytes of the words w (byte i of the run is byte i & 3 of w[i >> 2]).
 __device__ __forceinline__ void store_bytes(uint8_t *out, const uint32_t *w, int n, bool vec8) {
   if (vec8) {
-    store_nt(reinterpret_cast<uint2 *>(out), make_uint2(w[0], w[1]));
+    if ((reinterpret_cast<uintptr_t>(out) & 7) == 0)
+      store_nt(reinterpret_cast<uint2 *>(out), make_uint2(w[0], w[1]));
+    else
+      *reinterpret_cast<u2_any *>(out) = make_uint2(w[0], w[1]);
     return;
   }
   for (int i = 0; i < n; ++i) out[i] = static_cast<uint8_t>(w[i >> 2] >> (8 * (i & 3)));
@@ -404,8 +410,6 @@ __device__ __forceinline__ void rgb_to_yuv420_rows(const RgbYuvArgs &a, int bx, 
   if (x0 >= a.width) return;
   const int count = min(kBlockW, a.width - x0);
   const bool full = count == kBlockW;
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
-  const bool y_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 7) == 0;
   const int pairs = (a.height + 1) / 2;
   const int nchroma = (count + 1) / 2;
   const size_t cw = static_cast<size_t>((a.width + 1) / 2);
@@ -416,7 +420,7 @@ __device__ __forceinline__ void rgb_to_yuv420_rows(const RgbYuvArgs &a, int bx, 
       if (y >= a.height) break;
       uint32_t w[2 * SCN];
       load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
-                    full && in_al, count, w);
+                    full, count, w);
       const Rgb8 p = split_px8<SCN>(w, a.bgr);
       uint32_t yw[2] = {0, 0};
 #pragma unroll
@@ -425,7 +429,7 @@ __device__ __forceinline__ void rgb_to_yuv420_rows(const RgbYuvArgs &a, int bx, 
                                                  102760 * p.b[k] + (1 << 19) + (16 << 20)) >> 20;
         yw[k >> 2] |= (Y & 0xFF) << (8 * (k & 3));
       }
-      store_bytes(a.dst + static_cast<size_t>(y) * a.dst_stride + x0, yw, count, full && y_al);
+      store_bytes(a.dst + static_cast<size_t>(y) * a.dst_stride + x0, yw, count, full);
       if (r == 1) continue;
       // chroma of the block's top-left pixels 0, 2, 4, 6
       uint32_t uw = 0, vw = 0;
@@ -452,8 +456,7 @@ __device__ __forceinline__ void rgb_to_yuv420_rows(const RgbYuvArgs &a, int bx, 
                             ((uw >> 8) & 0xFF0000u) | (vw & 0xFF000000u);
         const uint32_t uv[2] = {lo, hi};
         uint8_t *out = a.uv + static_cast<size_t>(yp) * a.uv_stride + x0;
-        const bool al = ((reinterpret_cast<uintptr_t>(a.uv) | a.uv_stride) & 7) == 0;
-        store_bytes(out, uv, 2 * nchroma, nchroma == 4 && al);
+        store_bytes(out, uv, 2 * nchroma, nchroma == 4);
       } else {
         // planar: U row h/4 of the chroma area (+ cw when h/2 is odd), V row
         // (h + height + 1)/4 (+ cw when (h + height + 1)/2 is odd), h = 2 yp
@@ -483,12 +486,10 @@ __device__ __forceinline__ void rgb_to_yuv422_rows(const RgbYuvArgs &a, int bx, 
   if (x0 >= a.width) return;
   const int count = min(kBlockW, a.width - x0);  // even: width is even
   const bool full = count == kBlockW;
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
   for (int y = y_begin; y < a.height; y += y_step) {
     uint32_t w[2 * SCN];
     load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
-                  full && in_al, count, w);
+                  full, count, w);
     const Rgb8 p = split_px8<SCN>(w, a.bgr);
     uint32_t q[4];
 #pragma unroll
@@ -505,8 +506,10 @@ __device__ __forceinline__ void rgb_to_yuv422_rows(const RgbYuvArgs &a, int bx, 
              ((u & 0xFF) << (8 * a.u_off)) | ((v & 0xFF) << (8 * a.v_off));
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 2 * x0;
-    if (full && out_al)
+    if (full && (reinterpret_cast<uintptr_t>(out) & 15) == 0)
       store_nt(reinterpret_cast<uint4 *>(out), make_uint4(q[0], q[1], q[2], q[3]));
+    else if (full)
+      *reinterpret_cast<u4_any *>(out) = make_uint4(q[0], q[1], q[2], q[3]);
     else
       store_bytes(out, q, 2 * count, false);
   }
@@ -519,12 +522,10 @@ __device__ __forceinline__ void rgb_to_yuv444_rows(const RgbYuvArgs &a, int bx, 
   if (x0 >= a.width) return;
   const int count = min(kBlockW, a.width - x0);
   const bool full = count == kBlockW;
-  const bool in_al = ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 7) == 0;
-  const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 7) == 0;
   for (int y = y_begin; y < a.height; y += y_step) {
     uint32_t w[2 * SCN];
     load_px8<SCN>(a.src + static_cast<size_t>(y) * a.src_stride + static_cast<size_t>(x0) * SCN,
-                  full && in_al, count, w);
+                  full, count, w);
     const Rgb8 p = split_px8<SCN>(w, a.bgr);
     uint32_t o[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -537,11 +538,16 @@ __device__ __forceinline__ void rgb_to_yuv444_rows(const RgbYuvArgs &a, int bx, 
       for (int j = 0; j < 3; ++j) o[(3 * k + j) >> 2] |= c[j] << (8 * ((3 * k + j) & 3));
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 3 * x0;
-    if (full && out_al) {
+    if (full && (reinterpret_cast<uintptr_t>(out) & 7) == 0) {
       uint2 *q = reinterpret_cast<uint2 *>(out);
       store_nt(q, make_uint2(o[0], o[1]));
       store_nt(q + 1, make_uint2(o[2], o[3]));
       store_nt(q + 2, make_uint2(o[4], o[5]));
+    } else if (full) {
+      u2_any *q = reinterpret_cast<u2_any *>(out);
+      q[0] = make_uint2(o[0], o[1]);
+      q[1] = make_uint2(o[2], o[3]);
+      q[2] = make_uint2(o[4], o[5]);
     } else {
       store_bytes(out, o, 3 * count, false);
     }
