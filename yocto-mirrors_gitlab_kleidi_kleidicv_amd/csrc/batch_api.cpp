@@ -233,8 +233,10 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
         j->tiles_x = static_cast<int>(div_up(a.src_width, kUp2x4TilePx));
         return j->tiles_x * (int64_t{a.src_height} + 1);
       } else {
-        j->band = t.resize_band_rows ? static_cast<int>(t.resize_band_rows)
-                                     : rsz::band_rows(a);
+        // no taller than the band whose source window fits the LDS stage:
+        // taller bands fall back to per-byte global reads
+        j->band = rsz::band_rows(a);
+        if (t.resize_band_rows) j->band = std::min(j->band, static_cast<int>(t.resize_band_rows));
         j->tiles_x = static_cast<int>(div_up(n, kResizeTileE));
       }
       return j->tiles_x * div_up(a.dst_height, j->band);

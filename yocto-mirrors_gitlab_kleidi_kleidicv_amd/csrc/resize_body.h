@@ -433,9 +433,9 @@ __device__ __forceinline__ int block_reduce(int v, bool is_max, int *scratch) {
   return r;
 }
 
-// LDS of one staged tile: the window plus 8 spare bytes (the 8-byte reads of
-// the narrow path may start in the window's last dword) and the reduction
-// scratch.
+// LDS of one staged tile: the window plus 16 spare bytes (the 8- and 16-byte
+// reads of the narrow and wide paths may start in the window's last dwords)
+// and the reduction scratch.
 constexpr int kStagedSmem = kStageBytes + 16 + 4 * (kThreads / 64);
 
 // Destination bytes [bx*kTileE, +kTileE) of rows [by*band, +band).
@@ -522,6 +522,10 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   }
   const int tbase = tmin & ~3;
   const bool narrow = staged && tmax >= 0 && tmax - tbase < 8;
+  // Taps within 16 bytes (2:1 downsizing of 1 and 3 channels): four dword
+  // reads per row; each tap pair is picked from the low or the high 8 bytes
+  // by two v_perm_b32 whose other selectors read zero, then OR-ed.
+  const bool wide = staged && !narrow && tmax >= 0 && tmax - tbase < 16;
   // Byte selectors for v_perm_b32 over the 8-byte window {hi, lo}: two taps
   // per selector land in the low bytes of a 16-bit pair; 0x0C selects zero,
   // which also realises the vector code's "index beyond the load reads 0".
@@ -535,6 +539,28 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   };
   const uint32_t sel_l01 = sel(left, 1, 0), sel_l23 = sel(left, 1, 2);
   const uint32_t sel_r01 = sel(right, 2, 0), sel_r23 = sel(right, 2, 2);
+  // wide form: selectors over {t1, t0} (upper = false) and {t3, t2} (upper)
+  auto sel_half = [&](const int *idx, int flag, int k0, bool upper) {
+    uint32_t v = 0x0C000C00u;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k = k0 + h;
+      const int o = (zero[k] & flag) ? -1 : idx[k] - tbase;
+      const bool mine = o >= 0 && (upper ? o >= 8 : o < 8);
+      v |= static_cast<uint32_t>(mine ? (upper ? o - 8 : o) : 0x0C) << (16 * h);
+    }
+    return v;
+  };
+  uint32_t wsel[4][2];  // l01, l23, r01, r23 x {low 8 bytes, high 8 bytes}
+  if (wide) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      wsel[0][u] = sel_half(left, 1, 0, u);
+      wsel[1][u] = sel_half(left, 1, 2, u);
+      wsel[2][u] = sel_half(right, 2, 0, u);
+      wsel[3][u] = sel_half(right, 2, 2, u);
+    }
+  }
   const us2 wx01 = us2{static_cast<uint16_t>(wx[0]), static_cast<uint16_t>(wx[1])};
   const us2 wx23 = us2{static_cast<uint16_t>(wx[2]), static_cast<uint16_t>(wx[3])};
 
@@ -557,6 +583,21 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
                                      as_us2(__builtin_amdgcn_perm(b1, b0, sel_l23)),
                                      as_us2(__builtin_amdgcn_perm(b1, b0, sel_r23)), wx23, wy2);
       // bytes 0 and 2 of each pair word -> one dword
+      packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
+    } else if (wide) {
+      const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
+      const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
+      const uint32_t t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
+      const uint32_t b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
+      auto pick = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, int s_) {
+        return as_us2(__builtin_amdgcn_perm(x1, x0, wsel[s_][0]) |
+                      __builtin_amdgcn_perm(x3, x2, wsel[s_][1]));
+      };
+      const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
+      const us2 r01 = combine2<MODE>(pick(t0, t1, t2, t3, 0), pick(t0, t1, t2, t3, 2),
+                                     pick(b0, b1, b2, b3, 0), pick(b0, b1, b2, b3, 2), wx01, wy2);
+      const us2 r23 = combine2<MODE>(pick(t0, t1, t2, t3, 1), pick(t0, t1, t2, t3, 3),
+                                     pick(b0, b1, b2, b3, 1), pick(b0, b1, b2, b3, 3), wx23, wy2);
       packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
     } else if (staged) {
       const uint8_t *t = win + (top - r_lo) * pitch;
