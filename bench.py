@@ -312,43 +312,84 @@ def host_api(K, np, s_host, kx, ky, T, ops, reps=3):
 
 
 def cpu_baseline(np, s_host, kx, ky, T, ops, min_seconds=10.0, max_passes=4):
-    """Single-threaded oracle on the host cores: full 4K passes of the op set,
-    repeated until about min_seconds of CPU work (at most max_passes)."""
+    """The C++ oracle on the host cores, each op split into row stripes run by
+    a thread pool (the stripe scheduling of kleidicv_thread.cpp:60-75 with one
+    stripe per thread): full 4K passes of the op set, repeated until about
+    min_seconds of wall time (at most max_passes)."""
     import oracle_lib as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    calls = _cpu_calls(O, np, s_host, kx, ky, T)
     t_total, pix, passes = 0.0, 0, 0
-    while passes < max_passes and (passes == 0 or t_total < min_seconds):
-        passes += 1
-        t, p = _cpu_pass(O, np, s_host, kx, ky, T, ops)
-        t_total += t
-        pix += p
-    return {"value": round(pix / t_total / 1e6, 2), "unit": "Mpix/s", "cores": 1,
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        while passes < max_passes and (passes == 0 or t_total < min_seconds):
+            passes += 1
+            for name, dst_pix, _ in ops:
+                rows, fn = calls[name]
+                bands = [(rows * i // threads, rows * (i + 1) // threads) for i in range(threads)]
+                t0 = time.perf_counter()
+                errs = list(pool.map(lambda b: fn(*b), [b for b in bands if b[1] > b[0]]))
+                t_total += time.perf_counter() - t0
+                assert all(e == 0 for e in errs), (name, errs)
+                pix += dst_pix
+    return {"value": round(pix / t_total / 1e6, 2), "unit": "Mpix/s", "cores": threads,
             "kind": "port",
-            "sample": "%d full 3840x2160 pass(es) of each of the %d ops, scalar C++ oracle, "
-                      "1 thread (%.1f s)" % (passes, len(ops), t_total)}
+            "sample": "%d full 3840x2160 pass(es) of each of the %d ops, C++ oracle in %d "
+                      "row stripes on %d threads (%.1f s)" % (passes, len(ops), threads,
+                                                              threads, t_total)}
 
 
-def _cpu_pass(O, np, s_host, kx, ky, T, ops):
-    t_total, pix = 0.0, 0
-    for name, dst_pix, _ in ops:
-        t0 = time.perf_counter()
-        if name == "separable_7x7_1ch":
-            O.separable_filter_2d(s_host["sep_src"], kx, ky, "REPLICATE")
-        elif name == "median_5x5_1ch":
-            O.median_blur(s_host["med_src"], 5, "REPLICATE")
-        elif name == "resize_2x_up_4ch":
-            O.resize_linear(s_host["up_src"], 2 * W, 2 * H)
-        elif name == "resize_0.5x_4ch":
-            O.resize_linear(s_host["down_src"], W // 2, H // 2)
-        elif name == "remap_s16point5_1ch":
-            O.remap_s16point5(s_host["remap_src"], s_host["mapxy"],
-                              s_host["mapfrac"].view(np.uint16), "CONSTANT")
-        elif name == "warp_perspective_4ch":
-            O.warp_perspective(s_host["warp_src"], T, W, H, "LINEAR", "REPLICATE")
-        elif name == "nv12_to_rgb":
-            O.yuv_sp_to_rgb(s_host["nv12_y"], s_host["nv12_uv"], 0x02)
-        t_total += time.perf_counter() - t0
-        pix += dst_pix
-    return t_total, pix
+def _cpu_calls(O, np, s, kx, ky, T):
+    """Per op: (rows to stripe, fn(y_begin, y_end) -> error) over the oracle's
+    row-stripe entry points, or row-offset pointers where an op's output rows
+    are independent (remap, NV12 row pairs)."""
+    import ctypes as C
+    L = O.lib()
+    sz = C.c_size_t
+
+    def P(a, off=0):
+        return C.c_void_p(a.ctypes.data + off)
+    kxa = np.ascontiguousarray(kx, np.uint8)
+    kya = np.ascontiguousarray(ky, np.uint8)
+    sep_src, med_src = s["sep_src"], s["med_src"]
+    sep_dst, med_dst = np.empty_like(sep_src), np.empty_like(med_src)
+    up_src, down_src = s["up_src"], s["down_src"]
+    up_dst = np.empty((2 * H, 2 * W, 4), np.uint8)
+    down_dst = np.empty((H // 2, W // 2, 4), np.uint8)
+    rsrc, mxy, mfr = s["remap_src"], s["mapxy"], s["mapfrac"].view(np.uint16)
+    rdst = np.empty((H, W), np.uint8)
+    bv0 = np.zeros(4, np.uint8)
+    wsrc = s["warp_src"]
+    wdst = np.empty((H, W, 4), np.uint8)
+    t32 = np.ascontiguousarray(T, np.float32)
+    ny, nuv = s["nv12_y"], s["nv12_uv"]
+    ndst = np.empty((H, W, 3), np.uint8)
+    return {
+        "separable_7x7_1ch": (H, lambda a, b: L.orc_separable_filter_2d_u8_stripe(
+            P(sep_src), sz(W), P(sep_dst), sz(W), sz(W), sz(H), sz(1), P(kxa), sz(len(kxa)),
+            P(kya), sz(len(kya)), C.c_int(O.BORDER["REPLICATE"]), sz(a), sz(b))),
+        "median_5x5_1ch": (H, lambda a, b: L.orc_median_blur_u8_stripe(
+            P(med_src), sz(W), P(med_dst), sz(W), sz(W), sz(H), sz(1), sz(5),
+            C.c_int(O.BORDER["REPLICATE"]), sz(a), sz(b))),
+        "resize_2x_up_4ch": (2 * H, lambda a, b: L.orc_resize_linear_u8_stripe(
+            P(up_src), sz(4 * W), sz(W), sz(H), P(up_dst), sz(8 * W), sz(2 * W), sz(2 * H),
+            sz(4), sz(a), sz(b))),
+        "resize_0.5x_4ch": (H // 2, lambda a, b: L.orc_resize_linear_u8_stripe(
+            P(down_src), sz(4 * W), sz(W), sz(H), P(down_dst), sz(2 * W), sz(W // 2),
+            sz(H // 2), sz(4), sz(a), sz(b))),
+        "remap_s16point5_1ch": (H, lambda a, b: L.orc_remap_s16point5_u8(
+            P(rsrc), sz(W), sz(W), sz(H), P(rdst, a * W), sz(W), sz(W), sz(b - a), sz(1),
+            P(mxy, a * 4 * W), sz(4 * W), P(mfr, a * 2 * W), sz(2 * W),
+            C.c_int(O.BORDER["CONSTANT"]), P(bv0))),
+        "warp_perspective_4ch": (H, lambda a, b: L.orc_warp_perspective_u8_stripe(
+            P(wsrc), sz(4 * W), sz(W), sz(H), P(wdst), sz(4 * W), sz(W), sz(H), P(t32), sz(4),
+            C.c_int(O.INTERP["LINEAR"]), C.c_int(O.BORDER["REPLICATE"]), P(bv0), sz(a),
+            sz(b))),
+        # row pairs: a stripe of pairs [a, b) is luma rows [2a, 2b)
+        "nv12_to_rgb": (H // 2, lambda a, b: L.orc_yuv_semiplanar_to_rgb_u8(
+            P(ny, 2 * a * W), sz(W), P(nuv, a * W), sz(W), P(ndst, 2 * a * 3 * W), sz(3 * W),
+            sz(W), sz(2 * (b - a)), C.c_int(0x02))),
+    }
 
 
 def bench_batched(K, torch, WL, dev, stream, steps, warmup):

@@ -724,15 +724,16 @@ static bool resize_implemented(size_t sw, size_t sh, size_t dw, size_t dh,
 // generic table pinned against the direct-coordinate formula within the
 // reference unit test's xfrac tolerance (test_resize_linear_generic_u8.cpp:
 // 19-133, 339, 367-388) and bit-exact for exact ratios.
-extern "C" int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
-                                    size_t sw, size_t sh, uint8_t *dst,
-                                    size_t dst_stride, size_t dw, size_t dh,
-                                    size_t C) {
+extern "C" int orc_resize_linear_u8_stripe(const uint8_t *src, size_t src_stride,
+                                           size_t sw, size_t sh, uint8_t *dst,
+                                           size_t dst_stride, size_t dw, size_t dh,
+                                           size_t C, size_t y_begin, size_t y_end) {
   if (!resize_implemented(sw, sh, dw, dh, C)) return kErrNotImplemented;
   if (!src || !dst) return kErrNull;
   if (sw == 0 || sh == 0) return kErrOk;
+  y_end = std::min(y_end, dh);
   if (sw == 2 * dw && sh == 2 * dh) {
-    for (size_t y = 0; y < dh; ++y)
+    for (size_t y = y_begin; y < y_end; ++y)
       for (size_t x = 0; x < dw; ++x)
         for (size_t c = 0; c < C; ++c) {
           const uint8_t *r0 = src + 2 * y * src_stride;
@@ -757,7 +758,7 @@ extern "C" int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
       if (s + 1 > n - 1) { i1 = i0; }
     };
     const int shift = f == 2 ? 4 : 6;
-    for (long y = 0; y < static_cast<long>(dh); ++y) {
+    for (long y = static_cast<long>(y_begin); y < static_cast<long>(y_end); ++y) {
       long y0, y1; int wy0, wy1;
       taps(y, static_cast<long>(sh), y0, y1, wy0, wy1);
       for (long x = 0; x < static_cast<long>(dw); ++x) {
@@ -773,7 +774,7 @@ extern "C" int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
     return kErrOk;
   }
   std::vector<Lane> cols = generic_columns(sw, dw, C, plan_generic(sw, dw, C));
-  for (size_t dy = 0; dy < dh; ++dy) {
+  for (size_t dy = y_begin; dy < y_end; ++dy) {
     int64_t sy_fixp = aligned_scale(dy, sh, dh);
     int64_t sy = sy_fixp >> kFixpBits;
     const int64_t max_sy = static_cast<int64_t>(sh) - 1;
@@ -798,6 +799,14 @@ extern "C" int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
     }
   }
   return kErrOk;
+}
+
+extern "C" int orc_resize_linear_u8(const uint8_t *src, size_t src_stride,
+                                    size_t sw, size_t sh, uint8_t *dst,
+                                    size_t dst_stride, size_t dw, size_t dh,
+                                    size_t C) {
+  return orc_resize_linear_u8_stripe(src, src_stride, sw, sh, dst, dst_stride, dw, dh, C, 0,
+                                     dh);
 }
 
 // ---------------------------------------------------------------------------
@@ -993,12 +1002,13 @@ extern "C" int orc_remap_f32_u8(const uint8_t *src, size_t src_stride,
 // Channels 2..4: per-plane extension of the 1-channel semantics.
 // Parity: pinned by the SimpleResize KAT (test_warp_perspective.cpp:700-742,
 // exact) and within +-1 of a double-precision reference (:239-300).
-extern "C" int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
-                                       size_t sw, size_t sh, uint8_t *dst,
-                                       size_t dst_stride, size_t dw, size_t dh,
-                                       const float T[9], size_t C,
-                                       int interpolation, int border,
-                                       const uint8_t *border_value) {
+extern "C" int orc_warp_perspective_u8_stripe(const uint8_t *src, size_t src_stride,
+                                              size_t sw, size_t sh, uint8_t *dst,
+                                              size_t dst_stride, size_t dw, size_t dh,
+                                              const float T[9], size_t C,
+                                              int interpolation, int border,
+                                              const uint8_t *border_value, size_t y_begin,
+                                              size_t y_end) {
   if (!(dw >= 8 && (interpolation == 0 || interpolation == 1) &&
         (border == kBorderReplicate || border == kBorderConstant) && C >= 1 &&
         C <= 4))
@@ -1025,7 +1035,7 @@ extern "C" int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
     mxw = std::max(w, mxw);
   }
   const bool check_zero = mn * mxw <= 0.0f;
-  for (size_t y = 0; y < dh; ++y) {
+  for (size_t y = y_begin; y < std::min(y_end, dh); ++y) {
     float fy = static_cast<float>(y);
     float tx0 = std::fmaf(T[1], fy, T[2]);
     float ty0 = std::fmaf(T[4], fy, T[5]);
@@ -1044,6 +1054,16 @@ extern "C" int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
     }
   }
   return kErrOk;
+}
+
+extern "C" int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
+                                       size_t sw, size_t sh, uint8_t *dst,
+                                       size_t dst_stride, size_t dw, size_t dh,
+                                       const float T[9], size_t C,
+                                       int interpolation, int border,
+                                       const uint8_t *border_value) {
+  return orc_warp_perspective_u8_stripe(src, src_stride, sw, sh, dst, dst_stride, dw, dh, T, C,
+                                        interpolation, border, border_value, 0, dh);
 }
 
 // ---------------------------------------------------------------------------

@@ -144,6 +144,20 @@ int orc_separable_filter_2d_u8_stripe(
     size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
     int border, size_t y_begin, size_t y_end);
 
+int orc_resize_linear_u8_stripe(const uint8_t *src, size_t src_stride,
+                                size_t src_width, size_t src_height,
+                                uint8_t *dst, size_t dst_stride,
+                                size_t dst_width, size_t dst_height,
+                                size_t channels, size_t y_begin, size_t y_end);
+int orc_warp_perspective_u8_stripe(const uint8_t *src, size_t src_stride,
+                                   size_t src_width, size_t src_height,
+                                   uint8_t *dst, size_t dst_stride,
+                                   size_t dst_width, size_t dst_height,
+                                   const float transformation[9],
+                                   size_t channels, int interpolation,
+                                   int border, const uint8_t *border_value,
+                                   size_t y_begin, size_t y_end);
+
 #ifdef __cplusplus
 }
 #endif

@@ -490,3 +490,29 @@ def test_rgb_to_yuv_routing_errors():
     assert O.rgb_to_yuv(src, 0x02)[0] == 1            # NV12 through the one-buffer call
     assert O.rgb_to_yuv(src, 0x83)[0] == 1
     assert O.rgb_to_yuv_semiplanar(src, 0x03)[0] == 1
+
+
+def test_row_stripes_equal_whole_calls(rng):
+    # the stripe entry points the CPU baseline schedules: every band of
+    # destination rows equals the same rows of the whole-image call
+    L = O.lib()
+    sz = C.c_size_t
+    src = rng.integers(0, 256, (41, 57, 4), dtype=np.uint8)
+    for dw, dh in ((114, 82), (28, 20), (43, 29)):
+        _, whole = O.resize_linear(src, dw, dh)
+        got = np.zeros_like(whole)
+        for a, b in ((0, 5), (5, 6), (6, dh)):
+            assert L.orc_resize_linear_u8_stripe(
+                C.c_void_p(src.ctypes.data), sz(57 * 4), sz(57), sz(41), C.c_void_p(got.ctypes.data),
+                sz(dw * 4), sz(dw), sz(dh), sz(4), sz(a), sz(b)) == 0
+        np.testing.assert_array_equal(got, whole)
+    t = np.array([[0.9, 0.1, 2.0], [-0.1, 0.95, 1.0], [1e-4, -1e-4, 1.0]], np.float32)
+    _, whole = O.warp_perspective(src, t, 50, 30, "LINEAR", "REPLICATE")
+    got = np.zeros_like(whole)
+    bv = np.zeros(4, np.uint8)
+    for a, b in ((0, 7), (7, 30)):
+        assert L.orc_warp_perspective_u8_stripe(
+            C.c_void_p(src.ctypes.data), sz(57 * 4), sz(57), sz(41), C.c_void_p(got.ctypes.data),
+            sz(200), sz(50), sz(30), C.c_void_p(t.ctypes.data), sz(4), C.c_int(1), C.c_int(1),
+            C.c_void_p(bv.ctypes.data), sz(a), sz(b)) == 0
+    np.testing.assert_array_equal(got, whole)
