@@ -475,7 +475,7 @@ typedef struct {
 
 /* Scheduling parameters; results never depend on them.
  *   blocks_per_cu: persistent workgroups per compute unit for each stage
- *       kernel (1..16)
+ *       kernel (1..16), capped at what that kernel keeps resident at once
  *   resize_band_rows: destination rows per resize work item (0: as many as
  *       fit the LDS window, at most 64)
  *   xcd_order: 1 = hand consecutive work items (neighbouring tiles) to the

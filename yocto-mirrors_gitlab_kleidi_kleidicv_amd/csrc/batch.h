@@ -30,7 +30,9 @@ enum Family : int {
 
 // Resize path of a job (rsz::Path as an int so the job stays trivially
 // copyable on both sides).
-enum ResizeKind : int { kRsGeneric = 0, kRsExact2, kRsExact4, kRsQuarter, kRsUp2x4 };
+// kRsDirect is a kernel variant only: a kRsGeneric job whose lerp reads
+// straight from global memory (rsz::generic_reads_direct).
+enum ResizeKind : int { kRsGeneric = 0, kRsExact2, kRsExact4, kRsQuarter, kRsUp2x4, kRsDirect };
 
 struct CopyArgs {
   const uint8_t *src;
@@ -99,6 +101,10 @@ constexpr int kTablesPerItem = 256;           // table entries
 const void *segment_kernel(Family family, int variant);
 hipError_t launch_segment(Family family, int variant, const Segment &seg, int grid,
                           hipStream_t stream);
+// Workgroups of that kernel one compute unit keeps resident at once (its
+// register and LDS footprint decide), or 0 when the pair does not exist or
+// the runtime cannot say.
+int segment_blocks_per_cu(Family family, int variant);
 
 }  // namespace kcv
 

@@ -137,6 +137,22 @@ __global__ __launch_bounds__(kThreads) void batch_resize_kernel(Segment s) {
   });
 }
 
+// kRsDirect: generic lerp jobs that read straight from global memory.  No
+// LDS, so the kernel's occupancy is not held to the staged window's.
+__global__ __launch_bounds__(kThreads) void batch_resize_direct_kernel(Segment s) {
+  for_each_item(s, [&](const Job &job, int bx, int by) {
+    const ResizeJob &rj = job.u.resize;
+    const int quads = (rj.a.dst_width * rj.a.channels + 3) / 4;
+    int lanes = kThreads;
+    while (lanes > 1 && lanes / 2 >= quads) lanes /= 2;
+    const int y0 = by * job.band;
+    rsz::resize_direct_rows<rsz::kLerpGeneric>(
+        rj.a, rj.cols, rj.rows, bx * rsz::kTileE + static_cast<int>(threadIdx.x % lanes) * 4,
+        y0 + static_cast<int>(threadIdx.x / lanes), min(y0 + job.band, rj.a.dst_height),
+        kThreads / lanes);
+  });
+}
+
 template <int C, int BORDER>
 __global__ __launch_bounds__(kThreads) void batch_remap_s16_kernel(Segment s) {
   for_each_item(s, [&](const Job &job, int bx, int by) {
@@ -251,6 +267,7 @@ const void *segment_kernel(Family family, int variant) {
              : variant == kRsQuarter + 8 * 3 ? KCV_K(batch_quarter_kernel<3>)
              : variant == kRsQuarter + 8 * 4 ? KCV_K(batch_quarter_kernel<4>)
              : variant == kRsUp2x4   ? KCV_K(batch_resize_kernel<kRsUp2x4>)
+             : variant == kRsDirect  ? KCV_K(batch_resize_direct_kernel)
                                      : nullptr;
     case kFamRemapS16:
       return variant == 1 * 8 + kReplicate   ? KCV_K(batch_remap_s16_kernel<1, kReplicate>)
@@ -267,6 +284,15 @@ const void *segment_kernel(Family family, int variant) {
     default:
       return nullptr;
   }
+}
+
+int segment_blocks_per_cu(Family family, int variant) {
+  const void *fn = segment_kernel(family, variant);
+  int n = 0;
+  if (fn == nullptr || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kThreads, 0) !=
+                           hipSuccess)
+    return 0;
+  return n;
 }
 
 hipError_t launch_segment(Family family, int variant, const Segment &seg, int grid,

@@ -139,7 +139,9 @@ kleidicv_error_t prepare_op(const kleidicv_hip_op_desc &d, Prepared *p, bool *sk
       j.u.resize.kind = p->resize_kind;
       // the quarter kernels are specialised per channel count
       j.variant = p->resize_kind == kcv::kRsQuarter ? kcv::kRsQuarter + 8 * a.channels
-                                                     : p->resize_kind;
+                  : p->resize_kind == kcv::kRsGeneric && kcv::rsz::generic_reads_direct(a)
+                      ? kcv::kRsDirect
+                      : p->resize_kind;
       p->table_bytes = kcv::resize_workspace_bytes(a);
       break;
     }
@@ -382,7 +384,13 @@ kleidicv_error_t make_plan(const std::vector<Prepared> &prep,
       sp.seg.njobs = static_cast<int>(end - begin);
       sp.seg.nitems = static_cast<uint32_t>(count);
       sp.seg.xcd_order = static_cast<int>(t.xcd_order);
-      int64_t grid = std::min<int64_t>(count, int64_t{cus} * t.blocks_per_cu);
+      // Items are split statically over the grid, so a grid larger than the
+      // kernel can keep resident would run its last workgroups as a second,
+      // mostly empty wave: cap it at the kernel's own occupancy.
+      int64_t per_cu = t.blocks_per_cu;
+      const int resident = kcv::segment_blocks_per_cu(sp.family, sp.variant);
+      if (resident > 0) per_cu = std::min<int64_t>(per_cu, resident);
+      int64_t grid = std::min<int64_t>(count, int64_t{cus} * per_cu);
       if (t.xcd_order) grid = div_up(grid, 8) * 8;
       sp.grid = static_cast<int>(grid);
       out->segs.push_back(sp);

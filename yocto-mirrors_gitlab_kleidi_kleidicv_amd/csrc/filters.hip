@@ -40,7 +40,7 @@ void launch_channels(const SepArgs &a, dim3 grid, int tx, hipStream_t s) {
 }
 
 template <int K, int C>
-__global__ __launch_bounds__(kThreads) void median_kernel(MedianArgs a, int tiles_x) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(K == 5 && C <= 2 ? 8 : 1))) void median_kernel(MedianArgs a, int tiles_x) {
   __shared__ __align__(16) uint8_t smem[median_smem_bytes<K>()];
   int bx, by;
   tile_of(tiles_x, bx, by);

@@ -25,6 +25,15 @@ __global__ __launch_bounds__(kThreads) void resize_staged_kernel(
   resize_staged_tile<MODE>(a, cols, rows, band, blockIdx.x, blockIdx.y, smem);
 }
 
+// The generic lerp when it reads straight from global memory: no LDS, so
+// registers alone set the occupancy.
+__global__ __launch_bounds__(kThreads) void resize_direct_kernel(
+    ResizeArgs a, const int2 *__restrict__ cols, const int2 *__restrict__ rows, int band) {
+  const int y0 = blockIdx.y * band;
+  resize_direct_rows<kLerpGeneric>(a, cols, rows, blockIdx.x * kTileE + threadIdx.x * 4, y0,
+                                   min(y0 + band, a.dst_height), 1);
+}
+
 __global__ __launch_bounds__(kThreads) void resize_up2x4_kernel(ResizeArgs a) {
   for (int g = blockIdx.y; g <= a.src_height; g += gridDim.y) up2x4_group(a, blockIdx.x, g);
 }
@@ -42,7 +51,10 @@ hipError_t launch_staged(const ResizeArgs &a, const int2 *cols, const int2 *rows
   const int band = band_rows(a);
   const dim3 grid(static_cast<unsigned>((n + kTileE - 1) / kTileE),
                   static_cast<unsigned>((a.dst_height + band - 1) / band));
-  resize_staged_kernel<MODE><<<grid, kThreads, 0, stream>>>(a, cols, rows, band);
+  if (MODE == kLerpGeneric && generic_reads_direct(a))
+    resize_direct_kernel<<<grid, kThreads, 0, stream>>>(a, cols, rows, band);
+  else
+    resize_staged_kernel<MODE><<<grid, kThreads, 0, stream>>>(a, cols, rows, band);
   return hipGetLastError();
 }
 

@@ -433,50 +433,230 @@ __device__ __forceinline__ int block_reduce(int v, bool is_max, int *scratch) {
   return r;
 }
 
+// Column taps of destination bytes e0 .. e0 + 3 from the generic table:
+// left and right source bytes, horizontal weight, and flags (bit 0: the left
+// tap reads 0, bit 1: the right tap reads 0; both set past the row's end).
+struct QuadTaps {
+  int left[4], right[4], wx[4], zero[4];
+};
+
+__device__ __forceinline__ QuadTaps load_taps(const int2 *__restrict__ cols, int e0, int n) {
+  QuadTaps q;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int e = e0 + k;
+    const int2 c = cols[min(e, n - 1)];
+    q.left[k] = c.x;
+    q.right[k] = c.x + (c.y >> 16);
+    q.wx[k] = c.y & 0xFF;
+    q.zero[k] = e < n ? (c.y >> 8) & 3 : 3;
+  }
+  return q;
+}
+
+// Least and greatest byte index read by the taps (tmax < 0: none read).
+__device__ __forceinline__ void tap_span(const QuadTaps &q, int &tmin, int &tmax) {
+  tmin = 0x7FFFFFFF;
+  tmax = -1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (!(q.zero[k] & 1)) { tmin = min(tmin, q.left[k]); tmax = max(tmax, q.left[k]); }
+    if (!(q.zero[k] & 2)) { tmin = min(tmin, q.right[k]); tmax = max(tmax, q.right[k]); }
+  }
+}
+
+// Byte selector of v_perm_b32 over an 8-byte window {hi, lo} that starts at
+// byte `base`: taps k0 and k0 + 1 of idx land in the low bytes of a 16-bit
+// pair; 0x0C selects zero, which also realises the vector code's "index
+// beyond the load reads 0".
+__device__ __forceinline__ uint32_t pair_sel(const int *idx, const int *zero, int flag,
+                                             int k0, int base) {
+  uint32_t v = 0x0C000C00u;
+  v |= (zero[k0] & flag) ? 0x0Cu : static_cast<uint32_t>(idx[k0] - base);
+  v |= ((zero[k0 + 1] & flag) ? 0x0Cu : static_cast<uint32_t>(idx[k0 + 1] - base)) << 16;
+  return v;
+}
+
+// The same over a 16-byte window: the selector for its low 8 bytes (upper
+// false) or its high 8 bytes; taps in the other half select zero, so the
+// two v_perm results are OR-ed.
+__device__ __forceinline__ uint32_t pair_sel_half(const int *idx, const int *zero, int flag,
+                                                  int k0, int base, bool upper) {
+  uint32_t v = 0x0C000C00u;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = k0 + h;
+    const int o = (zero[k] & flag) ? -1 : idx[k] - base;
+    const bool mine = o >= 0 && (upper ? o >= 8 : o < 8);
+    v |= static_cast<uint32_t>(mine ? (upper ? o - 8 : o) : 0x0C) << (16 * h);
+  }
+  return v;
+}
+
+// Four outputs from 8-byte windows {t1, t0} (top row) and {b1, b0} (bottom
+// row); sel = {l01, l23, r01, r23}.
+template <int MODE>
+__device__ __forceinline__ uint32_t lerp_quad8(uint32_t t0, uint32_t t1, uint32_t b0,
+                                               uint32_t b1, const uint32_t *sel, us2 wx01,
+                                               us2 wx23, int wy) {
+  const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
+  const us2 r01 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel[0])),
+                                 as_us2(__builtin_amdgcn_perm(t1, t0, sel[2])),
+                                 as_us2(__builtin_amdgcn_perm(b1, b0, sel[0])),
+                                 as_us2(__builtin_amdgcn_perm(b1, b0, sel[2])), wx01, wy2);
+  const us2 r23 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel[1])),
+                                 as_us2(__builtin_amdgcn_perm(t1, t0, sel[3])),
+                                 as_us2(__builtin_amdgcn_perm(b1, b0, sel[1])),
+                                 as_us2(__builtin_amdgcn_perm(b1, b0, sel[3])), wx23, wy2);
+  // bytes 0 and 2 of each pair word -> one dword
+  return __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
+}
+
+// Four outputs from 16-byte windows t[0..3] and b[0..3]; wsel[s][half] for
+// s = l01, l23, r01, r23.
+template <int MODE>
+__device__ __forceinline__ uint32_t lerp_quad16(const uint32_t *t, const uint32_t *b,
+                                                const uint32_t (*wsel)[2], us2 wx01, us2 wx23,
+                                                int wy) {
+  auto pick = [&](const uint32_t *x, int s_) {
+    return as_us2(__builtin_amdgcn_perm(x[1], x[0], wsel[s_][0]) |
+                  __builtin_amdgcn_perm(x[3], x[2], wsel[s_][1]));
+  };
+  const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
+  const us2 r01 = combine2<MODE>(pick(t, 0), pick(t, 2), pick(b, 0), pick(b, 2), wx01, wy2);
+  const us2 r23 = combine2<MODE>(pick(t, 1), pick(t, 3), pick(b, 1), pick(b, 3), wx23, wy2);
+  return __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
+}
+
+// Destination bytes e0 .. e0 + 3 of row y (fewer at the row's end).
+__device__ __forceinline__ void store_dst_quad(const ResizeArgs &a, int y, int e0, int n,
+                                               uint32_t packed) {
+  uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
+  if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
+    store_nt(reinterpret_cast<uint32_t *>(out + e0), packed);
+  } else {
+    for (int k = 0; k < 4 && e0 + k < n; ++k)
+      out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
+  }
+}
+
+// Whether the generic lerp of `a` reads its taps straight from global
+// memory (resize_direct_rows) instead of through an LDS stage: downsizing
+// rows by 1.5 or more.
+__host__ __device__ inline bool generic_reads_direct(const ResizeArgs &a) {
+  return 2 * static_cast<int64_t>(a.src_height) >= 3 * static_cast<int64_t>(a.dst_height);
+}
+
+// Generic lerp read straight from global memory: destination bytes
+// e0 .. e0 + 3 of rows y_begin, y_begin + y_step, ... < y_end.  Used for
+// downsizing by 1.5 or more, where a source row serves about one destination
+// row and an LDS stage would buy no reuse: each row is one 8- or 16-byte
+// load at any alignment (gfx950 vector memory runs in unaligned mode), the
+// window start pulled back from the row's end so that it stays inside the
+// row; taps spread wider than 16 bytes read byte by byte.
+template <int MODE>
+__device__ __forceinline__ void resize_direct_rows(const ResizeArgs &a,
+                                                   const int2 *__restrict__ cols,
+                                                   const int2 *__restrict__ rows, int e0,
+                                                   int y_begin, int y_end, int y_step) {
+  const int n = a.dst_width * a.channels;
+  if (e0 >= n) return;
+  const int row_bytes = a.src_width * a.channels;
+  const QuadTaps q = load_taps(cols, e0, n);
+  int tmin, tmax;
+  tap_span(q, tmin, tmax);
+  const bool narrow = tmax >= 0 && tmax - tmin < 8 && row_bytes >= 8;
+  const bool wide = tmax >= 0 && !narrow && tmax - tmin < 16 && row_bytes >= 16;
+  // taps lie in [tmin, tmax] within [0, row_bytes): offsets from `base`
+  // stay below the window size
+  const int base = narrow ? min(tmin, row_bytes - 8) : wide ? min(tmin, row_bytes - 16) : 0;
+  uint32_t sel[4], wsel[4][2];
+  if (narrow) {
+    sel[0] = pair_sel(q.left, q.zero, 1, 0, base);
+    sel[1] = pair_sel(q.left, q.zero, 1, 2, base);
+    sel[2] = pair_sel(q.right, q.zero, 2, 0, base);
+    sel[3] = pair_sel(q.right, q.zero, 2, 2, base);
+  } else if (wide) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      wsel[0][u] = pair_sel_half(q.left, q.zero, 1, 0, base, u);
+      wsel[1][u] = pair_sel_half(q.left, q.zero, 1, 2, base, u);
+      wsel[2][u] = pair_sel_half(q.right, q.zero, 2, 0, base, u);
+      wsel[3][u] = pair_sel_half(q.right, q.zero, 2, 2, base, u);
+    }
+  }
+  const us2 wx01 = us2{static_cast<uint16_t>(q.wx[0]), static_cast<uint16_t>(q.wx[1])};
+  const us2 wx23 = us2{static_cast<uint16_t>(q.wx[2]), static_cast<uint16_t>(q.wx[3])};
+  for (int y = y_begin; y < y_end; y += y_step) {
+    const int2 r = rows[y];
+    const int top = r.x, bot = r.y & 0xFFFFFF;
+    const int wy = (r.y >> 24) & 0xFF;
+    const uint8_t *tp = a.src + static_cast<size_t>(top) * a.src_stride;
+    const uint8_t *bp = a.src + static_cast<size_t>(bot) * a.src_stride;
+    uint32_t packed = 0;
+    if (narrow) {
+      const uint2 t = *reinterpret_cast<const u2_any *>(tp + base);
+      const uint2 b = *reinterpret_cast<const u2_any *>(bp + base);
+      packed = lerp_quad8<MODE>(t.x, t.y, b.x, b.y, sel, wx01, wx23, wy);
+    } else if (wide) {
+      const uint4 tv = *reinterpret_cast<const u4_any *>(tp + base);
+      const uint4 bv = *reinterpret_cast<const u4_any *>(bp + base);
+      const uint32_t t[4] = {tv.x, tv.y, tv.z, tv.w}, b[4] = {bv.x, bv.y, bv.z, bv.w};
+      packed = lerp_quad16<MODE>(t, b, wsel, wx01, wx23, wy);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pa = (q.zero[k] & 1) ? 0 : tp[q.left[k]];
+        const int pc = (q.zero[k] & 1) ? 0 : bp[q.left[k]];
+        const int pb = (q.zero[k] & 2) ? 0 : tp[q.right[k]];
+        const int pd = (q.zero[k] & 2) ? 0 : bp[q.right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, q.wx[k], wy) & 0xFFu) << (8 * k);
+      }
+    }
+    store_dst_quad(a, y, e0, n, packed);
+  }
+}
+
 // LDS of one staged tile: the window plus 16 spare bytes (the 8- and 16-byte
 // reads of the narrow and wide paths may start in the window's last dwords)
 // and the reduction scratch.
 constexpr int kStagedSmem = kStageBytes + 16 + 4 * (kThreads / 64);
 
 // Destination bytes [bx*kTileE, +kTileE) of rows [by*band, +band).
-template <int MODE>
 // `lanes` threads (a power of two) share each destination row: kThreads for
 // rows of a whole tile or more; for narrower rows fewer, and the band's
 // rows are dealt to the kThreads / lanes groups of them.
+template <int MODE>
 __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
                                                    const int2 *__restrict__ cols,
                                                    const int2 *__restrict__ rows,
                                                    int band, int bx, int by,
                                                    uint8_t *smem, int lanes = kThreads) {
-  uint8_t *win = smem;
-  int *scratch = reinterpret_cast<int *>(smem + kStageBytes + 16);
   const int n = a.dst_width * a.channels;
   const int row_bytes = a.src_width * a.channels;
   const int e0 = bx * kTileE + static_cast<int>(threadIdx.x % lanes) * 4;
   const int row_lane = static_cast<int>(threadIdx.x / lanes), row_step = kThreads / lanes;
+  const int y_begin = by * band;
+  const int y_end = min(y_begin + band, a.dst_height);
+  // The choice depends on the image only, so the whole workgroup takes it
+  // and the direct path (no barriers, no LDS) never splits a barrier.
+  if (MODE == kLerpGeneric && generic_reads_direct(a)) {
+    resize_direct_rows<MODE>(a, cols, rows, e0, y_begin + row_lane, y_end, row_step);
+    return;
+  }
+  uint8_t *win = smem;
+  int *scratch = reinterpret_cast<int *>(smem + kStageBytes + 16);
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
 
-  int left[4], right[4], wx[4], zero[4];
-  int lo = 0x7FFFFFFF, hi = -1;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int e = e0 + k;
-    const int2 c = cols[min(e, n - 1)];
-    left[k] = c.x;
-    right[k] = c.x + (c.y >> 16);
-    wx[k] = c.y & 0xFF;
-    zero[k] = e < n ? (c.y >> 8) & 3 : 3;
-    if (!(zero[k] & 1)) { lo = min(lo, left[k]); hi = max(hi, left[k]); }
-    if (!(zero[k] & 2)) { lo = min(lo, right[k]); hi = max(hi, right[k]); }
-  }
+  QuadTaps q = load_taps(cols, e0, n);
+  int lo, hi;
+  tap_span(q, lo, hi);
   lo = block_reduce(lo, false, scratch);
   hi = block_reduce(hi, true, scratch);
   if (hi < lo) { lo = 0; hi = 0; }
   const int lo4 = lo & ~3;
   const int pitch = ((hi + 1 - lo4) + 15) & ~15;  // whole 16-byte chunks
 
-  const int y_begin = by * band;
-  const int y_end = min(y_begin + band, a.dst_height);
   const int r_lo = rows[y_begin].x;
   const int r_hi = rows[y_end - 1].y & 0xFFFFFF;
   const int nrows = r_hi - r_lo + 1;
@@ -485,150 +665,92 @@ __device__ __forceinline__ void resize_staged_tile(const ResizeArgs &a,
   if (staged) {
     // 16-byte loads at any alignment (unaligned mode); the row's last
     // partial chunk byte by byte
-    typedef uint4 __attribute__((aligned(1))) u4u;
     const int chunks = pitch >> 4;
     for (int i = threadIdx.x; i < nrows * chunks; i += kThreads) {
-      const int r = i / chunks, q = i - r * chunks;
+      const int r = i / chunks, c = i - r * chunks;
       const uint8_t *srow = a.src + static_cast<size_t>(r_lo + r) * a.src_stride;
-      const int off = lo4 + 16 * q;
+      const int off = lo4 + 16 * c;
       uint4 v;
       if (off + 16 <= row_bytes) {
-        v = *reinterpret_cast<const u4u *>(srow + off);
+        v = *reinterpret_cast<const u4_any *>(srow + off);
       } else {
         uint32_t w[4] = {0, 0, 0, 0};
         for (int b = 0; b < 16 && off + b < row_bytes; ++b)
           w[b >> 2] |= static_cast<uint32_t>(srow[off + b]) << (8 * (b & 3));
         v = make_uint4(w[0], w[1], w[2], w[3]);
       }
-      *reinterpret_cast<uint4 *>(win + r * pitch + 16 * q) = v;
+      *reinterpret_cast<uint4 *>(win + r * pitch + 16 * c) = v;
     }
     __syncthreads();
   }
   if (e0 >= n) return;  // no barrier follows in this tile
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    left[k] -= lo4;
-    right[k] -= lo4;
+    q.left[k] -= lo4;
+    q.right[k] -= lo4;
   }
 
   // When a thread's taps lie within 8 bytes of a dword boundary (upsizing,
   // mild downsizing) each source row is one 8-byte LDS read plus bit-field
   // extracts instead of eight byte reads.
-  int tmin = 0x7FFFFFFF, tmax = -1;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (!(zero[k] & 1)) { tmin = min(tmin, left[k]); tmax = max(tmax, left[k]); }
-    if (!(zero[k] & 2)) { tmin = min(tmin, right[k]); tmax = max(tmax, right[k]); }
-  }
+  int tmin, tmax;
+  tap_span(q, tmin, tmax);
   const int tbase = tmin & ~3;
   const bool narrow = staged && tmax >= 0 && tmax - tbase < 8;
   // Taps within 16 bytes (2:1 downsizing of 1 and 3 channels): four dword
   // reads per row; each tap pair is picked from the low or the high 8 bytes
   // by two v_perm_b32 whose other selectors read zero, then OR-ed.
   const bool wide = staged && !narrow && tmax >= 0 && tmax - tbase < 16;
-  // Byte selectors for v_perm_b32 over the 8-byte window {hi, lo}: two taps
-  // per selector land in the low bytes of a 16-bit pair; 0x0C selects zero,
-  // which also realises the vector code's "index beyond the load reads 0".
-  auto sel = [&](const int *idx, int flag, int k0) {
-    uint32_t v = 0x0C000C00u;
-    if (!(zero[k0] & flag)) v |= static_cast<uint32_t>(idx[k0] - tbase);
-    else v |= 0x0Cu;
-    if (!(zero[k0 + 1] & flag)) v |= static_cast<uint32_t>(idx[k0 + 1] - tbase) << 16;
-    else v |= 0x0Cu << 16;
-    return v;
-  };
-  const uint32_t sel_l01 = sel(left, 1, 0), sel_l23 = sel(left, 1, 2);
-  const uint32_t sel_r01 = sel(right, 2, 0), sel_r23 = sel(right, 2, 2);
-  // wide form: selectors over {t1, t0} (upper = false) and {t3, t2} (upper)
-  auto sel_half = [&](const int *idx, int flag, int k0, bool upper) {
-    uint32_t v = 0x0C000C00u;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int k = k0 + h;
-      const int o = (zero[k] & flag) ? -1 : idx[k] - tbase;
-      const bool mine = o >= 0 && (upper ? o >= 8 : o < 8);
-      v |= static_cast<uint32_t>(mine ? (upper ? o - 8 : o) : 0x0C) << (16 * h);
-    }
-    return v;
-  };
-  uint32_t wsel[4][2];  // l01, l23, r01, r23 x {low 8 bytes, high 8 bytes}
-  if (wide) {
+  uint32_t sel[4], wsel[4][2];  // l01, l23, r01, r23 (x {low, high 8 bytes})
+  if (narrow) {
+    sel[0] = pair_sel(q.left, q.zero, 1, 0, tbase);
+    sel[1] = pair_sel(q.left, q.zero, 1, 2, tbase);
+    sel[2] = pair_sel(q.right, q.zero, 2, 0, tbase);
+    sel[3] = pair_sel(q.right, q.zero, 2, 2, tbase);
+  } else if (wide) {
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
-      wsel[0][u] = sel_half(left, 1, 0, u);
-      wsel[1][u] = sel_half(left, 1, 2, u);
-      wsel[2][u] = sel_half(right, 2, 0, u);
-      wsel[3][u] = sel_half(right, 2, 2, u);
+      wsel[0][u] = pair_sel_half(q.left, q.zero, 1, 0, tbase, u);
+      wsel[1][u] = pair_sel_half(q.left, q.zero, 1, 2, tbase, u);
+      wsel[2][u] = pair_sel_half(q.right, q.zero, 2, 0, tbase, u);
+      wsel[3][u] = pair_sel_half(q.right, q.zero, 2, 2, tbase, u);
     }
   }
-  const us2 wx01 = us2{static_cast<uint16_t>(wx[0]), static_cast<uint16_t>(wx[1])};
-  const us2 wx23 = us2{static_cast<uint16_t>(wx[2]), static_cast<uint16_t>(wx[3])};
+  const us2 wx01 = us2{static_cast<uint16_t>(q.wx[0]), static_cast<uint16_t>(q.wx[1])};
+  const us2 wx23 = us2{static_cast<uint16_t>(q.wx[2]), static_cast<uint16_t>(q.wx[3])};
 
   for (int y = y_begin + row_lane; y < y_end; y += row_step) {
     const int2 r = rows[y];
     const int top = r.x, bot = r.y & 0xFFFFFF;
     const int wy = (r.y >> 24) & 0xFF;
     uint32_t packed = 0;
-    if (narrow) {
+    if (narrow || wide) {
       const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
       const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
-      const uint32_t t0 = tp[0], t1 = tp[1], b0 = bp[0], b1 = bp[1];
-      const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
-      const us2 r01 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l01)),
-                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r01)),
-                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l01)),
-                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r01)), wx01, wy2);
-      const us2 r23 = combine2<MODE>(as_us2(__builtin_amdgcn_perm(t1, t0, sel_l23)),
-                                     as_us2(__builtin_amdgcn_perm(t1, t0, sel_r23)),
-                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_l23)),
-                                     as_us2(__builtin_amdgcn_perm(b1, b0, sel_r23)), wx23, wy2);
-      // bytes 0 and 2 of each pair word -> one dword
-      packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
-    } else if (wide) {
-      const uint32_t *tp = reinterpret_cast<const uint32_t *>(win + (top - r_lo) * pitch + tbase);
-      const uint32_t *bp = reinterpret_cast<const uint32_t *>(win + (bot - r_lo) * pitch + tbase);
-      const uint32_t t0 = tp[0], t1 = tp[1], t2 = tp[2], t3 = tp[3];
-      const uint32_t b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
-      auto pick = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, int s_) {
-        return as_us2(__builtin_amdgcn_perm(x1, x0, wsel[s_][0]) |
-                      __builtin_amdgcn_perm(x3, x2, wsel[s_][1]));
-      };
-      const us2 wy2 = us2{static_cast<uint16_t>(wy), static_cast<uint16_t>(wy)};
-      const us2 r01 = combine2<MODE>(pick(t0, t1, t2, t3, 0), pick(t0, t1, t2, t3, 2),
-                                     pick(b0, b1, b2, b3, 0), pick(b0, b1, b2, b3, 2), wx01, wy2);
-      const us2 r23 = combine2<MODE>(pick(t0, t1, t2, t3, 1), pick(t0, t1, t2, t3, 3),
-                                     pick(b0, b1, b2, b3, 1), pick(b0, b1, b2, b3, 3), wx23, wy2);
-      packed = __builtin_amdgcn_perm(as_u32(r23), as_u32(r01), 0x06040200u);
-    } else if (staged) {
-      const uint8_t *t = win + (top - r_lo) * pitch;
-      const uint8_t *b = win + (bot - r_lo) * pitch;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
-        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
-        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
-        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
-        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+      if (narrow) {
+        packed = lerp_quad8<MODE>(tp[0], tp[1], bp[0], bp[1], sel, wx01, wx23, wy);
+      } else {
+        const uint32_t t[4] = {tp[0], tp[1], tp[2], tp[3]};
+        const uint32_t b[4] = {bp[0], bp[1], bp[2], bp[3]};
+        packed = lerp_quad16<MODE>(t, b, wsel, wx01, wx23, wy);
       }
     } else {
-      const uint8_t *t = a.src + static_cast<size_t>(top) * a.src_stride + lo4;
-      const uint8_t *b = a.src + static_cast<size_t>(bot) * a.src_stride + lo4;
+      // per-byte reads from the LDS window, or from global memory where the
+      // window does not fit
+      const uint8_t *t = staged ? win + (top - r_lo) * pitch
+                                : a.src + static_cast<size_t>(top) * a.src_stride + lo4;
+      const uint8_t *b = staged ? win + (bot - r_lo) * pitch
+                                : a.src + static_cast<size_t>(bot) * a.src_stride + lo4;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        const int pa = (zero[k] & 1) ? 0 : t[left[k]];
-        const int pc = (zero[k] & 1) ? 0 : b[left[k]];
-        const int pb = (zero[k] & 2) ? 0 : t[right[k]];
-        const int pd = (zero[k] & 2) ? 0 : b[right[k]];
-        packed |= (combine<MODE>(pa, pb, pc, pd, wx[k], wy) & 0xFFu) << (8 * k);
+        const int pa = (q.zero[k] & 1) ? 0 : t[q.left[k]];
+        const int pc = (q.zero[k] & 1) ? 0 : b[q.left[k]];
+        const int pb = (q.zero[k] & 2) ? 0 : t[q.right[k]];
+        const int pd = (q.zero[k] & 2) ? 0 : b[q.right[k]];
+        packed |= (combine<MODE>(pa, pb, pc, pd, q.wx[k], wy) & 0xFFu) << (8 * k);
       }
     }
-    uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
-    if (e0 + 3 < n && (reinterpret_cast<uintptr_t>(out + e0) & 3) == 0) {
-      store_nt(reinterpret_cast<uint32_t *>(out + e0), packed);
-    } else {
-      for (int k = 0; k < 4 && e0 + k < n; ++k)
-        out[e0 + k] = static_cast<uint8_t>(packed >> (8 * k));
-    }
+    store_dst_quad(a, y, e0, n, packed);
   }
 }
 
