@@ -159,8 +159,8 @@ extern "C" int kcv_selfcheck_rounding(unsigned long long *mismatches) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(mismatches, bad, sizeof(*bad), hipMemcpyDeviceToHost);
-  hipFree(bad);
-  return e == hipSuccess ? 0 : 1;
+  const hipError_t freed = hipFree(bad);
+  return e == hipSuccess && freed == hipSuccess ? 0 : 1;
 }
 
 // Self-check of the reciprocal used by warp perspective (not part of the
@@ -178,6 +178,6 @@ extern "C" int kcv_selfcheck_reciprocal(unsigned long long *mismatches) {
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(mismatches, bad, sizeof(*bad), hipMemcpyDeviceToHost);
-  hipFree(bad);
-  return e == hipSuccess ? 0 : 1;
+  const hipError_t freed = hipFree(bad);
+  return e == hipSuccess && freed == hipSuccess ? 0 : 1;
 }
