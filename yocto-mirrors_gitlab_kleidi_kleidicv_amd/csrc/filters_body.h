@@ -273,7 +273,8 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   auto mac = [](sep_us2 acc, sep_us2 x, sep_us2 w) {
     if (MODE == kSepSaturate) {
       sep_us2 r;
-      asm("v_pk_mad_u16 %0, %1, %2, %3 clamp" : "=v"(r) : "v"(x), "v"(w), "v"(acc));
+      // the taps are uniform: an SGPR operand saves a VGPR copy per use
+      asm("v_pk_mad_u16 %0, %1, %2, %3 clamp" : "=v"(r) : "v"(x), "s"(w), "v"(acc));
       return r;
     }
     return acc + x * w;
@@ -286,7 +287,8 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
     for (int k = 0; k <= K; ++k) {
       const uint32_t v = win32[(2 * p + k) * kQuads + q];
       const sep_us2 e = __builtin_bit_cast(sep_us2, v & 0x00FF00FFu);
-      const sep_us2 o = __builtin_bit_cast(sep_us2, (v >> 8) & 0x00FF00FFu);
+      // bytes 1 and 3 into the low bytes of the halves: one v_perm_b32
+      const sep_us2 o = __builtin_bit_cast(sep_us2, __builtin_amdgcn_perm(0u, v, 0x0C030C01u));
       if (k < K) {
         e0 = mac(e0, e, wy[k]);
         o0 = mac(o0, o, wy[k]);
