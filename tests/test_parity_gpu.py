@@ -314,6 +314,29 @@ def test_resize_kats(kat):
     assert_same(K.resize_linear(src, exp.shape[1], exp.shape[0]), exp)
 
 
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+def test_resize_generic_direct_reads(rng, c):
+    # rows shrinking by 1.5 or more read their taps straight from global
+    # memory: 8-byte windows, 16-byte windows, per-byte reads for wide tap
+    # spreads, windows pulled back from the row's end, odd row pitches
+    min_w = 16 // c + (1 if 16 % c else 0)
+    for sw, sh, dw, dh in [(min_w, 40, max(min_w // 2, 1), 20), (min_w + 1, 9, min_w // 2 + 1, 6),
+                           (257, 63, 129, 32), (301, 90, 151, 45), (515, 31, 172, 11),
+                           (1031, 17, 211, 5), (777, 120, 389, 80), (2049, 33, 1025, 17)]:
+        err, _ = O.resize_linear(np.zeros((sh, sw, c), np.uint8), dw, dh)
+        if err != 0:
+            continue
+        resize_case(rng, sw, sh, dw, dh, c)
+        src = padded_rows(rng, sh, sw, c, 5)
+        err, want = O.resize_linear(np.ascontiguousarray(src), dw, dh)
+        assert err == 0
+        dst = np.full((dh, dw * c + 3), 0xEE, np.uint8)
+        view = dst[:, :dw * c].reshape(dh, dw, c) if c > 1 else dst[:, :dw]
+        K.resize_linear(src, dw, dh, dst=view)
+        assert_same(view, want, "padded %dx%d->%dx%d c%d" % (sw, sh, dw, dh, c))
+        assert np.all(dst[:, dw * c:] == 0xEE)
+
+
 def test_resize_pyramid_odd_sizes(rng):
     # cfg3-style 0.5x pyramids on odd sizes leave the quarter path
     for _ in range(12):
