@@ -180,23 +180,36 @@ def extended_outputs(torch, dev):
     }
 
 
+EXTENDED_INPUTS = ("i420", "yuyv", "yuv444", "rgb", "sep_src", "med_src", "down_src")
+
+
 def bench_extended(K, torch, sets, dev, stream, steps, warmup):
-    """Each extended op alone: `steps` launches over the rotating input sets
-    between two events on the launch stream; ms per launch."""
+    """Each extended op alone: `steps` launches between two events on the
+    launch stream; ms per launch.  Every launch, warmup included, reads its
+    own copy of the inputs (copies of the input sets as needed), and a 512 MiB
+    fill before each op evicts the Infinity Cache (256 MB), so no timed launch
+    finds its input cached by an earlier one: inputs come from HBM, as in
+    the step."""
     out = extended_outputs(torch, dev)
+    ext = [{k: s[k] for k in EXTENDED_INPUTS} for s in sets]
+    while len(ext) < warmup + steps:
+        ext.append({k: v.clone() for k, v in ext[len(ext) % len(sets)].items()})
+    scrub = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
     res = {}
-    names = [o[0] for o in extended_ops(K, sets[0], out, stream)]
+    names = [o[0] for o in extended_ops(K, ext[0], out, stream)]
     for k, name in enumerate(names):
-        calls = [extended_ops(K, s, out, stream)[k] for s in sets]
+        calls = [extended_ops(K, s, out, stream)[k] for s in ext]
         _, pix, nbytes, _ = calls[0]
+        scrub.fill_(k)
+        torch.cuda.synchronize()
         for i in range(warmup):
-            calls[i % len(calls)][3]()
+            calls[i][3]()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         e0.record(stream)
         for i in range(steps):
-            calls[i % len(calls)][3]()
+            calls[warmup + i][3]()
         e1.record(stream)
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / steps
