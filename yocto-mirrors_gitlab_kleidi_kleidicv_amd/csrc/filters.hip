@@ -62,52 +62,76 @@ __global__ __launch_bounds__(kThreads) void blur_down_kernel(SepArgs a, int tile
 constexpr int kArbRows = 16;        // output rows per thread in the vertical pass
 constexpr int kArbRowBytes = 1024;  // output bytes per workgroup in the horizontal pass
 
-// Vertical pass: thread = one dword column x kArbRows output rows.  Each
-// source row of the span is loaded once and added to every output row it
-// reaches, in 16-bit lanes (sums stay below 2^16: the taps sum to 256).
-__global__ __launch_bounds__(kThreads) void gauss_arb_vertical_kernel(GaussArbArgs a) {
-  __shared__ uint8_t taps[256];
-  taps[threadIdx.x] = a.taps[threadIdx.x];
-  __syncthreads();
+// Vertical pass: thread = one dword column (4 bytes) x kArbRows output rows.
+// Source rows from y0 - R are taken in groups of four: the four row dwords of
+// a group are transposed by v_perm_b32 into one dword per column holding
+// that column's four rows, and each output row r adds v_dot4_u32_u8 of it
+// with a weight dword.  Relative to y0 - R, group g byte b is row offset
+// 4g + b, whose tap for output row r = 4q + s is t[4(g - q) + b - s]: the
+// weight is phase[s][g - q], built on the host (zero bytes outside the
+// kernel), and uniform, so it is read with scalar loads.  The sums are exact
+// in 32 bits and round to u8 as the reference's vertical pass does.
+constexpr int kArbVDwords = (3 + 254) / 4 + 1;  // row groups per output row at K = 255
+
+struct GaussArbVertical {
+  GaussArbArgs a;
+  uint32_t phase[4 * kArbVDwords];
+};
+
+__global__ __launch_bounds__(kThreads) void gauss_arb_vertical_kernel(GaussArbVertical v) {
+  const GaussArbArgs &a = v.a;
   const int row_bytes = a.width * a.channels;
   const int e = 4 * (blockIdx.x * kThreads + threadIdx.x);
   if (e >= row_bytes) return;
-  const int R = a.ksize / 2;
+  const int K = a.ksize, R = K / 2;
+  const int nd = (K + 2) / 4 + 1;              // weight dwords per phase
+  const int ng = (kArbRows + K + 2) / 4;       // groups over rows y0 - R .. y0 + kArbRows - 1 + R
   const bool vec = e + 4 <= row_bytes &&
                    ((reinterpret_cast<uintptr_t>(a.src) | a.src_stride) & 3) == 0;
   for (int y0 = blockIdx.y * kArbRows; y0 < a.height; y0 += gridDim.y * kArbRows) {
-    sep_us2 lo[kArbRows], hi[kArbRows];
+    uint32_t acc[kArbRows][4];
 #pragma unroll
-    for (int r = 0; r < kArbRows; ++r) lo[r] = hi[r] = sep_us2{0, 0};
-    for (int j = y0 - R; j < y0 + kArbRows + R; ++j) {
-      const uint8_t *row = a.src + static_cast<size_t>(min(max(j, 0), a.height - 1)) * a.src_stride;
-      uint32_t v = 0;
-      if (vec) {
-        v = *reinterpret_cast<const uint32_t *>(row + e);
-      } else {
-        for (int b = 0; b < 4 && e + b < row_bytes; ++b)
-          v |= static_cast<uint32_t>(row[e + b]) << (8 * b);
+    for (int r = 0; r < kArbRows; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[r][c] = 0;
+    for (int g = 0; g < ng; ++g) {
+      uint32_t d[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int j = min(max(y0 - R + 4 * g + k, 0), a.height - 1);
+        const uint8_t *row = a.src + static_cast<size_t>(j) * a.src_stride;
+        if (vec) {
+          d[k] = *reinterpret_cast<const uint32_t *>(row + e);
+        } else {
+          d[k] = 0;
+          for (int b = 0; b < 4 && e + b < row_bytes; ++b)
+            d[k] |= static_cast<uint32_t>(row[e + b]) << (8 * b);
+        }
       }
-      const sep_us2 ev = __builtin_bit_cast(sep_us2, v & 0x00FF00FFu);
-      const sep_us2 od = __builtin_bit_cast(sep_us2, (v >> 8) & 0x00FF00FFu);
+      // 4x4 byte transpose: col[c] = (d0.c, d1.c, d2.c, d3.c)
+      const uint32_t lo01 = __builtin_amdgcn_perm(d[1], d[0], 0x05010400u);
+      const uint32_t hi01 = __builtin_amdgcn_perm(d[1], d[0], 0x07030602u);
+      const uint32_t lo23 = __builtin_amdgcn_perm(d[3], d[2], 0x05010400u);
+      const uint32_t hi23 = __builtin_amdgcn_perm(d[3], d[2], 0x07030602u);
+      const uint32_t col[4] = {__builtin_amdgcn_perm(lo23, lo01, 0x05040100u),
+                               __builtin_amdgcn_perm(lo23, lo01, 0x07060302u),
+                               __builtin_amdgcn_perm(hi23, hi01, 0x05040100u),
+                               __builtin_amdgcn_perm(hi23, hi01, 0x07060302u)};
 #pragma unroll
       for (int r = 0; r < kArbRows; ++r) {
-        const int i = j - y0 - r + R;  // tap index for output row y0 + r
-        if (i >= 0 && i < a.ksize) {
-          const uint16_t t = taps[i];
-          lo[r] += ev * sep_us2{t, t};
-          hi[r] += od * sep_us2{t, t};
-        }
+        const int m = g - r / 4;
+        if (m < 0 || m >= nd) continue;
+        const uint32_t w = v.phase[(r & 3) * kArbVDwords + m];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_amdgcn_udot4(col[c], w, acc[r][c], false);
       }
     }
 #pragma unroll
     for (int r = 0; r < kArbRows; ++r) {
       const int y = y0 + r;
       if (y >= a.height) break;
-      const sep_us2 l = (lo[r] + sep_us2{128, 128}) >> 8;
-      const sep_us2 h = (hi[r] + sep_us2{128, 128}) >> 8;
-      const uint32_t lw = __builtin_bit_cast(uint32_t, l), hw = __builtin_bit_cast(uint32_t, h);
-      const uint32_t out = (lw & 0x00FF00FFu) | ((hw & 0x00FF00FFu) << 8);
+      const uint32_t out = ((acc[r][0] + 128) >> 8) | (((acc[r][1] + 128) >> 8) << 8) |
+                           (((acc[r][2] + 128) >> 8) << 16) | (((acc[r][3] + 128) >> 8) << 24);
       uint8_t *o = a.tmp + static_cast<size_t>(y) * a.dst_stride + e;
       if (vec && ((reinterpret_cast<uintptr_t>(o) & 3) == 0)) {
         *reinterpret_cast<uint32_t *>(o) = out;
@@ -118,20 +142,43 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_vertical_kernel(GaussArbAr
   }
 }
 
-// Horizontal pass: a workgroup takes kArbRowBytes output bytes of one row,
-// stages the intermediate span they read (border pixels replicated) in LDS
-// and each thread sums the taps of 4 output bytes.
+// Horizontal pass: a workgroup takes kArbRowBytes output bytes of one row
+// and stages the intermediate span they read (border pixels replicated) in
+// LDS; each thread makes 4 consecutive output bytes o0 + p from the aligned
+// span dwords at o0 with v_dot4_u32_u8.  Output o0 + p sums taps[i] *
+// span[o0 + p + i * C]: against span dword m, byte b, that is the tap
+// i = (4m + b - p) / C where 4m + b - p is a non-negative multiple of C below
+// K * C, so each phase p has its own weight dwords (zero bytes elsewhere),
+// built once per workgroup.  The sums are exact in 32 bits.
+constexpr int kArbPhaseDwords = (3 + 254 * 4) / 4 + 1;  // K = 255, C = 4
+
 __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArbArgs a) {
   __shared__ uint8_t taps[256];
-  __shared__ uint8_t span[kArbRowBytes + 2 * 127 * 4];
+  __shared__ uint32_t wq[4 * kArbPhaseDwords];
+  // span dwords read past n (up to 4 * kArbPhaseDwords bytes after a thread's
+  // first) meet zero weights only
+  __shared__ __align__(16) uint8_t span[kArbRowBytes + 4 * kArbPhaseDwords];
   taps[threadIdx.x] = a.taps[threadIdx.x];
-  const int C = a.channels, R = a.ksize / 2;
+  const int C = a.channels, K = a.ksize, R = K / 2;
+  const int nd = (3 + (K - 1) * C) / 4 + 1;  // span dwords per output
+  __syncthreads();
+  for (int i = threadIdx.x; i < 4 * nd; i += kThreads) {
+    const int p = i / nd, m = i - p * nd;
+    uint32_t w = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int d = 4 * m + b - p;
+      if (d >= 0 && d % C == 0 && d / C < K) w |= static_cast<uint32_t>(taps[d / C]) << (8 * b);
+    }
+    wq[i] = w;
+  }
   const int row_bytes = a.width * C;
   const int e0 = blockIdx.x * kArbRowBytes;
   const int n = min(kArbRowBytes, row_bytes - e0) + 2 * R * C;  // span bytes
+  const int o0 = 4 * threadIdx.x;
   for (int y = blockIdx.y; y < a.height; y += gridDim.y) {
     const uint8_t *row = a.tmp + static_cast<size_t>(y) * a.dst_stride;
-    __syncthreads();  // the previous row's span is no longer read
+    __syncthreads();  // the previous row's span is no longer read (and wq is built)
     for (int b = threadIdx.x; b < n; b += kThreads) {
       const int eb = e0 - R * C + b;  // byte position, may lie outside the row
       const int x = eb >= 0 ? eb / C : -((C - 1 - eb) / C);
@@ -139,14 +186,19 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArb
       span[b] = row[min(max(x, 0), a.width - 1) * C + c];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int o = 4 * threadIdx.x + k;  // output byte within the segment
-      if (e0 + o >= row_bytes) break;
-      uint32_t acc = 0;
-      for (int i = 0; i < a.ksize; ++i) acc += taps[i] * static_cast<uint32_t>(span[o + i * C]);
-      a.dst[static_cast<size_t>(y) * a.dst_stride + e0 + o] = static_cast<uint8_t>((acc + 128) >> 8);
+    if (e0 + o0 >= row_bytes) continue;
+    const uint32_t *s32 = reinterpret_cast<const uint32_t *>(span) + threadIdx.x;
+    uint32_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+    for (int m = 0; m < nd; ++m) {
+      const uint32_t d = s32[m];
+      acc0 = __builtin_amdgcn_udot4(d, wq[m], acc0, false);
+      acc1 = __builtin_amdgcn_udot4(d, wq[nd + m], acc1, false);
+      acc2 = __builtin_amdgcn_udot4(d, wq[2 * nd + m], acc2, false);
+      acc3 = __builtin_amdgcn_udot4(d, wq[3 * nd + m], acc3, false);
     }
+    const uint32_t out = ((acc0 + 128) >> 8) | (((acc1 + 128) >> 8) << 8) |
+                         (((acc2 + 128) >> 8) << 16) | (((acc3 + 128) >> 8) << 24);
+    store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + o0, row_bytes, out);
   }
 }
 
@@ -275,8 +327,20 @@ hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream) 
   if (a.ksize < 3 || a.ksize > 255 || (a.ksize & 1) == 0) return hipErrorInvalidValue;
   const int row_bytes = a.width * a.channels;
   const int vy = min((a.height + kArbRows - 1) / kArbRows, 65535);
+  GaussArbVertical v;
+  v.a = a;
+  // phase[s][m] byte b: tap 4m + b - s (zero outside [0, K))
+  for (int s = 0; s < 4; ++s)
+    for (int m = 0; m < kArbVDwords; ++m) {
+      uint32_t w = 0;
+      for (int b = 0; b < 4; ++b) {
+        const int i = 4 * m + b - s;
+        if (i >= 0 && i < a.ksize) w |= static_cast<uint32_t>(a.taps[i]) << (8 * b);
+      }
+      v.phase[s * kArbVDwords + m] = w;
+    }
   gauss_arb_vertical_kernel<<<dim3((row_bytes + 4 * kThreads - 1) / (4 * kThreads), vy),
-                              kThreads, 0, stream>>>(a);
+                              kThreads, 0, stream>>>(v);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   gauss_arb_horizontal_kernel<<<dim3((row_bytes + kArbRowBytes - 1) / kArbRowBytes,
