@@ -243,6 +243,60 @@ extern kleidicv_error_t (*kleidicv_remap_f32_u8)(
     size_t mapy_stride, kleidicv_interpolation_type_t interpolation,
     kleidicv_border_type_t border_type, const uint8_t *border_value);
 
+/* Backend-selection twins (kleidicv.h:21-35): the reference's `_sme` names
+ * choose its SME backend and keep the semantics and parameter contract of
+ * the unsuffixed functions.  A GPU has one backend, so each twin runs
+ * exactly the unsuffixed entry point. */
+
+/* kleidicv.h:1583-1585 */
+kleidicv_error_t kleidicv_yuv_to_rgb_u8_sme(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride,
+                                            size_t width, size_t height,
+                                            kleidicv_color_conversion_t color_format);
+/* kleidicv.h:1744-1746 */
+kleidicv_error_t kleidicv_rgb_to_yuv_u8_sme(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride,
+                                            size_t width, size_t height,
+                                            kleidicv_color_conversion_t color_format);
+/* kleidicv.h:1816-1819 */
+kleidicv_error_t kleidicv_rgb_to_yuv_semiplanar_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride,
+    uint8_t *uv_dst, size_t uv_stride, size_t width, size_t height,
+    kleidicv_color_conversion_t color_format);
+/* kleidicv.h:2097-2100 */
+kleidicv_error_t kleidicv_separable_filter_2d_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type);
+/* kleidicv.h:2165-2168 */
+kleidicv_error_t kleidicv_gaussian_blur_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, float sigma_x, float sigma_y,
+    kleidicv_border_type_t border_type);
+/* kleidicv.h:2220-2223 */
+kleidicv_error_t kleidicv_blur_and_downsample_u8_sme(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t channels,
+    kleidicv_border_type_t border_type);
+/* kleidicv.h:2273 */
+kleidicv_error_t kleidicv_median_blur_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, size_t kernel_width,
+    size_t kernel_height, kleidicv_border_type_t border_type);
+/* kleidicv.h:2489-2492 */
+kleidicv_error_t kleidicv_resize_linear_u8_sme(const uint8_t *src,
+                                               size_t src_stride, size_t src_width,
+                                               size_t src_height, uint8_t *dst,
+                                               size_t dst_stride, size_t dst_width,
+                                               size_t dst_height, size_t channels);
+/* kleidicv.h:1647-1652: a function-pointer data symbol like its twin */
+extern kleidicv_error_t (*kleidicv_yuv_semiplanar_to_rgb_u8_sme)(
+    const uint8_t *src_y, size_t src_y_stride, const uint8_t *src_uv,
+    size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
+    size_t height, kleidicv_color_conversion_t color_format);
+
 /* ------------------------------------------------------------------------
  * 2. Threaded API (kleidicv_thread/include/kleidicv_thread/kleidicv_thread.h)
  * ------------------------------------------------------------------------ */

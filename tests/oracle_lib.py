@@ -201,3 +201,127 @@ def yuv_sp_to_rgb(y, uv, color_format=0x02):
     err = lib().orc_yuv_semiplanar_to_rgb_u8(_p(y), *_sz(w), _p(uv), *_sz(uv.shape[1]), _p(dst),
                                              *_sz(w * oc, w, h), C.c_int(color_format))
     return err, dst
+
+
+# ---------------------------------------------------------------------------
+# Full-size images: the oracle's row-stripe entry points (destination rows
+# [y_begin, y_end), each reading the whole source) on a thread pool, so a
+# 4K or 8K comparison takes seconds.  ctypes releases the GIL during each
+# call.  The stripes are disjoint, so the result equals the whole call.
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _striped(rows, fn):
+    from concurrent.futures import ThreadPoolExecutor
+    n = _threads()
+    bands = [(rows * i // n, rows * (i + 1) // n) for i in range(n)]
+    bands = [b for b in bands if b[1] > b[0]]
+    with ThreadPoolExecutor(max_workers=n) as pool:
+        errs = list(pool.map(lambda b: fn(*b), bands))
+    bad = [e for e in errs if e != 0]
+    return bad[0] if bad else 0
+
+
+def gaussian_blur_mt(src, kernel, sigma=0.0, border="REVERSE"):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    err = _striped(h, lambda a, b: lib().orc_gaussian_blur_u8_stripe(
+        _p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c, kernel), C.c_float(sigma),
+        C.c_int(BORDER[border]), *_sz(a, b)))
+    return err, dst.reshape(src.shape)
+
+
+def median_blur_mt(src, kernel, border="REPLICATE"):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    err = _striped(h, lambda a, b: lib().orc_median_blur_u8_stripe(
+        _p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c, kernel), C.c_int(BORDER[border]),
+        *_sz(a, b)))
+    return err, dst.reshape(src.shape)
+
+
+def separable_filter_2d_mt(src, kx, ky, border="REPLICATE"):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros_like(s)
+    kx = np.ascontiguousarray(kx, dtype=np.uint8)
+    ky = np.ascontiguousarray(ky, dtype=np.uint8)
+    err = _striped(h, lambda a, b: lib().orc_separable_filter_2d_u8_stripe(
+        _p(s), *_sz(w * c), _p(dst), *_sz(w * c, w, h, c), _p(kx), *_sz(len(kx)), _p(ky),
+        *_sz(len(ky)), C.c_int(BORDER[border]), *_sz(a, b)))
+    return err, dst.reshape(src.shape)
+
+
+def resize_linear_mt(src, dst_w, dst_h):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros((dst_h, dst_w, c), dtype=np.uint8)
+    err = _striped(dst_h, lambda a, b: lib().orc_resize_linear_u8_stripe(
+        _p(s), *_sz(w * c, w, h), _p(dst), *_sz(dst_w * c, dst_w, dst_h, c, a, b)))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def warp_perspective_mt(src, transform, dst_w, dst_h, interpolation="LINEAR",
+                        border="REPLICATE", border_value=(0, 0, 0, 0)):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dst = np.zeros((dst_h, dst_w, c), dtype=np.uint8)
+    t = np.ascontiguousarray(transform, dtype=np.float32)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = _striped(dst_h, lambda a, b: lib().orc_warp_perspective_u8_stripe(
+        _p(s), *_sz(w * c, w, h), _p(dst), *_sz(dst_w * c, dst_w, dst_h), _p(t), *_sz(c),
+        C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]), _p(bv), *_sz(a, b)))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def remap_s16point5_mt(src, mapxy, mapfrac, border="CONSTANT", border_value=(0, 0, 0, 0)):
+    # destination rows are independent: a stripe is a row-offset view
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dh, dw = mapfrac.shape
+    dst = np.zeros((dh, dw, c), dtype=np.uint8)
+    mapxy = np.ascontiguousarray(mapxy, dtype=np.int16)
+    mapfrac = np.ascontiguousarray(mapfrac, dtype=np.uint16)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = _striped(dh, lambda a, b: lib().orc_remap_s16point5_u8(
+        _p(s), *_sz(w * c, w, h), C.c_void_p(dst.ctypes.data + a * dw * c),
+        *_sz(dw * c, dw, b - a, c), C.c_void_p(mapxy.ctypes.data + a * dw * 4), *_sz(dw * 4),
+        C.c_void_p(mapfrac.ctypes.data + a * dw * 2), *_sz(dw * 2), C.c_int(BORDER[border]),
+        _p(bv)))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def remap_f32_mt(src, mapx, mapy, interpolation="LINEAR", border="REPLICATE",
+                 border_value=(0, 0, 0, 0)):
+    s = hwc(np.ascontiguousarray(src, dtype=np.uint8))
+    h, w, c = s.shape
+    dh, dw = mapx.shape
+    dst = np.zeros((dh, dw, c), dtype=np.uint8)
+    mapx = np.ascontiguousarray(mapx, dtype=np.float32)
+    mapy = np.ascontiguousarray(mapy, dtype=np.float32)
+    bv = np.array(border_value, dtype=np.uint8)
+    err = _striped(dh, lambda a, b: lib().orc_remap_f32_u8(
+        _p(s), *_sz(w * c, w, h), C.c_void_p(dst.ctypes.data + a * dw * c),
+        *_sz(dw * c, dw, b - a, c), C.c_void_p(mapx.ctypes.data + a * dw * 4), *_sz(dw * 4),
+        C.c_void_p(mapy.ctypes.data + a * dw * 4), *_sz(dw * 4),
+        C.c_int(INTERP[interpolation]), C.c_int(BORDER[border]), _p(bv)))
+    return err, (dst if src.ndim == 3 else dst[:, :, 0])
+
+
+def yuv_sp_to_rgb_mt(y, uv, color_format=0x02):
+    # a stripe of chroma rows [a, b) is luma rows [2a, 2b)
+    y = np.ascontiguousarray(y, dtype=np.uint8)
+    uv = np.ascontiguousarray(uv, dtype=np.uint8)
+    h, w = y.shape
+    oc = 4 if color_format & 0x40 else 3
+    dst = np.zeros((h, w, oc), dtype=np.uint8)
+    pairs = (h + 1) // 2
+    err = _striped(pairs, lambda a, b: lib().orc_yuv_semiplanar_to_rgb_u8(
+        C.c_void_p(y.ctypes.data + 2 * a * w), *_sz(w),
+        C.c_void_p(uv.ctypes.data + a * uv.shape[1]), *_sz(uv.shape[1]),
+        C.c_void_p(dst.ctypes.data + 2 * a * w * oc), *_sz(w * oc, w, min(h, 2 * b) - 2 * a),
+        C.c_int(color_format)))
+    return err, dst

@@ -35,7 +35,7 @@ def exported():
 
 def test_library_exports_every_declared_symbol():
     funcs, data = declared_symbols()
-    assert len(funcs) >= 20 and len(data) == 3
+    assert len(funcs) >= 20 and len(data) == 4
     syms = exported()
     for f in funcs:
         assert syms.get(f) == "T", f
@@ -53,6 +53,45 @@ def test_function_pointer_symbols_are_initialised():
     for name in ("kleidicv_yuv_semiplanar_to_rgb_u8", "kleidicv_remap_s16point5_u8",
                  "kleidicv_remap_f32_u8"):
         assert C.c_void_p.in_dll(lib, name).value != 0
+
+
+SME_TWINS = ["kleidicv_yuv_to_rgb_u8", "kleidicv_rgb_to_yuv_u8",
+             "kleidicv_rgb_to_yuv_semiplanar_u8", "kleidicv_separable_filter_2d_u8",
+             "kleidicv_gaussian_blur_u8", "kleidicv_blur_and_downsample_u8",
+             "kleidicv_median_blur_u8", "kleidicv_resize_linear_u8"]
+
+
+def test_sme_twins_exported_with_the_same_behaviour():
+    # kleidicv.h:1583,1647,1744,1816,2097,2165,2220,2273,2489: the `_sme`
+    # backend-selection names keep the semantics of the unsuffixed ones
+    funcs, data = declared_symbols()
+    syms = exported()
+    for name in SME_TWINS:
+        assert name + "_sme" in funcs, name
+        assert syms.get(name + "_sme") == "T", name
+    assert "kleidicv_yuv_semiplanar_to_rgb_u8_sme" in data
+    lib = K.lib()
+    a = C.c_void_p.in_dll(lib, "kleidicv_yuv_semiplanar_to_rgb_u8").value
+    b = C.c_void_p.in_dll(lib, "kleidicv_yuv_semiplanar_to_rgb_u8_sme").value
+    assert a == b != 0
+    # same error codes for the same invalid arguments (no device needed)
+    buf = np.zeros(64 * 64, np.uint8)
+    p = buf.ctypes.data
+    for suffix in ("", "_sme"):
+        g = getattr(lib, "kleidicv_gaussian_blur_u8" + suffix)
+        args = [C.c_void_p(p), C.c_size_t(64), C.c_void_p(p + 1), C.c_size_t(64),
+                C.c_size_t(16), C.c_size_t(16), C.c_size_t(1), C.c_size_t(5), C.c_size_t(5),
+                C.c_float(0), C.c_float(0)]
+        assert g(*args, C.c_int(0)) == K.NOT_IMPLEMENTED
+        assert g(C.c_void_p(0), *args[1:], C.c_int(1)) == K.NULL_POINTER
+        m = getattr(lib, "kleidicv_median_blur_u8" + suffix)
+        assert m(C.c_void_p(p), C.c_size_t(64), C.c_void_p(p), C.c_size_t(64), C.c_size_t(16),
+                 C.c_size_t(16), C.c_size_t(1), C.c_size_t(5), C.c_size_t(5),
+                 C.c_int(1)) == K.NOT_IMPLEMENTED  # in place
+        r = getattr(lib, "kleidicv_resize_linear_u8" + suffix)
+        assert r(C.c_void_p(0), C.c_size_t(16), C.c_size_t(16), C.c_size_t(16),
+                 C.c_void_p(p), C.c_size_t(16), C.c_size_t(8), C.c_size_t(8),
+                 C.c_size_t(1)) == K.NULL_POINTER
 
 
 def _gauss(src, sstride, dst, dstride, w, h, c, kw, kh, sx, sy, border):
@@ -206,6 +245,14 @@ def test_remap_error_codes():
     assert fn(p, 16, 16, 4, p, 16, 16, 4, 1, p, 63, p, 32, 1, None) == K.ALIGNMENT
     assert fn(p, 16, 16, 4, p, 16, 16, 4, 2, p, 64, p, 32, 1, None) == K.NOT_IMPLEMENTED
     assert fn(p, 16, 16, 4, p, 7, 7, 4, 1, p, 64, p, 32, 1, None) == K.NOT_IMPLEMENTED
+    # an empty source is declined before any launch (the reference reads out
+    # of bounds there), for both borders and channel counts
+    for c in (1, 4):
+        for border in (0, 1):
+            assert fn(p, 64, 0, 4, p, 64, 16, 4, c, p, 64, p, 32, border, p) == K.NOT_IMPLEMENTED
+            assert fn(p, 64, 16, 0, p, 64, 16, 4, c, p, 64, p, 32, border, p) == K.NOT_IMPLEMENTED
+    # with no destination rows there is nothing to read: OK, as the reference
+    assert fn(p, 64, 0, 0, p, 64, 16, 0, 1, p, 64, p, 32, 1, p) == K.OK
 
 
 def test_warp_error_codes():

@@ -121,18 +121,20 @@ def test_pyramid_batch_order_and_tuning_invariant():
 
 @pytest.mark.skip(reason="graph replay of a batch is switched off: its first run here "
                          "ended in a host segmentation fault inside hipGraphLaunch and the "
-                         "cause is not yet known; kleidicv_hip_batch_create rejects "
-                         "use_graph=1")
+                         "cause could not be shown from the code; kleidicv_hip_batch_create "
+                         "rejects use_graph=1")
 def test_graph_replays_agree_with_eager_work_between():
     recs = WL.pyramid_batch(torch, dev(), n=6, seed=13, lo=64, hi=160)
     stream = torch.cuda.Stream()
-    with K.Batch(WL.pyramid_descs(recs), keep=recs) as b:
+    torch.cuda.synchronize()
+    with K.Batch(WL.pyramid_descs(recs), tuning=K.Tuning(4, 0, 1, 1), keep=recs) as b:
         b.run(stream)
         torch.cuda.synchronize()
         want = pyramid_outputs(recs)
         for rep in range(3):
             clear_pyramid(recs)
             scratch = torch.ones(1 << 20, device=dev()).sum()  # eager work between replays
+            stream.wait_stream(torch.cuda.current_stream())
             b.run(stream)
             torch.cuda.synchronize()
             assert float(scratch) == float(1 << 20)

@@ -19,10 +19,15 @@ using kcv::is_device_pointer;
 using kcv::runtime;
 
 // The device API launches on the caller's stream as given; NULL is HIP's
-// null stream, as everywhere in HIP.  The drop-in API uses the library's
+// null stream, as everywhere in HIP.  The drop-in API behaves like the
+// reference's synchronous CPU call: it starts after all work already queued
+// on the device (a kernel on any stream may still be writing `src`, and the
+// library's stream is ordered against none of them), runs on the library's
 // own stream and waits for it.
 hipStream_t pick_stream(void *stream, bool sync) {
-  return sync ? runtime().stream : static_cast<hipStream_t>(stream);
+  if (!sync) return static_cast<hipStream_t>(stream);
+  (void)hipDeviceSynchronize();
+  return runtime().stream;
 }
 
 // ---------------------------------------------------------------------------
@@ -56,7 +61,7 @@ class Session {
       return KLEIDICV_OK;
     }
     void *d = nullptr;
-    KCV_TRY(from_hip(hipMallocAsync(&d, bytes, stream_)));
+    KCV_TRY(from_hip(kcv::device_alloc(&d, bytes, stream_, true)));
     bufs_[n_++] = Staged{p, d, bytes};
     KCV_TRY(from_hip(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, stream_)));
     *dev = static_cast<const T *>(d);
@@ -75,7 +80,7 @@ class Session {
       return KLEIDICV_OK;
     }
     void *d = nullptr;
-    KCV_TRY(from_hip(hipMallocAsync(&d, bytes, stream_)));
+    KCV_TRY(from_hip(kcv::device_alloc(&d, bytes, stream_, true)));
     bufs_[n_] = Staged{p, d, bytes};
     out_[n_] = true;
     ++n_;
@@ -88,7 +93,7 @@ class Session {
   kleidicv_error_t scratch(size_t bytes, void **dev) {
     *dev = nullptr;
     if (bytes == 0) return KLEIDICV_OK;
-    KCV_TRY(from_hip(hipMallocAsync(dev, bytes, stream_)));
+    KCV_TRY(from_hip(kcv::device_alloc(dev, bytes, stream_, true)));
     bufs_[n_++] = Staged{*dev, *dev, bytes};
     return KLEIDICV_OK;
   }
@@ -546,6 +551,80 @@ kleidicv_error_t kleidicv_warp_perspective_u8(
   return warp_impl(src, src_stride, src_width, src_height, dst, dst_stride, dst_width,
                    dst_height, transformation, channels, interpolation, border_type,
                    border_value, nullptr, true);
+}
+
+// Backend-selection twins (kleidicv.h:21-35): same semantics, one backend.
+kleidicv_error_t (*kleidicv_yuv_semiplanar_to_rgb_u8_sme)(
+    const uint8_t *, size_t, const uint8_t *, size_t, uint8_t *, size_t, size_t,
+    size_t, kleidicv_color_conversion_t) = &yuv_sp_entry;
+
+kleidicv_error_t kleidicv_yuv_to_rgb_u8_sme(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride, size_t width,
+                                            size_t height,
+                                            kleidicv_color_conversion_t color_format) {
+  return kleidicv_yuv_to_rgb_u8(src, src_stride, dst, dst_stride, width, height,
+                                color_format);
+}
+
+kleidicv_error_t kleidicv_rgb_to_yuv_u8_sme(const uint8_t *src, size_t src_stride,
+                                            uint8_t *dst, size_t dst_stride, size_t width,
+                                            size_t height,
+                                            kleidicv_color_conversion_t color_format) {
+  return kleidicv_rgb_to_yuv_u8(src, src_stride, dst, dst_stride, width, height,
+                                color_format);
+}
+
+kleidicv_error_t kleidicv_rgb_to_yuv_semiplanar_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *y_dst, size_t y_stride, uint8_t *uv_dst,
+    size_t uv_stride, size_t width, size_t height, kleidicv_color_conversion_t color_format) {
+  return kleidicv_rgb_to_yuv_semiplanar_u8(src, src_stride, y_dst, y_stride, uv_dst,
+                                           uv_stride, width, height, color_format);
+}
+
+kleidicv_error_t kleidicv_separable_filter_2d_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride,
+    size_t width, size_t height, size_t channels, const uint8_t *kernel_x,
+    size_t kernel_width, const uint8_t *kernel_y, size_t kernel_height,
+    kleidicv_border_type_t border_type) {
+  return kleidicv_separable_filter_2d_u8(src, src_stride, dst, dst_stride, width, height,
+                                         channels, kernel_x, kernel_width, kernel_y,
+                                         kernel_height, border_type);
+}
+
+kleidicv_error_t kleidicv_gaussian_blur_u8_sme(
+    const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride, size_t width,
+    size_t height, size_t channels, size_t kernel_width, size_t kernel_height,
+    float sigma_x, float sigma_y, kleidicv_border_type_t border_type) {
+  return kleidicv_gaussian_blur_u8(src, src_stride, dst, dst_stride, width, height,
+                                   channels, kernel_width, kernel_height, sigma_x,
+                                   sigma_y, border_type);
+}
+
+kleidicv_error_t kleidicv_blur_and_downsample_u8_sme(const uint8_t *src, size_t src_stride,
+                                                     size_t src_width, size_t src_height,
+                                                     uint8_t *dst, size_t dst_stride,
+                                                     size_t channels,
+                                                     kleidicv_border_type_t border_type) {
+  return kleidicv_blur_and_downsample_u8(src, src_stride, src_width, src_height, dst,
+                                         dst_stride, channels, border_type);
+}
+
+kleidicv_error_t kleidicv_median_blur_u8_sme(const uint8_t *src, size_t src_stride,
+                                             uint8_t *dst, size_t dst_stride, size_t width,
+                                             size_t height, size_t channels,
+                                             size_t kernel_width, size_t kernel_height,
+                                             kleidicv_border_type_t border_type) {
+  return kleidicv_median_blur_u8(src, src_stride, dst, dst_stride, width, height, channels,
+                                 kernel_width, kernel_height, border_type);
+}
+
+kleidicv_error_t kleidicv_resize_linear_u8_sme(const uint8_t *src, size_t src_stride,
+                                               size_t src_width, size_t src_height,
+                                               uint8_t *dst, size_t dst_stride,
+                                               size_t dst_width, size_t dst_height,
+                                               size_t channels) {
+  return kleidicv_resize_linear_u8(src, src_stride, src_width, src_height, dst, dst_stride,
+                                   dst_width, dst_height, channels);
 }
 
 // Threaded API: the stripe scheduler is replaced by the GPU grid.

@@ -591,15 +591,29 @@ struct kleidicv_hip_batch {
     plan = Plan{};
   }
 
+  // Builds the plan for `t` into a new allocation and swaps it in only once
+  // it is complete on the device: on any failure the batch keeps its
+  // previous plan, tuning and work lists, and still runs as before.  The
+  // caller makes sure no run still reads the old work lists.
   kleidicv_error_t build(const kleidicv_hip_batch_tuning &t) {
-    release();
-    tuning = t;
     Plan sizing;
     KCV_TRY(make_plan(prep, t, nullptr, &sizing));
-    if (sizing.bytes == 0) return KLEIDICV_OK;
-    KCV_TRY(from_hip(hipMalloc(&dev, sizing.bytes)));
-    KCV_TRY(make_plan(prep, t, static_cast<char *>(dev), &plan));
-    return upload(plan, dev);
+    void *ndev = nullptr;
+    Plan nplan;
+    if (sizing.bytes != 0) {
+      KCV_TRY(from_hip(kcv::device_alloc(&ndev, sizing.bytes, nullptr, false)));
+      kleidicv_error_t e = make_plan(prep, t, static_cast<char *>(ndev), &nplan);
+      if (e == KLEIDICV_OK) e = upload(nplan, ndev);
+      if (e != KLEIDICV_OK) {
+        (void)hipFree(ndev);
+        return e;
+      }
+    }
+    release();
+    dev = ndev;
+    plan = std::move(nplan);
+    tuning = t;
+    return KLEIDICV_OK;
   }
 };
 

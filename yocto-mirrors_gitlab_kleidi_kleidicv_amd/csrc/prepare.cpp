@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 
 namespace kcv {
@@ -46,6 +47,23 @@ bool is_device_pointer(const void *p) {
     return false;
   }
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+namespace {
+// Device allocations still to succeed before the forced failure; negative
+// when no failure is armed.
+std::atomic<long> g_alloc_fail_after{-1};
+}  // namespace
+
+hipError_t device_alloc(void **p, size_t bytes, hipStream_t stream, bool stream_ordered) {
+  long n = g_alloc_fail_after.load();
+  while (n >= 0 && !g_alloc_fail_after.compare_exchange_weak(n, n - 1)) {
+  }
+  if (n == 0) {
+    *p = nullptr;
+    return hipErrorOutOfMemory;
+  }
+  return stream_ordered ? hipMallocAsync(p, bytes, stream) : hipMalloc(p, bytes);
 }
 
 namespace {
@@ -391,6 +409,10 @@ kleidicv_error_t prepare_remap_s16point5(
          border_type == KLEIDICV_BORDER_TYPE_CONSTANT)))
     return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   if (dst_height == 0) return KLEIDICV_OK;
+  // An empty source leaves no pixel to clamp to (the reference reads out of
+  // bounds there); declined as remap_f32_is_implemented (remap.h:55-74)
+  // declines it.
+  if (src_width == 0 || src_height == 0) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
   if (!runtime().ok) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
 
   *a = RemapS16Args{};
@@ -760,3 +782,10 @@ kleidicv_error_t prepare_rgb_to_yuv_sp(const uint8_t *src, size_t src_stride, ui
 }
 
 }  // namespace kcv
+
+// Test hook (not part of the KleidiCV API): the device allocation that
+// follows `after` successful ones fails with out-of-memory, once; a negative
+// value disarms it.
+extern "C" void kcv_test_fail_device_allocation(long after) {
+  kcv::g_alloc_fail_after.store(after);
+}

@@ -33,6 +33,13 @@ Runtime &runtime();
 kleidicv_error_t from_hip(hipError_t e);
 bool is_device_pointer(const void *p);
 
+// Every device allocation the library makes goes through here: stream-ordered
+// (hipMallocAsync on `stream`) or not (hipMalloc).  A test hook can make one
+// of them fail, as the reference's tests make malloc fail
+// (test/framework/wrap_malloc.cpp), to check that callers see
+// KLEIDICV_ERROR_ALLOCATION and keep their state.
+hipError_t device_alloc(void **p, size_t bytes, hipStream_t stream, bool stream_ordered);
+
 #define KCV_TRY(expr)                          \
   do {                                         \
     kleidicv_error_t kcv_e = (expr);           \

@@ -320,6 +320,12 @@ def rgb_to_yuv_semiplanar(src, color_format, y=None, uv=None, stream=None, devic
     return y, uv
 
 
+def fail_device_allocation(after):
+    """Test hook: the library's device allocation that follows `after`
+    successful ones fails with out-of-memory (once); negative disarms."""
+    lib().kcv_test_fail_device_allocation(C.c_long(int(after)))
+
+
 def synchronize(stream=None):
     _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
 

@@ -117,6 +117,16 @@ def homography(rng, w, h, max_deg=15.0, max_persp=1e-4):
     return (T / T[2, 2]).astype(np.float32)
 
 
+def plain_op_params(w, h, seed=7):
+    """Seeded parameters of the plain 4K op set: 7x7 separable taps
+    (integers in [0, 2]) and the warp homography (rotation <= 15 degrees,
+    perspective terms <= 1e-4)."""
+    rng = np.random.default_rng(seed)
+    kx = rng.integers(0, 3, 7).astype(np.uint8)
+    ky = rng.integers(0, 3, 7).astype(np.uint8)
+    return kx, ky, homography(rng, w, h)
+
+
 def smooth_maps(torch, gen, dev, w, h, amplitude):
     """Float maps x + dx, y + dy with |dx|, |dy| <= amplitude (sum of 4 waves)."""
     yy, xx = torch.meshgrid(torch.arange(h, device=dev, dtype=torch.float32),
@@ -128,6 +138,25 @@ def smooth_maps(torch, gen, dev, w, h, amplitude):
     dy = sum(amp * torch.cos((0.002 + 0.02 * r[4 * i + 1]) * xx + (0.002 + 0.02 * r[4 * i]) * yy
                              + 6.28 * r[4 * i + 3]) for i in range(4))
     return (xx + dx).contiguous(), (yy + dy).contiguous()
+
+
+def s16point5_maps(torch, gen, dev, w, h, amplitude=40.0):
+    """OpenCV convertMaps-style fixed-point maps of a smooth displacement
+    (sum of 4 seeded waves, |dx|, |dy| <= amplitude): interleaved s16 (x, y)
+    integer parts and u16 5-bit fractions (fy << 5 | fx)."""
+    yy, xx = torch.meshgrid(torch.arange(h, device=dev, dtype=torch.float64),
+                            torch.arange(w, device=dev, dtype=torch.float64), indexing="ij")
+    r = torch.rand(16, generator=gen, device=dev, dtype=torch.float64)
+    amp = amplitude / 4.0
+    fx = sum(amp * torch.sin((0.001 + 0.01 * r[4 * i]) * xx + (0.001 + 0.01 * r[4 * i + 1]) * yy
+                             + 6.28 * r[4 * i + 2]) for i in range(4))
+    fy = sum(amp * torch.cos((0.001 + 0.01 * r[4 * i + 1]) * xx + (0.001 + 0.01 * r[4 * i]) * yy
+                             + 6.28 * r[4 * i + 3]) for i in range(4))
+    ix = torch.floor((xx + fx) * 32).to(torch.int64)
+    iy = torch.floor((yy + fy) * 32).to(torch.int64)
+    mapxy = torch.stack([ix >> 5, iy >> 5], dim=-1).to(torch.int16).contiguous()
+    mapfrac = ((ix & 31) | ((iy & 31) << 5)).to(torch.int16).contiguous()
+    return mapxy, mapfrac
 
 
 def transform_batch(torch, dev, n=512, seed=5, w=1280, h=720, channels=4, amplitude=20.0):
