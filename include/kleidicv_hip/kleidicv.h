@@ -463,6 +463,23 @@ kleidicv_error_t kleidicv_hip_yuv_semiplanar_to_rgb_u8(
     size_t src_uv_stride, uint8_t *dst, size_t dst_stride, size_t width,
     size_t height, kleidicv_color_conversion_t color_format, void *stream);
 
+/* No-allocation resize.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * validation, results and error codes as kleidicv_hip_resize_linear_u8, but
+ * the coordinate tables of the generic and exact-2x/4x paths go to a fixed
+ * device array of KLEIDICV_HIP_NOALLOC_SCRATCH_BYTES that each GPU holds
+ * until the process ends, and no allocator is called.  The tables take
+ * (dst_width * channels + dst_height) * 8 bytes; a call that needs more
+ * returns KLEIDICV_ERROR_RANGE and launches nothing.  Calls on one GPU share
+ * the array: calls that overlap on the device (other streams, other host
+ * threads) race and must be ordered by the caller. */
+#define KLEIDICV_HIP_NOALLOC_SCRATCH_BYTES ((size_t)4 << 20)
+kleidicv_error_t kleidicv_hip_resize_linear_u8_noalloc(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height,
+    uint8_t *dst, size_t dst_stride, size_t dst_width, size_t dst_height,
+    size_t channels, void *stream);
+
 /* The library's own non-blocking stream (used by the drop-in entry points),
  * and a helper that waits for a stream (NULL = HIP's null stream). */
 void *kleidicv_hip_default_stream(void);

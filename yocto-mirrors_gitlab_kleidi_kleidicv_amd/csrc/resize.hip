@@ -1,5 +1,10 @@
-// Per-image launches of the linear resize (bodies in resize_body.h).
+// Per-image launches of the linear resize (bodies in resize_body.h), and the
+// resize entry point that allocates nothing.
 
+#include <mutex>
+
+#include "kleidicv_hip/kleidicv.h"
+#include "prepare.h"
 #include "resize_body.h"
 
 namespace kcv {
@@ -111,3 +116,61 @@ hipError_t launch_resize(const ResizeArgs &a, void *workspace,
 }
 
 }  // namespace kcv
+
+// ---------------------------------------------------------------------------
+// No-allocation resize.  The coordinate tables of the generic and exact
+// paths live in `scratch`, a fixed array that every GPU holds from module
+// load until the process ends.  In host code `scratch` names a host-side
+// stand-in, so its device address is looked up once per device and only
+// that address is passed to the kernels.  The tables kernel writes every
+// entry the lerp kernel reads, on the same stream, so nothing needs
+// clearing; two calls that overlap on one GPU (different streams, or
+// different host threads) share the array and race.
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = KLEIDICV_HIP_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+struct ScratchSlot {
+  std::once_flag once;
+  void *ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+
+// Device address of this device's `scratch` (the current device), or null.
+void *scratch_address(int device) {
+  static ScratchSlot slots[kMaxDevices];
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  ScratchSlot &s = slots[device];
+  std::call_once(s.once, [&s] {
+    void *p = nullptr;
+    s.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (s.err == hipSuccess) s.ptr = p;
+    else (void)hipGetLastError();
+  });
+  return s.ptr;
+}
+
+}  // namespace
+
+extern "C" kleidicv_error_t kleidicv_hip_resize_linear_u8_noalloc(
+    const uint8_t *src, size_t src_stride, size_t src_width, size_t src_height, uint8_t *dst,
+    size_t dst_stride, size_t dst_width, size_t dst_height, size_t channels, void *stream) {
+  kcv::ResizeArgs a;
+  kcv::Action act;
+  KCV_TRY(kcv::prepare_resize(src, src_stride, src_width, src_height, dst, dst_stride,
+                              dst_width, dst_height, channels, &a, &act));
+  if (act == kcv::Action::kNone) return KLEIDICV_OK;
+  const size_t need = kcv::resize_workspace_bytes(a);
+  if (need > kNoAllocScratchBytes) return KLEIDICV_ERROR_RANGE;
+  void *ws = nullptr;
+  if (need > 0) {
+    int device = 0;
+    KCV_TRY(kcv::from_hip(hipGetDevice(&device)));
+    ws = scratch_address(device);
+    if (ws == nullptr) return KLEIDICV_ERROR_NOT_IMPLEMENTED;
+  }
+  return kcv::from_hip(kcv::launch_resize(a, ws, static_cast<hipStream_t>(stream)));
+}
