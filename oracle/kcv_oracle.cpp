@@ -63,15 +63,18 @@ extern "C" long orc_border_index(long index, long limit, int border) {
 
 // ---------------------------------------------------------------------------
 // a4: sigma half-kernel (include/kleidicv/filters/sigma.h:27-79).
-// `value = hk * multiplier - error` is one C expression in the reference and
-// contracts to a fused multiply-subtract under clang's default fp-contract.
-// Parity: "import-only" for the exact tap table (host expf); pinned against
-// the reference KATs within their stated +-1 (test_gaussian_blur.cpp:777-785).
+// The default sigma `ksize * 0.15F + 0.35F` and `value = hk * multiplier -
+// error` are single C expressions in the reference and contract to fused
+// multiply-adds under clang's default fp-contract on AArch64 (the default
+// sigma differs from the unfused one for 35 odd sizes; the taps change only
+// for 235x235).  Parity: pinned against a double-precision model of the same
+// generator (every entry equal, or at a rounding boundary) and the reference
+// KATs within their stated +-1 (test_gaussian_blur.cpp:777-785).
 extern "C" int orc_gaussian_half_kernel(uint8_t *half_kernel, size_t half_size,
                                         float sigma) {
   const size_t mid = half_size - 1;
   const size_t ksize = mid * 2 + 1;
-  if (sigma == 0.0f) sigma = static_cast<float>(ksize) * 0.15f + 0.35f;
+  if (sigma == 0.0f) sigma = std::fmaf(static_cast<float>(ksize), 0.15f, 0.35f);
   float tmp[255];
   float coefficient = 1 / -(2 * sigma * sigma);
   float sum = 0.0f;
@@ -960,10 +963,15 @@ uint8_t sample_nearest(const Gather &g, float xf, float yf, size_t c) {
 // a11: src/transform/remap_f32_neon.cpp:14-220.  Channels 3 and 4 apply the
 // 1-channel arithmetic to each plane with its own border value (extension;
 // the reference returns NOT_IMPLEMENTED, remap.h:55-74).
-// Parity: "import-only" for exact agreement with the Arm build (float);
-// pinned in tests/ by the reference's +-1 float/double tolerance
-// (test_remap.cpp:1183-1244) and by integer-coordinate maps where every
-// fraction is 0 (exact copies).
+// Parity: pinned in tests/ exactly by an exact-rational model of the
+// reference's operation sequence (transform_neon.h:189-337; every float step
+// rounded once, AArch64 conversion rules): on dyadic maps, where the result
+// is fixed by the mathematics and exact .5 ties tell the rounding modes
+// apart; on NaN, infinite, negative and far-edge coordinates; and on
+// arbitrary maps, including searched cases where the fused and unfused
+// lerps split.  Also within the reference's +-1 of a double model
+// (test_remap.cpp:1183-1244).  That the Arm build contracts vmlaq_f32 as
+// here stays import-only.
 extern "C" int orc_remap_f32_u8(const uint8_t *src, size_t src_stride,
                                 size_t sw, size_t sh, uint8_t *dst,
                                 size_t dst_stride, size_t dw, size_t dh,
@@ -1001,7 +1009,10 @@ extern "C" int orc_remap_f32_u8(const uint8_t *src, size_t src_stride,
 // vmlaq_n_f32 (fused), 1/tw is an IEEE division, xf = tx*iw.
 // Channels 2..4: per-plane extension of the 1-channel semantics.
 // Parity: pinned by the SimpleResize KAT (test_warp_perspective.cpp:700-742,
-// exact) and within +-1 of a double-precision reference (:239-300).
+// exact), exactly by the exact-rational model of the operation sequence on
+// dyadic affine transforms (w = 1 or 2: every step exact) and on
+// perspective ones, and within +-1 of a double-precision reference
+// (:239-300).
 extern "C" int orc_warp_perspective_u8_stripe(const uint8_t *src, size_t src_stride,
                                               size_t sw, size_t sh, uint8_t *dst,
                                               size_t dst_stride, size_t dw, size_t dh,
@@ -1069,8 +1080,10 @@ extern "C" int orc_warp_perspective_u8(const uint8_t *src, size_t src_stride,
 // ---------------------------------------------------------------------------
 // a13: src/conversions/yuv420sp_to_rgb_neon.cpp:100-160 and
 // yuv420_to_rgb_neon.h:17-259 with constants yuv42x_coefficients.h:95-105.
-// Parity: pinned by the closed form of test_yuv420sp_to_rgb.cpp:112-175,
-// written independently in tests/, on its fixed inputs (:42-53).
+// Parity: pinned exactly by the integer formula of
+// test_yuv420sp_to_rgb.cpp:112-160 in tests/, with the weights read from
+// yuv42x_coefficients.h:95-105 and the test by tests/golden/extract_fixtures.py,
+// on random planes and on every (U, V) pair; and on the fixed inputs (:42-53).
 extern "C" int orc_yuv_semiplanar_to_rgb_u8(const uint8_t *src_y,
                                             size_t src_y_stride,
                                             const uint8_t *src_uv,

@@ -16,6 +16,8 @@ Sources (paths relative to the reference root):
   test/api/test_resize_linear.cpp:1016-1187       ResizeLinearU8Params tables
   test/unit_neon/test_resize_linear_generic_u8.cpp:251-389  generator sizes
   test/api/test_blur_and_downsample.cpp:179-206   BlurAndDownsample.Minimal_u8
+  kleidicv/src/conversions/yuv42x_coefficients.h:95-105  YUV -> RGB weights
+  test/api/test_yuv420sp_to_rgb.cpp:112-160       the test's integer formula
 """
 
 import json
@@ -119,6 +121,34 @@ def nv12(ref):
     return {"y": y, "uv": uv}
 
 
+def nv12_coefficients(ref):
+    """The 20-bit YUV -> RGB weights, read twice: from the library's table
+    (kYWeight, kUVWeights) and from the integer formula of the NV12 test's
+    expected-value function; the two must agree."""
+    text = open(os.path.join(ref, "kleidicv/src/conversions/yuv42x_coefficients.h")).read()
+    y = int(re.search(r"kYWeight\s*=\s*/\*[^*]*\*/\s*(-?\d+)", text).group(1))
+    uv = _ints(re.sub(r"/\*.*?\*/", "", _block(text, "kUVWeights[4] =")))
+    lib = {"y": y, "rv": uv[0], "gu": uv[1], "gv": uv[2], "bu": uv[3]}
+    t = open(os.path.join(ref, "test/api/test_yuv420sp_to_rgb.cpp")).read()
+    body = _block(t, "void calculate_expected(")
+    r = re.search(r"int32_t r = \(\((\d+) \* y\) \+ \((\d+) \* v\) \+ \(1 << (\d+)\)\) >> (\d+)",
+                  body)
+    g = re.search(r"int32_t g =\s*\(\((\d+) \* y\) - \((\d+) \* u\) - \((\d+) \* v\) \+ "
+                  r"\(1 << (\d+)\)\) >> (\d+)", body)
+    b = re.search(r"int32_t b = \(\((\d+) \* y\) \+ \((\d+) \* u\) \+ \(1 << (\d+)\)\) >> (\d+)",
+                  body)
+    test = {"y": int(r.group(1)), "rv": int(r.group(2)), "gu": -int(g.group(2)),
+            "gv": -int(g.group(3)), "bu": int(b.group(2)), "shift": int(r.group(4))}
+    assert int(g.group(1)) == test["y"] == int(b.group(1))
+    assert int(r.group(3)) + 1 == test["shift"] == int(g.group(5)) == int(b.group(4))
+    for k in lib:
+        assert lib[k] == test[k], (k, lib[k], test[k])
+    # the test's offsets: Y' = max(Y - 16, 0), U' = U - 128, V' = V - 128
+    assert "std::max(0, y - 16)" in body and "u -= 128" in body and "v -= 128" in body
+    test.update({"y_offset": 16, "uv_offset": 128})
+    return test
+
+
 def resize_params(ref):
     text = open(os.path.join(ref, "test/api/test_resize_linear.cpp")).read()
     i = text.index("INSTANTIATE_TEST_SUITE_P(\n    ResizeLinear, ResizeLinearU8Params")
@@ -158,6 +188,7 @@ def main():
         "resize_u8_params": resize_params(ref),
         "resize_generator_sizes": generator_sizes(ref),
         "blur_and_downsample_minimal": blur_and_downsample(ref),
+        "nv12_coefficients": nv12_coefficients(ref),
     }
     here = os.path.dirname(os.path.abspath(__file__))
     with open(os.path.join(here, "reference_kats.json"), "w") as f:

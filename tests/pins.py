@@ -269,3 +269,222 @@ def rgb_to_yuv420_reference_formula(rgb):
     v = np.clip((460324 * r0 - 385875 * g0 - 74448 * b0 + (1 << 19) + (128 << 20)) >> 20,
                 0, 255)
     return y.astype(np.uint8), u.astype(np.uint8), v.astype(np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# NV12 / NV21 -> RGB: the integer formula of the reference's NV12 test
+# (test/api/test_yuv420sp_to_rgb.cpp:112-160) with the weights read from the
+# reference's files by tests/golden/extract_fixtures.py.
+def nv12_reference_formula(y, uv, fmt, k):
+    h, w = y.shape
+    vu = bool(fmt & 0x10)
+    Y = np.maximum(y.astype(np.int64) - k["y_offset"], 0)
+    pair = (np.arange(w) - np.arange(w) % 2)
+    rows = uv[np.arange(h) // 2].astype(np.int64)
+    U = rows[:, pair + (1 if vu else 0)] - k["uv_offset"]
+    V = rows[:, pair + (0 if vu else 1)] - k["uv_offset"]
+    half = 1 << (k["shift"] - 1)
+    R = (k["y"] * Y + k["rv"] * V + half) >> k["shift"]
+    G = (k["y"] * Y + k["gu"] * U + k["gv"] * V + half) >> k["shift"]
+    B = (k["y"] * Y + k["bu"] * U + half) >> k["shift"]
+    if fmt & 0x20:
+        R, B = B, R
+    ch = [R, G, B] + ([np.full_like(R, 255)] if fmt & 0x40 else [])
+    return np.clip(np.stack(ch, axis=-1), 0, 255).astype(np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# Sigma half-kernel (include/kleidicv/filters/sigma.h:27-79) in double
+# precision.  The float table can differ from it only where a value sits at
+# a rounding boundary; `follow` takes the float table's choice there and
+# carries on, so every later entry is still checked.
+def default_sigma(ksize, fused=True):
+    """sigma.h:38: ksize * 0.15F + 0.35F in float, one rounding when fused
+    (clang contracts the expression on AArch64)."""
+    a, b, c = (float(np.float32(v)) for v in (ksize, 0.15, 0.35))
+    if fused:
+        return _f32(_F(a) * _F(b) + _F(c))
+    return float(np.float32(np.float32(np.float32(a) * np.float32(b)) + np.float32(c)))
+
+
+def gaussian_half_kernel_double(half_size, sigma, follow=None):
+    mid = half_size - 1
+    ksize = 2 * mid + 1
+    if sigma == 0.0:
+        sigma = default_sigma(ksize)
+    vals = [np.exp(-(j * j) / (2.0 * sigma * sigma)) for j in range(mid, 0, -1)]
+    mult = 256.0 / (2.0 * sum(vals) + 1.0)
+    out, near, err = [], [], 0.0
+    for i in range(mid):
+        v = vals[i] * mult - err
+        r = float(np.floor(v + 0.5))
+        near.append(abs(v - np.floor(v) - 0.5))
+        if follow is not None and follow[i] != r:
+            r = float(follow[i])
+        out.append(r)
+        err = r - v
+    m = mult - err
+    near.append(abs(m - np.floor(m) - 0.5))
+    return [int(v) for v in out] + [int(np.floor(m + 0.5))], near
+
+
+# ---------------------------------------------------------------------------
+# Float gathers (remap f32 and warp perspective).  The reference's arithmetic
+# (src/transform/transform_neon.h:189-337, warp_perspective_neon.cpp:55-102)
+# evaluated exactly with rationals: each float operation rounds once to the
+# nearest float32 (ties to even), and the fused forms (vmla, and the row
+# terms T1*y+T2 that clang contracts on AArch64) round once for the whole
+# multiply-add.  `fused=False` rounds the product separately.  The integer
+# conversions follow the AArch64 instruction semantics: FCVTMU/FCVTMS floor,
+# FCVTAU/FCVTAS round half away from zero, NaN to 0, saturating.
+from fractions import Fraction as _F
+import math as _m
+
+
+def _f32(q):
+    """Round a rational (or +-inf) to the nearest float32, ties to even."""
+    if isinstance(q, float):
+        return float(np.float32(q))
+    if q == 0:
+        return 0.0
+    s = -1 if q < 0 else 1
+    a = abs(q)
+    e = a.numerator.bit_length() - a.denominator.bit_length()
+    if _F(2) ** e > a:
+        e -= 1
+    e = max(e, -126)                 # subnormals share the smallest exponent
+    scale = _F(2) ** (e - 23)
+    n = a / scale
+    fl = n.numerator // n.denominator
+    rem = n - fl
+    if rem > _F(1, 2) or (rem == _F(1, 2) and fl % 2 == 1):
+        fl += 1
+    v = fl * scale
+    if v >= _F(2) ** 128:
+        return s * _m.inf
+    return s * float(v)
+
+
+def _q(x):
+    return _F(x) if _m.isfinite(x) else x
+
+
+def _add(a, b):
+    if not (_m.isfinite(a) and _m.isfinite(b)):
+        return a + b
+    return _f32(_F(a) + _F(b))
+
+
+def _mul(a, b):
+    if not (_m.isfinite(a) and _m.isfinite(b)):
+        return a * b
+    return _f32(_F(a) * _F(b))
+
+
+def _fma(a, b, c, fused):
+    if not fused:
+        return _add(_mul(a, b), c)
+    if not (_m.isfinite(a) and _m.isfinite(b) and _m.isfinite(c)):
+        return a * b + c
+    return _f32(_F(a) * _F(b) + _F(c))
+
+
+def _div(a, b):
+    if b == 0.0 or not (_m.isfinite(a) and _m.isfinite(b)):
+        return float(np.float32(np.float64(a) / np.float64(b))) if b != 0.0 else (
+            _m.copysign(_m.inf, a) * _m.copysign(1.0, b) if a != 0 else _m.nan)
+    return _f32(_F(a) / _F(b))
+
+
+def _floor(x):
+    return x if not _m.isfinite(x) else float(_m.floor(x))
+
+
+def _cvt(x, signed, mode):
+    lo, hi = (-(1 << 31), (1 << 31) - 1) if signed else (0, (1 << 32) - 1)
+    if _m.isnan(x):
+        return 0
+    if _m.isinf(x):
+        return hi if x > 0 else lo
+    if mode == "m":
+        v = _m.floor(x)
+    else:  # round half away from zero
+        v = _m.floor(abs(x) + 0.5) * (1 if x >= 0 else -1)
+        if abs(x) % 1 == 0.5:
+            v = (_m.floor(abs(x)) + 1) * (1 if x >= 0 else -1)
+    return int(min(max(v, lo), hi))
+
+
+def _sample(s, xf, yf, interp, border, bv, fused):
+    """One destination pixel (all channels) of the reference's gather."""
+    h, w, c = s.shape
+    xmax, ymax = w - 1, h - 1
+
+    def px(x, y):
+        return [float(v) for v in s[y, x]]
+
+    if interp == "NEAREST":
+        if border == "REPLICATE":
+            x = min(_cvt(xf, False, "a"), xmax)
+            y = min(_cvt(yf, False, "a"), ymax)
+            return px(x, y)
+        x = _cvt(xf, True, "a") & 0xFFFFFFFF
+        y = _cvt(yf, True, "a") & 0xFFFFFFFF
+        return px(x, y) if x <= xmax and y <= ymax else [float(v) for v in bv[:c]]
+    xfrac = _add(xf, -_floor(xf)) if _m.isfinite(xf) else _m.nan
+    yfrac = _add(yf, -_floor(yf)) if _m.isfinite(yf) else _m.nan
+    if border == "REPLICATE":
+        x0 = min(_cvt(xf, False, "m"), xmax)
+        y0 = min(_cvt(yf, False, "m"), ymax)
+        x1 = x0 + 1 if (xf >= 0 and x0 < xmax) else x0
+        y1 = y0 + 1 if (yf >= 0 and y0 < ymax) else y0
+        a, b, cc, d = px(x0, y0), px(x1, y0), px(x0, y1), px(x1, y1)
+    else:
+        x0 = _cvt(xf, True, "m") & 0xFFFFFFFF
+        y0 = _cvt(yf, True, "m") & 0xFFFFFFFF
+        x1, y1 = (x0 + 1) & 0xFFFFFFFF, (y0 + 1) & 0xFFFFFFFF
+        bvals = [float(v) for v in bv[:c]]
+
+        def get(x, y):
+            return px(x, y) if x <= xmax and y <= ymax else bvals
+        a, b, cc, d = get(x0, y0), get(x1, y0), get(x0, y1), get(x1, y1)
+    out = []
+    for k in range(c):
+        line0 = _fma(_add(b[k], -a[k]), xfrac, a[k], fused)
+        line1 = _fma(_add(d[k], -cc[k]), xfrac, cc[k], fused)
+        r = _fma(_add(line1, -line0), yfrac, line0, fused)
+        out.append(float(_cvt(r, False, "a")))
+    return out
+
+
+def remap_f32_exact(src, mapx, mapy, interp, border, bv, fused=True):
+    s = _hwc(src)
+    dh, dw = mapx.shape
+    out = np.zeros((dh, dw, s.shape[2]), np.uint8)
+    for y in range(dh):
+        for x in range(dw):
+            out[y, x] = _sample(s, float(mapx[y, x]), float(mapy[y, x]), interp, border, bv,
+                                fused)
+    return out if src.ndim == 3 else out[..., 0]
+
+
+def warp_exact(src, t, dst_w, dst_h, interp, border, bv, fused=True):
+    """warp_perspective_neon.cpp:55-102: row terms T1*y+T2 (fused as clang
+    contracts them), column terms with vmla, iw = 1/tw correctly rounded (0
+    when tw == 0), xf = tx * iw."""
+    T = [float(np.float32(v)) for v in np.asarray(t, np.float32).reshape(9)]
+    s = _hwc(src)
+    out = np.zeros((dst_h, dst_w, s.shape[2]), np.uint8)
+    for y in range(dst_h):
+        fy = float(y)
+        tx0 = _fma(T[1], fy, T[2], fused)
+        ty0 = _fma(T[4], fy, T[5], fused)
+        tw0 = _fma(T[7], fy, T[8], fused)
+        for x in range(dst_w):
+            fx = float(x)
+            tx = _fma(fx, T[0], tx0, fused)
+            ty = _fma(fx, T[3], ty0, fused)
+            tw = _fma(fx, T[6], tw0, fused)
+            iw = 0.0 if tw == 0.0 else _div(1.0, tw)
+            out[y, x] = _sample(s, _mul(tx, iw), _mul(ty, iw), interp, border, bv, fused)
+    return out if src.ndim == 3 else out[..., 0]

@@ -516,3 +516,233 @@ def test_row_stripes_equal_whole_calls(rng):
             sz(200), sz(50), sz(30), C.c_void_p(t.ctypes.data), sz(4), C.c_int(1), C.c_int(1),
             C.c_void_p(bv.ctypes.data), sz(a), sz(b)) == 0
     np.testing.assert_array_equal(got, whole)
+
+
+# ---------------------------------------------------------------------------
+# Exact pins of the NV12 chroma terms, the sigma tap table and the float
+# gathers' rounding and contraction.
+@pytest.mark.parametrize("fmt", [0x02, 0x22, 0x42, 0x62, 0x12, 0x32, 0x52, 0x72])
+def test_nv12_equals_reference_test_formula(rng, fmt):
+    # test_yuv420sp_to_rgb.cpp:112-160 with the weights of
+    # yuv42x_coefficients.h:95-105, both read from the reference's files
+    k = KATS["nv12_coefficients"]
+    for h, w in [(2, 2), (7, 9), (16, 33), (31, 64)]:
+        y = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        uv = rng.integers(0, 256, ((h + 1) // 2, 2 * ((w + 1) // 2)), dtype=np.uint8)
+        err, out = O.yuv_sp_to_rgb(y, uv, fmt)
+        assert err == 0
+        np.testing.assert_array_equal(out, pins.nv12_reference_formula(y, uv, fmt, k))
+    # every (U, V) pair once, with Y at both ends and in the middle: a wrong
+    # weight or rounding constant changes some of these outputs
+    uu, vv = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    uv = np.stack([uu, vv], axis=-1).reshape(256, 512).astype(np.uint8)
+    for yl in (0, 16, 17, 128, 235, 255):
+        y = np.full((512, 512), yl, np.uint8)
+        err, out = O.yuv_sp_to_rgb(y, uv, fmt)
+        np.testing.assert_array_equal(out, pins.nv12_reference_formula(y, uv, fmt, k))
+
+
+def test_nv12_pin_rejects_a_mistyped_weight(rng):
+    # the pin is discriminating: one weight off by one changes outputs
+    k = dict(KATS["nv12_coefficients"])
+    y = rng.integers(0, 256, (64, 64), dtype=np.uint8)
+    uv = rng.integers(0, 256, (32, 64), dtype=np.uint8)
+    _, out = O.yuv_sp_to_rgb(y, uv, 0x02)
+    for name in ("rv", "gu", "gv", "bu"):
+        bad = dict(k, **{name: k[name] + 1024})
+        assert not np.array_equal(out, pins.nv12_reference_formula(y, uv, 0x02, bad)), name
+
+
+def _half_kernel(k, sigma):
+    import ctypes as C
+    half = np.zeros(k // 2 + 1, np.uint8)
+    ok = O.lib().orc_gaussian_half_kernel(half.ctypes.data, C.c_size_t(k // 2 + 1),
+                                          C.c_float(sigma))
+    return bool(ok), [int(v) for v in half]
+
+
+# sigma values (size, sigma) where the float table and the double model
+# round one entry differently, the value lying within 1e-4 of the half-way
+# point; every other entry of every table on the grid is equal
+SIGMA_BOUNDARY_CASES_LIMIT = 12
+
+
+def test_sigma_half_kernel_equals_double_model_up_to_rounding_boundaries():
+    grid = [0.0] + [round(0.05 * i, 2) for i in range(1, 241)]
+    sizes = [3, 5, 7, 9, 15, 21, 11, 31, 63, 127, 235, 255]
+    boundary = []
+    for k in sizes:
+        for sigma in grid:
+            ok, table = _half_kernel(k, np.float32(sigma))
+            want, near = pins.gaussian_half_kernel_double(k // 2 + 1, float(np.float32(sigma)),
+                                                          follow=table)
+            if not ok:
+                assert want[-1] == 256, (k, sigma)  # all energy in the centre tap
+                continue
+            for i, (a, b) in enumerate(zip(table, want)):
+                if a != b or (i < k // 2 and table[i] != want[i]):
+                    assert near[i] < 1e-4, (k, sigma, i, a, b, near[i])
+            # a followed entry is one whose rounding the two precisions split
+            plain, _ = pins.gaussian_half_kernel_double(k // 2 + 1, float(np.float32(sigma)))
+            if plain != table:
+                boundary.append((k, sigma))
+    assert len(boundary) <= SIGMA_BOUNDARY_CASES_LIMIT, boundary
+
+
+def test_sigma_default_is_the_contracted_expression():
+    # ksize * 0.15F + 0.35F rounds once on AArch64; at 235x235 the unfused
+    # sigma gives different taps, so the oracle's table tells the two apart
+    s_f, s_u = pins.default_sigma(235), pins.default_sigma(235, fused=False)
+    assert s_f != s_u
+    ok, table = _half_kernel(235, 0.0)
+    assert ok
+    assert table == _half_kernel(235, s_f)[1] != _half_kernel(235, s_u)[1]
+    want, _ = pins.gaussian_half_kernel_double(118, s_f)
+    assert table == want
+
+
+def dyadic(rng, lo, hi, shape, bits=4):
+    return (np.round(rng.uniform(lo, hi, shape) * (1 << bits)) / (1 << bits)).astype(np.float32)
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+@pytest.mark.parametrize("channels", [1, 2, 4])
+def test_remap_f32_exact_on_dyadic_coordinates(rng, interp, border, channels):
+    # with fractions of 1/16 every float step is exact, so fused and unfused
+    # lerps agree and the result is the exact rational rounded half away from
+    # zero: exact .5 ties (frequent here) tell FCVTA from ties-to-even and
+    # truncation, negative coordinates tell floor from truncation
+    h, w = 9, 13
+    src = rng.integers(0, 256, (h, w, channels), dtype=np.uint8)
+    mapx = dyadic(rng, -3, w + 2, (12, 20))
+    mapy = dyadic(rng, -3, h + 2, (12, 20))
+    bv = (11, 22, 33, 44)
+    err, out = O.remap_f32(src, mapx, mapy, interp, border, bv)
+    assert err == 0
+    want = pins.remap_f32_exact(src, mapx, mapy, interp, border, bv)
+    np.testing.assert_array_equal(out, want)
+    # half-way fractions do occur, so the rounding mode is exercised
+    assert (np.abs(mapx * 2 - np.round(mapx * 2)) == 0).mean() > 0.05
+
+
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_f32_special_coordinates(border):
+    # NaN, infinities, ties, negative values and the far edge: results the
+    # AArch64 conversion rules fix (NaN -> 0 for the taps and the lerp, ties
+    # away from zero, saturating conversions)
+    src = (np.arange(4 * 6).reshape(4, 6) * 9 + 3).astype(np.uint8)
+    vals = [np.nan, np.inf, -np.inf, -0.5, -0.25, -1.0, 0.5, 1.5, 2.5, 4.75, 5.0, 5.5, 6.0,
+            1e30, -1e30, 3.0]
+    xs, ys = np.meshgrid(np.array(vals, np.float32), np.array(vals, np.float32))
+    for interp in ("LINEAR", "NEAREST"):
+        err, out = O.remap_f32(src, xs.astype(np.float32), ys.astype(np.float32), interp,
+                               border, (77,))
+        assert err == 0
+        want = pins.remap_f32_exact(src, xs, ys, interp, border, (77,))
+        np.testing.assert_array_equal(out, want)
+    # spot values derived by hand: NaN gives 0 when interpolating
+    _, out = O.remap_f32(src, np.full((1, 4), np.nan, np.float32), np.zeros((1, 4), np.float32),
+                         "LINEAR", border, (77,))
+    assert (out == 0).all()
+
+
+def test_remap_f32_fuses_its_multiply_adds(rng):
+    # The fused (vmla) and unfused lerps round differently in about one
+    # pixel in 10^6, when a result lies within an ulp of a half-way point.
+    # Such cases are found by a vectorised float32 search over 2x2 patches
+    # (coordinates in [0, 1): full-precision fractions) and confirmed with
+    # the exact model; on each, the oracle must give the fused result.
+    srng = np.random.default_rng(5)
+    n = 4_000_000
+    a, b, c, d = (srng.integers(0, 256, n).astype(np.float32) for _ in range(4))
+    x, y = srng.random(n, dtype=np.float32), srng.random(n, dtype=np.float32)
+
+    def fused(p, q, r):
+        return (p.astype(np.float64) * q.astype(np.float64) + r).astype(np.float32)
+
+    def unfused(p, q, r):
+        return (p * q).astype(np.float32) + r
+
+    def lerp(fma):
+        l0, l1 = fma(b - a, x, a), fma(d - c, x, c)
+        return np.floor(fma((l1 - l0).astype(np.float32), y, l0).astype(np.float64) + 0.5)
+    found = np.nonzero(lerp(fused) != lerp(unfused))[0]
+    checked = 0
+    for i in found:
+        src = np.array([[a[i], b[i]], [c[i], d[i]]], np.uint8)
+        mx = np.full((1, 4), x[i], np.float32)
+        my = np.full((1, 4), y[i], np.float32)
+        f = pins.remap_f32_exact(src, mx, my, "LINEAR", "REPLICATE", (0,), fused=True)
+        u = pins.remap_f32_exact(src, mx, my, "LINEAR", "REPLICATE", (0,), fused=False)
+        if np.array_equal(f, u):
+            continue  # the float64 search rounded twice; not a real split
+        _, out = O.remap_f32(src, mx, my, "LINEAR", "REPLICATE", (0,))
+        np.testing.assert_array_equal(out, f)
+        checked += 1
+    assert checked >= 3, (len(found), checked)
+    # and on ordinary coordinates the oracle equals the fused model
+    src = rng.integers(0, 256, (16, 24), dtype=np.uint8)
+    mapx = rng.uniform(-1, 24, (12, 20)).astype(np.float32)
+    mapy = rng.uniform(-1, 16, (12, 20)).astype(np.float32)
+    for border in ("REPLICATE", "CONSTANT"):
+        _, out = O.remap_f32(src, mapx, mapy, "LINEAR", border, (5,))
+        np.testing.assert_array_equal(
+            out, pins.remap_f32_exact(src, mapx, mapy, "LINEAR", border, (5,)))
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_warp_exact_on_dyadic_transforms(rng, interp, border):
+    # affine transforms with dyadic entries and w = 1 or 2: every step is
+    # exact, so the result is fixed by the mathematics and the rounding rules
+    src = rng.integers(0, 256, (20, 28, 3), dtype=np.uint8)
+    for tw in (1.0, 2.0):
+        T = np.array([[1.25 * tw, 0.375 * tw, -2.5 * tw], [-0.25 * tw, 0.875 * tw, 1.5 * tw],
+                      [0.0, 0.0, tw]], np.float32)
+        err, out = O.warp_perspective(src, T, 24, 18, interp, border, (9, 8, 7))
+        assert err == 0
+        np.testing.assert_array_equal(
+            out, pins.warp_exact(src, T, 24, 18, interp, border, (9, 8, 7)))
+
+
+@pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
+def test_warp_equals_fused_model(rng, interp):
+    # perspective homographies: the oracle equals the exactly-rounded model
+    # of the reference's operation sequence with fused multiply-adds
+    src = rng.integers(0, 256, (30, 40), dtype=np.uint8)
+    T = random_homography(rng, 40, 30, persp=2e-3)
+    for border in ("REPLICATE", "CONSTANT"):
+        _, out = O.warp_perspective(src, T, 36, 26, interp, border, (77,))
+        np.testing.assert_array_equal(
+            out, pins.warp_exact(src, T, 36, 26, interp, border, (77,)))
+
+
+def test_threaded_oracle_helpers_equal_whole_calls(rng):
+    # the full-size GPU tests check against the oracle's row stripes on a
+    # thread pool; the stripes must give exactly the whole call
+    src = rng.integers(0, 256, (133, 157, 3), dtype=np.uint8)
+    kx, ky = [1, 2, 0, 1, 0, 0, 1], [0, 1, 0, 0, 1, 1, 0]
+    T = random_homography(rng, 157, 133, persp=1e-3)
+    pairs = [
+        (O.gaussian_blur(src, 5, 0.0, "REFLECT"), O.gaussian_blur_mt(src, 5, 0.0, "REFLECT")),
+        (O.median_blur(src, 5, "REVERSE"), O.median_blur_mt(src, 5, "REVERSE")),
+        (O.separable_filter_2d(src, kx, ky), O.separable_filter_2d_mt(src, kx, ky)),
+        (O.resize_linear(src, 301, 250), O.resize_linear_mt(src, 301, 250)),
+        (O.warp_perspective(src, T, 140, 120, "LINEAR", "CONSTANT", (1, 2, 3)),
+         O.warp_perspective_mt(src, T, 140, 120, "LINEAR", "CONSTANT", (1, 2, 3))),
+    ]
+    mxy = rng.integers(-10, 170, (120, 140, 2)).astype(np.int16)
+    mfr = rng.integers(0, 1024, (120, 140)).astype(np.uint16)
+    s1 = np.ascontiguousarray(src[..., 0])
+    pairs.append((O.remap_s16point5(s1, mxy, mfr, "CONSTANT", (5,)),
+                  O.remap_s16point5_mt(s1, mxy, mfr, "CONSTANT", (5,))))
+    mx = rng.uniform(-5, 165, (120, 140)).astype(np.float32)
+    my = rng.uniform(-5, 140, (120, 140)).astype(np.float32)
+    pairs.append((O.remap_f32(src, mx, my), O.remap_f32_mt(src, mx, my)))
+    y = rng.integers(0, 256, (133, 158), dtype=np.uint8)
+    uv = rng.integers(0, 256, (67, 158), dtype=np.uint8)
+    pairs.append((O.yuv_sp_to_rgb(y, uv, 0x52), O.yuv_sp_to_rgb_mt(y, uv, 0x52)))
+    for i, ((e1, a), (e2, b)) in enumerate(pairs):
+        assert e1 == e2 == 0, i
+        np.testing.assert_array_equal(a, b, err_msg=str(i))

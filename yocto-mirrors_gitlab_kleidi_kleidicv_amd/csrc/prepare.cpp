@@ -111,12 +111,13 @@ bool gaussian_is_implemented(size_t w, size_t h, size_t kw, size_t kh, float sx,
 }
 
 // Host half-kernel for sigma != 0 (include/kleidicv/filters/sigma.h:27-79).
-// The reference computes `hk * multiplier - error` as one contracted
-// expression, hence the explicit fused multiply-subtract.
+// The reference computes the default sigma `ksize * 0.15 + 0.35` and
+// `hk * multiplier - error` as single expressions that clang contracts on
+// AArch64, hence the explicit fused multiply-adds.
 bool gaussian_half_kernel(uint8_t *half, size_t half_size, float sigma) {
   const size_t mid = half_size - 1;
   const size_t ksize = mid * 2 + 1;
-  if (sigma == 0.0f) sigma = static_cast<float>(ksize) * 0.15f + 0.35f;
+  if (sigma == 0.0f) sigma = std::fmaf(static_cast<float>(ksize), 0.15f, 0.35f);
   float vals[128];
   const float coefficient = 1 / -(2 * sigma * sigma);
   float sum = 0.0f;
