@@ -9,12 +9,20 @@ image per launch, inputs resident in HBM:
   warp_perspective (4 ch, seeded homography: rotation <= 15 deg,
                     perspective terms <= 1e-4, LINEAR, REPLICATE)
   yuv_semiplanar_to_rgb (NV12 -> RGB)
-Inputs rotate over several distinct sets (> 512 MB in total) so reads come
-from HBM rather than the 256 MB Infinity Cache.  Kernel time per op is taken
-inside the timed steps with HIP events on the stream the kernels run on.
+`value` is destination Mpix/s over whole steps (all seven ops back to back
+on one stream; inputs and outputs rotate over sets of more than 512 MB in
+total, so reads and writes reach HBM rather than the 256 MB Infinity
+Cache).
 
-Prints one JSON line (metric, value in destination Mpix/s, per-op GB/s and
-% of the 8 TB/s HBM peak, the dominant kernel's roofline, a CPU baseline).
+Per op, the kernel's average launch duration is measured on its own: a run
+of launches back to back on the launch stream between two HIP events, each
+launch reading inputs and writing outputs that no launch of the last
+640 MB touched.  rocprofv3 --kernel-trace durations of the same kernels
+come from a child run made before this process touches the GPU, beside
+the HBM counters (FETCH_SIZE, WRITE_SIZE).  The same measurement is made
+for a 1920x1080 Gaussian 5x5 (sigma 0, REVERSE).
+
+Prints one JSON line.
 """
 
 import argparse
@@ -35,7 +43,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 W, H = 3840, 2160
+GW, GH = 1920, 1080
 METRIC = "Mpix/s + % HBM peak per op; batched/served vs plain Mpix/s; p99 latency"
+ROTATE_BYTES = 640 << 20  # distinct bytes each op's isolated launches cycle through
 
 
 def parse():
@@ -43,13 +53,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--sets", type=int, default=3, help="distinct input sets rotated")
+    ap.add_argument("--sets", type=int, default=3, help="distinct input/output sets of the step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--ops", default="", help="comma-separated subset of ops (profiling)")
+    ap.add_argument("--isolated-only", action="store_true",
+                    help="only the per-op isolated launches (profiling child runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 512 MiB (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--no-rocprof", action="store_true",
+                    help="skip the rocprofv3 kernel-trace pass")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched workloads (pyramid and transform batches)")
     ap.add_argument("--no-serving", action="store_true",
@@ -58,56 +72,6 @@ def parse():
                     help="skip the drop-in (host buffer) entry point timings")
     ap.add_argument("--serving-requests", type=int, default=1500)
     return ap.parse_args()
-
-
-def make_inputs(torch, gen, dev):
-    """One seeded input set for every op, generated on the device."""
-    def u8(*shape):
-        return torch.randint(0, 256, shape, dtype=torch.uint8, device=dev, generator=gen)
-
-    s = {}
-    s["sep_src"] = u8(H, W)
-    s["med_src"] = u8(H, W)
-    s["up_src"] = u8(H, W, 4)
-    s["down_src"] = u8(H, W, 4)
-    s["remap_src"] = u8(H, W)
-    s["warp_src"] = u8(H, W, 4)
-    s["nv12_y"] = u8(H, W)
-    s["nv12_uv"] = u8(H // 2, W)
-    # smooth displacement, amplitude <= 40 px: sum of 4 seeded sinusoids
-    yy, xx = torch.meshgrid(torch.arange(H, device=dev, dtype=torch.float64),
-                            torch.arange(W, device=dev, dtype=torch.float64), indexing="ij")
-    r = torch.rand(16, generator=gen, device=dev, dtype=torch.float64)
-    fx = sum(10.0 * torch.sin((0.001 + 0.01 * r[4 * i]) * xx + (0.001 + 0.01 * r[4 * i + 1]) * yy
-                              + 6.28 * r[4 * i + 2]) for i in range(4))
-    fy = sum(10.0 * torch.cos((0.001 + 0.01 * r[4 * i + 1]) * xx + (0.001 + 0.01 * r[4 * i]) * yy
-                              + 6.28 * r[4 * i + 3]) for i in range(4))
-    ix = torch.floor((xx + fx) * 32).to(torch.int64)
-    iy = torch.floor((yy + fy) * 32).to(torch.int64)
-    s["mapxy"] = torch.stack([ix >> 5, iy >> 5], dim=-1).to(torch.int16).contiguous()
-    s["mapfrac"] = ((ix & 31) | ((iy & 31) << 5)).to(torch.int16).contiguous()
-    # inputs of the extended (non-step) ops, drawn last
-    s["i420"] = u8((3 * H) // 2, W)
-    s["yuyv"] = u8(H, 2 * W)
-    s["yuv444"] = u8(H, 3 * W)
-    s["rgb"] = u8(H, W, 3)
-    return s
-
-
-def seeded_params(np):
-    rng = np.random.default_rng(7)
-    kx = rng.integers(0, 3, 7).astype(np.uint8)
-    ky = rng.integers(0, 3, 7).astype(np.uint8)
-    a = np.deg2rad(rng.uniform(-15, 15))
-    cx, cy = W / 2, H / 2
-    R = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
-    Tc = np.array([[1, 0, cx], [0, 1, cy], [0, 0, 1]])
-    Ti = np.array([[1, 0, -cx], [0, 1, -cy], [0, 0, 1]])
-    P = np.array([[1, 0, 0], [0, 1, 0],
-                  [rng.uniform(-1e-4, 1e-4), rng.uniform(-1e-4, 1e-4), 1]])
-    T = (Tc @ R @ Ti) @ P
-    T = T / T[2, 2]
-    return kx, ky, T.astype(np.float32)
 
 
 # destination pixels and algorithmic (compulsory) bytes per launch
@@ -121,48 +85,207 @@ OPS = [
     ("warp_perspective_4ch", W * H, 8 * W * H),
     ("nv12_to_rgb", W * H, (W * H * 9) // 2),
 ]
-
+GAUSS_1080P = ("gaussian_5x5_1080p", GW * GH, 2 * GW * GH)
+ALL_OPS = OPS + [GAUSS_1080P]
 
 # Library kernels launched by each op (substrings of the kernel names).
 OP_KERNELS = {
-    "separable_7x7_1ch": ["separable_kernel"],
-    "median_5x5_1ch": ["median_kernel"],
+    "separable_7x7_1ch": ["separable_kernel<2, 7, 1>"],
+    "median_5x5_1ch": ["median_kernel<5, 1>"],
     "resize_2x_up_4ch": ["resize_up2x4_kernel"],
-    "resize_0.5x_4ch": ["resize_quarter_kernel"],
-    "remap_s16point5_1ch": ["remap_s16point5_kernel"],
-    "warp_perspective_4ch": ["warp_kernel"],
-    "nv12_to_rgb": ["nv12_kernel"],
+    "resize_0.5x_4ch": ["resize_quarter_kernel<4>"],
+    "remap_s16point5_1ch": ["remap_s16point5_kernel<1, 0>"],
+    "warp_perspective_4ch": ["warp_kernel<4, 1, 1>"],
+    "nv12_to_rgb": ["nv12_kernel<3>"],
+    "gaussian_5x5_1080p": ["separable_kernel<0, 5, 1>"],
+}
+
+# input and output tensors of one launch: name -> ([(key, shape)], output shape)
+def op_shapes(name):
+    return {
+        "separable_7x7_1ch": ([("src", (H, W))], (H, W)),
+        "median_5x5_1ch": ([("src", (H, W))], (H, W)),
+        "resize_2x_up_4ch": ([("src", (H, W, 4))], (2 * H, 2 * W, 4)),
+        "resize_0.5x_4ch": ([("src", (H, W, 4))], (H // 2, W // 2, 4)),
+        "remap_s16point5_1ch": ([("src", (H, W)), ("maps", None)], (H, W)),
+        "warp_perspective_4ch": ([("src", (H, W, 4))], (H, W, 4)),
+        "nv12_to_rgb": ([("y", (H, W)), ("uv", (H // 2, W))], (H, W, 3)),
+        "gaussian_5x5_1080p": ([("src", (GH, GW))], (GH, GW)),
+    }[name]
+
+
+def call_op(K, name, inp, out, params, stream):
+    kx, ky, T = params
+    if name == "separable_7x7_1ch":
+        K.separable_filter_2d(inp["src"], kx, ky, "REPLICATE", dst=out, stream=stream,
+                              device_api=True)
+    elif name == "median_5x5_1ch":
+        K.median_blur(inp["src"], 5, "REPLICATE", dst=out, stream=stream, device_api=True)
+    elif name == "resize_2x_up_4ch":
+        K.resize_linear(inp["src"], 2 * W, 2 * H, dst=out, stream=stream, device_api=True)
+    elif name == "resize_0.5x_4ch":
+        K.resize_linear(inp["src"], W // 2, H // 2, dst=out, stream=stream, device_api=True)
+    elif name == "remap_s16point5_1ch":
+        mapxy, mapfrac = inp["maps"]
+        K.remap_s16point5(inp["src"], mapxy, mapfrac, "CONSTANT", (0, 0, 0, 0), dst=out,
+                          stream=stream, device_api=True)
+    elif name == "warp_perspective_4ch":
+        K.warp_perspective(inp["src"], T, W, H, "LINEAR", "REPLICATE", dst=out, stream=stream,
+                           device_api=True)
+    elif name == "nv12_to_rgb":
+        K.yuv_sp_to_rgb(inp["y"], inp["uv"], K.NV12_TO_RGB, dst=out, stream=stream,
+                        device_api=True)
+    elif name == "gaussian_5x5_1080p":
+        K.gaussian_blur(inp["src"], 5, 0.0, "REVERSE", dst=out, stream=stream, device_api=True)
+
+
+class InputPool:
+    """Seeded uniform u8 bytes on the device; images are views at distinct
+    offsets, so cycling through them touches fresh memory each launch."""
+
+    def __init__(self, torch, dev, gen, nbytes):
+        self.buf = torch.randint(0, 256, (nbytes,), dtype=torch.uint8, device=dev, generator=gen)
+        self.off = 0
+
+    def image(self, shape):
+        n = 1
+        for d in shape:
+            n *= d
+        if self.off + n > self.buf.numel():
+            self.off = 0
+        v = self.buf[self.off:self.off + n].view(shape)
+        self.off += (n + 4095) // 4096 * 4096
+        return v
+
+
+def op_inputs(torch, dev, gen, pool, WL, name, count):
+    """`count` distinct input sets of one op."""
+    specs, _ = op_shapes(name)
+    sets = []
+    for _ in range(count):
+        s = {}
+        for key, shape in specs:
+            if key == "maps":
+                s[key] = WL.s16point5_maps(torch, gen, dev, W, H, amplitude=40.0)
+            else:
+                s[key] = pool.image(shape)
+        sets.append(s)
+    return sets
+
+
+def input_bytes(name):
+    specs, _ = op_shapes(name)
+    n = 0
+    for key, shape in specs:
+        if key == "maps":
+            n += 6 * W * H
+        else:
+            m = 1
+            for d in shape:
+                m *= d
+            n += m
+    return n
+
+
+def out_bytes(name):
+    m = 1
+    for d in op_shapes(name)[1]:
+        m *= d
+    return m
+
+
+def bench_isolated(K, torch, WL, dev, stream, names, params, launches, warmup):
+    """Per op: `launches` back-to-back launches between two HIP events on
+    the launch stream; ms per launch.  Inputs and outputs cycle through at
+    least ROTATE_BYTES of distinct memory each."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4321)
+    pool = InputPool(torch, dev, gen, ROTATE_BYTES + (64 << 20))
+    res = {}
+    for name in names:
+        n_in = max(3, min(256, -(-ROTATE_BYTES // input_bytes(name))))
+        n_out = max(2, min(256, -(-ROTATE_BYTES // out_bytes(name))))
+        ins = op_inputs(torch, dev, gen, pool, WL, name, n_in)
+        outs = [torch.empty(op_shapes(name)[1], dtype=torch.uint8, device=dev)
+                for _ in range(n_out)]
+        torch.cuda.synchronize()
+        for i in range(warmup):
+            call_op(K, name, ins[i % n_in], outs[i % n_out], params, stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for i in range(launches):
+            k = warmup + i
+            call_op(K, name, ins[k % n_in], outs[k % n_out], params, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        res[name] = {"ms": e0.elapsed_time(e1) / launches, "launches": launches,
+                     "input_sets": n_in, "output_sets": n_out}
+        if name == "separable_7x7_1ch":
+            res[name]["saturated_fraction"] = round(float((outs[0] == 255).float().mean()), 4)
+        del ins, outs
+        torch.cuda.empty_cache()
+    return res
+
+
+def make_step_sets(torch, gen, dev, WL, n):
+    """`n` input sets and output sets of the whole op set for the step."""
+    sets, outs = [], []
+    for _ in range(n):
+        s = {}
+        for name, _, _ in OPS:
+            specs, _ = op_shapes(name)
+            d = {}
+            for key, shape in specs:
+                if key == "maps":
+                    d[key] = WL.s16point5_maps(torch, gen, dev, W, H, amplitude=40.0)
+                else:
+                    d[key] = torch.randint(0, 256, shape, dtype=torch.uint8, device=dev,
+                                           generator=gen)
+            s[name] = d
+        sets.append(s)
+        outs.append({name: torch.empty(op_shapes(name)[1], dtype=torch.uint8, device=dev)
+                     for name, _, _ in OPS})
+    return sets, outs
+
+
+# Entry points beyond the headline step, timed one op at a time on
+# 3840x2160 inputs: name, dst pixels, algorithmic bytes, call.
+EXTENDED_INPUTS = {
+    "i420_to_rgb": ((3 * H) // 2, W), "yuyv_to_rgb": (H, 2 * W), "yuv444_to_rgba": (H, 3 * W),
+    "rgb_to_nv12": (H, W, 3), "gaussian_31x31_1ch": (H, W), "median_9x9_1ch": (H, W),
+    "blur_and_downsample_1ch": (H, W), "blur_and_downsample_4ch": (H, W, 4),
 }
 
 
-# Entry points beyond the headline step, timed one op at a time on the same
-# 3840x2160 inputs: name, dst pixels, algorithmic bytes, call.
 def extended_ops(K, s, out, stream):
-    def bd(src, name):
-        return lambda: K.blur_and_downsample(src, "REVERSE", dst=out[name], stream=stream,
+    def bd(name):
+        return lambda: K.blur_and_downsample(s[name], "REVERSE", dst=out[name], stream=stream,
                                              device_api=True)
-    def yuv(src, name, fmt):
-        return lambda: K.yuv_to_rgb(src, W, H, fmt, dst=out[name], stream=stream,
+
+    def yuv(name, fmt):
+        return lambda: K.yuv_to_rgb(s[name], W, H, fmt, dst=out[name], stream=stream,
                                     device_api=True)
     return [
-        ("i420_to_rgb", W * H, (W * H * 3) // 2 + 3 * W * H, yuv(s["i420"], "i420_to_rgb", 0x03)),
-        ("yuyv_to_rgb", W * H, 2 * W * H + 3 * W * H, yuv(s["yuyv"], "yuyv_to_rgb", 0x04)),
-        ("yuv444_to_rgba", W * H, 3 * W * H + 4 * W * H,
-         yuv(s["yuv444"], "yuv444_to_rgba", 0x41)),
+        ("i420_to_rgb", W * H, (W * H * 3) // 2 + 3 * W * H, yuv("i420_to_rgb", 0x03)),
+        ("yuyv_to_rgb", W * H, 2 * W * H + 3 * W * H, yuv("yuyv_to_rgb", 0x04)),
+        ("yuv444_to_rgba", W * H, 3 * W * H + 4 * W * H, yuv("yuv444_to_rgba", 0x41)),
         ("rgb_to_nv12", W * H, 3 * W * H + (W * H * 3) // 2,
-         lambda: K.rgb_to_yuv_semiplanar(s["rgb"], 0x02, y=out["rgb_to_nv12"][0],
+         lambda: K.rgb_to_yuv_semiplanar(s["rgb_to_nv12"], 0x02, y=out["rgb_to_nv12"][0],
                                          uv=out["rgb_to_nv12"][1], stream=stream,
                                          device_api=True)),
         ("gaussian_31x31_1ch", W * H, 2 * W * H,
-         lambda: K.gaussian_blur(s["sep_src"], 31, 0.0, "REPLICATE", dst=out["gaussian_31x31_1ch"],
-                                 stream=stream, device_api=True)),
+         lambda: K.gaussian_blur(s["gaussian_31x31_1ch"], 31, 0.0, "REPLICATE",
+                                 dst=out["gaussian_31x31_1ch"], stream=stream,
+                                 device_api=True)),
         ("median_9x9_1ch", W * H, 2 * W * H,
-         lambda: K.median_blur(s["med_src"], 9, "REPLICATE", dst=out["median_9x9_1ch"],
+         lambda: K.median_blur(s["median_9x9_1ch"], 9, "REPLICATE", dst=out["median_9x9_1ch"],
                                stream=stream, device_api=True)),
         ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
-         bd(s["sep_src"], "blur_and_downsample_1ch")),
+         bd("blur_and_downsample_1ch")),
         ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
-         bd(s["down_src"], "blur_and_downsample_4ch")),
+         bd("blur_and_downsample_4ch")),
     ]
 
 
@@ -180,27 +303,24 @@ def extended_outputs(torch, dev):
     }
 
 
-EXTENDED_INPUTS = ("i420", "yuyv", "yuv444", "rgb", "sep_src", "med_src", "down_src")
-
-
-def bench_extended(K, torch, sets, dev, stream, steps, warmup):
+def bench_extended(K, torch, dev, stream, steps, warmup):
     """Each extended op alone: `steps` launches between two events on the
     launch stream; ms per launch.  Every launch, warmup included, reads its
-    own copy of the inputs (copies of the input sets as needed), and a 512 MiB
-    fill before each op evicts the Infinity Cache (256 MB), so no timed launch
-    finds its input cached by an earlier one: inputs come from HBM, as in
-    the step."""
+    own inputs (views into a seeded pool of 640 MB and more), so no timed
+    launch finds its input cached by an earlier one."""
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(99)
     out = extended_outputs(torch, dev)
-    ext = [{k: s[k] for k in EXTENDED_INPUTS} for s in sets]
-    while len(ext) < warmup + steps:
-        ext.append({k: v.clone() for k, v in ext[len(ext) % len(sets)].items()})
-    scrub = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
     res = {}
-    names = [o[0] for o in extended_ops(K, ext[0], out, stream)]
-    for k, name in enumerate(names):
-        calls = [extended_ops(K, s, out, stream)[k] for s in ext]
+    for k, (name, shape) in enumerate(EXTENDED_INPUTS.items()):
+        n = 1
+        for d in shape:
+            n *= d
+        count = warmup + steps
+        pool = InputPool(torch, dev, gen, max(ROTATE_BYTES, count * ((n + 4095) // 4096 * 4096)))
+        views = [{name: pool.image(shape)} for _ in range(count)]
+        calls = [[c for c in extended_ops(K, v, out, stream) if c[0] == name][0] for v in views]
         _, pix, nbytes, _ = calls[0]
-        scrub.fill_(k)
         torch.cuda.synchronize()
         for i in range(warmup):
             calls[i][3]()
@@ -217,15 +337,20 @@ def bench_extended(K, torch, sets, dev, stream, steps, warmup):
         res[name] = {"ms": round(ms, 4), "mpix_s": round(pix / (ms * 1e-3) / 1e6, 1),
                      "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                      "bytes": nbytes}
+        del pool, views, calls
+    torch.cuda.empty_cache()
     return res
+
+
+def _child(args_list):
+    return [sys.executable, os.path.abspath(__file__)] + args_list
 
 
 def _pmc_pass(exe, counter, extra, td, tag):
     """One rocprofv3 --pmc run of this script; mean counter value per kernel name."""
     out = os.path.join(td, counter + "_" + tag)
     env = dict(os.environ, TMPDIR="/tmp")
-    cmd = [exe, "--pmc", counter, "-f", "csv", "-d", out, "-o", "pmc", "--",
-           sys.executable, os.path.abspath(__file__)] + extra
+    cmd = [exe, "--pmc", counter, "-f", "csv", "-d", out, "-o", "pmc", "--"] + _child(extra)
     subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
                    stderr=subprocess.DEVNULL, timeout=240, check=True)
     files = glob.glob(os.path.join(out, "**", "*counter_collection.csv"), recursive=True)
@@ -236,11 +361,15 @@ def _pmc_pass(exe, counter, extra, td, tag):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def measure_traffic(args, ops):
+def _isolated_args(names):
+    return ["--isolated-only", "--steps", "6", "--warmup", "2", "--ops", ",".join(names)]
+
+
+def measure_traffic(names):
     """HBM bytes per launch of each op from two rocprofv3 --pmc passes.
 
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are read in separate runs of
-    this script over the same ops, and each is divided by what the same
+    this script's isolated launches, and each is divided by what the same
     counter reads per byte for a plain 512 MiB device copy.  Runs before this
     process touches the GPU, so the profiled children never share it with
     the timed run.
@@ -248,9 +377,6 @@ def measure_traffic(args, ops):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None, "rocprofv3 not found"
-    names = ",".join(o[0] for o in ops)
-    op_args = ["--steps", "2", "--warmup", "1", "--sets", str(args.sets), "--no-cpu-baseline",
-               "--no-traffic", "--ops", names]
     try:
         with tempfile.TemporaryDirectory(dir="/tmp") as td:
             per = {}
@@ -260,8 +386,8 @@ def measure_traffic(args, ops):
                 if not kib:
                     return None, "copy kernel not seen by rocprofv3"
                 factor = max(kib) * 1024.0 / (512 << 20)
-                vals = _pmc_pass(exe, counter, op_args, td, "ops")
-                for name, _, _ in ops:
+                vals = _pmc_pass(exe, counter, _isolated_args(names), td, "ops")
+                for name in names:
                     tot = 0.0
                     for sub in OP_KERNELS[name]:
                         tot += sum(v for k, v in vals.items() if sub in k and "kcv::" in k)
@@ -272,43 +398,53 @@ def measure_traffic(args, ops):
     return per, None
 
 
-def run_op(K, torch, name, s, out, kx, ky, T, stream):
-    if name == "separable_7x7_1ch":
-        K.separable_filter_2d(s["sep_src"], kx, ky, "REPLICATE", dst=out[name], stream=stream,
-                              device_api=True)
-    elif name == "median_5x5_1ch":
-        K.median_blur(s["med_src"], 5, "REPLICATE", dst=out[name], stream=stream, device_api=True)
-    elif name == "resize_2x_up_4ch":
-        K.resize_linear(s["up_src"], 2 * W, 2 * H, dst=out[name], stream=stream, device_api=True)
-    elif name == "resize_0.5x_4ch":
-        K.resize_linear(s["down_src"], W // 2, H // 2, dst=out[name], stream=stream,
-                        device_api=True)
-    elif name == "remap_s16point5_1ch":
-        K.remap_s16point5(s["remap_src"], s["mapxy"], s["mapfrac"], "CONSTANT", (0, 0, 0, 0),
-                          dst=out[name], stream=stream, device_api=True)
-    elif name == "warp_perspective_4ch":
-        K.warp_perspective(s["warp_src"], T, W, H, "LINEAR", "REPLICATE", dst=out[name],
-                           stream=stream, device_api=True)
-    elif name == "nv12_to_rgb":
-        K.yuv_sp_to_rgb(s["nv12_y"], s["nv12_uv"], K.NV12_TO_RGB, dst=out[name], stream=stream,
-                        device_api=True)
+def measure_rocprof(names):
+    """Average kernel duration (us) per op from rocprofv3 --kernel-trace
+    --stats over this script's isolated launches (a child run)."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    try:
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            env = dict(os.environ, TMPDIR="/tmp")
+            cmd = [exe, "--kernel-trace", "--stats", "-f", "csv", "-d", td, "-o", "kt",
+                   "--"] + _child(_isolated_args(names))
+            subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=240, check=True)
+            files = glob.glob(os.path.join(td, "**", "*kernel_stats.csv"), recursive=True)
+            rows = [r for f in files for r in csv.DictReader(open(f))]
+    except (subprocess.SubprocessError, OSError) as e:
+        return None, "rocprofv3 pass failed: %s" % type(e).__name__
+    out = {}
+    for name in names:
+        hits = [r for r in rows if any(s in r["Name"] for s in OP_KERNELS[name])]
+        if hits:
+            calls = sum(int(r["Calls"]) for r in hits)
+            total = sum(float(r["TotalDurationNs"]) for r in hits)
+            out[name] = {"avg_us": round(total / calls / 1e3, 2), "calls": calls}
+    return out, None
 
 
-def host_api(K, np, s_host, kx, ky, T, ops, reps=3):
+def host_api(K, np, s_host, params, ops, reps=3):
     """The drop-in entry points on host (pageable numpy) buffers: each call
     copies its inputs to the device, runs, copies the result back and waits.
     Mpix/s including the host-device copies."""
+    kx, ky, T = params
     calls = {
-        "separable_7x7_1ch": lambda: K.separable_filter_2d(s_host["sep_src"], kx, ky,
-                                                           "REPLICATE"),
-        "median_5x5_1ch": lambda: K.median_blur(s_host["med_src"], 5, "REPLICATE"),
-        "resize_2x_up_4ch": lambda: K.resize_linear(s_host["up_src"], 2 * W, 2 * H),
-        "resize_0.5x_4ch": lambda: K.resize_linear(s_host["down_src"], W // 2, H // 2),
+        "separable_7x7_1ch": lambda: K.separable_filter_2d(s_host["separable_7x7_1ch"]["src"],
+                                                           kx, ky, "REPLICATE"),
+        "median_5x5_1ch": lambda: K.median_blur(s_host["median_5x5_1ch"]["src"], 5, "REPLICATE"),
+        "resize_2x_up_4ch": lambda: K.resize_linear(s_host["resize_2x_up_4ch"]["src"], 2 * W,
+                                                    2 * H),
+        "resize_0.5x_4ch": lambda: K.resize_linear(s_host["resize_0.5x_4ch"]["src"], W // 2,
+                                                   H // 2),
         "remap_s16point5_1ch": lambda: K.remap_s16point5(
-            s_host["remap_src"], s_host["mapxy"], s_host["mapfrac"].view(np.uint16), "CONSTANT"),
-        "warp_perspective_4ch": lambda: K.warp_perspective(s_host["warp_src"], T, W, H, "LINEAR",
-                                                           "REPLICATE"),
-        "nv12_to_rgb": lambda: K.yuv_sp_to_rgb(s_host["nv12_y"], s_host["nv12_uv"], 0x02),
+            s_host["remap_s16point5_1ch"]["src"], s_host["remap_s16point5_1ch"]["maps"][0],
+            s_host["remap_s16point5_1ch"]["maps"][1].view(np.uint16), "CONSTANT"),
+        "warp_perspective_4ch": lambda: K.warp_perspective(
+            s_host["warp_perspective_4ch"]["src"], T, W, H, "LINEAR", "REPLICATE"),
+        "nv12_to_rgb": lambda: K.yuv_sp_to_rgb(s_host["nv12_to_rgb"]["y"],
+                                               s_host["nv12_to_rgb"]["uv"], 0x02),
     }
     out, t_all, pix_all = {}, 0.0, 0
     for name, dst_pix, _ in ops:
@@ -324,39 +460,79 @@ def host_api(K, np, s_host, kx, ky, T, ops, reps=3):
     return out
 
 
-def cpu_baseline(np, s_host, kx, ky, T, ops, min_seconds=10.0, max_passes=4):
-    """The C++ oracle on the host cores, each op split into row stripes run by
-    a thread pool (the stripe scheduling of kleidicv_thread.cpp:60-75 with one
-    stripe per thread): full 4K passes of the op set, repeated until about
-    min_seconds of wall time (at most max_passes)."""
+def cpu_threads():
+    # the GPU box gives each GPU 16 host cores, whatever os.cpu_count()
+    # reports for the whole host
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_baseline(np, s_host, params, ops, runs=25):
+    """The C++ oracle on the host cores, each op split into one row stripe
+    per thread (the stripe scheduling of kleidicv_thread.cpp:60-75) on a
+    thread pool: every op runs `runs` times on its full 3840x2160 input, the
+    median run time per op is kept, and the rate is destination pixels over
+    the sum of the medians."""
     import oracle_lib as O
-    threads = max(1, min(16, os.cpu_count() or 1))
-    calls = _cpu_calls(O, np, s_host, kx, ky, T)
-    t_total, pix, passes = 0.0, 0, 0
+    threads = cpu_threads()
+    calls = _cpu_calls(O, np, s_host, params)
     from concurrent.futures import ThreadPoolExecutor
+    med, total = {}, 0.0
     with ThreadPoolExecutor(max_workers=threads) as pool:
-        while passes < max_passes and (passes == 0 or t_total < min_seconds):
-            passes += 1
-            for name, dst_pix, _ in ops:
-                rows, fn = calls[name]
-                bands = [(rows * i // threads, rows * (i + 1) // threads) for i in range(threads)]
+        for name, _, _ in ops:
+            rows, fn = calls[name]
+            bands = [(rows * i // threads, rows * (i + 1) // threads) for i in range(threads)]
+            bands = [b for b in bands if b[1] > b[0]]
+            ts = []
+            for _ in range(runs + 1):
                 t0 = time.perf_counter()
-                errs = list(pool.map(lambda b: fn(*b), [b for b in bands if b[1] > b[0]]))
-                t_total += time.perf_counter() - t0
+                errs = list(pool.map(lambda b: fn(*b), bands))
+                ts.append(time.perf_counter() - t0)
                 assert all(e == 0 for e in errs), (name, errs)
-                pix += dst_pix
-    return {"value": round(pix / t_total / 1e6, 2), "unit": "Mpix/s", "cores": threads,
-            "kind": "port",
-            "sample": "%d full 3840x2160 pass(es) of each of the %d ops, C++ oracle in %d "
-                      "row stripes on %d threads (%.1f s)" % (passes, len(ops), threads,
-                                                              threads, t_total)}
+            ts = sorted(ts[1:])  # the first run warms the caches
+            med[name] = ts[len(ts) // 2]
+            total += sum(ts)
+    pix = sum(p for n, p, _ in ops)
+    value = pix / sum(med.values()) / 1e6
+    return {"value": round(value, 2), "unit": "Mpix/s", "cores": threads, "kind": "port",
+            "sample": "each of the %d ops on its full 3840x2160 input, %d runs per op (median "
+                      "kept), C++ oracle in %d row stripes on %d threads (%.1f s of runs)"
+                      % (len(ops), runs, threads, threads, total),
+            "per_op_ms": {n: round(t * 1e3, 2) for n, t in med.items()}}
 
 
-def _cpu_calls(O, np, s, kx, ky, T):
+def cpu_gaussian_1080p(np, src, runs=25):
+    import oracle_lib as O
+    import ctypes as C
+    L = O.lib()
+    dst = np.empty_like(src)
+    threads = cpu_threads()
+    sz = C.c_size_t
+    p_src, p_dst = C.c_void_p(src.ctypes.data), C.c_void_p(dst.ctypes.data)
+
+    def fn(a, b):
+        return L.orc_gaussian_blur_u8_stripe(p_src, sz(GW), p_dst, sz(GW), sz(GW), sz(GH),
+                                             sz(1), sz(5), C.c_float(0.0),
+                                             C.c_int(O.BORDER["REVERSE"]), sz(a), sz(b))
+    from concurrent.futures import ThreadPoolExecutor
+    bands = [(GH * i // threads, GH * (i + 1) // threads) for i in range(threads)]
+    ts = []
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        for _ in range(runs + 1):
+            t0 = time.perf_counter()
+            assert all(e == 0 for e in pool.map(lambda b: fn(*b), bands))
+            ts.append(time.perf_counter() - t0)
+    ts = sorted(ts[1:])
+    t = ts[len(ts) // 2]
+    return {"value": round(GW * GH / t / 1e6, 2), "unit": "Mpix/s", "cores": threads,
+            "kind": "port", "sample": "%d runs of the 1920x1080 image (median kept)" % runs}
+
+
+def _cpu_calls(O, np, s, params):
     """Per op: (rows to stripe, fn(y_begin, y_end) -> error) over the oracle's
     row-stripe entry points, or row-offset pointers where an op's output rows
     are independent (remap, NV12 row pairs)."""
     import ctypes as C
+    kx, ky, T = params
     L = O.lib()
     sz = C.c_size_t
 
@@ -364,18 +540,19 @@ def _cpu_calls(O, np, s, kx, ky, T):
         return C.c_void_p(a.ctypes.data + off)
     kxa = np.ascontiguousarray(kx, np.uint8)
     kya = np.ascontiguousarray(ky, np.uint8)
-    sep_src, med_src = s["sep_src"], s["med_src"]
+    sep_src, med_src = s["separable_7x7_1ch"]["src"], s["median_5x5_1ch"]["src"]
     sep_dst, med_dst = np.empty_like(sep_src), np.empty_like(med_src)
-    up_src, down_src = s["up_src"], s["down_src"]
+    up_src, down_src = s["resize_2x_up_4ch"]["src"], s["resize_0.5x_4ch"]["src"]
     up_dst = np.empty((2 * H, 2 * W, 4), np.uint8)
     down_dst = np.empty((H // 2, W // 2, 4), np.uint8)
-    rsrc, mxy, mfr = s["remap_src"], s["mapxy"], s["mapfrac"].view(np.uint16)
+    rsrc = s["remap_s16point5_1ch"]["src"]
+    mxy, mfr = s["remap_s16point5_1ch"]["maps"][0], s["remap_s16point5_1ch"]["maps"][1].view(np.uint16)
     rdst = np.empty((H, W), np.uint8)
     bv0 = np.zeros(4, np.uint8)
-    wsrc = s["warp_src"]
+    wsrc = s["warp_perspective_4ch"]["src"]
     wdst = np.empty((H, W, 4), np.uint8)
     t32 = np.ascontiguousarray(T, np.float32)
-    ny, nuv = s["nv12_y"], s["nv12_uv"]
+    ny, nuv = s["nv12_to_rgb"]["y"], s["nv12_to_rgb"]["uv"]
     ndst = np.empty((H, W, 3), np.uint8)
     return {
         "separable_7x7_1ch": (H, lambda a, b: L.orc_separable_filter_2d_u8_stripe(
@@ -531,16 +708,37 @@ def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 1e9)):
     return out
 
 
+def rate(pix, nbytes, ms):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "mpix_s": round(pix / (ms * 1e-3) / 1e6, 1),
+            "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4), "bytes": nbytes}
+
+
+def host_sets(s):
+    """numpy copies of one step set (maps as a pair of arrays)."""
+    out = {}
+    for name, d in s.items():
+        out[name] = {k: (tuple(t.cpu().numpy() for t in v) if isinstance(v, tuple)
+                         else v.cpu().numpy()) for k, v in d.items()}
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every op is per image and shards by "
               "image/request with no exchange; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    ops = [o for o in OPS if not args.ops or o[0] in args.ops.split(",")]
+    wanted = args.ops.split(",") if args.ops else None
+    names = [o[0] for o in ALL_OPS if wanted is None or o[0] in wanted]
+    step_ops = [o for o in OPS if wanted is None or o[0] in wanted]
     traffic, traffic_note = None, "not measured (--no-traffic)"
-    if not args.no_traffic and not args.copy_calibration:
-        traffic, traffic_note = measure_traffic(args, ops)
+    rocp, rocp_note = None, "not measured (--no-rocprof)"
+    if not args.copy_calibration and not args.isolated_only:
+        if not args.no_traffic:
+            traffic, traffic_note = measure_traffic(names)
+        if not args.no_rocprof:
+            rocp, rocp_note = measure_rocprof(names)
 
     import numpy as np
     import torch
@@ -558,85 +756,69 @@ def main():
         return
 
     import kleidicv_hip as K
+    import workloads as WL
 
     dev = torch.device("cuda:0")
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(1234)
-    kx, ky, T = seeded_params(np)
-    sets = [make_inputs(torch, gen, dev) for _ in range(args.sets)]
-    out = {
-        "separable_7x7_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
-        "median_5x5_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
-        "resize_2x_up_4ch": torch.empty(2 * H, 2 * W, 4, dtype=torch.uint8, device=dev),
-        "resize_0.5x_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
-        "remap_s16point5_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
-        "warp_perspective_4ch": torch.empty(H, W, 4, dtype=torch.uint8, device=dev),
-        "nv12_to_rgb": torch.empty(H, W, 3, dtype=torch.uint8, device=dev),
-    }
+    params = WL.plain_op_params(W, H)
     # A dedicated stream: the library launches on exactly this stream and the
     # timing events are recorded on it.
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
-    for i in range(args.warmup):
-        for name, _, _ in ops:
-            run_op(K, torch, name, sets[i % len(sets)], out, kx, ky, T, stream)
-    torch.cuda.synchronize()
+    if args.isolated_only:
+        bench_isolated(K, torch, WL, dev, stream, names, params, args.steps, args.warmup)
+        torch.cuda.synchronize()
+        print(json.dumps({"isolated": names, "launches": args.steps}))
+        return
 
-    # Whole-step time: every op of a step back to back on one stream, with
-    # nothing between the launches.
+    # Whole steps: every op of a step back to back on one stream, nothing
+    # between the launches; step i uses input and output set i % sets.
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1234)
+    sets, outs = make_step_sets(torch, gen, dev, WL, args.sets)
+    for i in range(args.warmup):
+        for name, _, _ in step_ops:
+            call_op(K, name, sets[i % len(sets)][name], outs[i % len(outs)][name], params,
+                    stream)
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     start.record(stream)
     for i in range(args.steps):
-        s = sets[i % len(sets)]
-        for name, _, _ in ops:
-            run_op(K, torch, name, s, out, kx, ky, T, stream)
+        for name, _, _ in step_ops:
+            call_op(K, name, sets[i % len(sets)][name], outs[i % len(outs)][name], params,
+                    stream)
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = start.elapsed_time(stop) / args.steps
+    s_host = host_sets(sets[0])
+    del outs
+    torch.cuda.empty_cache()
 
-    # Per-op duration: the same steps again with an event recorded on the
-    # launch stream between consecutive ops.  Each recorded event adds a
-    # fixed delay (the next op is dispatched only after the timestamp is
-    # written), which the marked steps show as their excess over the plain
-    # steps above; that excess is shared evenly among the ops, so the per-op
-    # times add up to the plain step time.  What remains of an op's time is
-    # its kernels' duration plus the gap between two launches, the figure
-    # rocprofv3 --kernel-trace reports (less that gap).
-    marks = [[torch.cuda.Event(enable_timing=True) for _ in range(len(ops) + 1)]
-             for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    for i in range(args.steps):
-        s = sets[i % len(sets)]
-        marks[i][0].record(stream)
-        for k, (name, _, _) in enumerate(ops):
-            run_op(K, torch, name, s, out, kx, ky, T, stream)
-            marks[i][k + 1].record(stream)
-    torch.cuda.synchronize()
-    marked = {}
-    for k, (name, _, _) in enumerate(ops):
-        marked[name] = sum(m[k].elapsed_time(m[k + 1]) for m in marks) / args.steps
-    event_delay = max(0.0, (sum(marked.values()) - step_ms) / len(ops))
-    op_ms = {name: marked[name] - event_delay for name in marked}
-
+    # Each op on its own: average launch duration over back-to-back launches
+    # on fresh inputs and outputs.
+    iso = bench_isolated(K, torch, WL, dev, stream, names, params, max(args.steps, 20),
+                         args.warmup)
     per_op = {}
-    total_pix = 0
-    for name, dst_pix, nbytes in ops:
-        ms = op_ms[name]
-        gbs = nbytes / (ms * 1e-3) / 1e9
-        per_op[name] = {"ms": round(ms, 4), "mpix_s": round(dst_pix / (ms * 1e-3) / 1e6, 1),
-                        "gb_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                        "bytes": nbytes}
+    for name, pix, nbytes in ALL_OPS:
+        if name not in iso:
+            continue
+        d = rate(pix, nbytes, iso[name]["ms"])
+        d["launches"] = iso[name]["launches"]
+        if rocp and name in rocp:
+            d["rocprof_us"] = rocp[name]["avg_us"]
+            d["rocprof_gb_s"] = round(nbytes / (rocp[name]["avg_us"] * 1e-6) / 1e9, 1)
         if traffic is not None:
-            per_op[name]["traffic"] = int(traffic[name]["FETCH_SIZE"] +
-                                          traffic[name]["WRITE_SIZE"])
-        total_pix += dst_pix
+            d["traffic"] = int(traffic[name]["FETCH_SIZE"] + traffic[name]["WRITE_SIZE"])
+        per_op[name] = d
+    step_names = [o[0] for o in step_ops]
+    total_pix = sum(p for _, p, _ in step_ops)
     value = total_pix / (step_ms * 1e-3) / 1e6
-    dom = max(per_op, key=lambda n: per_op[n]["ms"])
+    dom = max(step_names, key=lambda n: per_op[n]["ms"])
     d = per_op[dom]
+    kx, ky, _ = params
+    sat = iso.get("separable_7x7_1ch", {}).get("saturated_fraction")
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -652,12 +834,22 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded uniform u8 pixels, seeded maps and homography)",
         "config": {"workload": "full plain op set, one 3840x2160 u8 image per launch "
-                               "(%d ops per step, %d rotating input sets)" % (len(ops), args.sets),
-                   "ops": [o[0] for o in ops]},
-        "per_op": per_op,
-        "per_op_note": "kernel time of each op inside the timed steps: event-bracketed time "
-                       "less the per-event delay %.4f ms (marked minus plain step time, per op)"
-                       % event_delay,
+                               "(%d ops per step, %d rotating input and output sets)"
+                               % (len(step_ops), args.sets),
+                   "ops": step_names,
+                   "separable_taps": {"kx": [int(v) for v in kx], "ky": [int(v) for v in ky],
+                                      "saturated_output_fraction": sat,
+                                      "note": "seeded integers in [0, 2]; the reference's "
+                                              "separable filter has no normalising shift, so "
+                                              "random u8 pixels saturate to 255 almost "
+                                              "everywhere (the arithmetic runs in full)"}},
+        "per_op": {n: per_op[n] for n in step_names},
+        "per_op_note": "ms: one op alone, mean launch-to-launch time over back-to-back "
+                       "launches on the launch stream (HIP events), every launch on inputs "
+                       "and outputs that the last 640 MB of launches did not touch; "
+                       "rocprof_us: rocprofv3 --kernel-trace average duration of the same "
+                       "launches in a child run; traffic: FETCH_SIZE + WRITE_SIZE bytes per "
+                       "launch",
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gb_s"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
                      "traffic": d.get("traffic")},
@@ -668,21 +860,39 @@ def main():
         result["roofline"]["traffic_note"] = (
             "HBM bytes per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE, each divided by its "
             "reading per byte of a plain 512 MiB copy (factors %s)" % json.dumps(traffic["_factor"]))
+    if rocp is None:
+        result["roofline"]["rocprof_note"] = rocp_note
+    else:
+        result["roofline"]["rocprof_us"] = d.get("rocprof_us")
+    if GAUSS_1080P[0] in per_op:
+        g = dict(per_op[GAUSS_1080P[0]])
+        g["workload"] = ("Gaussian 5x5, sigma 0 (binomial 1-4-6-4-1), REVERSE border, 1 channel "
+                         "u8 1920x1080, seeded uniform pixels")
+        if not args.no_host_api:
+            src = np.random.default_rng(20250601).integers(0, 256, (GH, GW), dtype=np.uint8)
+            K.gaussian_blur(src, 5, 0.0, "REVERSE")
+            t0 = time.perf_counter()
+            for _ in range(5):
+                K.gaussian_blur(src, 5, 0.0, "REVERSE")
+            t = (time.perf_counter() - t0) / 5
+            g["including_host_copies"] = {"ms": round(t * 1e3, 3),
+                                          "mpix_s": round(GW * GH / t / 1e6, 1)}
+        if not args.no_cpu_baseline:
+            src = np.random.default_rng(20250601).integers(0, 256, (GH, GW), dtype=np.uint8)
+            g["cpu_baseline"] = cpu_gaussian_1080p(np, src)
+        result["gaussian_5x5_1080p"] = g
     if not args.ops:
-        result["extended"] = bench_extended(K, torch, sets, dev, stream, args.steps,
-                                            args.warmup)
+        result["extended"] = bench_extended(K, torch, dev, stream, args.steps, args.warmup)
         result["extended_note"] = ("entry points outside the headline step, each timed alone "
-                                   "on the same 3840x2160 inputs (launch-to-launch time)")
+                                   "on 3840x2160 inputs (launch-to-launch time)")
     if not args.no_batched and not args.ops:
-        import workloads as WL
         result["batched"] = bench_batched(K, torch, WL, dev, stream, args.steps, args.warmup)
     if not args.no_serving and not args.ops:
         result["serving"] = bench_serving(torch, dev, args.serving_requests)
-    s_host = {k: v.cpu().numpy() for k, v in sets[0].items()}
     if not args.ops and not args.no_host_api:
-        result["including_host_copies"] = host_api(K, np, s_host, kx, ky, T, ops)
+        result["including_host_copies"] = host_api(K, np, s_host, params, step_ops)
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(np, s_host, kx, ky, T, ops)
+        result["cpu_baseline"] = cpu_baseline(np, s_host, params, step_ops)
     print(json.dumps(result))
 
 

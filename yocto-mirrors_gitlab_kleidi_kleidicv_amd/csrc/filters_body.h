@@ -59,17 +59,41 @@ struct Window {
   static constexpr int kRows = kTileH + 2 * R;
 };
 
+// Column border map as one branch-free formula for the four filter
+// borders: a position left of the row maps to sl * x + bl, one right of it
+// to sr * x + br, then everything is clamped (REPLICATE is the clamp alone).
+// Equal to map_index for overhangs up to the row length.
+struct ColMap {
+  int sl, bl, sr, br, n;
+};
+
+__device__ __forceinline__ ColMap col_map(int border, int n) {
+  switch (border) {
+    case kReflect: return ColMap{-1, -1, -1, 2 * n - 1, n};
+    case kWrap: return ColMap{1, n, 1, -n, n};
+    case kReverse: return ColMap{-1, 0, -1, 2 * n - 2, n};
+    default: return ColMap{0, 0, 0, n - 1, n};
+  }
+}
+
+__device__ __forceinline__ int map_col(const ColMap &m, int x) {
+  const int r = x < 0 ? m.sl * x + m.bl : (x >= m.n ? m.sr * x + m.br : x);
+  return min(max(r, 0), m.n - 1);
+}
+
 // Stages rows [y0-R, y0+kTileH+R) x bytes [e0-pad, e0+kTileW+pad) of the
 // border-extended image into `win` (row pitch Window<R>::kCols).  Rows are
 // mapped through the border first, so every 16-byte chunk that lies inside
 // the row is one 16-byte load (edge tiles and unaligned rows included); only
-// chunks that cross the left or right edge are assembled byte by byte.  Each thread issues all
-// of its loads before its first LDS store, so their latencies overlap.
-template <int R>
+// chunks that cross the left or right edge are assembled byte by byte,
+// through the branch-free column map.  The two kinds go to separate
+// registers: all of a thread's 16-byte loads are issued first and stay in
+// flight while the edge bytes are fetched, and nothing waits on a load
+// before the LDS stores.
+template <int R, int C>
 __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
                                              size_t stride, int width,
-                                             int height, int channels,
-                                             int border, int e0, int y0) {
+                                             int height, int border, int e0, int y0) {
   using G = Window<R>;
   constexpr int kChunks = G::kCols / 16;
   constexpr int kTotal = G::kRows * kChunks;
@@ -77,38 +101,51 @@ __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
   // rows and strides of any alignment: 16-byte loads at byte addresses
   // (gfx950 vector memory runs in unaligned mode)
   typedef uint4 __attribute__((aligned(1))) u4u;
-  const int row_bytes = width * channels;
+  const int row_bytes = width * C;
   const int c0 = e0 - G::kPad;
-  uint4 v[kPerThread];
+  uint4 v[kPerThread], ev[kPerThread];
+  bool inside[kPerThread];
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const int i = threadIdx.x + k * kThreads;
-    if (i >= kTotal) break;
     const int r = i / kChunks, q = i - r * kChunks;
-    const int y = map_index(y0 - R + r, height, border);
-    const uint8_t *srow = src + static_cast<size_t>(y) * stride;
+    const uint8_t *srow = src + static_cast<size_t>(map_index(y0 - R + r, height, border)) * stride;
     const int e = c0 + 16 * q;
-    if (e >= 0 && e + 16 <= row_bytes) {
-      v[k] = *reinterpret_cast<const u4u *>(srow + e);
-    } else {
+    inside[k] = i < kTotal && e >= 0 && e + 16 <= row_bytes;
+    v[k] = inside[k] ? *reinterpret_cast<const u4u *>(srow + e) : make_uint4(0, 0, 0, 0);
+  }
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const int i = threadIdx.x + k * kThreads;
+    const int r = i / kChunks, q = i - r * kChunks;
+    const int e = c0 + 16 * q;
+    ev[k] = make_uint4(0, 0, 0, 0);
+    if (i < kTotal && !inside[k]) {
+      const uint8_t *srow = src + static_cast<size_t>(map_index(y0 - R + r, height, border)) *
+                                      stride;
+      const ColMap cm = col_map(border, width);
       uint32_t w[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int b = 0; b < 16; ++b) {
         const int eb = e + b;
-        const int x = eb >= 0 ? eb / channels : -((channels - 1 - eb) / channels);
-        const int c = eb - x * channels;
-        const int xm = map_index(x, width, border);
-        w[b >> 2] |= static_cast<uint32_t>(srow[xm * channels + c]) << (8 * (b & 3));
+        const int x = eb >= 0 ? eb / C : -((C - 1 - eb) / C);
+        const int c = eb - x * C;
+        w[b >> 2] |= static_cast<uint32_t>(srow[map_col(cm, x) * C + c]) << (8 * (b & 3));
       }
-      v[k] = make_uint4(w[0], w[1], w[2], w[3]);
+      ev[k] = make_uint4(w[0], w[1], w[2], w[3]);
     }
   }
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const int i = threadIdx.x + k * kThreads;
-    if (i >= kTotal) break;
-    const int r = i / kChunks, q = i - r * kChunks;
-    *reinterpret_cast<uint4 *>(win + r * G::kCols + 16 * q) = v[k];
+    if (i < kTotal) {
+      const int r = i / kChunks, q = i - r * kChunks;
+      // a bitwise blend rather than a select: the two arrays stay in registers
+      const uint32_t m = inside[k] ? 0xFFFFFFFFu : 0u;
+      *reinterpret_cast<uint4 *>(win + r * G::kCols + 16 * q) =
+          make_uint4((v[k].x & m) | ev[k].x, (v[k].y & m) | ev[k].y, (v[k].z & m) | ev[k].z,
+                     (v[k].w & m) | ev[k].w);
+    }
   }
 }
 
@@ -255,8 +292,7 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   const int e0 = bx * kTileW;
   const int y0 = by * kTileH;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0,
-                  y0);
+  stage_window<R, C>(win, a.src, a.src_stride, a.width, a.height, a.border, e0, y0);
   __syncthreads();
 
   // Vertical pass: each item is one window dword column over two output
@@ -354,7 +390,7 @@ __device__ __forceinline__ void blur_down_tile(const SepArgs &a, int bx, int by,
   const int o0 = bx * Down<C>::kOutW;  // first output byte of the tile
   const int y0 = 2 * kDownRows * by;   // first source row of the tile
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  stage_window<2>(win, a.src, a.src_stride, a.width, a.height, C, a.border, 2 * o0, y0);
+  stage_window<2, C>(win, a.src, a.src_stride, a.width, a.height, a.border, 2 * o0, y0);
   __syncthreads();
 
   // vertical pass: output row r uses window rows 2r .. 2r + 4
@@ -642,7 +678,7 @@ __device__ __forceinline__ void median_tile_c(const MedianArgs &a, int bx, int b
   const int e0 = bx * kTileW;
   const int y0 = by * kTileH;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  stage_window<R>(win, a.src, a.src_stride, a.width, a.height, C, a.border, e0, y0);
+  stage_window<R, C>(win, a.src, a.src_stride, a.width, a.height, a.border, e0, y0);
   __syncthreads();
 
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
