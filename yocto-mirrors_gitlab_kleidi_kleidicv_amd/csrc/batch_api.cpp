@@ -245,18 +245,18 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
     }
     case kFamRemapS16: {
       const RemapS16Args &a = j->u.remap_s16;
-      j->tiles_x = static_cast<int>(div_up(a.dst_width, kGatherTileW));
-      return j->tiles_x * div_up(a.dst_height, kGatherTileH);
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, gather_tile_w(a.channels)));
+      return j->tiles_x * div_up(a.dst_height, gather_tile_h(a.channels));
     }
     case kFamRemapF32: {
       const RemapF32Args &a = j->u.remap_f32;
-      j->tiles_x = static_cast<int>(div_up(a.dst_width, kGatherTileW));
-      return j->tiles_x * div_up(a.dst_height, kGatherTileH);
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, gather_tile_w(a.channels)));
+      return j->tiles_x * div_up(a.dst_height, gather_tile_h(a.channels));
     }
     case kFamWarp: {
       const WarpArgs &a = j->u.warp;
-      j->tiles_x = static_cast<int>(div_up(a.dst_width, kGatherTileW));
-      return j->tiles_x * div_up(a.dst_height, kGatherTileH);
+      j->tiles_x = static_cast<int>(div_up(a.dst_width, gather_tile_w(a.channels)));
+      return j->tiles_x * div_up(a.dst_height, gather_tile_h(a.channels));
     }
     case kFamYuv: {
       const YuvArgs &a = j->u.yuv;

@@ -60,8 +60,14 @@ typedef uint2 __attribute__((aligned(1))) u2_any;
 typedef uint4 __attribute__((aligned(1))) u4_any;
 
 // Destination tile of the gather kernels (remap s16point5, remap f32, warp
-// perspective), per-image and batched.
-constexpr int kGatherTileW = 64, kGatherTileH = 16;  // 4 pixels per thread
+// perspective), per-image and batched: 256 threads of 4 horizontally
+// adjacent pixels.  One-channel tiles are 128 x 8 so that each tile row is
+// a whole 128-byte line (a 64-pixel row wrote half lines, which reached HBM
+// as 1.37x the output bytes); wider pixels keep 64 x 16.
+__host__ __device__ constexpr int gather_tile_w(int channels) { return channels == 1 ? 128 : 64; }
+__host__ __device__ constexpr int gather_tile_h(int channels) {
+  return 1024 / gather_tile_w(channels);
+}
 
 // Largest 1-D grid used with xcd_tile (blocks of 256 threads).
 constexpr int64_t kMaxLinearTiles = int64_t{1} << 23;

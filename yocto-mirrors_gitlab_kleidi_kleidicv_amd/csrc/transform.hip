@@ -28,17 +28,17 @@ using namespace xform;
 template <int C, int BORDER>
 __global__ __launch_bounds__(kThreads) void remap_s16point5_kernel(RemapS16Args a,
                                                                    int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (remap_s16point5_tile<C, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, gather_tile_h(C), (remap_s16point5_tile<C, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
 __global__ __launch_bounds__(kThreads) void remap_f32_kernel(RemapF32Args a, int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, gather_tile_h(C), (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
 __global__ __launch_bounds__(kThreads) void warp_kernel(WarpArgs a, int tiles_x) {
-  KCV_FOR_TILES(tiles_x, a.dst_height, kGatherTileH, (warp_tile<C, INTERP, BORDER>(a, bx, by)));
+  KCV_FOR_TILES(tiles_x, a.dst_height, gather_tile_h(C), (warp_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 struct TileGrid {
@@ -46,8 +46,8 @@ struct TileGrid {
   int tiles_x;
 };
 
-TileGrid tile_grid(int width, int height) {
-  constexpr int tile_w = kGatherTileW, tile_h = kGatherTileH;
+TileGrid tile_grid(int width, int height, int channels) {
+  const int tile_w = gather_tile_w(channels), tile_h = gather_tile_h(channels);
   const int tx = (width + tile_w - 1) / tile_w;
   const int ty = (height + tile_h - 1) / tile_h;
   if (int64_t{tx} * ty <= kMaxLinearTiles) return {dim3(tx * ty), tx};
@@ -84,7 +84,7 @@ TileGrid tile_grid(int width, int height) {
 
 hipError_t launch_remap_s16point5(const RemapS16Args &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height, a.channels);
   if (a.channels == 1) {
     if (a.border == kReplicate)
       remap_s16point5_kernel<1, kReplicate><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x);
@@ -101,7 +101,7 @@ hipError_t launch_remap_s16point5(const RemapS16Args &a, hipStream_t stream) {
 
 hipError_t launch_remap_f32(const RemapF32Args &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height, a.channels);
   KCV_FLOAT_DISPATCH(remap_f32_kernel, a, g.tiles_x, a.channels, a.interpolation, a.border,
                      g.grid, stream);
   return hipGetLastError();
@@ -109,7 +109,7 @@ hipError_t launch_remap_f32(const RemapF32Args &a, hipStream_t stream) {
 
 hipError_t launch_warp_perspective(const WarpArgs &a, hipStream_t stream) {
   if (a.dst_width <= 0 || a.dst_height <= 0) return hipSuccess;
-  const TileGrid g = tile_grid(a.dst_width, a.dst_height);
+  const TileGrid g = tile_grid(a.dst_width, a.dst_height, a.channels);
   KCV_FLOAT_DISPATCH(warp_kernel, a, g.tiles_x, a.channels, a.interpolation, a.border,
                      g.grid, stream);
   return hipGetLastError();

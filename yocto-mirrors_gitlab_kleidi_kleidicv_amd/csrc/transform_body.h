@@ -186,11 +186,12 @@ __device__ __forceinline__ uint32_t remap_s16_px_narrow(const RemapS16Args &a, u
   return out;
 }
 
-// Destination tile (bx, by) of kGatherTileW x kGatherTileH pixels.
+// Destination tile (bx, by) of gather_tile_w(C) x gather_tile_h(C) pixels.
 template <int C, int BORDER>
 __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int bx, int by) {
-  const int q = threadIdx.x % (kGatherTileW / 4), r = threadIdx.x / (kGatherTileW / 4);
-  const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
+  constexpr int TW = gather_tile_w(C), TH = gather_tile_h(C);
+  const int q = threadIdx.x % (TW / 4), r = threadIdx.x / (TW / 4);
+  const int x = bx * TW + 4 * q, y = by * TH + r;
   if (x >= a.dst_width || y >= a.dst_height) return;
   const int n = min(4, a.dst_width - x);
   uint32_t bv = 0;
@@ -529,7 +530,7 @@ __device__ __forceinline__ void store_quad_px(uint8_t *d, int n, const uint32_t 
   for (int k = 0; k < n; ++k) store_px<C>(d + k * C, o[k], (al & 3) == 0 && C == 4);
 }
 
-// Shared body of the float gathers over a kGatherTileW x kGatherTileH tile:
+// Shared body of the float gathers over a gather_tile_w(C) x gather_tile_h(C) tile:
 // a thread owns 4 horizontally adjacent pixels of one row; `coords(y, x, n,
 // xf, yf)` fills the source positions of those pixels.
 template <int C, int INTERP, int BORDER, bool BUF>
@@ -551,8 +552,9 @@ template <int C, int INTERP, int BORDER, class Coords>
 __device__ __forceinline__ void gather_tiles(const Src &s, uint8_t *dst, size_t dst_stride,
                                              int dst_width, int dst_height, int bx,
                                              int by, const Coords &coords) {
-  const int q = threadIdx.x % (kGatherTileW / 4), r = threadIdx.x / (kGatherTileW / 4);
-  const int x = bx * kGatherTileW + 4 * q, y = by * kGatherTileH + r;
+  constexpr int TW = gather_tile_w(C), TH = gather_tile_h(C);
+  const int q = threadIdx.x % (TW / 4), r = threadIdx.x / (TW / 4);
+  const int x = bx * TW + 4 * q, y = by * TH + r;
   if (x >= dst_width || y >= dst_height) return;
   const int n = min(4, dst_width - x);
   float xf[4], yf[4];
