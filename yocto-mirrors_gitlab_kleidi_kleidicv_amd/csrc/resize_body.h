@@ -832,8 +832,13 @@ __device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
 // fraction 192, output 2m+1 takes m and m+1 with fraction 64, indices
 // clamped to the image (the generic tables give the same clamped taps, and
 // lerping a value with itself returns it).  L(a, b, 192) = (a + 3b + 2) >> 2
-// and L(a, b, 64) = (3a + b + 2) >> 2 exactly, so both passes run on two
-// channels per 32-bit register (16-bit lanes, at most 1022, no carries).
+// and L(a, b, 64) = (3a + b + 2) >> 2 exactly.  Per byte, (3a + b + 2) >> 2
+// equals (a + m + r) >> 1 with the rounded mean m = (a + b + 1) >> 1 and
+// r = 1 when a + b is even (the tests check this over all byte pairs), and
+// v_lerp_u8 computes (x + y + r) >> 1 on the four bytes of a dword at once.
+// So a lerp of a 4-channel pixel is one v_lerp_u8 with the pair's mean,
+// and the mean (one v_lerp_u8 and one v_bitop3_b32) serves both lerps of
+// the pair, L(a, b, 64) and L(a, b, 192).
 //
 // Row group g in [0, h] produces output rows 2g-1 (vertical fraction 64) and
 // 2g (fraction 192) from source rows max(g-1, 0) and min(g, h-1); group 0
@@ -842,8 +847,19 @@ __device__ __forceinline__ void resize_quarter_rows(const ResizeArgs &a, int bx,
 // lanes of a wave store one contiguous 1 KiB run per output row.
 constexpr int kUp2x4ThreadPx = 2;
 
-__device__ __forceinline__ uint32_t lerp_q3(uint32_t a, uint32_t b) {  // (3a + b + 2) >> 2
-  return (((a << 1) + a + b + 0x00020002u) >> 2) & 0x00FF00FFu;
+// Rounded mean of the bytes of a and b, and the rounding bits of the
+// three-to-one lerps of the pair.
+struct ByteMean {
+  uint32_t m, r;
+};
+
+__device__ __forceinline__ ByteMean byte_mean(uint32_t a, uint32_t b) {
+  return {__builtin_amdgcn_lerp(a, b, 0x01010101u), ~(a ^ b) & 0x01010101u};
+}
+
+// (3a + b + 2) >> 2 per byte, given the mean of a and b.
+__device__ __forceinline__ uint32_t lerp_q3(uint32_t a, const ByteMean &ab) {
+  return __builtin_amdgcn_lerp(a, ab.m, ab.r);
 }
 
 __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) {
@@ -877,26 +893,24 @@ __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) 
   b[3] = px(bot, m0 + 2);
   const int n_out = min(4, 2 * (w - m0));  // output pixels of this thread per row
   const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
+  // vertical lerps: v64 = L(top, bottom, 64) for row 2g-1, v192 for row 2g
+  uint32_t v64[4], v192[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const ByteMean tb = byte_mean(t[k], b[k]);
+    v64[k] = lerp_q3(t[k], tb);
+    v192[k] = lerp_q3(b[k], tb);
+  }
 #pragma unroll
   for (int half = 0; half < 2; ++half) {
     const int y = 2 * g - 1 + half;  // half 0: fraction 64, half 1: fraction 192
     if (y < 0 || y >= a.dst_height) continue;
-    // vertical lerp of the 4 pixels, channels {0,2} and {1,3}
-    uint32_t lo[4], hi[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const uint32_t tl = t[k] & 0x00FF00FFu, th = (t[k] >> 8) & 0x00FF00FFu;
-      const uint32_t bl = b[k] & 0x00FF00FFu, bh = (b[k] >> 8) & 0x00FF00FFu;
-      lo[k] = half == 0 ? lerp_q3(tl, bl) : lerp_q3(bl, tl);
-      hi[k] = half == 0 ? lerp_q3(th, bh) : lerp_q3(bh, th);
-    }
-    uint32_t o[4];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      // output 2m: L(V[m-1], V[m], 192); output 2m+1: L(V[m], V[m+1], 64)
-      o[2 * k] = lerp_q3(lo[k + 1], lo[k]) | (lerp_q3(hi[k + 1], hi[k]) << 8);
-      o[2 * k + 1] = lerp_q3(lo[k + 1], lo[k + 2]) | (lerp_q3(hi[k + 1], hi[k + 2]) << 8);
-    }
+    const uint32_t *v = half == 0 ? v64 : v192;
+    // output 2m: L(V[m-1], V[m], 192); output 2m+1: L(V[m], V[m+1], 64)
+    const ByteMean p01 = byte_mean(v[0], v[1]), p12 = byte_mean(v[1], v[2]),
+                   p23 = byte_mean(v[2], v[3]);
+    const uint32_t o[4] = {lerp_q3(v[1], p01), lerp_q3(v[1], p12), lerp_q3(v[2], p12),
+                           lerp_q3(v[2], p23)};
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride + 8 * static_cast<size_t>(m0);
     if (n_out == 4 && out_al) {
       store_nt(reinterpret_cast<uint4 *>(out), make_uint4(o[0], o[1], o[2], o[3]));
