@@ -161,14 +161,14 @@ __global__ __launch_bounds__(kThreads) void batch_remap_s16_kernel(Segment s) {
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void batch_remap_f32_kernel(Segment s) {
+__global__ __launch_bounds__(kThreads, float_gather_blocks_per_cu(C)) void batch_remap_f32_kernel(Segment s) {
   for_each_item(s, [&](const Job &job, int bx, int by) {
     xform::remap_f32_tile<C, INTERP, BORDER>(job.u.remap_f32, bx, by);
   });
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void batch_warp_kernel(Segment s) {
+__global__ __launch_bounds__(kThreads, float_gather_blocks_per_cu(C)) void batch_warp_kernel(Segment s) {
   for_each_item(s, [&](const Job &job, int bx, int by) {
     xform::warp_tile<C, INTERP, BORDER>(job.u.warp, bx, by);
   });

@@ -69,6 +69,16 @@ __host__ __device__ constexpr int gather_tile_h(int channels) {
   return 1024 / gather_tile_w(channels);
 }
 
+// Occupancy floor of the 4-channel float gather kernels (warp perspective,
+// remap with float maps): 8 workgroups of 4 waves per CU, i.e. at most 64
+// VGPRs and 8 waves per SIMD.  Their exact float lerps keep the VALU busy,
+// and more resident waves hide more of the gather latency (4K RGBA warp:
+// 26.8 -> 24.7 us against the compiler's 68-VGPR choice).  Fewer channels
+// would spill at 64 VGPRs and keep the compiler's choice (1: no floor).
+__host__ __device__ constexpr int float_gather_blocks_per_cu(int channels) {
+  return channels == 4 ? 8 : 1;
+}
+
 // Largest 1-D grid used with xcd_tile (blocks of 256 threads).
 constexpr int64_t kMaxLinearTiles = int64_t{1} << 23;
 

@@ -32,12 +32,13 @@ __global__ __launch_bounds__(kThreads) void remap_s16point5_kernel(RemapS16Args 
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void remap_f32_kernel(RemapF32Args a, int tiles_x) {
+__global__ __launch_bounds__(kThreads, float_gather_blocks_per_cu(C)) void remap_f32_kernel(RemapF32Args a,
+                                                                              int tiles_x) {
   KCV_FOR_TILES(tiles_x, a.dst_height, gather_tile_h(C), (remap_f32_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 
 template <int C, int INTERP, int BORDER>
-__global__ __launch_bounds__(kThreads) void warp_kernel(WarpArgs a, int tiles_x) {
+__global__ __launch_bounds__(kThreads, float_gather_blocks_per_cu(C)) void warp_kernel(WarpArgs a, int tiles_x) {
   KCV_FOR_TILES(tiles_x, a.dst_height, gather_tile_h(C), (warp_tile<C, INTERP, BORDER>(a, bx, by)));
 }
 

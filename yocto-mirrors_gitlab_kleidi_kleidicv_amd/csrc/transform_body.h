@@ -444,8 +444,10 @@ __device__ __forceinline__ uint32_t round_lerp_u8(float v) {
   return round_half_away_u8(fmaxf(v, 0.0f));
 }
 
-// lerp_2d (transform_neon.h:331-337) per channel.
-template <int C>
+// lerp_2d (transform_neon.h:331-337) per channel.  FINITE: the fractions
+// are known finite (then the result lies in [0, 255] and needs no NaN
+// mapping).
+template <int C, bool FINITE = false>
 __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
   uint32_t out = 0;
 #pragma unroll
@@ -457,7 +459,7 @@ __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
     const float line0 = fmaf(pb - pa, q.xfrac, pa);
     const float line1 = fmaf(pd - pc, q.xfrac, pc);
     const float res = fmaf(line1 - line0, q.yfrac, line0);
-    out |= round_lerp_u8(res) << (8 * c);
+    out |= (FINITE ? round_half_away_u8(res) : round_lerp_u8(res)) << (8 * c);
   }
   return out;
 }
@@ -533,11 +535,36 @@ __device__ __forceinline__ void store_quad_px(uint8_t *d, int n, const uint32_t 
 // Shared body of the float gathers over a gather_tile_w(C) x gather_tile_h(C) tile:
 // a thread owns 4 horizontally adjacent pixels of one row; `coords(y, x, n,
 // xf, yf)` fills the source positions of those pixels.
+//
+// Linear gathers of a thread whose four pixels all have 0 <= x < xmax and
+// 0 <= y < ymax (so every tap is inside the image, and NaNs are excluded)
+// take a short path: under either border rule the pair starts at floor(x)
+// with taps 0 and 1, the rows are floor(y) and floor(y) + 1, and no border
+// value or clamp applies.
 template <int C, int INTERP, int BORDER, bool BUF>
 __device__ __forceinline__ void gather_quad(const Src &s, const float (&xf)[4],
                                             const float (&yf)[4], uint32_t (&out)[4]) {
   if (INTERP == 1) {
+    const float xlim = static_cast<float>(s.xmax), ylim = static_cast<float>(s.ymax);
+    bool inner = true;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      inner &= (xf[k] >= 0.0f) & (xf[k] < xlim) & (yf[k] >= 0.0f) & (yf[k] < ylim);
     Quad q[4];
+    if (inner) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const float fx = floorf(xf[k]), fy = floorf(yf[k]);
+        const uint32_t x0 = static_cast<uint32_t>(fx), y0 = static_cast<uint32_t>(fy);
+        fetch_pair<C, BUF>(s, y0, x0, q[k].a, q[k].b);
+        fetch_pair<C, BUF>(s, y0 + 1, x0, q[k].c, q[k].d);
+        q[k].xfrac = xf[k] - fx;
+        q[k].yfrac = yf[k] - fy;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) out[k] = lerp_quad<C, true>(q[k]);
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < 4; ++k) q[k] = gather_linear<C, BORDER, BUF>(s, xf[k], yf[k]);
 #pragma unroll
@@ -594,26 +621,36 @@ __device__ __forceinline__ void remap_f32_tile(const RemapF32Args &a, int bx, in
                                   coords);
 }
 
+// warp_perspective_neon.cpp:55-76 and :98-102: the row terms are shared by
+// the pixels of a row, each pixel adds its column term with one fma.
+struct WarpRow {
+  float tx0, ty0, tw0;
+};
+
+__device__ __forceinline__ WarpRow warp_row(const WarpArgs &a, int y) {
+  const float fy = static_cast<float>(y);
+  return {fmaf(a.t[1], fy, a.t[2]), fmaf(a.t[4], fy, a.t[5]), fmaf(a.t[7], fy, a.t[8])};
+}
+
+__device__ __forceinline__ void warp_px(const WarpArgs &a, const WarpRow &r, int x, float &xf,
+                                        float &yf, float &tw) {
+  const float fx = static_cast<float>(static_cast<uint32_t>(x));
+  const float tx = fmaf(fx, a.t[0], r.tx0);
+  const float ty = fmaf(fx, a.t[3], r.ty0);
+  tw = fmaf(fx, a.t[6], r.tw0);
+  const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : recip_rn(tw);
+  xf = tx * iw;
+  yf = ty * iw;
+}
+
 template <int C, int INTERP, int BORDER>
 __device__ __forceinline__ void warp_tile(const WarpArgs &a, int bx, int by) {
   const Src s = make_src<C>(a.src, a.src_stride, a.src_width, a.src_height, a.border_value);
-  // warp_perspective_neon.cpp:55-76 and :98-102: the row terms are shared by
-  // the pixels of a row, each pixel adds its column term with one fma.
   auto coords = [&](int y, int x, int, float (&xf)[4], float (&yf)[4]) {
-    const float fy = static_cast<float>(y);
-    const float tx0 = fmaf(a.t[1], fy, a.t[2]);
-    const float ty0 = fmaf(a.t[4], fy, a.t[5]);
-    const float tw0 = fmaf(a.t[7], fy, a.t[8]);
+    const WarpRow row = warp_row(a, y);
+    float tw;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const float fx = static_cast<float>(static_cast<uint32_t>(x + k));
-      const float tx = fmaf(fx, a.t[0], tx0);
-      const float ty = fmaf(fx, a.t[3], ty0);
-      const float tw = fmaf(fx, a.t[6], tw0);
-      const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : recip_rn(tw);
-      xf[k] = tx * iw;
-      yf[k] = ty * iw;
-    }
+    for (int k = 0; k < 4; ++k) warp_px(a, row, x + k, xf[k], yf[k], tw);
   };
   gather_tiles<C, INTERP, BORDER>(s, a.dst, a.dst_stride, a.dst_width, a.dst_height, bx, by,
                                   coords);
