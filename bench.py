@@ -658,12 +658,16 @@ def bench_batched(K, torch, WL, dev, stream, steps, warmup):
     return results
 
 
-def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 1e9)):
+def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 100000.0, 1e9),
+                  small_qps_list=(100000.0, 1e9)):
     """Poisson request streams through the continuous-batching server and,
-    with the same arrivals, as plain per-request launches on one stream."""
+    with the same arrivals, as plain per-request launches on one stream:
+    frames 64x64..7680x4320 at each of qps_list, and frames 64x64..512x512
+    (where per-launch costs dominate) at each of small_qps_list."""
     import serving as S
     import workloads as WL
     corpus = WL.frame_corpus(torch, dev, n=48, seed=9)
+    small_corpus = WL.frame_corpus(torch, dev, n=48, seed=9, hi=(512, 512))
     torch.cuda.synchronize()
     # Outputs of both paths come from one 8 GB device ring and are released
     # at completion (nothing consumes them), so neither path pays an
@@ -673,35 +677,42 @@ def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 1e9)):
     budget = 64 << 30
     stream = torch.cuda.Stream()
     # warm-up of both paths (kernel loading, allocator caches)
-    slots, max_ops, large_px = 2, 16, 1 << 18
-    srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
-                   large_px=large_px, release_done=True)
-    S.run_served(srv, WL.serving_requests(corpus, 100, 1e6, seed=1))
-    S.run_plain(be, WL.serving_requests(corpus, 100, 1e6, seed=2), stream, release_done=True)
+    slots, max_ops, large_px, large_batch = 2, 64, 1 << 18, 64
+
+    def server():
+        return S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
+                        large_px=large_px, large_batch=large_batch, release_done=True)
+
+    for c in (corpus, small_corpus):
+        S.run_served(server(), WL.serving_requests(c, 100, 1e6, seed=1))
+        S.run_plain(be, WL.serving_requests(c, 100, 1e6, seed=2), stream, release_done=True)
     torch.cuda.synchronize()
     out = {"workload": "%d requests, Poisson arrivals; ops uniform over gaussian 5x5, median "
                        "5x5, resize 0.5x, warp_perspective, NV12->RGB; frames from a resident "
-                       "corpus of 48, W x H log-uniform 64x64..7680x4320, C in {1,3,4}; "
-                       "latency = arrival to host-observed completion" % n,
+                       "corpus of 48, W x H log-uniform 64x64..7680x4320 (small_*: "
+                       "64x64..512x512), C in {1,3,4}; latency = arrival to host-observed "
+                       "completion" % n,
            "hbm_budget_gb": budget / 2 ** 30, "output_ring_gb": arena / 2 ** 30,
-           "slots": slots, "max_batch_ops": max_ops, "large_px": large_px}
-    for qps in qps_list:
-        reqs = WL.serving_requests(corpus, n, qps, seed=11)
-        srv = S.Server(be, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
-                       large_px=large_px, release_done=True)
+           "slots": slots, "max_batch_ops": max_ops, "large_px": large_px,
+           "large_batch": large_batch}
+    runs = [(c, q, "" if c is corpus else "small_") for c, ql in
+            ((corpus, qps_list), (small_corpus, small_qps_list)) for q in ql]
+    for c, qps, prefix in runs:
+        reqs = WL.serving_requests(c, n, qps, seed=11)
+        srv = server()
         t0 = S.run_served(srv, reqs)
         torch.cuda.synchronize()
         served = S.latency_stats(reqs, t0)
         served["batches"] = srv.batches
         served["peak_resident_gb"] = round(srv.peak / 2 ** 30, 3)
         del reqs, srv
-        plain_reqs = WL.serving_requests(corpus, n, qps, seed=11)
+        plain_reqs = WL.serving_requests(c, n, qps, seed=11)
         t0 = S.run_plain(be, plain_reqs, stream, release_done=True)
         torch.cuda.synchronize()
         plain = S.latency_stats(plain_reqs, t0)
         del plain_reqs
         torch.cuda.empty_cache()
-        key = "burst" if qps >= 1e8 else "qps_%d" % int(qps)
+        key = prefix + ("burst" if qps >= 1e8 else "qps_%d" % int(qps))
         out[key] = {
             "served": served, "plain": plain,
             "mpix_s_ratio": round(served["mpix_s"] / plain["mpix_s"], 2)}

@@ -184,6 +184,25 @@ def test_large_requests_run_alone_on_slot_zero_and_small_ones_pass_them():
     assert sorted(r.rid for r in srv.completed) == [0, 1, 2, 10, 11, 12, 13, 14]
 
 
+def test_backlogged_large_requests_share_batch_launches():
+    be = FakeBackend(polls=4)
+    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=8, clock=Clock(),
+                   large_px=64 * 64, depth=1, large_batch=3)
+    big = [req(i, h=64, w=64) for i in range(6)]
+    for r in big[:5]:
+        srv.submit(r)
+    srv.pump()
+    assert be.launches == [[0, 1, 2]]          # backlog: one batch of large_batch
+    srv.drain()
+    assert be.launches == [[0, 1, 2], [3, 4]]  # the rest of the backlog, still in order
+    srv.submit(big[5])
+    srv.drain()
+    assert be.launches[-1] == ["alone", 5]     # no backlog: alone again
+    assert [r.rid for r in srv.completed] == list(range(6))
+    with pytest.raises(ValueError):
+        S.Server(be, large_batch=0)
+
+
 def test_a_slot_queues_up_to_depth_launches():
     be = FakeBackend(polls=50)
     srv = S.Server(be, hbm_budget=1 << 30, slots=1, max_batch_ops=2, clock=Clock(), depth=3)

@@ -44,14 +44,16 @@ def oracle(r):
     return O.yuv_sp_to_rgb(src, r.uv.cpu().numpy(), K.NV12_TO_RGB)[1]
 
 
-@pytest.mark.parametrize("slots,large_px", [(1, 1 << 18), (2, 1 << 18), (3, 1 << 15)])
-def test_served_results_equal_alone_results(slots, large_px):
+@pytest.mark.parametrize("slots,large_px,large_batch",
+                         [(1, 1 << 18, 1), (2, 1 << 18, 1), (3, 1 << 15, 1), (2, 1 << 15, 8)])
+def test_served_results_equal_alone_results(slots, large_px, large_batch):
     dev = torch.device("cuda:0")
     corpus = WL.frame_corpus(torch, dev, n=10, seed=31, lo=(64, 64), hi=(480, 360))
     torch.cuda.synchronize()
-    reqs = WL.serving_requests(corpus, 150, qps=30000.0, seed=32 + slots)
+    reqs = WL.serving_requests(corpus, 150, qps=30000.0, seed=32 + slots + large_batch)
     srv = S.Server(S.TorchBackend(torch, dev), hbm_budget=1 << 30, slots=slots,
-                   max_batch_ops=24, workspace_reserve=16 << 20, large_px=large_px)
+                   max_batch_ops=24, workspace_reserve=16 << 20, large_px=large_px,
+                   large_batch=large_batch)
     S.run_served(srv, reqs)
     torch.cuda.synchronize()
     assert len(srv.completed) == len(reqs)

@@ -5,7 +5,9 @@ time.  The server keeps every buffer resident on the device and admits a
 request once its output fits the HBM budget.  Admitted requests are split by
 size.  A large request (at least `large_px` output pixels) fills the GPU on
 its own, so it is launched alone through its single-image device entry point
-on the large-request slot, in arrival order.  Small requests are packed into
+on the large-request slot, in arrival order; when large requests back up
+(the device is busy and the host sets the pace), up to `large_batch` of them
+share one batch launch.  Small requests are packed into
 batches (kleidicv_hip_batch_launch: one persistent launch per kernel variant)
 on the other stream slots, so they never wait behind a large one and the
 launch cost is shared (a lone small request also runs through its own entry
@@ -41,7 +43,7 @@ class Request:
     Y plane (h, w) and `uv` the interleaved chroma plane (h/2, w)."""
 
     __slots__ = ("rid", "op", "src", "uv", "T", "arrival", "dst", "admitted", "launched",
-                 "done", "batch")
+                 "done", "batch", "_shape")
 
     def __init__(self, rid, op, src, uv=None, T=None, arrival=0.0):
         if op not in OPS:
@@ -51,22 +53,24 @@ class Request:
         self.dst = None
         self.admitted = self.launched = self.done = None
         self.batch = None
+        h, w = src.shape[0], src.shape[1]
+        c = src.shape[2] if len(src.shape) == 3 else 1
+        if op == "resize_half":
+            self._shape = ((h + 1) // 2, (w + 1) // 2, c)
+        elif op == "nv12":
+            self._shape = (h, w, 3)
+        else:
+            self._shape = (h, w, c)
 
     def out_shape(self):
-        h, w = self.src.shape[0], self.src.shape[1]
-        c = self.src.shape[2] if len(self.src.shape) == 3 else 1
-        if self.op == "resize_half":
-            return ((h + 1) // 2, (w + 1) // 2, c)
-        if self.op == "nv12":
-            return (h, w, 3)
-        return (h, w, c)
+        return self._shape
 
     def out_bytes(self):
-        s = self.out_shape()
+        s = self._shape
         return s[0] * s[1] * s[2]
 
     def out_pixels(self):
-        s = self.out_shape()
+        s = self._shape
         return s[0] * s[1]
 
     def desc(self):
@@ -218,7 +222,7 @@ class Server:
 
     def __init__(self, backend, hbm_budget=64 << 30, slots=2, max_batch_ops=512,
                  workspace_reserve=64 << 20, tuning=None, clock=time.perf_counter,
-                 large_px=1 << 18, depth=4, release_done=False):
+                 large_px=1 << 18, depth=4, release_done=False, large_batch=1):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         if slots * workspace_reserve >= hbm_budget:
@@ -242,6 +246,9 @@ class Server:
         self.ws = [None] * slots
         self.ws_bytes = [0] * slots
         self.large_px = large_px
+        if large_batch < 1:
+            raise ValueError("large_batch must be >= 1")
+        self.large_batch = large_batch
         self.queue = collections.deque()   # arrived, waiting for budget
         self.ready = collections.deque()   # admitted small requests, waiting for a slot
         self.ready_large = collections.deque()  # admitted large requests
@@ -298,9 +305,15 @@ class Server:
     def _launch_one(self, i, now):
         """Launches one request or batch on slot i; False when nothing fits."""
         if i == 0 and self.ready_large:
-            req = self.ready_large.popleft()
-            self.be.launch_one(req, self.streams[0])
-            self._started(0, [req], None, now)
+            # Large requests go in arrival order, one per launch; with a
+            # backlog (the device is busy and the host sets the pace) up to
+            # large_batch of them share one batch launch.
+            if self.large_batch == 1 or len(self.ready_large) == 1:
+                req = self.ready_large.popleft()
+                self.be.launch_one(req, self.streams[0])
+                self._started(0, [req], None, now)
+                return True
+            self._launch_batch(0, self.ready_large, self.large_batch, now)
             return True
         if not self.ready:
             return False
@@ -310,9 +323,14 @@ class Server:
             self.be.launch_one(req, self.streams[i])
             self._started(i, [req], None, now)
             return True
+        self._launch_batch(i, self.ready, self.max_batch_ops, now)
+        return True
+
+    def _launch_batch(self, i, queue, limit, now):
+        """One batch launch on slot i of up to `limit` requests from `queue`."""
         reqs = []
-        while self.ready and len(reqs) < self.max_batch_ops:
-            reqs.append(self.ready.popleft())
+        while queue and len(reqs) < limit:
+            reqs.append(queue.popleft())
         # A slot's workspace lives in its reserve.  Launch into it; when
         # the work lists need more, grow it within the reserve or halve
         # the batch (the rest goes back to the front of the queue).
@@ -331,10 +349,9 @@ class Server:
                 raise ValueError("work lists of request %d exceed the workspace reserve"
                                  % reqs[0].rid)
             keep = len(reqs) // 2
-            self.ready.extendleft(reversed(reqs[keep:]))
+            queue.extendleft(reversed(reqs[keep:]))
             reqs, descs = reqs[:keep], descs[:keep]
         self._started(i, reqs, self.ws[i], now)
-        return True
 
     def _started(self, i, reqs, workspace, now):
         ev = self.be.record(self.streams[i])
