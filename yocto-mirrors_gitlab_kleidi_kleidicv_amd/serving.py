@@ -5,9 +5,9 @@ time.  The server keeps every buffer resident on the device and admits a
 request once its output fits the HBM budget.  Admitted requests are split by
 size.  A large request (at least `large_px` output pixels) fills the GPU on
 its own, so it is launched alone through its single-image device entry point
-on the large-request slot, in arrival order; when large requests back up
-(the device is busy and the host sets the pace), up to `large_batch` of them
-share one batch launch.  Small requests are packed into
+on the large-request slot, in arrival order; when more large requests back
+up than the slot queues (the device is busy and the host sets the pace), up
+to `large_batch` of them share one batch launch.  Small requests are packed into
 batches (kleidicv_hip_batch_launch: one persistent launch per kernel variant)
 on the other stream slots, so they never wait behind a large one and the
 launch cost is shared (a lone small request also runs through its own entry
@@ -306,9 +306,11 @@ class Server:
         """Launches one request or batch on slot i; False when nothing fits."""
         if i == 0 and self.ready_large:
             # Large requests go in arrival order, one per launch; with a
-            # backlog (the device is busy and the host sets the pace) up to
-            # large_batch of them share one batch launch.
-            if self.large_batch == 1 or len(self.ready_large) == 1:
+            # backlog of more than `depth` (more than the slot's queue
+            # holds: the host sets the pace) up to large_batch of them share
+            # one batch launch.  A batched request completes with its whole
+            # batch, so below that backlog they stay alone for latency.
+            if self.large_batch == 1 or len(self.ready_large) <= self.depth:
                 req = self.ready_large.popleft()
                 self.be.launch_one(req, self.streams[0])
                 self._started(0, [req], None, now)
