@@ -887,10 +887,29 @@ __device__ __forceinline__ void up2x4_group(const ResizeArgs &a, int bx, int g) 
     t[1] = px(top, m0); t[2] = px(top, m0 + 1);
     b[1] = px(bot, m0); b[2] = px(bot, m0 + 1);
   }
-  t[0] = px(top, m0 - 1);
-  b[0] = px(bot, m0 - 1);
-  t[3] = px(top, m0 + 2);
-  b[3] = px(bot, m0 + 2);
+  // Pixels m0 - 1 and m0 + 2 are the neighbouring lanes' pixels (DPP wave
+  // shifts); only the wave's end lanes and the image edges load them.
+  // (A neighbour lane that exited past the image end is never read: the
+  // lane before it is then at the edge and loads.)
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const uint32_t tl = __builtin_amdgcn_update_dpp(0, t[2], 0x138, 0xF, 0xF, false);  // wave_shr:1
+  const uint32_t bl = __builtin_amdgcn_update_dpp(0, b[2], 0x138, 0xF, 0xF, false);
+  const uint32_t tr = __builtin_amdgcn_update_dpp(0, t[1], 0x130, 0xF, 0xF, false);  // wave_shl:1
+  const uint32_t br = __builtin_amdgcn_update_dpp(0, b[1], 0x130, 0xF, 0xF, false);
+  if (lane == 0 || m0 == 0) {
+    t[0] = px(top, m0 - 1);
+    b[0] = px(bot, m0 - 1);
+  } else {
+    t[0] = tl;
+    b[0] = bl;
+  }
+  if (lane == 63 || m0 + 2 >= w) {
+    t[3] = px(top, m0 + 2);
+    b[3] = px(bot, m0 + 2);
+  } else {
+    t[3] = tr;
+    b[3] = br;
+  }
   const int n_out = min(4, 2 * (w - m0));  // output pixels of this thread per row
   const bool out_al = ((reinterpret_cast<uintptr_t>(a.dst) | a.dst_stride) & 15) == 0;
   // vertical lerps: v64 = L(top, bottom, 64) for row 2g-1, v192 for row 2g
