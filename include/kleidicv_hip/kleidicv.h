@@ -612,6 +612,145 @@ kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size
 /* Waits for the device to finish outstanding work, then frees the batch. */
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch);
 
+/* ------------------------------------------------------------------------
+ * 5. Serving.  No counterpart in the reference: a continuous-batching
+ *    request loop for one GPU.  Requests (an op descriptor over device
+ *    memory, as for a batch) arrive at any time; the server admits each
+ *    once its output fits the HBM budget, takes its output from a device
+ *    ring, and launches it:
+ *      - a large request (at least large_px output pixels) alone through
+ *        its single-image entry point on slot 0, in arrival order; when more
+ *        of them wait than slot 0 queues, up to large_batch of them share
+ *        one batch launch;
+ *      - small requests packed, up to max_batch_ops at a time, into one
+ *        batch launch (kleidicv_hip_batch_launch) on the other slots (on
+ *        slot 0 too when no large request waits); a lone small request runs
+ *        alone.
+ *    Each slot is a stream with up to `depth` launches queued; a request
+ *    completes when its launch's event has completed.  Every request runs
+ *    the kernels of its single-image entry point, so its output is the
+ *    same alone and in any mix.
+ *
+ *    A server is driven by one host thread (not thread-safe).  Times are
+ *    seconds on the caller's clock (kleidicv_hip_server_clock() for real
+ *    time).
+ * ------------------------------------------------------------------------ */
+
+/*  hbm_budget: bytes of outputs and workspaces held at once (inputs are the
+ *      caller's and resident already)
+ *  workspace_reserve: work-list workspace of one slot, inside the budget
+ *  large_px: output pixels from which a request is large
+ *  slots: streams (>= 1); slot 0 takes the large requests
+ *  max_batch_ops: small requests per batch launch (>= 1)
+ *  depth: launches queued per slot (>= 1)
+ *  large_batch: backlogged large requests per batch launch (1: always alone)
+ *  release_done: 1 = an output returns to the ring when its request
+ *      completes (the caller has copied or does not need it); 0 = it stays
+ *      until kleidicv_hip_server_release()
+ *  plain: 1 = the per-request baseline: every request is launched alone on
+ *      slot 0 as soon as it is admitted (no batching, no queue limit)
+ *  standin_polls: 0 = the GPU.  n > 0 = a host stand-in for tests without
+ *      a GPU: nothing touches a device, pointers are never read, a batch
+ *      needs 1024 workspace bytes per op, and a launch completes on the
+ *      n-th poll of its event
+ *  tuning: scheduling of the batch launches */
+typedef struct {
+  size_t hbm_budget;
+  size_t workspace_reserve;
+  size_t large_px;
+  unsigned slots;
+  unsigned max_batch_ops;
+  unsigned depth;
+  unsigned large_batch;
+  unsigned release_done;
+  unsigned plain;
+  unsigned standin_polls;
+  kleidicv_hip_batch_tuning tuning;
+} kleidicv_hip_server_config;
+
+typedef enum {
+  KLEIDICV_HIP_REQUEST_QUEUED = 0,   /* arrived, waiting for budget */
+  KLEIDICV_HIP_REQUEST_ADMITTED = 1, /* output held, waiting for a slot */
+  KLEIDICV_HIP_REQUEST_LAUNCHED = 2,
+  KLEIDICV_HIP_REQUEST_DONE = 3,
+  KLEIDICV_HIP_REQUEST_FAILED = 4 /* `error` says why; nothing was written */
+} kleidicv_hip_request_state;
+
+typedef struct {
+  uint64_t id;
+  int state;     /* kleidicv_hip_request_state */
+  int error;     /* kleidicv_error_t of a failed request */
+  unsigned slot;
+  unsigned alone;  /* 1: launched through its single-image entry point */
+  uint64_t launch; /* launch number, from 1, in launch order */
+  double arrival, admitted, launched, done;
+  uint8_t *dst;
+  size_t dst_stride;
+  size_t out_bytes;
+} kleidicv_hip_request_info;
+
+typedef struct {
+  uint64_t launches;
+  size_t pending;         /* requests not yet completed */
+  size_t out_in_use;      /* bytes of admitted outputs */
+  size_t workspace_bytes; /* work-list workspaces of all slots */
+  size_t peak;            /* most bytes of outputs and workspaces held at once */
+  unsigned stalled;       /* 1: requests wait for ring space that only
+                             kleidicv_hip_server_release() can return */
+} kleidicv_hip_server_counters;
+
+typedef struct kleidicv_hip_server kleidicv_hip_server;
+
+/* ring/ring_bytes: device memory the outputs of requests without a `dst`
+ * are taken from (rows of dst_width * channels bytes; NV12: 3 or 4
+ * channels); may be NULL/0 when every request brings its own dst. */
+kleidicv_error_t kleidicv_hip_server_create(const kleidicv_hip_server_config *config,
+                                            void *ring, size_t ring_bytes,
+                                            kleidicv_hip_server **server);
+
+/* Queues `count` requests that arrived at arrival[i] (NULL: now is
+ * unknown, 0); their ids are *first_id .. *first_id + count - 1.  A request
+ * whose output can never fit the budget or the ring gives
+ * KLEIDICV_ERROR_RANGE and nothing is queued. */
+kleidicv_error_t kleidicv_hip_server_submit(kleidicv_hip_server *server,
+                                            const kleidicv_hip_op_desc *ops, size_t count,
+                                            const double *arrival, uint64_t *first_id);
+
+/* One scheduling step at time `now`: retires completed launches, admits,
+ * launches.  *pending (if not NULL) counts requests not yet completed. */
+kleidicv_error_t kleidicv_hip_server_pump(kleidicv_hip_server *server, double now,
+                                          size_t *pending);
+
+/* Copies up to `capacity` completed (done or failed) requests, in
+ * completion order, into `out` and forgets them; returns how many. */
+size_t kleidicv_hip_server_take_completed(kleidicv_hip_server *server,
+                                          kleidicv_hip_request_info *out, size_t capacity);
+
+/* State of a request not yet taken. */
+kleidicv_error_t kleidicv_hip_server_info(const kleidicv_hip_server *server, uint64_t id,
+                                          kleidicv_hip_request_info *info);
+
+/* Returns a completed request's output block to the ring. */
+kleidicv_error_t kleidicv_hip_server_release(kleidicv_hip_server *server, uint64_t id);
+
+kleidicv_error_t kleidicv_hip_server_stats(const kleidicv_hip_server *server,
+                                           kleidicv_hip_server_counters *stats);
+
+/* Replays a request stream in real time: request i is submitted once
+ * offsets[i] seconds have passed since the call began (offsets ascending),
+ * and the server is pumped until every request has completed.  results[i]
+ * receives request i's final info; *t0 the clock at the start. */
+kleidicv_error_t kleidicv_hip_server_replay(kleidicv_hip_server *server,
+                                            const kleidicv_hip_op_desc *ops, size_t count,
+                                            const double *offsets,
+                                            kleidicv_hip_request_info *results, double *t0);
+
+/* Monotonic clock in seconds. */
+double kleidicv_hip_server_clock(void);
+
+/* Waits for the server's launches to finish, then frees it. */
+void kleidicv_hip_server_destroy(kleidicv_hip_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -275,10 +275,10 @@ def test_served_full_size_range_equals_alone():
     torch.cuda.synchronize()
     reqs = WL.serving_requests(corpus, 200, qps=50000.0, seed=52)
     assert max(r.src.shape[0] * r.src.shape[1] for r in reqs) == 7680 * 4320
-    # outputs are plain torch allocations (no ring to run dry); the budget
-    # is released as requests complete
-    srv = S.Server(S.TorchBackend(torch, dev()), hbm_budget=16 << 30, slots=2,
-                   max_batch_ops=16, large_px=1 << 18)
+    # every output stays in the ring until the comparison below (the budget
+    # is released as requests complete)
+    srv = S.Server(torch, dev(), hbm_budget=16 << 30, slots=2, max_batch_ops=16,
+                   large_px=1 << 18, ring_bytes=12 << 30)
     S.run_served(srv, reqs)
     torch.cuda.synchronize()
     assert len(srv.completed) == len(reqs)

@@ -1,52 +1,15 @@
-"""Scheduling logic of the serving loop, on the host.
+"""Scheduling logic of the serving engine, on the host.
 
-The server runs against a host-side backend whose "device" work completes a
-set number of polls after launch, so admission, batching and completion
-order can be checked without a GPU.
+The native engine (kleidicv_hip_server_*) runs against its host stand-in,
+whose "device" work completes a set number of polls after launch, so
+admission, batching and completion order are checked without a GPU.
 """
 
 import numpy as np
 import pytest
 
+import kleidicv_hip as K
 import serving as S
-
-
-class FakeEvent:
-    def __init__(self, polls):
-        self.left = polls
-
-
-class FakeBackend:
-    def __init__(self, polls=3):
-        self.polls = polls
-        self.launches = []
-
-    def stream(self):
-        return object()
-
-    def alloc(self, shape):
-        return np.zeros(shape, np.uint8)
-
-    def workspace(self, nbytes):
-        return np.zeros(max(int(nbytes), 1), np.uint8)
-
-    def launch(self, descs, workspace, stream, tuning, requests):
-        assert len(descs) == len(requests)
-        need = 1024 * len(descs)
-        if workspace is None or workspace.size < need:
-            return need
-        self.launches.append([r.rid for r in requests])
-        return 0
-
-    def launch_one(self, request, stream):
-        self.launches.append(["alone", request.rid])
-
-    def record(self, stream):
-        return FakeEvent(self.polls)
-
-    def query(self, ev):
-        ev.left -= 1
-        return ev.left <= 0
 
 
 class Clock:
@@ -56,6 +19,27 @@ class Clock:
     def __call__(self):
         self.t += 1.0
         return self.t
+
+
+def standin(polls, **kw):
+    """The native engine on its host stand-in (no GPU); outputs return to
+    the ring at completion unless a test says otherwise."""
+    kw.setdefault("clock", Clock())
+    kw.setdefault("release_done", True)
+    return S.Server(standin_polls=polls, **kw)
+
+
+def launches(srv):
+    """Launches in launch order, rebuilt from the completed requests: a
+    batch as its request ids, a lone launch as ["alone", id]."""
+    by = {}
+    for r in srv.completed:
+        by.setdefault(r.batch, []).append(r)
+    out = []
+    for b in sorted(by):
+        rs = sorted(by[b], key=lambda r: r.sid)
+        out.append(["alone", rs[0].rid] if rs[0].alone else [r.rid for r in rs])
+    return out
 
 
 def frame(h, w, c=1):
@@ -77,29 +61,28 @@ def test_output_shapes():
 
 
 def test_every_request_completes_once_in_fifo_admission_order():
-    be = FakeBackend(polls=2)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=4, clock=Clock())
+    srv = standin(2, hbm_budget=1 << 30, slots=2, max_batch_ops=4)
     reqs = [req(i, S.OPS[i % len(S.OPS)]) for i in range(23)]
     for r in reqs:
         srv.submit(r)
     srv.drain()
     assert sorted(r.rid for r in srv.completed) == list(range(23))
-    admitted = sorted(reqs, key=lambda r: r.admitted)
-    assert [r.rid for r in admitted] == list(range(23))
-    assert all(len(b) <= 4 for b in be.launches)
-    assert sum(len(b) for b in be.launches) == 23
+    adm = [r.admitted for r in reqs]
+    assert adm == sorted(adm)                     # FIFO: never admitted before an earlier one
+    ls = launches(srv)
+    assert all(len(b) <= 4 for b in ls if b[0] != "alone")
+    assert sum(1 if b[0] == "alone" else len(b) for b in ls) == 23
     assert all(r.admitted <= r.launched < r.done for r in reqs)
-    assert srv.out_in_use == 0 and srv.in_use == sum(srv.ws_bytes)
+    assert srv.out_in_use == 0 and srv.in_use == srv.ws_bytes == 2 * (64 << 20)
+    assert srv.pending() == 0
 
 
 def test_budget_bounds_resident_bytes_and_blocks_the_queue_head():
-    be = FakeBackend(polls=3)
     one = 64 * 64
     reserve = 4096   # work lists of up to 4 ops per slot
     # outputs of 3 requests fit beside the two reserves; the 4th waits
     budget = 3 * one + 2 * reserve
-    srv = S.Server(be, hbm_budget=budget, slots=2, max_batch_ops=8,
-                   workspace_reserve=reserve, clock=Clock())
+    srv = standin(3, hbm_budget=budget, slots=2, max_batch_ops=8, workspace_reserve=reserve)
     for i in range(10):
         srv.submit(req(i))
     peak_seen = max_out = 0
@@ -111,153 +94,149 @@ def test_budget_bounds_resident_bytes_and_blocks_the_queue_head():
         assert srv.out_in_use <= 3 * one
     assert srv.peak <= budget and peak_seen <= budget and max_out == 3 * one
     assert len(srv.completed) == 10
-    assert all(len(b) <= 3 for b in be.launches)
-    with pytest.raises(ValueError):
-        big = S.Server(FakeBackend(), hbm_budget=3 * 4096 - 1, workspace_reserve=4096,
-                       clock=Clock())
+    assert all(len(b) <= 3 for b in launches(srv) if b[0] != "alone")
+    big = standin(3, hbm_budget=3 * 4096 - 1, workspace_reserve=4096)
+    with pytest.raises(K.KleidiCVError) as e:
         big.submit(req(0))   # a 4096-byte output does not fit beside two reserves
-        big.pump()
+    assert e.value.code == K.RANGE
+    with pytest.raises(K.KleidiCVError):   # the reserves leave nothing for outputs
+        standin(3, hbm_budget=2 * 4096, workspace_reserve=4096)
 
 
 def test_batches_larger_than_the_workspace_reserve_are_split():
-    be = FakeBackend(polls=2)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=1, max_batch_ops=64,
-                   workspace_reserve=3 * 1024, clock=Clock())
+    # the stand-in's work lists take 1024 bytes per op
+    srv = standin(2, hbm_budget=1 << 30, slots=1, max_batch_ops=64, workspace_reserve=3 * 1024)
     for i in range(10):
         srv.submit(req(i))
     srv.drain()
     assert sorted(r.rid for r in srv.completed) == list(range(10))
-    assert all(len(b) <= 3 for b in be.launches)
-    assert [rid for b in be.launches for rid in b] == list(range(10))
+    ls = launches(srv)
+    assert all(len(b) <= 3 for b in ls)
+    assert [rid for b in ls for rid in (b[1:] if b[0] == "alone" else b)] == list(range(10))
 
 
-def test_arrivals_during_a_batch_overlap_it_on_another_slot():
-    be = FakeBackend(polls=6)
-    clock = Clock()
-    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=64, clock=clock)
+def test_arrivals_during_a_batch_launch_while_it_runs():
+    srv = standin(6, hbm_budget=1 << 30, slots=2, max_batch_ops=64)
     first = [req(i) for i in range(5)]
     for r in first:
         srv.submit(r)
     srv.pump()                       # batch 1 launched
-    assert len(be.launches) == 1
+    assert srv.batches == 1
     late = [req(100 + i) for i in range(3)]
     for r in late:
         srv.submit(r)
     srv.pump()                       # batch 2 launched while batch 1 runs
-    assert len(be.launches) == 2 and be.launches[1] == [100, 101, 102]
+    assert srv.batches == 2
     assert all(r.done is None for r in first)
     srv.drain()
+    assert launches(srv) == [[0, 1, 2, 3, 4], [100, 101, 102]]
     assert max(r.launched for r in late) < min(r.done for r in first)
 
 
 def test_single_slot_of_depth_one_serialises_batches():
-    be = FakeBackend(polls=4)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=1, clock=Clock(), depth=1)
+    srv = standin(4, hbm_budget=1 << 30, slots=1, depth=1)
     for i in range(3):
         srv.submit(req(i))
     srv.pump()
     srv.submit(req(9))
     srv.pump()
-    assert len(be.launches) == 1      # the only slot is busy
+    assert srv.batches == 1           # the only slot is busy
     srv.drain()
-    assert be.launches == [[0, 1, 2], ["alone", 9]]   # a lone request runs alone
+    assert launches(srv) == [[0, 1, 2], ["alone", 9]]   # a lone request runs alone
 
 
 def test_large_requests_run_alone_on_slot_zero_and_small_ones_pass_them():
-    be = FakeBackend(polls=6)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=8, clock=Clock(),
-                   large_px=64 * 64, depth=1)
+    srv = standin(6, hbm_budget=1 << 30, slots=2, max_batch_ops=8, large_px=64 * 64, depth=1)
     big = [req(i, h=64, w=64) for i in range(3)]           # 4096 px: large
     small = [req(10 + i, h=16, w=16) for i in range(5)]    # small
     for r in big[:2] + small + big[2:]:
         srv.submit(r)
     srv.pump()
-    # slot 0 took the first large request alone, slot 1 the small batch
-    assert be.launches[0] == ["alone", 0]
-    assert be.launches[1] == [10, 11, 12, 13, 14]
     srv.drain()
-    alone = [b[1] for b in be.launches if b[0] == "alone"]
+    ls = launches(srv)
+    # slot 0 took the first large request alone, slot 1 the small batch
+    assert ls[0] == ["alone", 0] and big[0].slot == 0
+    assert ls[1] == [10, 11, 12, 13, 14] and small[0].slot == 1
+    alone = [b[1] for b in ls if b[0] == "alone"]
     assert alone == [0, 1, 2]                              # large: arrival order, one at a time
-    batched = [rid for b in be.launches if b[0] != "alone" for rid in b]
+    assert all(r.slot == 0 for r in big)
+    batched = [rid for b in ls if b[0] != "alone" for rid in b]
     assert sorted(batched) == [10, 11, 12, 13, 14]
     assert max(r.done for r in small) < min(r.done for r in big[1:])
     assert sorted(r.rid for r in srv.completed) == [0, 1, 2, 10, 11, 12, 13, 14]
 
 
 def test_backlogged_large_requests_share_batch_launches():
-    be = FakeBackend(polls=4)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=8, clock=Clock(),
-                   large_px=64 * 64, depth=1, large_batch=3)
+    srv = standin(4, hbm_budget=1 << 30, slots=2, max_batch_ops=8, large_px=64 * 64, depth=1,
+                  large_batch=3)
     big = [req(i, h=64, w=64) for i in range(6)]
     for r in big[:5]:
         srv.submit(r)
     srv.pump()
-    assert be.launches == [[0, 1, 2]]          # backlog: one batch of large_batch
+    assert srv.batches == 1                    # backlog: one batch of large_batch
     srv.drain()
-    assert be.launches == [[0, 1, 2], [3, 4]]  # the rest of the backlog, still in order
+    assert launches(srv) == [[0, 1, 2], [3, 4]]  # the rest of the backlog, still in order
     srv.submit(big[5])
     srv.drain()
-    assert be.launches[-1] == ["alone", 5]     # no backlog: alone again
+    assert launches(srv)[-1] == ["alone", 5]     # no backlog: alone again
     assert [r.rid for r in srv.completed] == list(range(6))
-    with pytest.raises(ValueError):
-        S.Server(be, large_batch=0)
+    with pytest.raises(K.KleidiCVError):
+        standin(4, large_batch=0)
 
 
 def test_a_slot_queues_up_to_depth_launches():
-    be = FakeBackend(polls=50)
-    srv = S.Server(be, hbm_budget=1 << 30, slots=1, max_batch_ops=2, clock=Clock(), depth=3)
-    for i in range(10):
-        srv.submit(req(i))
+    srv = standin(50, hbm_budget=1 << 30, slots=1, max_batch_ops=2, depth=3)
+    reqs = [req(i) for i in range(10)]
+    for r in reqs:
+        srv.submit(r)
     srv.pump()
-    assert be.launches == [[0, 1], [2, 3], [4, 5]]     # three queued, none retired yet
+    assert srv.batches == 3                            # three queued, none retired yet
     assert srv.pending() == 10
     srv.drain()
-    assert [rid for b in be.launches for rid in b] == list(range(10))
+    assert launches(srv) == [[0, 1], [2, 3], [4, 5], [6, 7], [8, 9]]
     assert [r.rid for r in srv.completed] == list(range(10))   # retired in launch order
 
 
-def test_device_arena_ring():
-    a = S.DeviceArena(1024, align=64)
-    b1, b2, b3 = a.take(300), a.take(300), a.take(300)   # 320 bytes each
-    assert [b1[0], b2[0], b3[0]] == [0, 320, 640]
-    assert a.take(200) is None                          # 64 bytes left at the end
-    a.release(b2)                                       # out of order: nothing returns yet
-    assert a.take(200) is None
-    a.release(b1)                                       # [0, 640) returns
-    b4 = a.take(500)                                    # wraps to the start
-    assert b4[0] == 0
-    assert a.take(200) is None                          # [512, 640) too small
-    a.release(b3)
-    a.release(b4)
-    assert a.take(1024)[0] == 0                         # empty ring starts over
-    with pytest.raises(ValueError):
-        S.DeviceArena(128).take(129)
+def test_output_ring_returns_space_oldest_first():
+    # 16x16 outputs are one 256-byte ring block each; the ring holds four
+    srv = standin(1, hbm_budget=1 << 30, slots=1, max_batch_ops=1, ring_bytes=1024,
+                  release_done=False)
+    reqs = [req(i, h=16, w=16) for i in range(6)]
+    for r in reqs:
+        srv.submit(r)
+    with pytest.raises(RuntimeError):   # 4 and 5 wait for ring space held by 0..3
+        srv.drain()
+    assert [r.rid for r in srv.completed] == [0, 1, 2, 3]
+    assert srv.pending() == 2 and srv.counters().stalled
+    srv.release(reqs[1])            # out of order: nothing returns yet
+    for _ in range(4):
+        srv.pump()
+    assert len(srv.completed) == 4
+    srv.release(reqs[0])            # blocks 0 and 1 return: room for one block
+    with pytest.raises(RuntimeError):   # (a wrapped head never meets the tail)
+        srv.drain()
+    assert [r.rid for r in srv.completed] == [0, 1, 2, 3, 4]
+    srv.release(reqs[2])
+    srv.drain()
+    assert [r.rid for r in srv.completed] == list(range(6))
+    with pytest.raises(K.KleidiCVError):
+        srv.release(reqs[0])        # already returned and forgotten
+    with pytest.raises(K.KleidiCVError) as e:
+        standin(1, ring_bytes=1024).submit(req(7, h=64, w=64))  # larger than the whole ring
+    assert e.value.code == K.RANGE
 
 
 def test_admission_waits_for_output_buffers():
-    class Limited(FakeBackend):
-        def __init__(self):
-            super().__init__(polls=3)
-            self.live = 0
-            self.peak = 0
-
-        def alloc(self, shape):
-            if self.live == 2:
-                return None
-            self.live += 1
-            self.peak = max(self.peak, self.live)
-            return np.zeros(shape, np.uint8)
-
-        def free(self, t):
-            self.live -= 1
-
-    be = Limited()
-    srv = S.Server(be, hbm_budget=1 << 30, slots=2, max_batch_ops=4, clock=Clock(),
-                   release_done=True)
+    # a ring of two 64x64 outputs, returned at completion
+    srv = standin(3, hbm_budget=1 << 30, slots=2, max_batch_ops=4, ring_bytes=2 * 4096,
+                  release_done=True)
     for i in range(7):
         srv.submit(req(i))
-    srv.drain()
-    assert be.peak == 2 and be.live == 0
+    peak = 0
+    while srv.pending():
+        srv.pump()
+        peak = max(peak, srv.out_in_use)
+    assert peak == 2 * 4096
     assert sorted(r.rid for r in srv.completed) == list(range(7))
 
 

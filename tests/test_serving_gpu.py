@@ -44,21 +44,23 @@ def oracle(r):
     return O.yuv_sp_to_rgb(src, r.uv.cpu().numpy(), K.NV12_TO_RGB)[1]
 
 
-@pytest.mark.parametrize("slots,large_px,large_batch",
-                         [(1, 1 << 18, 1), (2, 1 << 18, 1), (3, 1 << 15, 1), (2, 1 << 15, 8)])
-def test_served_results_equal_alone_results(slots, large_px, large_batch):
+@pytest.mark.parametrize("slots,large_px,large_batch,qps",
+                         [(1, 1 << 18, 1, 1e9), (2, 1 << 18, 1, 1e9), (3, 1 << 15, 1, 1e9),
+                          (2, 1 << 15, 8, 1e9), (2, 1 << 15, 8, 30000.0)])
+def test_served_results_equal_alone_results(slots, large_px, large_batch, qps):
     dev = torch.device("cuda:0")
     corpus = WL.frame_corpus(torch, dev, n=10, seed=31, lo=(64, 64), hi=(480, 360))
     torch.cuda.synchronize()
-    reqs = WL.serving_requests(corpus, 150, qps=30000.0, seed=32 + slots + large_batch)
-    srv = S.Server(S.TorchBackend(torch, dev), hbm_budget=1 << 30, slots=slots,
+    reqs = WL.serving_requests(corpus, 150, qps=qps, seed=32 + slots + large_batch)
+    srv = S.Server(torch, dev, hbm_budget=1 << 30, slots=slots,
                    max_batch_ops=24, workspace_reserve=16 << 20, large_px=large_px,
                    large_batch=large_batch)
     S.run_served(srv, reqs)
     torch.cuda.synchronize()
     assert len(srv.completed) == len(reqs)
-    assert srv.batches > 1 and max(sum(1 for r in reqs if r.batch == b)
-                                   for b in range(1, srv.batches + 1)) > 1
+    if qps > 1e6:  # a burst: requests share batch launches
+        assert srv.batches > 1 and max(sum(1 for r in reqs if r.batch == b)
+                                       for b in range(1, srv.batches + 1)) > 1
     stream = torch.cuda.Stream()
     alone = []
     for r in reqs:
@@ -82,11 +84,44 @@ def test_plain_baseline_runs_the_same_requests():
     corpus = WL.frame_corpus(torch, dev, n=6, seed=41, lo=(64, 64), hi=(320, 240))
     torch.cuda.synchronize()
     reqs = WL.serving_requests(corpus, 40, qps=50000.0, seed=42)
-    stream = torch.cuda.Stream()
-    t0 = S.run_plain(S.TorchBackend(torch, dev), reqs, stream)
+    t0 = S.run_plain(torch, dev, reqs, hbm_budget=1 << 30)
     torch.cuda.synchronize()
     stats = S.latency_stats(reqs, t0)
     assert stats["requests"] == 40 and stats["p99_ms"] >= stats["p50_ms"] > 0
+    assert all(r.alone and r.slot == 0 for r in reqs)
+    assert sorted(r.batch for r in reqs) == list(range(1, 41))
     for r in reqs[:10]:
         want = oracle(r)
         assert_same(r.dst.cpu().numpy().reshape(want.shape), want, "plain %s" % r.op)
+
+
+def test_invalid_request_fails_alone_and_the_rest_are_served():
+    # a median of even size is declined by validation (NOT_IMPLEMENTED, as
+    # the single-image entry point declines it); it completes as failed
+    # whether it is launched alone or inside a batch, and the others run
+    dev = torch.device("cuda:0")
+    corpus = WL.frame_corpus(torch, dev, n=4, seed=61, lo=(64, 64), hi=(200, 160))
+    torch.cuda.synchronize()
+    reqs = WL.serving_requests(corpus, 30, qps=1e9, seed=62)
+
+    class Bad(S.Request):
+        __slots__ = ()
+
+        def desc(self):
+            d = super().desc()
+            d.kernel_size = 4
+            return d
+
+    f = corpus[0]
+    bad = [Bad(100 + i, "median5", f["src"], T=f["T"], arrival=0.0) for i in range(2)]
+    srv = S.Server(torch, dev, hbm_budget=1 << 30, slots=2, max_batch_ops=8,
+                   workspace_reserve=16 << 20)
+    S.run_served(srv, bad[:1] + reqs + bad[1:])
+    torch.cuda.synchronize()
+    assert all(b.error == K.NOT_IMPLEMENTED for b in bad)
+    assert all(r.error is None for r in reqs)
+    stream = torch.cuda.Stream()
+    for r in reqs:
+        a = r.run_alone(torch.empty(r.out_shape(), dtype=torch.uint8, device=dev), stream)
+        torch.cuda.synchronize()
+        assert_same(r.dst.cpu().numpy(), a.cpu().numpy(), "request %d (%s)" % (r.rid, r.op))

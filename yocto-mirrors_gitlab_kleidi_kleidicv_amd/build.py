@@ -22,7 +22,7 @@ LIB = os.path.join(HERE, "libkleidicv_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["filters.hip", "resize.hip", "transform.hip", "yuv.hip", "batch.hip",
-           "prepare.cpp", "api.cpp", "batch_api.cpp", "hal.cpp"]
+           "prepare.cpp", "api.cpp", "batch_api.cpp", "serve.cpp", "hal.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

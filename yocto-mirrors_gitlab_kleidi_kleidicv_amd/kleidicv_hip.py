@@ -373,8 +373,12 @@ def _desc(op, src, dst, stage, **kw):
     d.stage = stage
     sh, sw, sc = _shape(src)
     d.src, d.src_stride, d.src_width, d.src_height = _ptr(src).value, _stride(src), sw, sh
-    dh, dw, _ = _shape(dst)
-    d.dst, d.dst_stride, d.dst_width, d.dst_height = _ptr(dst).value, _stride(dst), dw, dh
+    if isinstance(dst, tuple):
+        # an output shape only: the serving engine takes the buffer from its ring
+        d.dst, d.dst_stride, d.dst_width, d.dst_height = None, 0, dst[1], dst[0]
+    else:
+        dh, dw, _ = _shape(dst)
+        d.dst, d.dst_stride, d.dst_width, d.dst_height = _ptr(dst).value, _stride(dst), dw, dh
     d.channels = sc
     for k, v in kw.items():
         setattr(d, k, v)

@@ -1,22 +1,25 @@
 """Serving loop: continuous batching of image requests on one GPU.
 
 Requests (an op on a frame that is already in device memory) arrive at any
-time.  The server keeps every buffer resident on the device and admits a
-request once its output fits the HBM budget.  Admitted requests are split by
-size.  A large request (at least `large_px` output pixels) fills the GPU on
-its own, so it is launched alone through its single-image device entry point
-on the large-request slot, in arrival order; when more large requests back
-up than the slot queues (the device is busy and the host sets the pace), up
-to `large_batch` of them share one batch launch.  Small requests are packed into
-batches (kleidicv_hip_batch_launch: one persistent launch per kernel variant)
-on the other stream slots, so they never wait behind a large one and the
-launch cost is shared (a lone small request also runs through its own entry
-point); the large-request slot also takes small batches when no large
-request waits.  Each slot keeps up to `depth` launches queued on
-its stream, so the device never waits for the host between them, and
-batches on different slots overlap on the device.  A request completes
-when the host observes its launch's event; its output is then handed to the
-caller and its budget released.
+time.  The scheduling runs in the library's native serving engine
+(kleidicv_hip_server_*, csrc/serve.cpp); this module builds the requests'
+op descriptors and hands the results back as torch views.
+
+The engine keeps every buffer resident on the device and admits a request
+once its output fits the HBM budget, taking the output from a device ring.
+A large request (at least `large_px` output pixels) fills the GPU on its
+own, so it is launched alone through its single-image device entry point on
+the large-request slot, in arrival order; when more large requests back up
+than the slot queues (the device is busy and the host sets the pace), up to
+`large_batch` of them share one batch launch.  Small requests are packed
+into batches (kleidicv_hip_batch_launch: one persistent launch per kernel
+variant) on the other stream slots, so they never wait behind a large one
+and the launch cost is shared (a lone small request runs through its own
+entry point); the large-request slot also takes small batches when no large
+request waits.  Each slot keeps up to `depth` launches queued on its stream,
+so the device never waits for the host between them, and batches on
+different slots overlap on the device.  A request completes when the host
+observes its launch's event; its budget is then released.
 
 Ops served: gaussian blur 5x5 (sigma 0, REVERSE), median blur 5x5
 (REPLICATE), linear resize to ceil(w/2) x ceil(h/2), warp perspective
@@ -24,12 +27,11 @@ Ops served: gaussian blur 5x5 (sigma 0, REVERSE), median blur 5x5
 the same kernels as its single-image entry point, so a request's result is
 identical alone and in any mix.
 
-The device side is reached through a backend object (TorchBackend for the
-GPU), so the scheduling logic also runs against a host-side stand-in.
+The engine's host stand-in (standin_polls > 0) runs the same scheduler
+without a GPU, for tests.
 """
 
-import collections
-import time
+import ctypes as C
 
 import numpy as np
 
@@ -43,7 +45,7 @@ class Request:
     Y plane (h, w) and `uv` the interleaved chroma plane (h/2, w)."""
 
     __slots__ = ("rid", "op", "src", "uv", "T", "arrival", "dst", "admitted", "launched",
-                 "done", "batch", "_shape")
+                 "done", "batch", "slot", "alone", "error", "sid", "_shape")
 
     def __init__(self, rid, op, src, uv=None, T=None, arrival=0.0):
         if op not in OPS:
@@ -52,7 +54,7 @@ class Request:
         self.arrival = arrival
         self.dst = None
         self.admitted = self.launched = self.done = None
-        self.batch = None
+        self.batch = self.slot = self.alone = self.error = self.sid = None
         h, w = src.shape[0], src.shape[1]
         c = src.shape[2] if len(src.shape) == 3 else 1
         if op == "resize_half":
@@ -74,15 +76,17 @@ class Request:
         return s[0] * s[1]
 
     def desc(self):
+        """Op descriptor; without a dst the server takes the output from its ring."""
+        dst = self.dst if self.dst is not None else self._shape
         if self.op == "gaussian5":
-            return K.op_gaussian_blur(self.src, self.dst, 5, 0.0, "REVERSE")
+            return K.op_gaussian_blur(self.src, dst, 5, 0.0, "REVERSE")
         if self.op == "median5":
-            return K.op_median_blur(self.src, self.dst, 5, "REPLICATE")
+            return K.op_median_blur(self.src, dst, 5, "REPLICATE")
         if self.op == "resize_half":
-            return K.op_resize_linear(self.src, self.dst)
+            return K.op_resize_linear(self.src, dst)
         if self.op == "warp":
-            return K.op_warp_perspective(self.src, self.dst, self.T, "LINEAR", "REPLICATE")
-        return K.op_yuv_sp_to_rgb(self.src, self.uv, self.dst, K.NV12_TO_RGB)
+            return K.op_warp_perspective(self.src, dst, self.T, "LINEAR", "REPLICATE")
+        return K.op_yuv_sp_to_rgb(self.src, self.uv, dst, K.NV12_TO_RGB)
 
     def run_alone(self, dst, stream=None):
         """The same op through its single-image device entry point."""
@@ -102,331 +106,237 @@ class Request:
         return dst
 
 
-class DeviceArena:
-    """Byte ranges of one device allocation handed out as a ring.
-
-    Blocks are taken at the head in allocation order and may be released in
-    any order; the space of the oldest blocks returns once they and every
-    block before them are released.  Requests complete in roughly arrival
-    order, so the ring stays compact, and taking a block costs no device
-    allocation.
-    """
-
-    def __init__(self, size, align=256):
-        self.size, self.align = int(size), align
-        self.head = 0
-        self.blocks = collections.deque()   # [offset, nbytes, released]
-
-    def take(self, nbytes):
-        """Offset of a new block, or None while the ring is full."""
-        n = -(-int(nbytes) // self.align) * self.align
-        if n > self.size:
-            raise ValueError("block of %d bytes exceeds the arena (%d bytes)" % (n, self.size))
-        if not self.blocks:
-            self.head = 0
-        tail = self.blocks[0][0] if self.blocks else 0
-        if not self.blocks or self.head > tail:     # free: [head, size) and [0, tail)
-            if self.size - self.head >= n:
-                off = self.head
-            elif tail > n:                           # wrap; the end of the ring idles
-                off = 0
-            else:
-                return None
-        elif tail - self.head > n:                   # wrapped; free: [head, tail)
-            off = self.head
-        else:
-            return None
-        block = [off, n, False]
-        self.blocks.append(block)
-        self.head = off + n
-        return block
-
-    def release(self, block):
-        block[2] = True
-        while self.blocks and self.blocks[0][2]:
-            self.blocks.popleft()
+# kleidicv_hip_server_config / _request_info / _counters (kleidicv.h, section 5)
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("workspace_reserve", C.c_size_t),
+                ("large_px", C.c_size_t), ("slots", C.c_uint), ("max_batch_ops", C.c_uint),
+                ("depth", C.c_uint), ("large_batch", C.c_uint), ("release_done", C.c_uint),
+                ("plain", C.c_uint), ("standin_polls", C.c_uint), ("tuning", K.Tuning)]
 
 
-class TorchBackend:
-    """Device side through torch tensors, streams and events.
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("state", C.c_int), ("error", C.c_int),
+                ("slot", C.c_uint), ("alone", C.c_uint), ("launch", C.c_uint64),
+                ("arrival", C.c_double), ("admitted", C.c_double), ("launched", C.c_double),
+                ("done", C.c_double), ("dst", C.c_void_p), ("dst_stride", C.c_size_t),
+                ("out_bytes", C.c_size_t)]
 
-    With arena_bytes, output buffers are views into one device allocation
-    managed as a DeviceArena (no allocator call per request); alloc returns
-    None while it is full and free returns a buffer to it.  Without, each
-    output is a torch allocation and free does nothing.
-    """
 
-    def __init__(self, torch, device, arena_bytes=None):
-        self.torch, self.device = torch, device
-        self.arena = DeviceArena(arena_bytes) if arena_bytes else None
-        self.arena_buf = None
-        self.blocks = {}
+class Counters(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("pending", C.c_size_t),
+                ("out_in_use", C.c_size_t), ("workspace_bytes", C.c_size_t),
+                ("peak", C.c_size_t), ("stalled", C.c_uint)]
 
-    def stream(self):
-        return self.torch.cuda.Stream(device=self.device)
 
-    def alloc(self, shape):
-        if self.arena is None:
-            return self.torch.empty(shape, dtype=self.torch.uint8, device=self.device)
-        if self.arena_buf is None:
-            self.arena_buf = self.torch.empty(self.arena.size, dtype=self.torch.uint8,
-                                              device=self.device)
-        n = 1
-        for d in shape:
-            n *= d
-        block = self.arena.take(n)
-        if block is None:
-            return None
-        t = self.arena_buf[block[0]:block[0] + n].view(shape)
-        self.blocks[t.data_ptr()] = block
-        return t
+QUEUED, ADMITTED, LAUNCHED, DONE, FAILED = range(5)
 
-    def free(self, tensor):
-        if self.arena is not None:
-            self.arena.release(self.blocks.pop(tensor.data_ptr()))
 
-    def workspace(self, nbytes):
-        return self.torch.empty(max(int(nbytes), 1), dtype=self.torch.uint8,
-                                device=self.device)
+def _native():
+    L = K.lib()
+    if not getattr(L, "_server_types", False):
+        L.kleidicv_hip_server_clock.restype = C.c_double
+        L.kleidicv_hip_server_pump.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_size_t)]
+        L.kleidicv_hip_server_take_completed.restype = C.c_size_t
+        L.kleidicv_hip_server_take_completed.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.kleidicv_hip_server_info.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
+        L.kleidicv_hip_server_release.argtypes = [C.c_void_p, C.c_uint64]
+        L.kleidicv_hip_server_destroy.argtypes = [C.c_void_p]
+        L._server_types = True
+    return L
 
-    def launch(self, descs, workspace, stream, tuning, requests):
-        """Launches, or returns the workspace bytes needed (nothing launched)."""
-        return K.batch_launch(descs, workspace, stream, tuning)
 
-    def launch_one(self, request, stream):
-        """One request through its single-image device entry point."""
-        request.run_alone(request.dst, stream)
-
-    def record(self, stream):
-        ev = self.torch.cuda.Event()
-        ev.record(stream)
-        return ev
-
-    def query(self, event):
-        return event.query()
+def clock():
+    """The native server's monotonic clock, in seconds."""
+    return _native().kleidicv_hip_server_clock()
 
 
 class Server:
-    """Continuous-batching server over `slots` streams.
+    """Continuous-batching server (the library's native engine,
+    kleidicv_hip_server_*) over `slots` streams of one GPU.
 
     hbm_budget bounds the bytes of outputs and workspaces held at once (the
-    inputs are resident already).  Each slot's work-list workspace comes out
-    of a fixed reserve of workspace_reserve bytes; outputs are admitted
-    against the rest.  max_batch_ops bounds one batch.  Slot 0 is the
-    large-request slot; with one slot, everything runs on it.  depth bounds
-    the launches queued on one slot's stream.  A slot's work lists are
-    uploaded on its own stream, so a queued batch can reuse the workspace;
-    a replaced workspace stays referenced until the launches that read it
-    retire.
+    inputs are resident already); each slot's work-list workspace is a fixed
+    reserve of workspace_reserve bytes inside it.  Outputs come from a device
+    ring of ring_bytes (default: what the budget leaves, at most 8 GiB).
+    max_batch_ops bounds one batch; slot 0 is the large-request slot; depth
+    bounds the launches queued on one slot's stream.  plain=True turns the
+    server into the per-request baseline (every request alone on one stream
+    at admission).  standin_polls > 0 runs the same scheduler against a host
+    stand-in (no GPU): a launch completes on the n-th poll of its event.
+
+    `clock` gives the times passed to the engine (default: its own
+    monotonic clock).  Request fields admitted/launched/done/batch/slot/alone
+    are filled in as requests complete; completed requests are appended to
+    `completed`.
     """
 
-    def __init__(self, backend, hbm_budget=64 << 30, slots=2, max_batch_ops=512,
-                 workspace_reserve=64 << 20, tuning=None, clock=time.perf_counter,
-                 large_px=1 << 18, depth=4, release_done=False, large_batch=1):
-        if slots < 1:
-            raise ValueError("slots must be >= 1")
-        if slots * workspace_reserve >= hbm_budget:
-            raise ValueError("the workspace reserve leaves no budget for outputs")
-        self.be = backend
-        self.budget = int(hbm_budget)
-        self.reserve = int(workspace_reserve)
-        self.out_budget = self.budget - slots * self.reserve
-        self.out_in_use = 0
-        self.peak = 0
-        self.max_batch_ops = max_batch_ops
-        self.tuning = tuning
-        self.clock = clock
-        if depth < 1:
-            raise ValueError("depth must be >= 1")
-        self.depth = depth
-        self.release_done = release_done
-        self.streams = [backend.stream() for _ in range(slots)]
-        # per slot, in launch order: (event, requests, workspace read)
-        self.inflight = [collections.deque() for _ in range(slots)]
-        self.ws = [None] * slots
-        self.ws_bytes = [0] * slots
-        self.large_px = large_px
-        if large_batch < 1:
-            raise ValueError("large_batch must be >= 1")
-        self.large_batch = large_batch
-        self.queue = collections.deque()   # arrived, waiting for budget
-        self.ready = collections.deque()   # admitted small requests, waiting for a slot
-        self.ready_large = collections.deque()  # admitted large requests
+    def __init__(self, torch=None, device=None, hbm_budget=64 << 30, slots=2,
+                 max_batch_ops=512, workspace_reserve=64 << 20, tuning=None, large_px=1 << 18,
+                 depth=4, release_done=False, large_batch=1, ring_bytes=None, plain=False,
+                 standin_polls=0, clock=None):
+        L = _native()
+        self._L = L
+        self.standin = standin_polls > 0
+        self.clock = clock or L.kleidicv_hip_server_clock
+        cfg = ServerConfig(hbm_budget=int(hbm_budget), workspace_reserve=int(workspace_reserve),
+                           large_px=int(large_px), slots=int(slots),
+                           max_batch_ops=int(max_batch_ops), depth=int(depth),
+                           large_batch=int(large_batch), release_done=int(bool(release_done)),
+                           plain=int(bool(plain)), standin_polls=int(standin_polls))
+        if tuning is not None:
+            cfg.tuning = tuning
+        else:
+            cfg.tuning = K.Tuning(blocks_per_cu=8, resize_band_rows=0, xcd_order=1, use_graph=0)
+        if ring_bytes is None:
+            ring_bytes = min(max(int(hbm_budget) - int(slots) * int(workspace_reserve), 0),
+                             8 << 30)
+        self.ring = None
+        ring_ptr = 0
+        if not self.standin and ring_bytes:
+            self.ring = torch.empty(int(ring_bytes), dtype=torch.uint8, device=device)
+            ring_ptr = self.ring.data_ptr()
+        self._h = C.c_void_p(0)
+        K._check(L.kleidicv_hip_server_create(C.byref(cfg), C.c_void_p(ring_ptr),
+                                              C.c_size_t(int(ring_bytes)), C.byref(self._h)),
+                 "kleidicv_hip_server_create")
+        self.reqs = {}
         self.completed = []
-        self.batches = 0
+        self._infos = (RequestInfo * 256)()
 
-    # -- admission ----------------------------------------------------------
+    # -- requests ---------------------------------------------------------------
     def submit(self, req):
-        self.queue.append(req)
+        first = C.c_uint64(0)
+        arr = C.c_double(req.arrival)
+        K._check(self._L.kleidicv_hip_server_submit(self._h, C.byref(req.desc()),
+                                                   C.c_size_t(1), C.byref(arr),
+                                                   C.byref(first)),
+                 "kleidicv_hip_server_submit")
+        req.sid = first.value
+        self.reqs[req.sid] = req
 
-    def _admit(self, now):
-        # Admitted requests hold their output buffers; admit no more than the
-        # slots can queue, so allocation interleaves with launches.
-        cap = self.depth * self.max_batch_ops * len(self.streams)
-        while self.queue and len(self.ready) + len(self.ready_large) < cap:
-            req = self.queue[0]
-            need = req.out_bytes()
-            if need > self.out_budget:
-                raise ValueError("request %d needs %d bytes, more than the whole budget"
-                                 % (req.rid, need))
-            if self.out_in_use + need > self.out_budget:
-                return  # FIFO: later requests wait behind the head
-            dst = self.be.alloc(req.out_shape())
-            if dst is None:
-                return  # output buffers exhausted until earlier requests complete
-            self.queue.popleft()
-            self.out_in_use += need
-            self.peak = max(self.peak, self.in_use)
-            req.dst = dst
-            req.admitted = now
-            if req.out_pixels() >= self.large_px:
-                self.ready_large.append(req)
-            else:
-                self.ready.append(req)
-
-    # -- completion -----------------------------------------------------------
-    def _retire(self, now):
-        for q in self.inflight:
-            while q and self.be.query(q[0][0]):
-                _, reqs, _ = q.popleft()
-                for r in reqs:
-                    r.done = now
-                    self.out_in_use -= r.out_bytes()
-                    self.completed.append(r)
-                    if self.release_done:
-                        self.be.free(r.dst)
-
-    # -- batching -----------------------------------------------------------
-    def _launch(self, now):
-        for i in range(len(self.streams)):
-            while len(self.inflight[i]) < self.depth and self._launch_one(i, now):
-                pass
-
-    def _launch_one(self, i, now):
-        """Launches one request or batch on slot i; False when nothing fits."""
-        if i == 0 and self.ready_large:
-            # Large requests go in arrival order, one per launch; with a
-            # backlog of more than `depth` (more than the slot's queue
-            # holds: the host sets the pace) up to large_batch of them share
-            # one batch launch.  A batched request completes with its whole
-            # batch, so below that backlog they stay alone for latency.
-            if self.large_batch == 1 or len(self.ready_large) <= self.depth:
-                req = self.ready_large.popleft()
-                self.be.launch_one(req, self.streams[0])
-                self._started(0, [req], None, now)
-                return True
-            self._launch_batch(0, self.ready_large, self.large_batch, now)
-            return True
-        if not self.ready:
-            return False
-        if len(self.ready) == 1:
-            # a lone small request: its own entry point is cheaper than a batch
-            req = self.ready.popleft()
-            self.be.launch_one(req, self.streams[i])
-            self._started(i, [req], None, now)
-            return True
-        self._launch_batch(i, self.ready, self.max_batch_ops, now)
-        return True
-
-    def _launch_batch(self, i, queue, limit, now):
-        """One batch launch on slot i of up to `limit` requests from `queue`."""
-        reqs = []
-        while queue and len(reqs) < limit:
-            reqs.append(queue.popleft())
-        # A slot's workspace lives in its reserve.  Launch into it; when
-        # the work lists need more, grow it within the reserve or halve
-        # the batch (the rest goes back to the front of the queue).
-        descs = [r.desc() for r in reqs]
-        while True:
-            need = self.be.launch(descs, self.ws[i], self.streams[i], self.tuning, reqs)
-            if need == 0:
-                break
-            if need <= self.reserve:
-                grow = min(max(need, 2 * self.ws_bytes[i]), self.reserve)
-                self.ws[i] = self.be.workspace(grow)
-                self.ws_bytes[i] = grow
-                self.peak = max(self.peak, self.in_use)
-                continue
-            if len(reqs) == 1:
-                raise ValueError("work lists of request %d exceed the workspace reserve"
-                                 % reqs[0].rid)
-            keep = len(reqs) // 2
-            queue.extendleft(reversed(reqs[keep:]))
-            reqs, descs = reqs[:keep], descs[:keep]
-        self._started(i, reqs, self.ws[i], now)
-
-    def _started(self, i, reqs, workspace, now):
-        ev = self.be.record(self.streams[i])
-        self.batches += 1
-        for r in reqs:
-            r.launched = now
-            r.batch = self.batches
-        self.inflight[i].append((ev, reqs, workspace))
+    def _complete(self, req, info):
+        req.admitted = info.admitted
+        req.launched = info.launched
+        req.done = info.done
+        req.batch = info.launch
+        req.slot = info.slot
+        req.alone = bool(info.alone)
+        req.error = info.error if info.state == FAILED else None
+        if self.ring is not None and info.dst and info.state == DONE:
+            off = info.dst - self.ring.data_ptr()
+            req.dst = self.ring[off:off + req.out_bytes()].view(req.out_shape())
 
     def pump(self):
         """One scheduling tick; returns the number of requests still pending."""
-        now = self.clock()
-        self._retire(now)
-        self._admit(now)
-        self._launch(now)
-        return self.pending()
+        pending = C.c_size_t(0)
+        K._check(self._L.kleidicv_hip_server_pump(self._h, self.clock(), C.byref(pending)),
+                 "kleidicv_hip_server_pump")
+        while True:
+            n = self._L.kleidicv_hip_server_take_completed(self._h, self._infos, 256)
+            for k in range(n):
+                info = self._infos[k]
+                req = self.reqs.pop(info.id)
+                self._complete(req, info)
+                self.completed.append(req)
+            if n < 256:
+                break
+        return pending.value
+
+    def release(self, req):
+        """Returns a completed request's output to the ring (release_done=False)."""
+        K._check(self._L.kleidicv_hip_server_release(self._h, C.c_uint64(req.sid)),
+                 "kleidicv_hip_server_release")
+
+    def drain(self):
+        """Pumps until every request has completed.  Raises when the rest
+        waits for ring space that only release() can return."""
+        while self.pump():
+            if self.counters().stalled:
+                raise RuntimeError("requests wait for ring space held by completed "
+                                   "requests: release() their outputs")
+
+    def replay(self, requests):
+        """Real-time replay of `requests` (arrival offsets in seconds), all
+        in the engine: returns the clock at the start."""
+        order = sorted(requests, key=lambda r: r.arrival)
+        n = len(order)
+        descs = (K.OpDesc * n)(*[r.desc() for r in order])
+        offs = (C.c_double * n)(*[r.arrival for r in order])
+        res = (RequestInfo * n)()
+        t0 = C.c_double(0)
+        K._check(self._L.kleidicv_hip_server_replay(self._h, descs, C.c_size_t(n), offs, res,
+                                                   C.byref(t0)),
+                 "kleidicv_hip_server_replay")
+        for r, info in zip(order, res):
+            self._complete(r, info)
+            self.completed.append(r)
+        return t0.value
+
+    # -- accounting ---------------------------------------------------------------
+    def counters(self):
+        c = Counters()
+        K._check(self._L.kleidicv_hip_server_stats(self._h, C.byref(c)),
+                 "kleidicv_hip_server_stats")
+        return c
+
+    @property
+    def batches(self):
+        return self.counters().launches
+
+    @property
+    def peak(self):
+        return self.counters().peak
+
+    @property
+    def out_in_use(self):
+        return self.counters().out_in_use
+
+    @property
+    def ws_bytes(self):
+        return self.counters().workspace_bytes
 
     @property
     def in_use(self):
-        """Device bytes held now: admitted outputs plus workspaces."""
-        return self.out_in_use + sum(self.ws_bytes)
+        c = self.counters()
+        return c.out_in_use + c.workspace_bytes
 
     def pending(self):
-        return (len(self.queue) + len(self.ready) + len(self.ready_large) +
-                sum(len(e[1]) for q in self.inflight for e in q))
+        return self.counters().pending
 
-    def drain(self):
-        while self.pending():
-            self.pump()
+    def close(self):
+        if self._h:
+            self._L.kleidicv_hip_server_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def poisson_arrivals(rng, n, qps):
     return np.cumsum(rng.exponential(1.0 / qps, n))
 
 
-def run_served(server, requests, clock=time.perf_counter):
-    """Feed `requests` (with arrival offsets in seconds) in real time."""
-    t0 = clock()
-    i = 0
-    order = sorted(requests, key=lambda r: r.arrival)
-    while i < len(order) or server.pending():
-        now = clock() - t0
-        while i < len(order) and order[i].arrival <= now:
-            server.submit(order[i])
-            i += 1
-        server.pump()
-    return t0
+def run_served(server, requests):
+    """Feeds `requests` (arrival offsets in seconds) to `server` in real time,
+    inside the engine; returns the clock at the start."""
+    return server.replay(requests)
 
 
-def run_plain(backend, requests, stream, clock=time.perf_counter, release_done=False):
-    """Baseline: each request launched alone at its arrival on one stream.
-    A request whose output buffer is not available waits (arena backend)."""
-    t0 = clock()
-    order = sorted(requests, key=lambda r: r.arrival)
-    inflight = collections.deque()
-    i = 0
-    while i < len(order) or inflight:
-        now = clock() - t0
-        while i < len(order) and order[i].arrival <= now:
-            r = order[i]
-            r.dst = backend.alloc(r.out_shape())
-            if r.dst is None:
-                break
-            r.admitted = r.launched = clock()
-            r.run_alone(r.dst, stream)
-            inflight.append((backend.record(stream), r))
-            i += 1
-        while inflight and backend.query(inflight[0][0]):
-            r = inflight.popleft()[1]
-            r.done = clock()
-            if release_done:
-                backend.free(r.dst)
-    return t0
+def run_plain(torch, device, requests, hbm_budget=64 << 30, ring_bytes=None,
+              release_done=False):
+    """Baseline: each request launched alone through its single-image entry
+    point at its arrival, on one stream, by the same native loop (outputs
+    from the same kind of ring; a request whose output does not fit waits).
+    Returns the clock at the start."""
+    srv = Server(torch, device, hbm_budget=hbm_budget, slots=1, workspace_reserve=0,
+                 plain=True, release_done=release_done, ring_bytes=ring_bytes)
+    try:
+        return srv.replay(requests)
+    finally:
+        srv.close()
 
 
 def latency_stats(requests, t0):
