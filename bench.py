@@ -255,6 +255,7 @@ def make_step_sets(torch, gen, dev, WL, n):
 EXTENDED_INPUTS = {
     "i420_to_rgb": ((3 * H) // 2, W), "yuyv_to_rgb": (H, 2 * W), "yuv444_to_rgba": (H, 3 * W),
     "rgb_to_nv12": (H, W, 3), "gaussian_31x31_1ch": (H, W), "median_9x9_1ch": (H, W),
+    "median_31x31_1ch": (H, W),
     "blur_and_downsample_1ch": (H, W), "blur_and_downsample_4ch": (H, W, 4),
 }
 
@@ -282,6 +283,9 @@ def extended_ops(K, s, out, stream):
         ("median_9x9_1ch", W * H, 2 * W * H,
          lambda: K.median_blur(s["median_9x9_1ch"], 9, "REPLICATE", dst=out["median_9x9_1ch"],
                                stream=stream, device_api=True)),
+        ("median_31x31_1ch", W * H, 2 * W * H,
+         lambda: K.median_blur(s["median_31x31_1ch"], 31, "REPLICATE",
+                               dst=out["median_31x31_1ch"], stream=stream, device_api=True)),
         ("blur_and_downsample_1ch", (W // 2) * (H // 2), W * H + (W // 2) * (H // 2),
          bd("blur_and_downsample_1ch")),
         ("blur_and_downsample_4ch", (W // 2) * (H // 2), 4 * (W * H + (W // 2) * (H // 2)),
@@ -298,6 +302,7 @@ def extended_outputs(torch, dev):
                         torch.empty(H // 2, W, dtype=torch.uint8, device=dev)),
         "gaussian_31x31_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
         "median_9x9_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
+        "median_31x31_1ch": torch.empty(H, W, dtype=torch.uint8, device=dev),
         "blur_and_downsample_1ch": torch.empty(H // 2, W // 2, dtype=torch.uint8, device=dev),
         "blur_and_downsample_4ch": torch.empty(H // 2, W // 2, 4, dtype=torch.uint8, device=dev),
     }

@@ -197,6 +197,24 @@ def test_median_large(rng, k):
             assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
 
 
+@pytest.mark.parametrize("k", [13, 21])
+def test_median_hist_strips_and_moving_medians(rng, k):
+    # tall enough for many strips of rows per column; a vertical ramp with
+    # noise, full bright rows and dark columns moves the median far from one
+    # output row to the next
+    h, w = 700, 300
+    ramp = np.arange(h)[:, None] * 255 // (h - 1)
+    for c in (1, 3):
+        src = (ramp[:, :, None] + rng.integers(-40, 41, (h, w, c))).clip(0, 255)
+        src = src.astype(np.uint8)
+        src[::97] = 255
+        src[:, ::53] = 0
+        for border in ("REPLICATE", "REFLECT", "WRAP"):
+            got = K.median_blur(src, k, border)
+            _, want = O.median_blur(src, k, border)
+            assert_same(got, want, "k%d %s c%d" % (k, border, c))
+
+
 def _median_by_box_counts(src, k, border):
     """Exact median of every k x k window from box counts of (src <= v) for
     each level v (integral images), for sizes where sorting every window

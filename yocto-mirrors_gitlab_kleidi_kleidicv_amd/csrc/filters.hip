@@ -1,6 +1,9 @@
 // Per-image launches of the tiled 2-D filters (bodies in filters_body.h).
 
+#include <type_traits>
+
 #include "filters_body.h"
+#include "prepare.h"
 
 namespace kcv {
 namespace {
@@ -203,42 +206,168 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArb
 }
 
 // ---------------------------------------------------------------------------
-// Median of larger windows (13x13 .. 255x255; 9x9 and 11x11 still fit the
-// merge-network tile above in registers), the sizes the reference serves
-// with histograms (src/filters/median_blur_{small,large}_hist_neon.cpp).  Any
-// exact selection gives the same value; here one thread finds the median of
-// one output byte bit by bit: the largest t with #(window < t) <= (K*K-1)/2,
-// eight counting sweeps over the window, which neighbouring threads read from
-// the same cache lines.  Interior columns index the row directly; columns
-// within K/2 of an edge go through the border map.
+// Median of larger windows (the sizes the reference serves with histograms,
+// src/filters/median_blur_{small,large}_hist_neon.cpp).  Any exact selection
+// gives the same value; here each lane keeps the histogram of its own
+// window and slides it down a strip of output rows (Huang's update): per
+// output row it adds the window's new bottom row and removes its old top
+// row (2K taps) and moves the median to the new rank.
+//
+// One wave per workgroup, one output byte per lane.  A lane's 256 bins are
+// 16-bit counts packed two to a dword (bins 2i and 2i + 1 in dword i, low
+// half first; counts reach K*K <= 65025) and interleaved by lane (dword i of
+// lane l at i * 64 + l), so every LDS access of a wave hits 64 different
+// banks whatever the pixel values: 32 KiB per wave.  Adds and removes are
+// fire-and-forget LDS atomics (ds_add_u32).  `lt` counts the window's values
+// below the current median m; after each update m moves down while more
+// than (K*K-1)/2 values lie below it and up while no more than that lie at
+// or below it, reading the eight bins m-4 .. m+3 in one round (five dwords)
+// and stepping through them in registers.  A strip starts from a full K x K
+// histogram, so strips are at least K rows tall (the start costs K*K taps,
+// the walk 2K per row).
+constexpr int kHistLanes = 64;
+
+// Largest window served by the merge-network tile; larger ones slide
+// histograms.
+#ifndef KCV_MEDIAN_NETWORK_MAX
+#define KCV_MEDIAN_NETWORK_MAX 11
+#endif
+constexpr int kMedianNetworkMax = KCV_MEDIAN_NETWORK_MAX;
+
+// Rows per strip of the histogram median: enough workgroups for about four
+// rounds over the GPU (five 32 KiB histograms fit one CU's LDS), but at
+// least K rows (a strip's start costs K*K taps, each further row 2K) and
+// few enough strips for a grid dimension.
+int median_hist_strip(int blocks_x, int height, int k, int cus) {
+  const int64_t want = int64_t{4} * 5 * std::max(cus, 1);
+  int64_t strip = (int64_t{blocks_x} * height + want - 1) / want;
+  strip = std::max<int64_t>(strip, k);
+  strip = std::max<int64_t>(strip, (height + 65534) / 65535);
+  return static_cast<int>(std::min<int64_t>(strip, height));
+}
+
 template <int C>
-__global__ __launch_bounds__(kThreads) void median_large_kernel(MedianArgs a) {
+__global__ __launch_bounds__(kHistLanes) void median_hist_kernel(MedianArgs a, int strip) {
+  __shared__ uint32_t hist[128 * kHistLanes];
+  const int lane = threadIdx.x;
   const int row_bytes = a.width * C;
-  const int e = blockIdx.x * kThreads + threadIdx.x;
-  if (e >= row_bytes) return;
-  const int x = e / C, c = e - x * C;
+  const int e = blockIdx.x * kHistLanes + lane;
+  const bool active = e < row_bytes;
+  const int ec = active ? e : row_bytes - 1;  // idle lanes shadow the last byte
+  const int x = ec / C, c = ec - x * C;
   const int K = a.ksize, R = K / 2;
-  const int rank = (K * K - 1) / 2;
+  const int half = (K * K - 1) / 2;
+  const ColMap cm = col_map(a.border, a.width);
   const bool interior = x - R >= 0 && x + R < a.width;
-  for (int y = blockIdx.y; y < a.height; y += gridDim.y) {
-    uint32_t prefix = 0;
-    for (int bit = 7; bit >= 0; --bit) {
-      const uint32_t t = prefix | (1u << bit);
-      int count = 0;
-      for (int i = 0; i < K; ++i) {
-        const uint8_t *row =
-            a.src + static_cast<size_t>(map_index(y + i - R, a.height, a.border)) * a.src_stride;
-        if (interior) {
-          const uint8_t *p = row + (x - R) * C + c;
-          for (int j = 0; j < K; ++j) count += p[j * C] < t;
-        } else {
-          for (int j = 0; j < K; ++j)
-            count += row[map_index(x + j - R, a.width, a.border) * C + c] < t;
+  uint32_t *h = hist + lane;
+  for (int i = 0; i < 128; ++i) h[i * kHistLanes] = 0;
+  int m = 0, lt = 0;
+  // One tap value v added (SIGN 1) or removed (SIGN -1) unless `used` (the
+  // same for every lane) is false.
+  auto count = [&](uint32_t v, bool used, auto sign_tag) {
+    constexpr int SIGN = decltype(sign_tag)::value;
+    lt += used && v < static_cast<uint32_t>(m) ? SIGN : 0;
+    if (used) {
+      uint32_t *p = h + (v >> 1) * kHistLanes;
+      const uint32_t inc = 1u << (16 * (v & 1));
+      atomicAdd(p, SIGN > 0 ? inc : 0u - inc);
+    }
+  };
+  // Column offsets of the taps: direct for interior lanes, through the
+  // border map (as selects, no branches) near the left and right edges.
+  auto tap_offset = [&](int j) -> int {
+    const int xx = x + j - R;
+    const int l = cm.sl * xx + cm.bl, r = cm.sr * xx + cm.br;
+    const int mapped = min(max(xx < 0 ? l : (xx >= a.width ? r : xx), 0), a.width - 1);
+    return (interior ? xx : mapped) * C;
+  };
+  // Taps of window row yy added or removed.  One-channel interior lanes read
+  // their K contiguous bytes as dwords (the last dword is the window's last
+  // four bytes, its bytes already counted skipped); other lanes read bytes.
+  // Loads go out in groups (4 dwords or 16 bytes) before their LDS atomics;
+  // a group's unused slots re-read the last tap and are not counted.
+  auto update = [&](int yy, auto sign_tag) {
+    const uint8_t *row =
+        a.src + static_cast<size_t>(map_index(yy, a.height, a.border)) * a.src_stride + c;
+    if (C == 1 && interior) {
+      typedef uint32_t __attribute__((aligned(1))) u32u;
+      const uint8_t *p = row + x - R;
+      const int nd = (K + 3) / 4;
+      for (int t0 = 0; t0 < nd; t0 += 4) {
+        uint32_t d[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+          d[t] = *reinterpret_cast<const u32u *>(p + min(4 * min(t0 + t, nd - 1), K - 4));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int start = min(4 * (t0 + t), K - 4);
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            count((d[t] >> (8 * b)) & 0xFFu, t0 + t < nd && start + b >= 4 * (t0 + t),
+                  sign_tag);
         }
       }
-      if (count <= rank) prefix = t;
+    } else {
+      for (int j0 = 0; j0 < K; j0 += 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) v[t] = row[tap_offset(min(j0 + t, K - 1))];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) count(v[t], j0 + t < K, sign_tag);
+      }
     }
-    a.dst[static_cast<size_t>(y) * a.dst_stride + e] = static_cast<uint8_t>(prefix);
+  };
+  using Add = std::integral_constant<int, 1>;
+  using Remove = std::integral_constant<int, -1>;
+  const int y0 = blockIdx.y * strip;
+  const int y1 = min(y0 + strip, a.height);
+  for (int i = -R; i <= R; ++i) update(y0 + i, Add{});
+  // bins m-4 .. m+3 (those outside 0..255 are read clamped and never used)
+  auto bins8 = [&](uint32_t (&b)[8]) {
+    const int w0 = (m - 4) >> 1;  // arithmetic shift: floor
+    uint32_t w[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) w[i] = h[min(max(w0 + i, 0), 127) * kHistLanes];
+    const uint32_t sh = static_cast<uint32_t>(m & 1) * 16;  // (m - 4) odd: start high
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t pair = __builtin_amdgcn_alignbit(w[i + 1], w[i], sh);
+      b[2 * i] = pair & 0xFFFFu;
+      b[2 * i + 1] = pair >> 16;
+    }
+  };
+  for (int y = y0; y < y1; ++y) {
+    if (y > y0) {
+      update(y + R, Add{});
+      update(y - R - 1, Remove{});
+    }
+    for (bool again = true; again;) {
+      uint32_t b[8];
+      bins8(b);
+      const int m0 = m;
+      // down while more than `half` values lie below m (b[3] is bin m0-1)
+#pragma unroll
+      for (int t = 3; t >= 0; --t) {
+        if (lt > half) {
+          --m;
+          lt -= static_cast<int>(b[t]);
+        }
+      }
+      // up (only when m did not move down) while no more than `half` values
+      // lie at or below m (b[4] is bin m0)
+      bool up = m == m0;
+#pragma unroll
+      for (int t = 4; t < 8; ++t) {
+        if (up && lt + static_cast<int>(b[t]) <= half) {
+          lt += static_cast<int>(b[t]);
+          ++m;
+        } else {
+          up = false;
+        }
+      }
+      again = lt > half || up;  // ran past the eight bins read
+    }
+    if (active) a.dst[static_cast<size_t>(y) * a.dst_stride + e] = static_cast<uint8_t>(m);
   }
 }
 
@@ -351,14 +480,16 @@ hipError_t launch_gaussian_arbitrary(const GaussArbArgs &a, hipStream_t stream) 
 
 hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
   if (a.width <= 0 || a.height <= 0 || a.channels <= 0) return hipSuccess;
-  if (a.ksize > 11) {
+  if (a.ksize > kMedianNetworkMax) {
     if (a.ksize > 255 || (a.ksize & 1) == 0) return hipErrorInvalidValue;
-    const dim3 grid((a.width * a.channels + kThreads - 1) / kThreads, min(a.height, 65535));
+    const int bx = (a.width * a.channels + kHistLanes - 1) / kHistLanes;
+    const int strip = median_hist_strip(bx, a.height, a.ksize, runtime().compute_units);
+    const dim3 grid(bx, (a.height + strip - 1) / strip);
     switch (a.channels) {
-      case 1: median_large_kernel<1><<<grid, kThreads, 0, stream>>>(a); break;
-      case 2: median_large_kernel<2><<<grid, kThreads, 0, stream>>>(a); break;
-      case 3: median_large_kernel<3><<<grid, kThreads, 0, stream>>>(a); break;
-      case 4: median_large_kernel<4><<<grid, kThreads, 0, stream>>>(a); break;
+      case 1: median_hist_kernel<1><<<grid, kHistLanes, 0, stream>>>(a, strip); break;
+      case 2: median_hist_kernel<2><<<grid, kHistLanes, 0, stream>>>(a, strip); break;
+      case 3: median_hist_kernel<3><<<grid, kHistLanes, 0, stream>>>(a, strip); break;
+      case 4: median_hist_kernel<4><<<grid, kHistLanes, 0, stream>>>(a, strip); break;
       default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
