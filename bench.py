@@ -229,6 +229,32 @@ def bench_isolated(K, torch, WL, dev, stream, names, params, launches, warmup):
     return res
 
 
+def copy_ceiling(torch, dev, stream, traffic, launches=20):
+    """Time of a device-to-device copy that moves `traffic` bytes of HBM
+    (traffic / 2 read and written), launch to launch, each copy on memory
+    the last 640 MB of copies did not touch: the practical ceiling for a
+    kernel with that much traffic on this GPU (ms)."""
+    m = (int(traffic) // 2 + 4095) // 4096 * 4096
+    n = max(2, -(-ROTATE_BYTES // m))
+    src = torch.empty(n * m, dtype=torch.uint8, device=dev).fill_(7)
+    dst = torch.empty_like(src)
+    for i in range(3):
+        dst[i % n * m:(i % n + 1) * m].copy_(src[i % n * m:(i % n + 1) * m])
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for i in range(launches):
+        k = (3 + i) % n
+        dst[k * m:(k + 1) * m].copy_(src[k * m:(k + 1) * m])
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / launches
+    del src, dst
+    torch.cuda.empty_cache()
+    return ms
+
+
 def make_step_sets(torch, gen, dev, WL, n):
     """`n` input sets and output sets of the whole op set for the step."""
     sets, outs = [], []
@@ -835,6 +861,9 @@ def main():
             d["rocprof_gb_s"] = round(nbytes / (rocp[name]["avg_us"] * 1e-6) / 1e9, 1)
         if traffic is not None:
             d["traffic"] = int(traffic[name]["FETCH_SIZE"] + traffic[name]["WRITE_SIZE"])
+        c_ms = copy_ceiling(torch, dev, stream, d.get("traffic") or nbytes)
+        d["copy_ms"] = round(c_ms, 4)
+        d["frac_of_copy"] = round(c_ms / iso[name]["ms"], 3)
         per_op[name] = d
     step_names = [o[0] for o in step_ops]
     total_pix = sum(p for _, p, _ in step_ops)
@@ -873,7 +902,9 @@ def main():
                        "and outputs that the last 640 MB of launches did not touch; "
                        "rocprof_us: rocprofv3 --kernel-trace average duration of the same "
                        "launches in a child run; traffic: FETCH_SIZE + WRITE_SIZE bytes per "
-                       "launch",
+                       "launch; copy_ms: a device-to-device copy moving the same HBM traffic "
+                       "(half read, half written), timed the same way: the practical ceiling "
+                       "for that much traffic on this GPU; frac_of_copy = copy_ms / ms",
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": d["gb_s"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": d["hbm_frac"],
                      "traffic": d.get("traffic")},
