@@ -226,6 +226,7 @@ __global__ __launch_bounds__(kThreads) void gauss_arb_horizontal_kernel(GaussArb
 // histogram, so strips are at least K rows tall (the start costs K*K taps,
 // the walk 2K per row).
 constexpr int kHistLanes = 64;
+constexpr int kHistPipeTaps = 32;  // widest window row loaded one output row ahead
 
 // Largest window served by the merge-network tile; larger ones slide
 // histograms.
@@ -336,11 +337,8 @@ __global__ __launch_bounds__(kHistLanes) void median_hist_kernel(MedianArgs a, i
       b[2 * i + 1] = pair >> 16;
     }
   };
-  for (int y = y0; y < y1; ++y) {
-    if (y > y0) {
-      update(y + R, Add{});
-      update(y - R - 1, Remove{});
-    }
+  // moves m to the median of the current window
+  auto search = [&]() {
     for (bool again = true; again;) {
       uint32_t b[8];
       bins8(b);
@@ -367,7 +365,78 @@ __global__ __launch_bounds__(kHistLanes) void median_hist_kernel(MedianArgs a, i
       }
       again = lt > half || up;  // ran past the eight bins read
     }
+  };
+  auto store = [&](int y) {
     if (active) a.dst[static_cast<size_t>(y) * a.dst_stride + e] = static_cast<uint8_t>(m);
+  };
+  search();
+  store(y0);
+  if (K <= kHistPipeTaps) {
+    // Windows of up to kHistPipeTaps columns: the taps of the next output
+    // row's new and old window rows are loaded into registers while the
+    // current row's atomics and median search run (two register sets used
+    // in turn), so the loads' latency is hidden behind the LDS work.
+    auto fetch = [&](int yy, uint32_t (&t)[kHistPipeTaps]) {
+      const uint8_t *row =
+          a.src + static_cast<size_t>(map_index(yy, a.height, a.border)) * a.src_stride + c;
+      if (C == 1 && interior) {
+        typedef uint32_t __attribute__((aligned(1))) u32u;
+        const uint8_t *p = row + x - R;
+        const int nd = (K + 3) / 4;
+#pragma unroll
+        for (int i = 0; i < kHistPipeTaps / 4; ++i)
+          t[i] = *reinterpret_cast<const u32u *>(p + min(4 * min(i, nd - 1), K - 4));
+      } else {
+#pragma unroll
+        for (int i = 0; i < kHistPipeTaps; ++i) t[i] = row[tap_offset(min(i, K - 1))];
+      }
+    };
+    auto apply = [&](const uint32_t (&t)[kHistPipeTaps], auto sign_tag) {
+      if (C == 1 && interior) {
+        const int nd = (K + 3) / 4;
+#pragma unroll
+        for (int i = 0; i < kHistPipeTaps / 4; ++i) {
+          const int start = min(4 * i, K - 4);
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            count((t[i] >> (8 * b)) & 0xFFu, i < nd && start + b >= 4 * i, sign_tag);
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kHistPipeTaps; ++i) count(t[i], i < K, sign_tag);
+      }
+    };
+    uint32_t add0[kHistPipeTaps], rem0[kHistPipeTaps], add1[kHistPipeTaps],
+        rem1[kHistPipeTaps];
+    // output row y: apply (add, rem), prefetch row y + 1's into (nadd, nrem)
+    auto step = [&](int y, const uint32_t (&add)[kHistPipeTaps],
+                    const uint32_t (&rem)[kHistPipeTaps], uint32_t (&nadd)[kHistPipeTaps],
+                    uint32_t (&nrem)[kHistPipeTaps]) {
+      if (y + 1 < y1) {
+        fetch(y + 1 + R, nadd);
+        fetch(y - R, nrem);
+      }
+      apply(add, Add{});
+      apply(rem, Remove{});
+      search();
+      store(y);
+    };
+    if (y0 + 1 < y1) {
+      fetch(y0 + 1 + R, add0);
+      fetch(y0 - R, rem0);
+    }
+    for (int y = y0 + 1; y < y1; y += 2) {
+      step(y, add0, rem0, add1, rem1);
+      if (y + 1 >= y1) break;
+      step(y + 1, add1, rem1, add0, rem0);
+    }
+  } else {
+    for (int y = y0 + 1; y < y1; ++y) {
+      update(y + R, Add{});
+      update(y - R - 1, Remove{});
+      search();
+      store(y);
+    }
   }
 }
 
