@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--ops", default="", help="comma-separated subset of ops (profiling)")
     ap.add_argument("--isolated-only", action="store_true",
                     help="only the per-op isolated launches (profiling child runs)")
+    ap.add_argument("--extended-only", action="store_true",
+                    help="only the extended ops (those named by --ops, or all), each alone "
+                         "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 512 MiB (HBM counter calibration)")
     ap.add_argument("--no-traffic", action="store_true",
@@ -334,7 +337,7 @@ def extended_outputs(torch, dev):
     }
 
 
-def bench_extended(K, torch, dev, stream, steps, warmup):
+def bench_extended(K, torch, dev, stream, steps, warmup, only=None):
     """Each extended op alone: `steps` launches between two events on the
     launch stream; ms per launch.  Every launch, warmup included, reads its
     own inputs (views into a seeded pool of 640 MB and more), so no timed
@@ -344,6 +347,8 @@ def bench_extended(K, torch, dev, stream, steps, warmup):
     out = extended_outputs(torch, dev)
     res = {}
     for k, (name, shape) in enumerate(EXTENDED_INPUTS.items()):
+        if only and name not in only:
+            continue
         n = 1
         for d in shape:
             n *= d
@@ -784,7 +789,7 @@ def main():
     step_ops = [o for o in OPS if wanted is None or o[0] in wanted]
     traffic, traffic_note = None, "not measured (--no-traffic)"
     rocp, rocp_note = None, "not measured (--no-rocprof)"
-    if not args.copy_calibration and not args.isolated_only:
+    if not args.copy_calibration and not args.isolated_only and not args.extended_only:
         if not args.no_traffic:
             traffic, traffic_note = measure_traffic(names)
         if not args.no_rocprof:
@@ -814,6 +819,11 @@ def main():
     # timing events are recorded on it.
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
+    if args.extended_only:
+        only = args.ops.split(",") if args.ops else None
+        res = bench_extended(K, torch, dev, stream, args.steps, args.warmup, only)
+        print(json.dumps({"extended": res}))
+        return
     if args.isolated_only:
         bench_isolated(K, torch, WL, dev, stream, names, params, args.steps, args.warmup)
         torch.cuda.synchronize()
