@@ -656,6 +656,71 @@ struct RowMerge<K, 1> {
   }
 };
 
+}  // namespace filt
+}  // namespace kcv
+#include "median5_net.h"
+namespace kcv {
+namespace filt {
+
+// 5x5 median tile for one and two channels: each 16-bit half of a thread's
+// us2 lanes takes a 2-column x 4-row block of same-channel outputs (one
+// channel: bytes 0-1 and 2-3 of the thread's dword; two channels: channel
+// 0 in bytes 0 and 2, channel 1 in bytes 1 and 3), so the eight windows of
+// a block share one 6 x 8 footprint and median5_net_2x4 (558 min/max for
+// 16 outputs, against 668 for four columns of 4-row windows) finds all of
+// them.  Footprint values are paired from aligned LDS dwords by v_perm_b32.
+template <int C>
+__device__ __forceinline__ void median5_tile_2x4(const MedianArgs &a, int bx, int by,
+                                                 uint8_t *smem) {
+  static_assert(C == 1 || C == 2, "blocks of same-channel pairs in one dword");
+  constexpr int R = 2;
+  using G = Window<R>;
+  uint8_t *win = smem;
+  const int e0 = bx * kTileW;
+  const int y0 = by * kTileH;
+  __syncthreads();  // LDS may still hold the previous tile of a batched loop
+  stage_window<R, C>(win, a.src, a.src_stride, a.width, a.height, a.border, e0, y0);
+  __syncthreads();
+
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int r = 4 * ty;  // tile rows r .. r + 3; footprint rows r .. r + 7
+  if (y0 + r >= a.height) return;
+  // dwords holding bytes -4 .. 7 around the thread's first output byte
+  const uint32_t *w0 =
+      reinterpret_cast<const uint32_t *>(win + r * G::kCols + G::kPad) + tx - 1;
+  us2 f[8][6];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    uint32_t d[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) d[q] = w0[i * (G::kCols / 4) + q];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      // byte positions in d (0 .. 11) of footprint column c in both halves
+      const int px = C == 1 ? 2 + c : 2 * c;
+      const int py = C == 1 ? 4 + c : 2 * c + 1;
+      const int q = px >> 2;
+      const uint32_t sel = static_cast<uint32_t>(px & 3) | (0x0cu << 8) |
+                           (static_cast<uint32_t>(py - 4 * q) << 16) | (0x0cu << 24);
+      f[i][c] = __builtin_bit_cast(us2, __builtin_amdgcn_perm(d[q < 2 ? q + 1 : 2], d[q], sel));
+    }
+  }
+  us2 out[4][2];
+  median5_net_2x4(f, out);
+  const int row_bytes = a.width * C;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int y = y0 + r + h;
+    const uint32_t b0 = C == 1 ? out[h][0].x : out[h][0].x;
+    const uint32_t b1 = C == 1 ? out[h][1].x : out[h][0].y;
+    const uint32_t b2 = C == 1 ? out[h][0].y : out[h][1].x;
+    const uint32_t b3 = out[h][1].y;
+    if (y < a.height)
+      store_quad(a.dst + static_cast<size_t>(y) * a.dst_stride, e0 + 4 * tx, row_bytes,
+                 b0 | (b1 << 8) | (b2 << 16) | (b3 << 24));
+  }
+}
+
 // One 256-byte x 16-row median tile of a C-channel image at tile
 // coordinates (bx, by).  Each thread owns 4 output bytes (two us2 lanes) of
 // 4 consecutive rows.  The taps of a window row are read from LDS as
@@ -664,6 +729,10 @@ struct RowMerge<K, 1> {
 template <int K, int C>
 __device__ __forceinline__ void median_tile_c(const MedianArgs &a, int bx, int by,
                                               uint8_t *smem) {
+  if constexpr (K == 5 && C <= 2) {
+    median5_tile_2x4<C>(a, bx, by, smem);
+    return;
+  }
   constexpr int R = K / 2;
   constexpr int M = (K * K + 1) / 2;     // median rank (1-based)
   constexpr int LO = M - K - 1;          // first kept shared rank (0-based)
