@@ -32,3 +32,25 @@ def test_three_to_one_lerp_rounding_bit_is_needed():
     want = (3 * a + b + 2) >> 2
     assert not np.array_equal(byte_lerp(a, m, 0), want)
     assert not np.array_equal(byte_lerp(a, m, 1), want)
+
+
+def test_median5_network_header_is_the_checked_generator_output(tmp_path):
+    # median5_net.h must be exactly what gen_median5_net.py writes, and the
+    # generator's network must equal numpy's median on every window of
+    # random and low-entropy 6x8 footprints (its own check, run here)
+    import importlib.util
+    import os
+    csrc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "yocto-mirrors_gitlab_kleidi_kleidicv_amd", "csrc")
+    spec = importlib.util.spec_from_file_location("gen_median5_net",
+                                                  os.path.join(csrc, "gen_median5_net.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    net, outputs = gen.build()
+    live = gen.live_ops(net, outputs)
+    gen.check(net, live, outputs)
+    out = tmp_path / "median5_net.h"
+    gen.emit(net, live, outputs, str(out))
+    with open(os.path.join(csrc, "median5_net.h")) as fh:
+        assert out.read_text() == fh.read()
+    assert len(live) == 558 and len(outputs) == 8
