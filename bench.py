@@ -712,7 +712,7 @@ def bench_serving(torch, dev, n, qps_list=(2000.0, 20000.0, 100000.0, 1e9),
     # them), so neither pays an allocator call per request.
     ring = 8 << 30
     budget = 64 << 30
-    slots, max_ops, large_px, large_batch = 2, 64, 1 << 18, 64
+    slots, max_ops, large_px, large_batch = 4, 64, 1 << 18, 64
 
     def server():
         return S.Server(torch, dev, hbm_budget=budget, slots=slots, max_batch_ops=max_ops,
