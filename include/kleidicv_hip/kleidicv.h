@@ -612,6 +612,13 @@ kleidicv_error_t kleidicv_hip_batch_launch(const kleidicv_hip_op_desc *ops, size
 /* Waits for the device to finish outstanding work, then frees the batch. */
 void kleidicv_hip_batch_destroy(kleidicv_hip_batch *batch);
 
+/* The unbatched counterpart: each op through its single-image device entry
+ * point (kleidicv_hip_*), in order, on `stream` (stages are kept by the
+ * stream order).  Asynchronous.  On an error *failed_index (if not NULL)
+ * names the op; the ops before it have been launched. */
+kleidicv_error_t kleidicv_hip_run_ops(const kleidicv_hip_op_desc *ops, size_t count,
+                                      void *stream, size_t *failed_index);
+
 /* ------------------------------------------------------------------------
  * 5. Serving.  No counterpart in the reference: a continuous-batching
  *    request loop for one GPU.  Requests (an op descriptor over device

@@ -83,6 +83,31 @@ def test_pyramid_batch_matches_plain_and_oracle():
             prev = want_lv
 
 
+def test_run_ops_matches_plain_calls_and_names_a_failing_op():
+    # kleidicv_hip_run_ops launches each descriptor through its single-image
+    # entry point in order on one stream, so later pyramid stages read the
+    # earlier ones' outputs exactly as the per-image calls do
+    recs = WL.pyramid_batch(torch, dev(), n=12, seed=13, lo=64, hi=260, padded=True)
+    stream = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    WL.pyramid_plain(recs, stream)
+    torch.cuda.synchronize()
+    want = pyramid_outputs(recs)
+    clear_pyramid(recs)
+    torch.cuda.synchronize()
+    descs = WL.pyramid_descs(recs)
+    K.run_ops(descs, stream)
+    torch.cuda.synchronize()
+    for i, (g, w) in enumerate(zip(pyramid_outputs(recs), want)):
+        assert_same(g, w, "output %d" % i)
+    bad = [descs[1], descs[2], descs[4]]   # two resizes, then image 1's Gaussian
+    bad[2].kernel_size = 4                 # an even Gaussian size is declined
+    with pytest.raises(K.KleidiCVError) as e:
+        K.run_ops(bad, stream)
+    torch.cuda.synchronize()
+    assert e.value.code == K.NOT_IMPLEMENTED and e.value.failed_index == 2
+
+
 def test_pyramid_batch_order_and_tuning_invariant():
     recs = WL.pyramid_batch(torch, dev(), n=12, seed=12, lo=64, hi=200)
     descs = WL.pyramid_descs(recs)

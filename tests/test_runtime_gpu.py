@@ -71,10 +71,12 @@ def test_allocation_failure_returns_allocation_error():
     d_src = torch.from_numpy(src).to(dev())
     d_dst = torch.empty((121, 171, 3), dtype=torch.uint8, device=dev())
     torch.cuda.synchronize()
-    # device API: the tables allocation fails
+    # device API on a stream the library has not seen (its table buffer is
+    # allocated on first use): the tables allocation fails
+    fresh = torch.cuda.Stream()
     K.fail_device_allocation(0)
     with pytest.raises(K.KleidiCVError) as e:
-        K.resize_linear(d_src, 171, 121, dst=d_dst, device_api=True)
+        K.resize_linear(d_src, 171, 121, dst=d_dst, stream=fresh, device_api=True)
     assert e.value.code == K.ALLOCATION
     # drop-in on host buffers: each of the staging and table allocations in turn
     for after in range(3):
@@ -84,9 +86,32 @@ def test_allocation_failure_returns_allocation_error():
         assert e.value.code == K.ALLOCATION, after
     K.fail_device_allocation(-1)
     assert_same(K.resize_linear(src, 171, 121), want, "after the failures")
-    K.resize_linear(d_src, 171, 121, dst=d_dst, device_api=True)
+    K.resize_linear(d_src, 171, 121, dst=d_dst, stream=fresh, device_api=True)
     torch.cuda.synchronize()
     assert_same(d_dst.cpu().numpy(), want, "device API after the failures")
+
+
+def test_device_resize_reuses_its_stream_table_buffer():
+    # calls on one stream share a table buffer that grows on demand; back to
+    # back calls of different sizes (tables written and read in stream order)
+    # and calls on a second stream give the oracle's bytes
+    rng = np.random.default_rng(74)
+    cases = [((90, 130, 3), 171, 121), ((300, 500, 1), 777, 411), ((60, 50, 4), 37, 29),
+             ((300, 500, 1), 200, 120), ((90, 130, 3), 171, 121)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    jobs = []
+    for i, (shape, w, h) in enumerate(cases * 2):
+        src = rng.integers(0, 256, shape, dtype=np.uint8)
+        d_src = torch.from_numpy(src).to(dev())
+        d_dst = torch.empty((h, w, shape[2]), dtype=torch.uint8, device=dev())
+        jobs.append((src, d_src, d_dst, w, h, streams[i % 2]))
+    torch.cuda.synchronize()
+    for _, d_src, d_dst, w, h, st in jobs:
+        K.resize_linear(d_src, w, h, dst=d_dst, stream=st, device_api=True)
+    torch.cuda.synchronize()
+    for src, _, d_dst, w, h, _ in jobs:
+        _, want = O.resize_linear(src, w, h)
+        assert_same(d_dst.cpu().numpy(), want, "%s -> %dx%d" % (src.shape, w, h))
 
 
 def test_failed_batch_rebuild_keeps_the_previous_plan():

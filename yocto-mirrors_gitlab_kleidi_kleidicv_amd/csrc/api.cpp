@@ -5,7 +5,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <deque>
+#include <mutex>
 
 #include "kcv_internal.h"
 #include "kleidicv_hip/kleidicv.h"
@@ -114,6 +117,70 @@ class Session {
   int n_ = 0;
 };
 
+
+// ---------------------------------------------------------------------------
+// Resize coordinate tables of device-API calls: one grow-only device buffer
+// per (device, stream), reused by successive calls on that stream.  A
+// stream runs its work in order, so one call's tables kernel never
+// overwrites tables that an earlier call's resize kernel still reads.  The
+// entry's lock is held while a call enqueues both of its kernels, so calls
+// on one stream from several host threads do not interleave them.  A
+// per-call hipMallocAsync / hipFreeAsync pair cost about 25 us of host time
+// (the free waits), more than a small resize takes on the GPU.  A buffer
+// grows stream-ordered (the old one freed after the calls that read it) and
+// is kept until the process ends: per stream, the largest table size used.
+class TableCache {
+ public:
+  struct Lease {
+    std::unique_lock<std::mutex> lock;
+    void *ptr = nullptr;
+  };
+
+  kleidicv_error_t get(hipStream_t s, size_t bytes, Lease *out) {
+    int dev = 0;
+    KCV_TRY(from_hip(hipGetDevice(&dev)));
+    Entry *e = nullptr;
+    {
+      std::lock_guard<std::mutex> g(mutex_);
+      for (Entry &x : entries_)
+        if (x.dev == dev && x.stream == s) e = &x;
+      if (e == nullptr) {
+        entries_.emplace_back();
+        e = &entries_.back();
+        e->dev = dev;
+        e->stream = s;
+      }
+    }
+    std::unique_lock<std::mutex> lock(e->mutex);
+    if (e->bytes < bytes) {
+      const size_t grow = std::max(bytes, 2 * e->bytes);
+      void *p = nullptr;
+      KCV_TRY(from_hip(kcv::device_alloc(&p, grow, s, true)));
+      if (e->ptr) (void)hipFreeAsync(e->ptr, s);  // after the calls that read it
+      e->ptr = p;
+      e->bytes = grow;
+    }
+    out->ptr = e->ptr;
+    out->lock = std::move(lock);
+    return KLEIDICV_OK;
+  }
+
+ private:
+  struct Entry {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    std::mutex mutex;
+  };
+  std::mutex mutex_;
+  std::deque<Entry> entries_;  // stable addresses
+};
+
+TableCache &table_cache() {
+  static TableCache *c = new TableCache;  // lives until the process ends
+  return *c;
+}
 
 // ---------------------------------------------------------------------------
 // Entry point bodies: prepare (validate + fill arguments), then either launch
@@ -305,13 +372,16 @@ kleidicv_error_t resize_impl(const uint8_t *src, size_t src_stride, size_t src_w
                               dst_width, dst_height, channels, &a, &act));
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
+  const size_t table_bytes = kcv::resize_workspace_bytes(a);
+  if (!sync) {
+    if (table_bytes == 0) return from_hip(kcv::launch_resize(a, nullptr, s));
+    TableCache::Lease lease;
+    KCV_TRY(table_cache().get(s, table_bytes, &lease));
+    return from_hip(kcv::launch_resize(a, lease.ptr, s));
+  }
   Session ss(s);
   void *ws = nullptr;
-  KCV_TRY(ss.scratch(kcv::resize_workspace_bytes(a), &ws));
-  if (!sync) {
-    KCV_TRY(from_hip(kcv::launch_resize(a, ws, s)));
-    return KLEIDICV_OK;  // Session frees the workspace in stream order
-  }
+  KCV_TRY(ss.scratch(table_bytes, &ws));
   KCV_TRY(ss.input(src, extent(src_stride, src_width * channels, src_height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, dst_width * channels, dst_height, &a.dst));
   KCV_TRY(from_hip(kcv::launch_resize(a, ws, s)));

@@ -24,6 +24,19 @@ Runtime &runtime() {
                               rt.device) != hipSuccess)
       return;
     if (hipStreamCreateWithFlags(&rt.stream, hipStreamNonBlocking) != hipSuccess) return;
+    // Per-call scratch (resize tables, staged host buffers) is stream-ordered
+    // (hipMallocAsync / hipFreeAsync).  With the default pool's release
+    // threshold of 0 every free returns the memory to the driver and the
+    // next allocation maps it again, which waits for the device: about
+    // 20 us per small resize call.  Keeping freed blocks in the pool makes
+    // the pair a few microseconds; the pool holds at most the peak of
+    // outstanding scratch.
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, rt.device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
+    (void)hipGetLastError();
     rt.ok = true;
   });
   return rt;

@@ -526,6 +526,19 @@ struct kleidicv_hip_server {
 
 extern "C" {
 
+kleidicv_error_t kleidicv_hip_run_ops(const kleidicv_hip_op_desc *ops, size_t count,
+                                      void *stream, size_t *failed_index) {
+  if (ops == nullptr && count > 0) return KLEIDICV_ERROR_NULL_POINTER;
+  for (size_t i = 0; i < count; ++i) {
+    const kleidicv_error_t e = launch_alone(ops[i], static_cast<hipStream_t>(stream));
+    if (e != KLEIDICV_OK) {
+      if (failed_index) *failed_index = i;
+      return e;
+    }
+  }
+  return KLEIDICV_OK;
+}
+
 double kleidicv_hip_server_clock(void) { return now_seconds(); }
 
 kleidicv_error_t kleidicv_hip_server_create(const kleidicv_hip_server_config *config,

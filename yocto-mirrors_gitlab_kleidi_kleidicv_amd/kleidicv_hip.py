@@ -516,6 +516,19 @@ class Batch:
         self.close()
 
 
+def run_ops(descs, stream=None):
+    """Each descriptor through its single-image device entry point, in order
+    on `stream` (kleidicv_hip_run_ops): the unbatched counterpart of a batch."""
+    arr = (OpDesc * len(descs))(*descs)
+    failed = C.c_size_t(0)
+    code = lib().kleidicv_hip_run_ops(arr, C.c_size_t(len(descs)), _stream_arg(stream),
+                                      C.byref(failed))
+    if code != OK:
+        err = KleidiCVError(code, "kleidicv_hip_run_ops (op %d)" % failed.value)
+        err.failed_index = failed.value
+        raise err
+
+
 def batch_workspace_size(descs, tuning=None):
     """Device bytes kleidicv_hip_batch_launch needs for these descriptors."""
     arr = (OpDesc * len(descs))(*descs)
