@@ -444,14 +444,49 @@ __device__ __forceinline__ uint32_t round_lerp_u8(float v) {
   return round_half_away_u8(fmaxf(v, 0.0f));
 }
 
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+
+// Channels 2k and 2k + 1 of a pixel as two f16 values 1024 + byte: the byte
+// goes under an exponent byte 0x64, and f16 spacing is 1 on [1024, 2048), so
+// both are exact.
+__device__ __forceinline__ h16x2 biased_pair(uint32_t v, int k) {
+  return __builtin_bit_cast(h16x2, __builtin_amdgcn_perm(0x64646464u, v,
+                                                         k == 0 ? 0x04010400u : 0x04030402u));
+}
+
 // lerp_2d (transform_neon.h:331-337) per channel.  FINITE: the fractions
 // are known finite (then the result lies in [0, 255] and needs no NaN
 // mapping).
+//
+// Channels are taken in pairs: one packed f16 subtraction gives two exact tap
+// differences (|b - a| <= 255), another two exact tap values (1024 + a -
+// 1024), and each horizontal fma takes them as f16 operands (v_fma_mix_f32:
+// f16 inputs widened exactly, one rounding), so it equals the float fma of
+// the same values: 40 instead of 48 VALU instructions per 4-channel pixel.
+// An odd last channel takes float taps directly.
 template <int C, bool FINITE = false>
 __device__ __forceinline__ uint32_t lerp_quad(const Quad &q) {
   uint32_t out = 0;
+  const h16x2 bias = {static_cast<_Float16>(1024.0f), static_cast<_Float16>(1024.0f)};
 #pragma unroll
-  for (int c = 0; c < C; ++c) {
+  for (int k = 0; k < C / 2; ++k) {
+    const h16x2 a = biased_pair(q.a, k), b = biased_pair(q.b, k);
+    const h16x2 c = biased_pair(q.c, k), d = biased_pair(q.d, k);
+    const h16x2 dab = b - a, dcd = d - c;
+    h16x2 pa = a - bias, pc = c - bias;
+    // keeps each pair in one register, so one packed subtraction makes it
+    // (else each half gets an f16 subtraction of its own)
+    asm("" : "+v"(pa), "+v"(pc));
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const float line0 = fmaf(static_cast<float>(dab[j]), q.xfrac, static_cast<float>(pa[j]));
+      const float line1 = fmaf(static_cast<float>(dcd[j]), q.xfrac, static_cast<float>(pc[j]));
+      const float res = fmaf(line1 - line0, q.yfrac, line0);
+      out |= (FINITE ? round_half_away_u8(res) : round_lerp_u8(res)) << (8 * (2 * k + j));
+    }
+  }
+  if constexpr (C % 2 == 1) {
+    constexpr int c = C - 1;
     const float pa = static_cast<float>(byte_of(q.a, c));
     const float pb = static_cast<float>(byte_of(q.b, c));
     const float pc = static_cast<float>(byte_of(q.c, c));
