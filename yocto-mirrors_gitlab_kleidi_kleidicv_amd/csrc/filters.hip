@@ -447,16 +447,16 @@ struct TileGrid {
   int tiles_x;
 };
 
-TileGrid tile_grid(int width, int height, int channels) {
+TileGrid tile_grid(int width, int height, int channels, int tile_rows) {
   const int tx = (width * channels + kTileW - 1) / kTileW;
-  const int ty = (height + kTileH - 1) / kTileH;
+  const int ty = (height + tile_rows - 1) / tile_rows;
   if (int64_t{tx} * ty <= kMaxLinearTiles) return {dim3(tx * ty), tx};
   return {dim3(tx, ty), 0};
 }
 
 template <int MODE>
 hipError_t launch_mode(const SepArgs &a, hipStream_t s) {
-  const TileGrid g = tile_grid(a.width, a.height, a.channels);
+  const TileGrid g = tile_grid(a.width, a.height, a.channels, sep_tile_rows(a.ksize));
   const dim3 grid = g.grid;
   const int tx = g.tiles_x;
   switch (a.ksize) {
@@ -563,7 +563,7 @@ hipError_t launch_median(const MedianArgs &a, hipStream_t stream) {
     }
     return hipGetLastError();
   }
-  const TileGrid g = tile_grid(a.width, a.height, a.channels);
+  const TileGrid g = tile_grid(a.width, a.height, a.channels, kTileH);
   switch (a.ksize * 8 + a.channels) {
 #define KCV_M(K, C) \
   case K * 8 + C: median_kernel<K, C><<<g.grid, kThreads, 0, stream>>>(a, g.tiles_x); break;

@@ -52,11 +52,12 @@ __device__ __forceinline__ int map_index(int i, int n, int border) {
   return min(max(r, 0), n - 1);
 }
 
-template <int R>
+// LDS window of a tile of TH output rows for a radius-R filter.
+template <int R, int TH = kTileH>
 struct Window {
   static constexpr int kPad = 16 * ((R + 3) / 4);  // >= R*C bytes for C <= 4
   static constexpr int kCols = kTileW + 2 * kPad;
-  static constexpr int kRows = kTileH + 2 * R;
+  static constexpr int kRows = TH + 2 * R;
 };
 
 // Column border map as one branch-free formula for the four filter
@@ -81,7 +82,7 @@ __device__ __forceinline__ int map_col(const ColMap &m, int x) {
   return min(max(r, 0), m.n - 1);
 }
 
-// Stages rows [y0-R, y0+kTileH+R) x bytes [e0-pad, e0+kTileW+pad) of the
+// Stages rows [y0-R, y0+TH+R) x bytes [e0-pad, e0+kTileW+pad) of the
 // border-extended image into `win` (row pitch Window<R>::kCols).  Rows are
 // mapped through the border first, so every 16-byte chunk that lies inside
 // the row is one 16-byte load (edge tiles and unaligned rows included); only
@@ -90,11 +91,11 @@ __device__ __forceinline__ int map_col(const ColMap &m, int x) {
 // registers: all of a thread's 16-byte loads are issued first and stay in
 // flight while the edge bytes are fetched, and nothing waits on a load
 // before the LDS stores.
-template <int R, int C>
+template <int R, int C, int TH = kTileH>
 __device__ __forceinline__ void stage_window(uint8_t *win, const uint8_t *src,
                                              size_t stride, int width,
                                              int height, int border, int e0, int y0) {
-  using G = Window<R>;
+  using G = Window<R, TH>;
   constexpr int kChunks = G::kCols / 16;
   constexpr int kTotal = G::kRows * kChunks;
   constexpr int kPerThread = (kTotal + kThreads - 1) / kThreads;
@@ -185,20 +186,24 @@ __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
 typedef unsigned short sep_us2 __attribute__((ext_vector_type(2)));
 
 template <int K>
+using SepWindow = Window<K / 2, sep_tile_rows(K)>;
+
+template <int K>
 constexpr int separable_win_bytes() {
-  return (Window<K / 2>::kRows * Window<K / 2>::kCols + 15) / 16 * 16;
+  return (SepWindow<K>::kRows * SepWindow<K>::kCols + 15) / 16 * 16;
 }
 
 template <int K>
 constexpr int separable_smem_bytes() {
-  return separable_win_bytes<K>() + 2 * kTileH * Window<K / 2>::kCols;
+  return separable_win_bytes<K>() + 2 * sep_tile_rows(K) * SepWindow<K>::kCols;
 }
 
 template <int MODE, int K, int C>
 __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uint16_t *vert,
                                                      int e0, int y0) {
   constexpr int R = K / 2;
-  using G = Window<R>;
+  constexpr int TH = sep_tile_rows(K);
+  using G = SepWindow<K>;
   constexpr int RC = R * C;
   constexpr int D = (RC + 7) / 8;         // 8-value chunks each side
   constexpr int NV = 8 * (2 * D + 1);      // column sums held per thread
@@ -212,7 +217,7 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
   for (int i = 0; i < K / 2; ++i)
     wpair[i] = sep_us2{static_cast<uint16_t>(w[2 * i]), static_cast<uint16_t>(w[2 * i + 1])};
 #pragma unroll
-  for (int pass = 0; pass < kTileH / 8; ++pass) {
+  for (int pass = 0; pass < TH / 8; ++pass) {
     const int r = ty + 8 * pass;
     const int y = y0 + r;
     if (y >= a.height) break;
@@ -279,20 +284,21 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
   }
 }
 
-// One 256-byte x 16-row output tile at tile coordinates (bx, by).  `smem`
-// holds separable_smem_bytes<K>() bytes, 16-byte aligned.
+// One 256-byte x sep_tile_rows(K)-row output tile at tile coordinates
+// (bx, by).  `smem` holds separable_smem_bytes<K>() bytes, 16-byte aligned.
 template <int MODE, int K, int C>
 __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
                                                uint8_t *smem) {
   constexpr int R = K / 2;
-  using G = Window<R>;
+  constexpr int TH = sep_tile_rows(K);
+  using G = SepWindow<K>;
   uint8_t *win = smem;
   uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<K>());
 
   const int e0 = bx * kTileW;
-  const int y0 = by * kTileH;
+  const int y0 = by * TH;
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
-  stage_window<R, C>(win, a.src, a.src_stride, a.width, a.height, a.border, e0, y0);
+  stage_window<R, C, TH>(win, a.src, a.src_stride, a.width, a.height, a.border, e0, y0);
   __syncthreads();
 
   // Vertical pass: each item is one window dword column over two output
@@ -300,7 +306,7 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   // once for both rows; v_pk_mad_u16 does two lanes per op (with clamp in
   // the saturating mode).
   constexpr int kQuads = G::kCols / 4;
-  constexpr int kPairs = kTileH / 2;
+  constexpr int kPairs = TH / 2;
   const uint32_t *win32 = reinterpret_cast<const uint32_t *>(win);
   sep_us2 wy[K];
 #pragma unroll
@@ -376,8 +382,13 @@ struct Down {
   static constexpr int kOutW = 32 * kOB;  // output bytes per tile row
 };
 
+// The staged window: 20 source rows (radius 2 around 16).
+constexpr int blur_down_win_bytes() {
+  return (Window<2>::kRows * Window<2>::kCols + 15) / 16 * 16;
+}
+
 constexpr int blur_down_smem_bytes() {
-  return separable_win_bytes<5>() + 2 * kDownRows * Window<2>::kCols;
+  return blur_down_win_bytes() + 2 * kDownRows * Window<2>::kCols;
 }
 
 template <int C>
@@ -386,7 +397,7 @@ __device__ __forceinline__ void blur_down_tile(const SepArgs &a, int bx, int by,
   using G = Window<2>;
   constexpr int OB = Down<C>::kOB;
   uint8_t *win = smem;
-  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<5>());
+  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + blur_down_win_bytes());
   const int o0 = bx * Down<C>::kOutW;  // first output byte of the tile
   const int y0 = 2 * kDownRows * by;   // first source row of the tile
   __syncthreads();  // LDS may still hold the previous tile of a batched loop
