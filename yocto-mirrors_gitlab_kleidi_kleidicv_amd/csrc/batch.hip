@@ -142,9 +142,7 @@ __global__ __launch_bounds__(kThreads) void batch_resize_kernel(Segment s) {
 __global__ __launch_bounds__(kThreads) void batch_resize_direct_kernel(Segment s) {
   for_each_item(s, [&](const Job &job, int bx, int by) {
     const ResizeJob &rj = job.u.resize;
-    const int quads = (rj.a.dst_width * rj.a.channels + 3) / 4;
-    int lanes = kThreads;
-    while (lanes > 1 && lanes / 2 >= quads) lanes /= 2;
+    const int lanes = rsz::direct_lanes(rj.a);
     const int y0 = by * job.band;
     rsz::resize_direct_rows<rsz::kLerpGeneric>(
         rj.a, rj.cols, rj.rows, bx * rsz::kTileE + static_cast<int>(threadIdx.x % lanes) * 4,

@@ -235,9 +235,12 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
         j->tiles_x = static_cast<int>(div_up(a.src_width, kUp2x4TilePx));
         return j->tiles_x * (int64_t{a.src_height} + 1);
       } else {
-        // no taller than the band whose source window fits the LDS stage:
-        // taller bands fall back to per-byte global reads
-        j->band = rsz::band_rows(a);
+        // staged: no taller than the band whose source window fits the LDS
+        // stage (taller bands fall back to per-byte global reads); direct:
+        // one group of rows in flight per thread
+        j->band = rsz::generic_reads_direct(a)
+                      ? rsz::direct_band_rows(rsz::kThreads / rsz::direct_lanes(a))
+                      : rsz::band_rows(a);
         if (t.resize_band_rows) j->band = std::min(j->band, static_cast<int>(t.resize_band_rows));
         j->tiles_x = static_cast<int>(div_up(n, kResizeTileE));
       }

@@ -53,10 +53,11 @@ template <int MODE>
 hipError_t launch_staged(const ResizeArgs &a, const int2 *cols, const int2 *rows,
                          hipStream_t stream) {
   const int64_t n = int64_t{a.dst_width} * a.channels;
-  const int band = band_rows(a);
+  const bool direct = MODE == kLerpGeneric && generic_reads_direct(a);
+  const int band = direct ? direct_band_rows() : band_rows(a);
   const dim3 grid(static_cast<unsigned>((n + kTileE - 1) / kTileE),
                   static_cast<unsigned>((a.dst_height + band - 1) / band));
-  if (MODE == kLerpGeneric && generic_reads_direct(a))
+  if (direct)
     resize_direct_kernel<<<grid, kThreads, 0, stream>>>(a, cols, rows, band);
   else
     resize_staged_kernel<MODE><<<grid, kThreads, 0, stream>>>(a, cols, rows, band);

@@ -547,20 +547,40 @@ __host__ __device__ inline bool generic_reads_direct(const ResizeArgs &a) {
   return 2 * static_cast<int64_t>(a.src_height) >= 3 * static_cast<int64_t>(a.dst_height);
 }
 
+// Rows a thread of the direct path has in flight at once.
+constexpr int kDirectRows = 4;
+
+// Threads per destination row of a batched direct-path item: a power of two
+// no larger than needed for the row's 4-byte quads; the workgroup's other
+// threads take further rows of the band.
+__host__ __device__ inline int direct_lanes(const ResizeArgs &a) {
+  const int quads = (a.dst_width * a.channels + 3) / 4;
+  int lanes = kThreads;
+  while (lanes > 1 && lanes / 2 >= quads) lanes /= 2;
+  return lanes;
+}
+
+// Band of a direct-path work item: one group of kDirectRows rows for each
+// thread (rows_per_pass = workgroup threads / threads per row).
+inline int direct_band_rows(int rows_per_pass = 1) { return kDirectRows * rows_per_pass; }
+
 // Generic lerp read straight from global memory: destination bytes
 // e0 .. e0 + 3 of rows y_begin, y_begin + y_step, ... < y_end.  Used for
 // downsizing by 1.5 or more, where a source row serves about one destination
 // row and an LDS stage would buy no reuse: each row is one 8- or 16-byte
 // load at any alignment (gfx950 vector memory runs in unaligned mode), the
 // window start pulled back from the row's end so that it stays inside the
-// row; taps spread wider than 16 bytes read byte by byte.
+// row; taps spread wider than 16 bytes read byte by byte.  The loads of
+// kDirectRows rows are issued together (rows past the end repeat the last
+// row and are not stored), so a band costs one memory round trip, not one
+// per row.
 template <int MODE>
 __device__ __forceinline__ void resize_direct_rows(const ResizeArgs &a,
                                                    const int2 *__restrict__ cols,
                                                    const int2 *__restrict__ rows, int e0,
                                                    int y_begin, int y_end, int y_step) {
   const int n = a.dst_width * a.channels;
-  if (e0 >= n) return;
+  if (e0 >= n || y_begin >= y_end) return;
   const int row_bytes = a.src_width * a.channels;
   const QuadTaps q = load_taps(cols, e0, n);
   int tmin, tmax;
@@ -587,31 +607,57 @@ __device__ __forceinline__ void resize_direct_rows(const ResizeArgs &a,
   }
   const us2 wx01 = us2{static_cast<uint16_t>(q.wx[0]), static_cast<uint16_t>(q.wx[1])};
   const us2 wx23 = us2{static_cast<uint16_t>(q.wx[2]), static_cast<uint16_t>(q.wx[3])};
-  for (int y = y_begin; y < y_end; y += y_step) {
-    const int2 r = rows[y];
-    const int top = r.x, bot = r.y & 0xFFFFFF;
-    const int wy = (r.y >> 24) & 0xFF;
-    const uint8_t *tp = a.src + static_cast<size_t>(top) * a.src_stride;
-    const uint8_t *bp = a.src + static_cast<size_t>(bot) * a.src_stride;
-    uint32_t packed = 0;
-    if (narrow) {
-      const uint2 t = *reinterpret_cast<const u2_any *>(tp + base);
-      const uint2 b = *reinterpret_cast<const u2_any *>(bp + base);
-      packed = lerp_quad8<MODE>(t.x, t.y, b.x, b.y, sel, wx01, wx23, wy);
-    } else if (wide) {
-      const uint4 tv = *reinterpret_cast<const u4_any *>(tp + base);
-      const uint4 bv = *reinterpret_cast<const u4_any *>(bp + base);
-      const uint32_t t[4] = {tv.x, tv.y, tv.z, tv.w}, b[4] = {bv.x, bv.y, bv.z, bv.w};
-      packed = lerp_quad16<MODE>(t, b, wsel, wx01, wx23, wy);
-    } else {
+  auto row_of = [&](int y, int &wy) {
+    const int2 r = rows[min(y, y_end - 1)];
+    wy = (r.y >> 24) & 0xFF;
+    return r;
+  };
+  auto row_ptr = [&](int i) { return a.src + static_cast<size_t>(i) * a.src_stride; };
+  if (narrow || wide) {
+    for (int y = y_begin; y < y_end; y += kDirectRows * y_step) {
+      int wy[kDirectRows];
+      uint4 tv[kDirectRows], bv[kDirectRows];
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int pa = (q.zero[k] & 1) ? 0 : tp[q.left[k]];
-        const int pc = (q.zero[k] & 1) ? 0 : bp[q.left[k]];
-        const int pb = (q.zero[k] & 2) ? 0 : tp[q.right[k]];
-        const int pd = (q.zero[k] & 2) ? 0 : bp[q.right[k]];
-        packed |= (combine<MODE>(pa, pb, pc, pd, q.wx[k], wy) & 0xFFu) << (8 * k);
+      for (int i = 0; i < kDirectRows; ++i) {
+        const int2 r = row_of(y + i * y_step, wy[i]);
+        const uint8_t *tp = row_ptr(r.x) + base, *bp = row_ptr(r.y & 0xFFFFFF) + base;
+        if (narrow) {
+          const uint2 t = *reinterpret_cast<const u2_any *>(tp);
+          const uint2 b = *reinterpret_cast<const u2_any *>(bp);
+          tv[i] = make_uint4(t.x, t.y, 0, 0);
+          bv[i] = make_uint4(b.x, b.y, 0, 0);
+        } else {
+          tv[i] = *reinterpret_cast<const u4_any *>(tp);
+          bv[i] = *reinterpret_cast<const u4_any *>(bp);
+        }
       }
+#pragma unroll
+      for (int i = 0; i < kDirectRows; ++i) {
+        uint32_t packed;
+        if (narrow) {
+          packed = lerp_quad8<MODE>(tv[i].x, tv[i].y, bv[i].x, bv[i].y, sel, wx01, wx23, wy[i]);
+        } else {
+          const uint32_t t[4] = {tv[i].x, tv[i].y, tv[i].z, tv[i].w};
+          const uint32_t b[4] = {bv[i].x, bv[i].y, bv[i].z, bv[i].w};
+          packed = lerp_quad16<MODE>(t, b, wsel, wx01, wx23, wy[i]);
+        }
+        if (y + i * y_step < y_end) store_dst_quad(a, y + i * y_step, e0, n, packed);
+      }
+    }
+    return;
+  }
+  for (int y = y_begin; y < y_end; y += y_step) {
+    int wy;
+    const int2 r = row_of(y, wy);
+    const uint8_t *tp = row_ptr(r.x), *bp = row_ptr(r.y & 0xFFFFFF);
+    uint32_t packed = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int pa = (q.zero[k] & 1) ? 0 : tp[q.left[k]];
+      const int pc = (q.zero[k] & 1) ? 0 : bp[q.left[k]];
+      const int pb = (q.zero[k] & 2) ? 0 : tp[q.right[k]];
+      const int pd = (q.zero[k] & 2) ? 0 : bp[q.right[k]];
+      packed |= (combine<MODE>(pa, pb, pc, pd, q.wx[k], wy) & 0xFFu) << (8 * k);
     }
     store_dst_quad(a, y, e0, n, packed);
   }
