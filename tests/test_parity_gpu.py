@@ -491,6 +491,71 @@ def test_float_gathers_narrow_and_odd_stride(rng, c, interp, border):
         assert_same(got, want, "warp " + what)
 
 
+def device_image_at(torch, img, lead):
+    """Device copy of the (h, w[, c]) numpy view `img` (rows may be padded)
+    that starts `lead` bytes into its allocation and ends at the image's last
+    byte (no padding after the last row)."""
+    h = img.shape[0]
+    stride = img.strides[0]
+    row = img.shape[1] * (img.shape[2] if img.ndim == 3 else 1)
+    flat = np.zeros(lead + (h - 1) * stride + row, np.uint8)
+    for y in range(h):
+        flat[lead + y * stride:lead + y * stride + row] = np.ascontiguousarray(img[y]).reshape(-1)
+    dev = torch.from_numpy(flat).cuda()
+    shape = img.shape if img.ndim == 3 else img.shape + (1,)
+    v = torch.as_strided(dev, shape, (stride, shape[2], 1), lead)
+    return v if img.ndim == 3 else v[:, :, 0]
+
+
+@pytest.mark.parametrize("c", [1, 2, 3, 4])
+def test_gathers_at_unaligned_bases_and_image_end(rng, c):
+    # Device-API gathers on sources that start 0-3 bytes past an aligned
+    # address and end at the allocation's last byte, with taps crowding the
+    # image's last pixels: the aligned source windows are pulled back from
+    # the image's end and the lead bytes before its start are never used.
+    import torch
+    stream = torch.cuda.Stream()
+    for (sw, sh, pad) in [(2, 1, 0), (3, 2, 1), (13, 7, 0), (50, 9, 3)]:
+        img = padded_rows(rng, sh, sw, c, pad)
+        host = np.ascontiguousarray(img)
+        for lead in range(4):
+            src = device_image_at(torch, img, lead)
+            dw, dh = 37, 6
+            near = rng.random((dh, dw)) < 0.6
+            mx = np.where(near, rng.uniform(sw - 2.5, sw + 0.25, (dh, dw)),
+                          rng.uniform(-1.5, sw + 0.5, (dh, dw))).astype(np.float32)
+            my = np.where(near, rng.uniform(sh - 2.5, sh + 0.25, (dh, dw)),
+                          rng.uniform(-1.5, sh + 0.5, (dh, dw))).astype(np.float32)
+            bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+            what = "c%d %dx%d pad %d lead %d" % (c, sw, sh, pad, lead)
+            for border in ("REPLICATE", "CONSTANT"):
+                dst = torch.empty((dh, dw, c) if c > 1 else (dh, dw), dtype=torch.uint8,
+                                  device="cuda")
+                K.remap_f32(src, torch.from_numpy(mx).cuda(), torch.from_numpy(my).cuda(),
+                            "LINEAR", border, bv, dst=dst, stream=stream, device_api=True)
+                stream.synchronize()
+                _, want = O.remap_f32(host, mx, my, "LINEAR", border, bv)
+                assert_same(dst.cpu().numpy(), want, "remap_f32 %s %s" % (border, what))
+                T = np.array([[1.0, 0.01, sw - dw * 0.9], [0.0, 1.0, sh - dh - 0.6],
+                              [0.0, 1e-4, 1.0]], np.float32)
+                K.warp_perspective(src, T, dw, dh, "LINEAR", border, bv, dst=dst, stream=stream,
+                                   device_api=True)
+                stream.synchronize()
+                _, want = O.warp_perspective(host, T, dw, dh, "LINEAR", border, bv)
+                assert_same(dst.cpu().numpy(), want, "warp %s %s" % (border, what))
+                if c in (1, 4):
+                    ix = np.floor(mx * 32).astype(np.int64)
+                    iy = np.floor(my * 32).astype(np.int64)
+                    mapxy = np.stack([ix >> 5, iy >> 5], axis=-1).astype(np.int16)
+                    mapfrac = ((ix & 31) | ((iy & 31) << 5)).astype(np.uint16)
+                    K.remap_s16point5(src, torch.from_numpy(mapxy).cuda(),
+                                      torch.from_numpy(mapfrac).cuda(), border, bv, dst=dst,
+                                      stream=stream, device_api=True)
+                    stream.synchronize()
+                    _, want = O.remap_s16point5(host, mapxy, mapfrac, border, bv)
+                    assert_same(dst.cpu().numpy(), want, "remap_s16 %s %s" % (border, what))
+
+
 def test_warp_zero_weight_row(rng):
     # a homography whose weight crosses 0 inside the image takes the checked path
     src = rng.integers(0, 256, (40, 60), dtype=np.uint8)

@@ -69,6 +69,62 @@ __device__ __forceinline__ uint32_t byte_of(uint32_t v, int c) {
 // with a 32-bit offset.
 constexpr uint32_t kRawBufferWord3 = 0x00020000u;  // gfx9 raw buffer resource
 
+// A source image as a raw buffer resource read through aligned windows.
+// The resource starts at the image's first byte rounded down to a dword
+// (`lead` bytes before it) and covers whole dwords up to the one holding
+// the image's last byte (`records` bytes).  Every dword a load fetches
+// therefore holds at least one image byte and lies in a mapped page; dwords
+// of a window beyond `records` read as 0 without a memory access (buffer
+// loads of several dwords are range checked per dword).  Byte-granular
+// loads at odd offsets cost the texture units more than an aligned load of
+// the same bytes (the 4K one-channel s16point5 remap: 24.5 us with 2-byte
+// tap pairs, 22.1 with aligned 8-byte windows), so the tap bytes of a
+// gather are cut out of an aligned window instead.
+struct SrcWindow {
+  __amdgpu_buffer_rsrc_t rsrc;
+  uint32_t lead, records;
+};
+
+// Largest image span read through a buffer resource: offsets plus the lead
+// and the dword rounding stay below 2^31.
+constexpr uint64_t kBufferSpanLimit = (1ull << 31) - 8;
+
+// `span` bytes from `p` (span < kBufferSpanLimit).
+__device__ __forceinline__ SrcWindow make_window(const uint8_t *p, uint64_t span) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  SrcWindow w;
+  w.lead = static_cast<uint32_t>(a & 3u);
+  w.records = static_cast<uint32_t>((span + w.lead + 3) & ~uint64_t{3});
+  w.rsrc = __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(a - w.lead), 0,
+                                             static_cast<int>(w.records), kRawBufferWord3);
+  return w;
+}
+
+// Bytes off .. off + 3 of the image (those up to its last byte are valid)
+// from the aligned 8-byte window at the dword holding byte off.  A window
+// may reach past the resource's end: multi-dword buffer loads are range
+// checked per dword, so such a dword reads as 0 and is not fetched, and the
+// dwords holding wanted bytes are always inside.
+__device__ __forceinline__ uint32_t window4(const SrcWindow &w, uint32_t off) {
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  const uint32_t o = off + w.lead;
+  const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                w.rsrc, static_cast<int>(o & ~3u), 0, 0));
+  return __builtin_amdgcn_alignbyte(v.y, v.x, o & 3u);
+}
+
+// Bytes off .. off + 5 (the first four in lo, the next two in hi) from the
+// aligned 12-byte window at the dword holding byte off.
+__device__ __forceinline__ void window6(const SrcWindow &w, uint32_t off, uint32_t &lo,
+                                        uint32_t &hi) {
+  typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+  const uint32_t o = off + w.lead;
+  const u32x3 v = __builtin_bit_cast(u32x3, __builtin_amdgcn_raw_buffer_load_b96(
+                                                w.rsrc, static_cast<int>(o & ~3u), 0, 0));
+  lo = __builtin_amdgcn_alignbyte(v.y, v.x, o & 3u);
+  hi = __builtin_amdgcn_alignbyte(v.z, v.y, o & 3u) & 0xFFFFu;
+}
+
 __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
                                               uint32_t c, uint32_t d,
                                               uint32_t fx, uint32_t fy) {
@@ -80,7 +136,7 @@ __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
 // One destination pixel from its map entries (xy: x in the low, y in the
 // high 16 bits; fr: 5-bit x and y fractions).
 template <int C, int BORDER>
-__device__ __forceinline__ uint32_t remap_s16_px(__amdgpu_buffer_rsrc_t src, uint32_t stride,
+__device__ __forceinline__ uint32_t remap_s16_px(const SrcWindow &src, uint32_t stride,
                                                  int w, int h, uint32_t xy, uint32_t fr,
                                                  uint32_t bv) {
   const int mx = static_cast<int16_t>(xy & 0xFFFF);
@@ -123,16 +179,18 @@ __device__ __forceinline__ uint32_t remap_s16_px(__amdgpu_buffer_rsrc_t src, uin
   const uint32_t off1 = static_cast<uint32_t>(cy1) * stride + static_cast<uint32_t>(cx * C);
   uint32_t pa, pb, pc, pd;
   if (C == 1) {
-    const uint32_t t0 = __builtin_amdgcn_raw_buffer_load_b16(src, off0, 0, 0);
-    const uint32_t t1 = __builtin_amdgcn_raw_buffer_load_b16(src, off1, 0, 0);
+    const uint32_t t0 = window4(src, off0);
+    const uint32_t t1 = window4(src, off1);
     pa = (t0 >> (8 * i0)) & 0xFFu;
     pb = (t0 >> (8 * i1)) & 0xFFu;
     pc = (t1 >> (8 * i0)) & 0xFFu;
     pd = (t1 >> (8 * i1)) & 0xFFu;
   } else {
     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    const u32x2 t0 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(src, off0, 0, 0));
-    const u32x2 t1 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(src, off1, 0, 0));
+    const u32x2 t0 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                    src.rsrc, off0 + src.lead, 0, 0));
+    const u32x2 t1 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(
+                                                    src.rsrc, off1 + src.lead, 0, 0));
     pa = i0 ? t0.y : t0.x;
     pb = i1 ? t0.y : t0.x;
     pc = i0 ? t1.y : t1.x;
@@ -220,8 +278,7 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
   if (w >= 2) {
     const uint32_t span = static_cast<uint32_t>(h - 1) * static_cast<uint32_t>(a.src_stride) +
                           static_cast<uint32_t>(w * C);
-    const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint8_t *>(a.src), 0, static_cast<int>(span), kRawBufferWord3);
+    const SrcWindow src = make_window(a.src, span);
     const uint32_t stride = static_cast<uint32_t>(a.src_stride);
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = remap_s16_px<C, BORDER>(src, stride, w, h, xy[k], fr[k], bv);
@@ -288,28 +345,29 @@ __device__ __forceinline__ float recip_rn(float w) {
   return 1.0f / w;
 }
 
-// Source image of a float gather.  When the image spans less than 2 GiB its
-// bytes are read with buffer loads at 32-bit offsets (no 64-bit address
-// arithmetic per tap), else with global loads at 64-bit addresses; the
-// choice is uniform over a launch.  Loads may be unaligned (gfx950 global
-// and buffer accesses are byte-addressed in unaligned mode) and never read
-// past the last byte of the image.
+// Source image of a float gather.  When the image spans less than
+// kBufferSpanLimit bytes, its bytes are read through the
+// aligned-window buffer resource at 32-bit offsets (no 64-bit address
+// arithmetic per tap), else with global loads at 64-bit addresses, which
+// may be unaligned (gfx950 global accesses are byte-addressed in unaligned
+// mode) and never read past the last byte of the image; the choice is
+// uniform over a launch.
 struct Src {
   const uint8_t *p;
   size_t stride;
   uint32_t stride32;
   uint32_t xmax, ymax;
   uint32_t bv;  // border value bytes, one per channel
-  __amdgpu_buffer_rsrc_t rsrc;
+  SrcWindow win;  // meaningful when src_fits_buffer
 };
 
 template <bool BUF, int NB>
 __device__ __forceinline__ uint32_t fetch(const Src &s, uint32_t y, uint32_t xb) {
   if (BUF) {
-    const int off = static_cast<int>(y * s.stride32 + xb);
-    if (NB == 1) return __builtin_amdgcn_raw_buffer_load_b8(s.rsrc, off, 0, 0);
-    if (NB == 2) return __builtin_amdgcn_raw_buffer_load_b16(s.rsrc, off, 0, 0);
-    return __builtin_amdgcn_raw_buffer_load_b32(s.rsrc, off, 0, 0);
+    const int off = static_cast<int>(y * s.stride32 + xb + s.win.lead);
+    if (NB == 1) return __builtin_amdgcn_raw_buffer_load_b8(s.win.rsrc, off, 0, 0);
+    if (NB == 2) return __builtin_amdgcn_raw_buffer_load_b16(s.win.rsrc, off, 0, 0);
+    return __builtin_amdgcn_raw_buffer_load_b32(s.win.rsrc, off, 0, 0);
   }
   const uint8_t *q = s.p + static_cast<size_t>(y) * s.stride + xb;
   typedef uint16_t u16u __attribute__((aligned(1)));
@@ -323,8 +381,8 @@ template <bool BUF>
 __device__ __forceinline__ uint64_t fetch64(const Src &s, uint32_t y, uint32_t xb) {
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
   if (BUF) {
-    const int off = static_cast<int>(y * s.stride32 + xb);
-    const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(s.rsrc, off, 0, 0));
+    const int off = static_cast<int>(y * s.stride32 + xb + s.win.lead);
+    const u32x2 v = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(s.win.rsrc, off, 0, 0));
     return (static_cast<uint64_t>(v.y) << 32) | v.x;
   }
   typedef uint64_t u64u __attribute__((aligned(1)));
@@ -342,7 +400,9 @@ __device__ __forceinline__ uint32_t fetch_px(const Src &s, uint32_t y, uint32_t 
 }
 
 // Pixels cx and cx + 1 of row y (cx + 1 <= xmax), or pixel 0 twice for an
-// image one pixel wide.
+// image one pixel wide.  Through the buffer resource, 1-3 channel pairs are
+// cut out of one aligned window; 4-channel pairs are one 8-byte load at the
+// pixel, dword aligned when the image and its stride are.
 template <int C, bool BUF>
 __device__ __forceinline__ void fetch_pair(const Src &s, uint32_t y, uint32_t cx, uint32_t &p0,
                                            uint32_t &p1) {
@@ -351,6 +411,20 @@ __device__ __forceinline__ void fetch_pair(const Src &s, uint32_t y, uint32_t cx
     return;
   }
   const uint32_t xb = cx * C;
+  if (BUF && C < 4) {
+    const uint32_t off = y * s.stride32 + xb;
+    if (C == 3) {
+      uint32_t lo, hi;
+      window6(s.win, off, lo, hi);
+      p0 = lo & 0xFFFFFFu;
+      p1 = (lo >> 24) | (hi << 8);
+    } else {
+      const uint32_t t = window4(s.win, off);
+      p0 = C == 1 ? t & 0xFFu : t & 0xFFFFu;
+      p1 = C == 1 ? (t >> 8) & 0xFFu : t >> 16;
+    }
+    return;
+  }
   if (C == 1) {
     const uint32_t t = fetch<BUF, 2>(s, y, xb);
     p0 = t & 0xFFu;
@@ -527,15 +601,16 @@ __device__ __forceinline__ Src make_src(const uint8_t *p, size_t stride, int w, 
 #pragma unroll
   for (int c = 0; c < C; ++c) s.bv |= static_cast<uint32_t>(border_value[c]) << (8 * c);
   const uint64_t span = static_cast<uint64_t>(h - 1) * stride + static_cast<uint64_t>(w) * C;
-  s.rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(p), 0,
-                                             static_cast<int>(span < (1ull << 31) ? span : 0),
-                                             kRawBufferWord3);
+  s.win = make_window(p, span < kBufferSpanLimit ? span : 0);
   return s;
 }
 
+// Whether the gathers of `s` go through its buffer resource: the image
+// spans less than kBufferSpanLimit bytes.
 __device__ __forceinline__ bool src_fits_buffer(const Src &s, int c) {
-  return static_cast<uint64_t>(s.ymax) * s.stride + (static_cast<uint64_t>(s.xmax) + 1) * c <
-         (1ull << 31);
+  const uint64_t span =
+      static_cast<uint64_t>(s.ymax) * s.stride + (static_cast<uint64_t>(s.xmax) + 1) * c;
+  return span < kBufferSpanLimit;
 }
 
 // The 4 destination pixels (x .. x + n - 1) of this thread, C bytes each,
