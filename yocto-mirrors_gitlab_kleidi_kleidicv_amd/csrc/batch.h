@@ -87,7 +87,7 @@ struct Segment {
 // Work-item geometry shared by the host (item counts) and device (tile
 // decoding).
 constexpr int kFilterTileW = 256;  // bytes
-constexpr int kFilterTileH = 16;   // rows (median and copy; separable: sep_tile_rows)
+constexpr int kFilterTileH = 16;   // rows
 constexpr int kResizeTileE = 1024;            // destination bytes (staged)
 constexpr int kUp2x4TilePx = rsz::kUp2x4ThreadPx * 256;  // source pixels (exact 2x, 4 ch)
 constexpr int kYuvTileW = 8 * 256;            // pixels

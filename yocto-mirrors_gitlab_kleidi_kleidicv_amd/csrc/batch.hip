@@ -72,9 +72,9 @@ __device__ __forceinline__ void for_each_item(const Segment &s, const Body &body
 
 template <int MODE, int K, int C>
 __global__ __launch_bounds__(kThreads) void batch_sep_kernel(Segment s) {
-  __shared__ __align__(16) uint8_t smem[filt::separable_smem_bytes<K>()];
+  __shared__ __align__(16) uint8_t smem[filt::separable_smem_bytes<K, kFilterTileH>()];
   for_each_item(s, [&](const Job &job, int bx, int by) {
-    filt::separable_tile<MODE, K, C>(job.u.sep, bx, by, smem);
+    filt::separable_tile<MODE, K, C, kFilterTileH>(job.u.sep, bx, by, smem);
   });
 }
 

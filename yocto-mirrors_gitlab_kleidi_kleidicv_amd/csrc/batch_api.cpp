@@ -209,7 +209,7 @@ int64_t job_items(const Prepared &p, const kleidicv_hip_batch_tuning &t, kcv::Jo
     case kFamSep: {
       const SepArgs &a = j->u.sep;
       j->tiles_x = static_cast<int>(div_up(int64_t{a.width} * a.channels, kFilterTileW));
-      return j->tiles_x * div_up(a.height, sep_tile_rows(a.ksize));
+      return j->tiles_x * div_up(a.height, kFilterTileH);
     }
     case kFamMedian: {
       const MedianArgs &a = j->u.median;

@@ -25,10 +25,10 @@ __device__ __forceinline__ void tile_of(int tiles_x, int &bx, int &by) {
 
 template <int MODE, int K, int C>
 __global__ __launch_bounds__(kThreads) void separable_kernel(SepArgs a, int tiles_x) {
-  __shared__ __align__(16) uint8_t smem[separable_smem_bytes<K>()];
+  __shared__ __align__(16) uint8_t smem[separable_smem_bytes<K, sep_tile_rows(K)>()];
   int bx, by;
   tile_of(tiles_x, bx, by);
-  separable_tile<MODE, K, C>(a, bx, by, smem);
+  separable_tile<MODE, K, C, sep_tile_rows(K)>(a, bx, by, smem);
 }
 
 // One kernel per channel count: the tile body is specialised on it.

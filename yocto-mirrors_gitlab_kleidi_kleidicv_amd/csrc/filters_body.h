@@ -185,25 +185,27 @@ __device__ __forceinline__ void store_quad(uint8_t *row, int e, int row_bytes,
 // apart), and the last odd tap through the full-rate 24-bit multiplier.
 typedef unsigned short sep_us2 __attribute__((ext_vector_type(2)));
 
-template <int K>
-using SepWindow = Window<K / 2, sep_tile_rows(K)>;
+// TH: output rows of a tile (sep_tile_rows(K) for single-image launches,
+// kTileH in batches, where many images already fill the GPU and taller
+// tiles load fewer halo rows).
+template <int K, int TH>
+using SepWindow = Window<K / 2, TH>;
 
-template <int K>
+template <int K, int TH>
 constexpr int separable_win_bytes() {
-  return (SepWindow<K>::kRows * SepWindow<K>::kCols + 15) / 16 * 16;
+  return (SepWindow<K, TH>::kRows * SepWindow<K, TH>::kCols + 15) / 16 * 16;
 }
 
-template <int K>
+template <int K, int TH>
 constexpr int separable_smem_bytes() {
-  return separable_win_bytes<K>() + 2 * sep_tile_rows(K) * SepWindow<K>::kCols;
+  return separable_win_bytes<K, TH>() + 2 * TH * SepWindow<K, TH>::kCols;
 }
 
-template <int MODE, int K, int C>
+template <int MODE, int K, int C, int TH>
 __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uint16_t *vert,
                                                      int e0, int y0) {
   constexpr int R = K / 2;
-  constexpr int TH = sep_tile_rows(K);
-  using G = SepWindow<K>;
+  using G = SepWindow<K, TH>;
   constexpr int RC = R * C;
   constexpr int D = (RC + 7) / 8;         // 8-value chunks each side
   constexpr int NV = 8 * (2 * D + 1);      // column sums held per thread
@@ -284,16 +286,15 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
   }
 }
 
-// One 256-byte x sep_tile_rows(K)-row output tile at tile coordinates
-// (bx, by).  `smem` holds separable_smem_bytes<K>() bytes, 16-byte aligned.
-template <int MODE, int K, int C>
+// One 256-byte x TH-row output tile at tile coordinates (bx, by).  `smem`
+// holds separable_smem_bytes<K, TH>() bytes, 16-byte aligned.
+template <int MODE, int K, int C, int TH>
 __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
                                                uint8_t *smem) {
   constexpr int R = K / 2;
-  constexpr int TH = sep_tile_rows(K);
-  using G = SepWindow<K>;
+  using G = SepWindow<K, TH>;
   uint8_t *win = smem;
-  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<K>());
+  uint16_t *vert = reinterpret_cast<uint16_t *>(smem + separable_win_bytes<K, TH>());
 
   const int e0 = bx * kTileW;
   const int y0 = by * TH;
@@ -358,7 +359,7 @@ __device__ __forceinline__ void separable_tile(const SepArgs &a, int bx, int by,
   }
   __syncthreads();
 
-  separable_horizontal<MODE, K, C>(a, vert, e0, y0);
+  separable_horizontal<MODE, K, C, TH>(a, vert, e0, y0);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,9 +26,10 @@ enum SepMode : int {
   kSepSaturate = 2,   // min(255, sum) with unsigned taps, no shift
 };
 
-// Output rows of a separable-filter tile for a K x K kernel: 8 up to 7x7
-// (twice the workgroups of 16-row tiles, which a 1920x1080 image needs to
-// spread over all CUs; the window's 2R halo rows stay few), 16 above.
+// Output rows of a separable-filter tile of a single-image launch for a
+// K x K kernel: 8 up to 7x7 (twice the workgroups of 16-row tiles, which a
+// 1920x1080 image needs to spread over all CUs; the window's 2R halo rows
+// stay few), 16 above.  Batches keep 16-row tiles.
 constexpr int sep_tile_rows(int k) { return k <= 7 ? 8 : 16; }
 
 // Tile order for 1-D grids: the hardware dispatches workgroup b to XCD

@@ -395,7 +395,10 @@ __device__ __forceinline__ uint32_t fetch_px(const Src &s, uint32_t y, uint32_t 
   const uint32_t xb = x * C;
   if (C == 1) return fetch<BUF, 1>(s, y, xb);
   if (C == 2) return fetch<BUF, 2>(s, y, xb);
-  if (C == 3) return fetch<BUF, 2>(s, y, xb) | (fetch<BUF, 1>(s, y, xb + 2) << 16);
+  if (C == 3) {
+    if (BUF) return window4(s.win, y * s.stride32 + xb) & 0xFFFFFFu;
+    return fetch<BUF, 2>(s, y, xb) | (fetch<BUF, 1>(s, y, xb + 2) << 16);
+  }
   return fetch<BUF, 4>(s, y, xb);
 }
 
