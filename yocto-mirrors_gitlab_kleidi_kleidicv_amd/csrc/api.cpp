@@ -129,6 +129,11 @@ class Session {
 // (the free waits), more than a small resize takes on the GPU.  A buffer
 // grows stream-ordered (the old one freed after the calls that read it) and
 // is kept until the process ends: per stream, the largest table size used.
+// Entries are keyed by the stream handle.  This relies on HIP releasing a
+// destroyed stream's object only once the work queued on it has drained, so
+// that a later stream given the same handle cannot overlap the calls that
+// used the buffer.  (The HIP runtime that torch ships has no hipStreamGetId
+// to tell the two streams apart.)
 class TableCache {
  public:
   struct Lease {
