@@ -201,6 +201,16 @@ constexpr int separable_smem_bytes() {
   return separable_win_bytes<K, TH>() + 2 * TH * SepWindow<K, TH>::kCols;
 }
 
+// Whether the horizontal pass can accumulate in 16-bit lanes: binomial 3x3
+// and 5x5 sums stay below 2^16 (at most 255 * 16 * 16 + 128 for 5x5), and
+// the saturating mode's clamp at 65535 is left unchanged by its final
+// min(255, ...) (every term is non-negative).  Binomial 7x7 and 9x9 and the
+// sigma taps (whose error-diffused sum can reach 257) keep 32-bit sums.
+template <int MODE, int K>
+constexpr bool sep_horizontal_u16() {
+  return MODE == kSepSaturate || (MODE == kSepBinomial && K <= 5);
+}
+
 template <int MODE, int K, int C, int TH>
 __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uint16_t *vert,
                                                      int e0, int y0) {
@@ -240,6 +250,44 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
       return (i & 1) ? __builtin_amdgcn_alignbit(v[(i >> 1) + 1], v[i >> 1], 16) : v[i >> 1];
     };
     uint32_t lo = 0, hi = 0;
+    if constexpr (sep_horizontal_u16<MODE, K>()) {
+      // two outputs per 16-bit-lane multiply-add: outputs (b, b + 1) take
+      // column-sum pairs (i, i + 1), an aligned dword of v for even i
+      // (every tap when C is even), else one v_alignbit
+      auto mac = [](sep_us2 acc, sep_us2 x, sep_us2 wt) {
+        if (MODE == kSepSaturate) {
+          sep_us2 r;
+          asm("v_pk_mad_u16 %0, %1, %2, %3 clamp" : "=v"(r) : "v"(x), "s"(wt), "v"(acc));
+          return r;
+        }
+        return acc + x * wt;
+      };
+      const uint16_t rnd = MODE == kSepBinomial ? static_cast<uint16_t>(1u << (a.shift - 1)) : 0;
+      const uint16_t sh = static_cast<uint16_t>(a.shift);
+      uint32_t word[2];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        sep_us2 acc[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const int b = 4 * half + 2 * q;
+          sep_us2 s = sep_us2{0, 0};
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            s = mac(s, __builtin_bit_cast(sep_us2, col_pair(8 * D + b + (j - R) * C)),
+                    sep_us2{static_cast<uint16_t>(w[j]), static_cast<uint16_t>(w[j])});
+          if (MODE == kSepBinomial)
+            s = (s + sep_us2{rnd, rnd}) >> sep_us2{sh, sh};
+          else
+            s = __builtin_elementwise_min(s, sep_us2{255, 255});
+          acc[q] = s;
+        }
+        word[half] = __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, acc[1]),
+                                           __builtin_bit_cast(uint32_t, acc[0]), 0x06040200u);
+      }
+      lo = word[0];
+      hi = word[1];
+    } else {
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       uint32_t acc = 0;
@@ -271,6 +319,7 @@ __device__ __forceinline__ void separable_horizontal(const SepArgs &a, const uin
         out = min(acc, 255u);
       if (b < 4) lo |= (out & 0xFFu) << (8 * b);
       else hi |= (out & 0xFFu) << (8 * (b - 4));
+    }
     }
     uint8_t *out = a.dst + static_cast<size_t>(y) * a.dst_stride;
     const int e = e0 + 8 * tx;
