@@ -57,6 +57,23 @@ def test_gaussian_binomial(rng, k, border):
             assert_same(got, want, "k%d %s %dx%dx%d" % (k, border, w, h, c))
 
 
+@pytest.mark.parametrize("k", [3, 5, 7])
+def test_gaussian_binomial_extreme_values(rng, k):
+    # all-255 images and 0/255 patterns drive the column and row sums to
+    # their largest values (255 * 16 * 16 + 128 for 5x5, the top of the
+    # 16-bit horizontal pass; 32-bit sums for 7x7)
+    h, w = 37, 300
+    for c in (1, 2, 3, 4):
+        for name, src in (("full", np.full((h, w, c), 255, np.uint8)),
+                          ("stripes", np.where((np.arange(w) // 3) % 2 == 0, 255, 0)
+                           .astype(np.uint8)[None, :, None].repeat(h, 0).repeat(c, 2)),
+                          ("sparse", np.where(rng.random((h, w, c)) < 0.9, 255, 0).astype(np.uint8))):
+            for border in ("REPLICATE", "REVERSE"):
+                got = K.gaussian_blur(src, k, 0.0, border)
+                _, want = O.gaussian_blur(src, k, 0.0, border)
+                assert_same(got, want, "k%d %s %s c%d" % (k, name, border, c))
+
+
 @pytest.mark.parametrize("k,sigma", [(3, 0.8), (5, 2.25), (7, 1.3), (9, 3.0), (15, 0.0),
                                      (21, 0.0), (15, 4.0), (21, 6.5), (5, 0.1)])
 def test_gaussian_sigma(rng, k, sigma):
