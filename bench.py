@@ -497,8 +497,8 @@ def host_api(K, np, s_host, params, ops, reps=3):
 
 
 def cpu_threads():
-    # the GPU box gives each GPU 16 host cores, whatever os.cpu_count()
-    # reports for the whole host
+    # one GPU's share of an MI355X host is 16 cores, whatever
+    # os.cpu_count() reports for the whole machine
     return max(1, min(16, os.cpu_count() or 1))
 
 
