@@ -46,14 +46,47 @@ struct Staged {
   const void *host = nullptr;  // null when the caller's pointer is used
   void *dev = nullptr;
   size_t bytes = 0;
+  bool owned = false;  // a stream-ordered allocation of its own
 };
+
+// Device staging memory of the synchronous (host-pointer) calls: one
+// grow-only buffer on the library stream, carved up by each call.  A call
+// holds the lock from its first copy to its final synchronisation, so calls
+// from several host threads never share it at once.  A call that needs more
+// than the buffer holds gets stream-ordered allocations for the rest, as
+// before, and the buffer then grows to that call's total.  A pair of
+// hipMallocAsync / hipFreeAsync per staged range cost about 17 us of host
+// time per range.
+struct StagingArena {
+  std::mutex mutex;
+  char *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+StagingArena &staging_arena() {
+  static StagingArena *a = new StagingArena;  // lives until the process ends
+  return *a;
+}
 
 class Session {
  public:
-  explicit Session(hipStream_t s) : stream_(s) {}
+  // arena: carve device memory from the staging arena (synchronous calls on
+  // the library stream only)
+  Session(hipStream_t s, bool arena) : stream_(s) {
+    if (arena) lock_ = std::unique_lock<std::mutex>(staging_arena().mutex);
+  }
   ~Session() {
     for (int i = 0; i < n_; ++i)
-      if (bufs_[i].dev && bufs_[i].host) (void)hipFreeAsync(bufs_[i].dev, stream_);
+      if (bufs_[i].owned) (void)hipFreeAsync(bufs_[i].dev, stream_);
+    if (lock_.owns_lock() && total_ > staging_arena().bytes) {
+      StagingArena &a = staging_arena();
+      void *p = nullptr;
+      if (kcv::device_alloc(&p, total_, stream_, true) == hipSuccess) {
+        if (a.ptr) (void)hipFreeAsync(a.ptr, stream_);  // after the calls that used it
+        a.ptr = static_cast<char *>(p);
+        a.bytes = total_;
+      }
+    }
   }
 
   // Device view of an input range; copies it in when it is host memory.
@@ -64,8 +97,9 @@ class Session {
       return KLEIDICV_OK;
     }
     void *d = nullptr;
-    KCV_TRY(from_hip(kcv::device_alloc(&d, bytes, stream_, true)));
-    bufs_[n_++] = Staged{p, d, bytes};
+    bool owned = false;
+    KCV_TRY(take(bytes, &d, &owned));
+    bufs_[n_++] = Staged{p, d, bytes, owned};
     KCV_TRY(from_hip(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, stream_)));
     *dev = static_cast<const T *>(d);
     return KLEIDICV_OK;
@@ -83,8 +117,9 @@ class Session {
       return KLEIDICV_OK;
     }
     void *d = nullptr;
-    KCV_TRY(from_hip(kcv::device_alloc(&d, bytes, stream_, true)));
-    bufs_[n_] = Staged{p, d, bytes};
+    bool owned = false;
+    KCV_TRY(take(bytes, &d, &owned));
+    bufs_[n_] = Staged{p, d, bytes, owned};
     out_[n_] = true;
     ++n_;
     if (rows > 1 && stride != row_bytes)
@@ -96,8 +131,9 @@ class Session {
   kleidicv_error_t scratch(size_t bytes, void **dev) {
     *dev = nullptr;
     if (bytes == 0) return KLEIDICV_OK;
-    KCV_TRY(from_hip(kcv::device_alloc(dev, bytes, stream_, true)));
-    bufs_[n_++] = Staged{*dev, *dev, bytes};
+    bool owned = false;
+    KCV_TRY(take(bytes, dev, &owned));
+    bufs_[n_++] = Staged{nullptr, *dev, bytes, owned};
     return KLEIDICV_OK;
   }
 
@@ -111,7 +147,26 @@ class Session {
   }
 
  private:
+  // Device memory for one staged range: from the arena while it has room,
+  // else a stream-ordered allocation of its own.
+  kleidicv_error_t take(size_t bytes, void **d, bool *owned) {
+    const size_t need = (bytes + 255) & ~size_t{255};
+    total_ += need;
+    StagingArena &a = staging_arena();
+    if (lock_.owns_lock() && used_ + need <= a.bytes) {
+      *d = a.ptr + used_;
+      used_ += need;
+      *owned = false;
+      return KLEIDICV_OK;
+    }
+    KCV_TRY(from_hip(kcv::device_alloc(d, bytes, stream_, true)));
+    *owned = true;
+    return KLEIDICV_OK;
+  }
+
   hipStream_t stream_;
+  std::unique_lock<std::mutex> lock_;
+  size_t used_ = 0, total_ = 0;
   Staged bufs_[8];
   bool out_[8] = {};
   int n_ = 0;
@@ -208,7 +263,7 @@ kleidicv_error_t gaussian_impl(const uint8_t *src, size_t src_stride, uint8_t *d
   if (act == Action::kLaunchArbitrary) {
     // the vertical pass's u8 intermediate, with the destination's stride
     const size_t tmp_bytes = extent(dst_stride, row_bytes, height);
-    Session ss(s);
+    Session ss(s, sync);
     void *tmp = nullptr;
     KCV_TRY(ss.scratch(tmp_bytes, &tmp));
     arb.tmp = static_cast<uint8_t *>(tmp);
@@ -228,7 +283,7 @@ kleidicv_error_t gaussian_impl(const uint8_t *src, size_t src_stride, uint8_t *d
                                        height, hipMemcpyDefault, s));
     return from_hip(kcv::launch_separable(a, s));
   }
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, row_bytes, height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, row_bytes, height, &a.dst));
   if (act == Action::kCopy)
@@ -250,7 +305,7 @@ kleidicv_error_t blur_down_impl(const uint8_t *src, size_t src_stride, size_t wi
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_blur_down(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, width * channels, height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, (width + 1) / 2 * channels, (height + 1) / 2, &a.dst));
   KCV_TRY(from_hip(kcv::launch_blur_down(a, s)));
@@ -267,7 +322,7 @@ kleidicv_error_t yuv_impl(const uint8_t *src, size_t src_stride, uint8_t *dst,
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_yuv_to_rgb(a, s));
-  Session ss(s);
+  Session ss(s, true);
   const uint8_t *dsrc = nullptr;
   KCV_TRY(ss.input(src, a.src_extent, &dsrc));
   // the chroma plane pointers move with the luma plane
@@ -290,7 +345,7 @@ kleidicv_error_t rgb_to_yuv_impl(const uint8_t *src, size_t src_stride, uint8_t 
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_rgb_to_yuv(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, width * a.scn, height), &a.src));
   const size_t out_bytes = a.layout == kcv::kYuv444 ? 3 * width
                            : a.layout == kcv::kYuv422 ? 2 * width : width;
@@ -315,7 +370,7 @@ kleidicv_error_t rgb_to_yuv_sp_impl(const uint8_t *src, size_t src_stride, uint8
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_rgb_to_yuv(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, width * a.scn, height), &a.src));
   KCV_TRY(ss.output(y_dst, y_stride, width, height, &a.dst));
   // (w+1)/2 interleaved pairs per chroma row
@@ -340,7 +395,7 @@ kleidicv_error_t separable_impl(const uint8_t *src, size_t src_stride, uint8_t *
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_separable(a, s));
   const size_t row_bytes = width * channels;
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, row_bytes, height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, row_bytes, height, &a.dst));
   KCV_TRY(from_hip(kcv::launch_separable(a, s)));
@@ -360,7 +415,7 @@ kleidicv_error_t median_impl(const uint8_t *src, size_t src_stride, uint8_t *dst
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_median(a, s));
   const size_t row_bytes = width * channels;
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, row_bytes, height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, row_bytes, height, &a.dst));
   KCV_TRY(from_hip(kcv::launch_median(a, s)));
@@ -384,7 +439,7 @@ kleidicv_error_t resize_impl(const uint8_t *src, size_t src_stride, size_t src_w
     KCV_TRY(table_cache().get(s, table_bytes, &lease));
     return from_hip(kcv::launch_resize(a, lease.ptr, s));
   }
-  Session ss(s);
+  Session ss(s, true);
   void *ws = nullptr;
   KCV_TRY(ss.scratch(table_bytes, &ws));
   KCV_TRY(ss.input(src, extent(src_stride, src_width * channels, src_height), &a.src));
@@ -409,7 +464,7 @@ kleidicv_error_t remap_s16point5_impl(
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_remap_s16point5(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, src_width * channels, src_height), &a.src));
   KCV_TRY(ss.input(mapxy, extent(mapxy_stride, dst_width * 4, dst_height), &a.mapxy));
   KCV_TRY(ss.input(mapfrac, extent(mapfrac_stride, dst_width * 2, dst_height), &a.mapfrac));
@@ -435,7 +490,7 @@ kleidicv_error_t remap_f32_impl(const uint8_t *src, size_t src_stride,
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_remap_f32(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, src_width * channels, src_height), &a.src));
   KCV_TRY(ss.input(mapx, extent(mapx_stride, dst_width * 4, dst_height), &a.mapx));
   KCV_TRY(ss.input(mapy, extent(mapy_stride, dst_width * 4, dst_height), &a.mapy));
@@ -459,7 +514,7 @@ kleidicv_error_t warp_impl(const uint8_t *src, size_t src_stride, size_t src_wid
   if (act == Action::kNone) return KLEIDICV_OK;
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_warp_perspective(a, s));
-  Session ss(s);
+  Session ss(s, true);
   KCV_TRY(ss.input(src, extent(src_stride, src_width * channels, src_height), &a.src));
   KCV_TRY(ss.output(dst, dst_stride, dst_width * channels, dst_height, &a.dst));
   KCV_TRY(from_hip(kcv::launch_warp_perspective(a, s)));
@@ -479,7 +534,7 @@ kleidicv_error_t yuv_sp_impl(const uint8_t *src_y, size_t src_y_stride,
   hipStream_t s = pick_stream(stream, sync);
   if (!sync) return from_hip(kcv::launch_yuv_sp_to_rgb(a, s));
   const size_t oc = a.alpha ? 4 : 3;
-  Session ss(s);
+  Session ss(s, true);
   const size_t uv_bytes = (width + 1) / 2 * 2;
   KCV_TRY(ss.input(src_y, extent(src_y_stride, width, height), &a.y));
   KCV_TRY(ss.input(src_uv, extent(src_uv_stride, uv_bytes, (height + 1) / 2), &a.uv));
@@ -955,3 +1010,16 @@ kleidicv_error_t kleidicv_hip_synchronize(void *stream) {
 }
 
 }  // extern "C"
+
+// Test hook: drops the drop-in staging buffer, so that the next synchronous
+// call allocates its staging again (allocation-failure tests).
+extern "C" void kcv_test_release_staging() {
+  StagingArena &a = staging_arena();
+  std::lock_guard<std::mutex> g(a.mutex);
+  if (a.ptr) {
+    (void)hipFreeAsync(a.ptr, kcv::runtime().stream);
+    (void)hipStreamSynchronize(kcv::runtime().stream);
+  }
+  a.ptr = nullptr;
+  a.bytes = 0;
+}

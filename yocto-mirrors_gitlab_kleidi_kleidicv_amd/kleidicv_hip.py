@@ -326,6 +326,12 @@ def fail_device_allocation(after):
     lib().kcv_test_fail_device_allocation(C.c_long(int(after)))
 
 
+def release_staging():
+    """Test hook: the next drop-in call allocates its device staging anew
+    (the library otherwise keeps one staging buffer for those calls)."""
+    lib().kcv_test_release_staging()
+
+
 def synchronize(stream=None):
     _check(lib().kleidicv_hip_synchronize(_stream_arg(stream)), "kleidicv_hip_synchronize")
 
