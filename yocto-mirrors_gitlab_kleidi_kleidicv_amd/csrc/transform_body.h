@@ -340,11 +340,6 @@ __device__ __forceinline__ float recip_refined(float w) {
   return fmaf(fmaf(-w, y0, 1.0f), y0, y0);
 }
 
-__device__ __forceinline__ float recip_rn(float w) {
-  if (__builtin_amdgcn_ballot_w64(!recip_refined_ok(w)) == 0) return recip_refined(w);
-  return 1.0f / w;
-}
-
 // Source image of a float gather.  When the image spans less than
 // kBufferSpanLimit bytes, its bytes are read through the
 // aligned-window buffer resource at 32-bit offsets (no 64-bit address
@@ -745,25 +740,38 @@ __device__ __forceinline__ WarpRow warp_row(const WarpArgs &a, int y) {
   return {fmaf(a.t[1], fy, a.t[2]), fmaf(a.t[4], fy, a.t[5]), fmaf(a.t[7], fy, a.t[8])};
 }
 
-__device__ __forceinline__ void warp_px(const WarpArgs &a, const WarpRow &r, int x, float &xf,
-                                        float &yf, float &tw) {
-  const float fx = static_cast<float>(static_cast<uint32_t>(x));
-  const float tx = fmaf(fx, a.t[0], r.tx0);
-  const float ty = fmaf(fx, a.t[3], r.ty0);
-  tw = fmaf(fx, a.t[6], r.tw0);
-  const float iw = (a.check_zero && tw == 0.0f) ? 0.0f : recip_rn(tw);
-  xf = tx * iw;
-  yf = ty * iw;
-}
-
 template <int C, int INTERP, int BORDER>
 __device__ __forceinline__ void warp_tile(const WarpArgs &a, int bx, int by) {
   const Src s = make_src<C>(a.src, a.src_stride, a.src_width, a.src_height, a.border_value);
   auto coords = [&](int y, int x, int, float (&xf)[4], float (&yf)[4]) {
     const WarpRow row = warp_row(a, y);
-    float tw;
+    float tx[4], ty[4], tw[4];
+    bool ok = true;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) warp_px(a, row, x + k, xf[k], yf[k], tw);
+    for (int k = 0; k < 4; ++k) {
+      const float fx = static_cast<float>(static_cast<uint32_t>(x + k));
+      tx[k] = fmaf(fx, a.t[0], row.tx0);
+      ty[k] = fmaf(fx, a.t[3], row.ty0);
+      tw[k] = fmaf(fx, a.t[6], row.tw0);
+      ok &= recip_refined_ok(tw[k]);
+    }
+    // 1/w for the thread's four weights with one vote of the wave: the
+    // refined reciprocal when every lane's weights allow it (it equals the
+    // IEEE quotient there), else the division
+    float r[4];
+    if (__builtin_amdgcn_ballot_w64(!ok) == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = recip_refined(tw[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) r[k] = 1.0f / tw[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float iw = (a.check_zero && tw[k] == 0.0f) ? 0.0f : r[k];
+      xf[k] = tx[k] * iw;
+      yf[k] = ty[k] * iw;
+    }
   };
   gather_tiles<C, INTERP, BORDER>(s, a.dst, a.dst_stride, a.dst_width, a.dst_height, bx, by,
                                   coords);
