@@ -442,6 +442,36 @@ def test_remap_s16point5_narrow_and_odd_stride(rng, c, border):
         assert_same(got, want, "%s c%d %dx%d -> %dx%d" % (border, c, sw, sh, dw, dh))
 
 
+@pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])
+def test_remap_s16point5_one_channel_zooms_and_smooth_maps(rng, border):
+    # Zooms by s from 1/4 to 7 (a destination tile's taps spread over 1/4 to
+    # 7 times its size in the source), smooth maps reaching past the image
+    # edges, a map smooth in one half and scattered in the other, odd row
+    # pitches and partial tiles.
+    cases = []
+    for s in (0.25, 1.0, 2.5, 3.5, 3.6, 3.7, 3.8, 4.0, 7.0):
+        dw, dh = 150, 37
+        sw, sh = int(dw * s) + 8, int(dh * s) + 8
+        yy, xx = np.mgrid[0:dh, 0:dw].astype(np.float64)
+        off = rng.uniform(-6, 6, 2)
+        cases.append((padded_rows(rng, sh, sw, 1, 0), xx * s + off[0], yy * s + off[1]))
+    for (sw, sh, dw, dh, pad) in [(300, 120, 333, 41, 3), (131, 67, 200, 90, 0), (40, 20, 77, 13, 1)]:
+        mx, my = smooth_maps(rng, dw, dh, sw, sh, 20)
+        cases.append((padded_rows(rng, sh, sw, 1, pad), mx, my))
+    mx, my = smooth_maps(rng, 256, 64, 256, 64, 10)
+    half = np.zeros((64, 256), dtype=bool)
+    half[:, 128:] = rng.random((64, 128)) < 0.02
+    mx[half] = rng.uniform(-50, 300, int(half.sum()))
+    my[half] = rng.uniform(-50, 120, int(half.sum()))
+    cases.append((padded_rows(rng, 64, 256, 1, 0), mx, my))
+    for src, mx, my in cases:
+        mapxy, mapfrac = s16point5_maps(mx, my)
+        bv = tuple(int(v) for v in rng.integers(0, 256, 4))
+        got = K.remap_s16point5(src, mapxy, mapfrac, border, bv)
+        _, want = O.remap_s16point5(np.ascontiguousarray(src), mapxy, mapfrac, border, bv)
+        assert_same(got, want, "%s %s -> %s" % (border, src.shape, mapxy.shape))
+
+
 @pytest.mark.parametrize("c", [1, 2, 3, 4])
 @pytest.mark.parametrize("interp", ["LINEAR", "NEAREST"])
 @pytest.mark.parametrize("border", ["REPLICATE", "CONSTANT"])

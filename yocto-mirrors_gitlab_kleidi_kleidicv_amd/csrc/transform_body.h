@@ -133,29 +133,38 @@ __device__ __forceinline__ uint32_t bilinear5(uint32_t a, uint32_t b,
   return (line0 * (32 - fy) + line1 * fy + 512) >> 10;
 }
 
-// One destination pixel from its map entries (xy: x in the low, y in the
-// high 16 bits; fr: 5-bit x and y fractions).
-template <int C, int BORDER>
-__device__ __forceinline__ uint32_t remap_s16_px(const SrcWindow &src, uint32_t stride,
-                                                 int w, int h, uint32_t xy, uint32_t fr,
-                                                 uint32_t bv) {
+// Taps of one destination pixel from its map entries (xy: x in the low, y
+// in the high 16 bits; fr: 5-bit x and y fractions): the pair column cx
+// (taps cx + i0 and cx + i1 of rows r0 and r1), the fractions and, for the
+// constant border, which taps lie inside the image.  Rows are clamped into
+// the image for both borders (a constant-border tap outside it is replaced
+// by the border value, whatever row it reads).
+struct S16Tap {
+  int cx, r0, r1, i0, i1;
+  uint32_t fx, fy;
+  bool ina, inb, inc, ind;
+};
+
+template <int BORDER>
+__device__ __forceinline__ S16Tap s16_tap(int w, int h, uint32_t xy, uint32_t fr) {
   const int mx = static_cast<int16_t>(xy & 0xFFFF);
   const int my = static_cast<int16_t>(xy >> 16);
-  uint32_t fx = fr & 31u, fy = (fr >> 5) & 31u;
-  int cx, cy0, cy1, i0, i1;           // pair column, rows, tap index in the pair
-  bool ina = true, inb = true, inc = true, ind = true;
+  S16Tap t;
+  t.fx = fr & 31u;
+  t.fy = (fr >> 5) & 31u;
+  t.ina = t.inb = t.inc = t.ind = true;
+  t.cx = max(min(mx, w - 2), 0);
+  t.r0 = min(max(my, 0), h - 1);
   if (BORDER == kReplicate) {
     // RemapS16Point5Replicate (:16-137) / get_coordinates_replicate
     // (:621-640): fractions of negative coordinates are dropped, taps
     // clamp to the image.
-    if (mx < 0) fx = 0;
-    if (my < 0) fy = 0;
+    if (mx < 0) t.fx = 0;
+    if (my < 0) t.fy = 0;
     const int x0 = min(max(mx, 0), w - 1);
-    cy0 = min(max(my, 0), h - 1);
-    cy1 = cy0 + (cy0 < h - 1 ? 1 : 0);
-    cx = max(min(x0, w - 2), 0);
-    i0 = x0 - cx;
-    i1 = min(x0 + 1, w - 1) - cx;
+    t.r1 = min(t.r0 + 1, h - 1);
+    t.i0 = x0 - t.cx;
+    t.i1 = min(x0 + 1, w - 1) - t.cx;
   } else {
     // RemapS16Point5ConstantBorder (:321-425) / get_coordinates_constant
     // (:860-881): each corner tests its own range with 16-bit unsigned
@@ -165,49 +174,63 @@ __device__ __forceinline__ uint32_t remap_s16_px(const SrcWindow &src, uint32_t 
     const uint32_t uy1 = static_cast<uint16_t>(uy0 + 1);
     const bool x0in = ux0 < static_cast<uint32_t>(w), x1in = ux1 < static_cast<uint32_t>(w);
     const bool y0in = uy0 < static_cast<uint32_t>(h), y1in = uy1 < static_cast<uint32_t>(h);
-    ina = x0in && y0in;
-    inb = x1in && y0in;
-    inc = x0in && y1in;
-    ind = x1in && y1in;
-    cy0 = y0in ? static_cast<int>(uy0) : 0;
-    cy1 = y1in ? static_cast<int>(uy1) : 0;
-    cx = max(min(mx, w - 2), 0);
-    i0 = (mx - cx) & 1;  // meaningful only when the tap is inside
-    i1 = (mx + 1 - cx) & 1;
+    t.ina = x0in && y0in;
+    t.inb = x1in && y0in;
+    t.inc = x0in && y1in;
+    t.ind = x1in && y1in;
+    t.r1 = min(max(my + 1, 0), h - 1);
+    t.i0 = (mx - t.cx) & 1;  // meaningful only when the tap is inside
+    t.i1 = (mx + 1 - t.cx) & 1;
   }
-  const uint32_t off0 = static_cast<uint32_t>(cy0) * stride + static_cast<uint32_t>(cx * C);
-  const uint32_t off1 = static_cast<uint32_t>(cy1) * stride + static_cast<uint32_t>(cx * C);
+  return t;
+}
+
+// The pixel from its four tap values (C channels each).
+template <int C, int BORDER>
+__device__ __forceinline__ uint32_t s16_blend(const S16Tap &t, uint32_t pa, uint32_t pb,
+                                              uint32_t pc, uint32_t pd, uint32_t bv) {
+  if (BORDER != kReplicate) {
+    pa = t.ina ? pa : bv;
+    pb = t.inb ? pb : bv;
+    pc = t.inc ? pc : bv;
+    pd = t.ind ? pd : bv;
+  }
+  uint32_t out = 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+    out |= bilinear5(byte_of(pa, c), byte_of(pb, c), byte_of(pc, c), byte_of(pd, c), t.fx, t.fy)
+           << (8 * c);
+  return out;
+}
+
+// One destination pixel, its taps gathered from memory.
+template <int C, int BORDER>
+__device__ __forceinline__ uint32_t remap_s16_px(const SrcWindow &src, uint32_t stride,
+                                                 int w, int h, uint32_t xy, uint32_t fr,
+                                                 uint32_t bv) {
+  const S16Tap t = s16_tap<BORDER>(w, h, xy, fr);
+  const uint32_t off0 = static_cast<uint32_t>(t.r0) * stride + static_cast<uint32_t>(t.cx * C);
+  const uint32_t off1 = static_cast<uint32_t>(t.r1) * stride + static_cast<uint32_t>(t.cx * C);
   uint32_t pa, pb, pc, pd;
   if (C == 1) {
     const uint32_t t0 = window4(src, off0);
     const uint32_t t1 = window4(src, off1);
-    pa = (t0 >> (8 * i0)) & 0xFFu;
-    pb = (t0 >> (8 * i1)) & 0xFFu;
-    pc = (t1 >> (8 * i0)) & 0xFFu;
-    pd = (t1 >> (8 * i1)) & 0xFFu;
+    pa = (t0 >> (8 * t.i0)) & 0xFFu;
+    pb = (t0 >> (8 * t.i1)) & 0xFFu;
+    pc = (t1 >> (8 * t.i0)) & 0xFFu;
+    pd = (t1 >> (8 * t.i1)) & 0xFFu;
   } else {
     typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
     const u32x2 t0 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(
                                                     src.rsrc, off0 + src.lead, 0, 0));
     const u32x2 t1 = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(
                                                     src.rsrc, off1 + src.lead, 0, 0));
-    pa = i0 ? t0.y : t0.x;
-    pb = i1 ? t0.y : t0.x;
-    pc = i0 ? t1.y : t1.x;
-    pd = i1 ? t1.y : t1.x;
+    pa = t.i0 ? t0.y : t0.x;
+    pb = t.i1 ? t0.y : t0.x;
+    pc = t.i0 ? t1.y : t1.x;
+    pd = t.i1 ? t1.y : t1.x;
   }
-  if (BORDER != kReplicate) {
-    pa = ina ? pa : bv;
-    pb = inb ? pb : bv;
-    pc = inc ? pc : bv;
-    pd = ind ? pd : bv;
-  }
-  uint32_t out = 0;
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-    out |= bilinear5(byte_of(pa, c), byte_of(pb, c), byte_of(pc, c), byte_of(pd, c), fx, fy)
-           << (8 * c);
-  return out;
+  return s16_blend<C, BORDER>(t, pa, pb, pc, pd, bv);
 }
 
 // Same pixel for a source one pixel wide (no pair fits in a row).
@@ -244,21 +267,12 @@ __device__ __forceinline__ uint32_t remap_s16_px_narrow(const RemapS16Args &a, u
   return out;
 }
 
-// Destination tile (bx, by) of gather_tile_w(C) x gather_tile_h(C) pixels.
-template <int C, int BORDER>
-__device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int bx, int by) {
-  constexpr int TW = gather_tile_w(C), TH = gather_tile_h(C);
-  const int q = threadIdx.x % (TW / 4), r = threadIdx.x / (TW / 4);
-  const int x = bx * TW + 4 * q, y = by * TH + r;
-  if (x >= a.dst_width || y >= a.dst_height) return;
-  const int n = min(4, a.dst_width - x);
-  uint32_t bv = 0;
-#pragma unroll
-  for (int c = 0; c < C; ++c)
-    bv |= static_cast<uint32_t>(a.border_value[C == 4 ? c : 0]) << (8 * c);
+// Map entries of the pixels x .. x + 3 of row y (n of them inside the
+// destination; the others repeat pixel x).
+__device__ __forceinline__ void load_s16_maps(const RemapS16Args &a, int x, int y, int n,
+                                              uint32_t (&xy)[4], uint32_t (&fr)[4]) {
   const uint8_t *mrow = reinterpret_cast<const uint8_t *>(a.mapxy) + static_cast<size_t>(y) * a.mapxy_stride;
   const uint8_t *frow = reinterpret_cast<const uint8_t *>(a.mapfrac) + static_cast<size_t>(y) * a.mapfrac_stride;
-  uint32_t xy[4], fr[4];
   if (n == 4 && ((reinterpret_cast<uintptr_t>(mrow) & 15) | (reinterpret_cast<uintptr_t>(frow) & 7)) == 0) {
     const uint4 m = *reinterpret_cast<const uint4 *>(mrow + 4 * static_cast<size_t>(x));
     const uint2 f = *reinterpret_cast<const uint2 *>(frow + 2 * static_cast<size_t>(x));
@@ -273,6 +287,37 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
       fr[k] = *reinterpret_cast<const uint16_t *>(frow + 2 * static_cast<size_t>(xx));
     }
   }
+}
+
+// n pixels (C channels each) of a destination row from x on.
+template <int C>
+__device__ __forceinline__ void store_s16_px(const RemapS16Args &a, int x, int y, int n,
+                                             const uint32_t (&out)[4]) {
+  uint8_t *d = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x) * C;
+  if (C == 1 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 3) == 0) {
+    store_nt(reinterpret_cast<uint32_t *>(d), out[0] | (out[1] << 8) | (out[2] << 16) | (out[3] << 24));
+  } else if (C == 4 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 15) == 0) {
+    store_nt(reinterpret_cast<uint4 *>(d), make_uint4(out[0], out[1], out[2], out[3]));
+  } else {
+    const bool al = (reinterpret_cast<uintptr_t>(d) & 3) == 0;
+    for (int k = 0; k < n; ++k) store_px<C>(d + k * C, out[k], al);
+  }
+}
+
+// Destination tile (bx, by) of gather_tile_w(C) x gather_tile_h(C) pixels.
+template <int C, int BORDER>
+__device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int bx, int by) {
+  constexpr int TW = gather_tile_w(C), TH = gather_tile_h(C);
+  const int q = threadIdx.x % (TW / 4), r = threadIdx.x / (TW / 4);
+  const int x = bx * TW + 4 * q, y = by * TH + r;
+  if (x >= a.dst_width || y >= a.dst_height) return;
+  const int n = min(4, a.dst_width - x);
+  uint32_t bv = 0;
+#pragma unroll
+  for (int c = 0; c < C; ++c)
+    bv |= static_cast<uint32_t>(a.border_value[C == 4 ? c : 0]) << (8 * c);
+  uint32_t xy[4], fr[4];
+  load_s16_maps(a, x, y, n, xy, fr);
   const int w = a.src_width, h = a.src_height;
   uint32_t out[4];
   if (w >= 2) {
@@ -286,15 +331,7 @@ __device__ __forceinline__ void remap_s16point5_tile(const RemapS16Args &a, int 
 #pragma unroll
     for (int k = 0; k < 4; ++k) out[k] = remap_s16_px_narrow<C, BORDER>(a, xy[k], fr[k], bv);
   }
-  uint8_t *d = a.dst + static_cast<size_t>(y) * a.dst_stride + static_cast<size_t>(x) * C;
-  if (C == 1 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 3) == 0) {
-    store_nt(reinterpret_cast<uint32_t *>(d), out[0] | (out[1] << 8) | (out[2] << 16) | (out[3] << 24));
-  } else if (C == 4 && n == 4 && (reinterpret_cast<uintptr_t>(d) & 15) == 0) {
-    store_nt(reinterpret_cast<uint4 *>(d), make_uint4(out[0], out[1], out[2], out[3]));
-  } else {
-    const bool al = (reinterpret_cast<uintptr_t>(d) & 3) == 0;
-    for (int k = 0; k < n; ++k) store_px<C>(d + k * C, out[k], al);
-  }
+  store_s16_px<C>(a, x, y, n, out);
 }
 
 // ---------------------------------------------------------------------------
