@@ -515,8 +515,11 @@ __device__ __forceinline__ Quad gather_linear(const Src &s, float xf, float yf) 
     inb = x1in && y0in;
     inc = x0in && y1in;
     ind = x1in && y1in;
-    y0 = y0in ? ya : 0u;
-    y1 = y1in ? yb : 0u;
+    // rows clamped into the image: an inside tap reads its own row, an
+    // outside one a row next to its neighbours' (its value is replaced)
+    const int32_t ymax = static_cast<int32_t>(s.ymax), ty = min(iy, ymax);
+    y0 = static_cast<uint32_t>(max(ty, 0));
+    y1 = static_cast<uint32_t>(min(max(ty + 1, 0), ymax));
     cx = s.xmax ? static_cast<uint32_t>(max(min(ix, static_cast<int32_t>(s.xmax) - 1), 0)) : 0u;
     i0 = (x0 - cx) & 1u;  // meaningful only when the tap is inside
     i1 = (x1 - cx) & 1u;
